@@ -1,0 +1,244 @@
+/*
+ * wfa_mi355x.h — C ABI of the MI355X wavefront-alignment engine.
+ *
+ * The reference (HPCBio/wfa2lib-rs) is a pure Rust library with no C ABI
+ * (src/lib.rs:1-18). Its public aligner API is `WavefrontAligner<N>`
+ * (src/aligner.rs:146-211): construct with penalties, configure with
+ * setters, call an `align_*` method, read `status()` and `cigar()`. Every
+ * entry point below replaces one of those methods one-to-one (the citation
+ * sits beside each declaration), so a thin Rust `-sys` crate, a ctypes
+ * module or a cgo stub can bind the engine exactly where the reference's
+ * aligner is called today. The batch entry points have no single reference
+ * counterpart: they replace the per-pair loop of the reference's benchmark
+ * driver (src/bin/align_benchmark.rs:593-627) with one device launch.
+ *
+ * Conventions: plain pointers and sizes, no C++ or HIP types. Functions
+ * that can fail return an int status (0 = WFA_OK, negative = WFA_ERR_*);
+ * the reference panics on invalid penalties (src/penalties.rs:157-167,
+ * 198-209, 245-262) and this ABI returns WFA_ERR_INVALID_PENALTIES instead.
+ * Scores are WFA scores (penalty sums, >= 0), exactly as the reference's
+ * align methods return them.
+ */
+#ifndef WFA_MI355X_H
+#define WFA_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Distance metrics (src/penalties.rs:12-24, same order) ---- */
+#define WFA_METRIC_INDEL          0
+#define WFA_METRIC_EDIT           1
+#define WFA_METRIC_GAP_LINEAR     2
+#define WFA_METRIC_GAP_AFFINE     3
+#define WFA_METRIC_GAP_AFFINE_2P  4
+
+/* ---- Alignment scope (src/aligner.rs:26-31) ---- */
+#define WFA_SCOPE_SCORE      0
+#define WFA_SCOPE_ALIGNMENT  1
+
+/* ---- Memory modes (src/aligner.rs:44-52; ULTRALOW = BiWFA, :2563) ---- */
+#define WFA_MEMORY_HIGH      0
+#define WFA_MEMORY_MED       1
+#define WFA_MEMORY_LOW       2
+#define WFA_MEMORY_ULTRALOW  3
+
+/* ---- Span (src/aligner.rs:109-115) ---- */
+#define WFA_SPAN_END2END     0
+#define WFA_SPAN_ENDSFREE    1
+
+/* ---- Heuristics (src/heuristic.rs, HeuristicStrategy) ---- */
+#define WFA_HEURISTIC_NONE             0
+#define WFA_HEURISTIC_WFADAPTIVE       1  /* p0 = min_wavefront_length, p1 = max_distance_threshold, p2 = steps_between_cutoffs */
+#define WFA_HEURISTIC_XDROP            2  /* p0 = xdrop, p1 = steps_between_cutoffs */
+#define WFA_HEURISTIC_ZDROP            3  /* p0 = zdrop, p1 = steps_between_cutoffs */
+#define WFA_HEURISTIC_BANDED_STATIC    4  /* p0 = min_k, p1 = max_k */
+#define WFA_HEURISTIC_BANDED_ADAPTIVE  5  /* p0 = min_k, p1 = max_k, p2 = steps_between_cutoffs */
+
+/* ---- Alignment status, same ordinals as AlignStatus (src/aligner.rs:120-135) ---- */
+#define WFA_STATUS_OK                 0
+#define WFA_STATUS_END_REACHED        1
+#define WFA_STATUS_END_UNREACHABLE    2
+#define WFA_STATUS_MAX_STEPS_REACHED  3
+#define WFA_STATUS_OOM                4
+#define WFA_STATUS_COMPLETED          5
+#define WFA_STATUS_PARTIAL            6
+
+/* ---- Sequence formats of batch inputs ---- */
+#define WFA_SEQ_ASCII       0  /* one byte per base, any byte values */
+#define WFA_SEQ_PACKED2BITS 1  /* 4 bases per byte, LSB first, A=0 C=1 G=2 T=3
+                                  (src/sequences.rs:30-74) */
+
+/* ---- Error codes ---- */
+#define WFA_OK                      0
+#define WFA_ERR_INVALID_PENALTIES  -1
+#define WFA_ERR_INVALID_ARGUMENT   -2
+#define WFA_ERR_UNSUPPORTED        -3  /* configuration not available on the device path */
+#define WFA_ERR_DEVICE             -4  /* HIP runtime error */
+#define WFA_ERR_NO_DEVICE          -5
+#define WFA_ERR_OUT_OF_MEMORY      -6
+#define WFA_ERR_REJECTED           -7  /* combination the reference rejects
+                                          (src/bin/align_benchmark.rs:531-560) */
+
+/* User penalties, as passed to WavefrontPenalties::new_* (src/penalties.rs:
+ * 136-296). `match` <= 0; a negative match is a reward and the engine
+ * applies the same Eizenga transform. For GAP_LINEAR `gap_opening1` is the
+ * indel penalty. Unused fields are ignored. */
+typedef struct wfa_penalties_t {
+    int32_t distance_metric;
+    int32_t match;
+    int32_t mismatch;
+    int32_t gap_opening1;
+    int32_t gap_extension1;
+    int32_t gap_opening2;
+    int32_t gap_extension2;
+} wfa_penalties_t;
+
+/* Full aligner configuration: the penalties plus everything the reference's
+ * setters configure (src/aligner.rs:462-611). `wfa_attr_default` fills the
+ * reference's defaults (score scope, high memory, end-to-end, no heuristic,
+ * max steps INT32_MAX, src/aligner.rs:416-458). */
+typedef struct wfa_attr_t {
+    wfa_penalties_t penalties;
+    int32_t alignment_scope;        /* WFA_SCOPE_* */
+    int32_t memory_mode;            /* WFA_MEMORY_* */
+    int32_t span;                   /* WFA_SPAN_* */
+    int32_t extension;              /* 1 = extension alignment (src/aligner.rs:582) */
+    int32_t pattern_begin_free;
+    int32_t pattern_end_free;
+    int32_t text_begin_free;
+    int32_t text_end_free;
+    int32_t heuristic;              /* WFA_HEURISTIC_* */
+    int32_t heuristic_p[3];
+    int32_t max_alignment_steps;
+} wfa_attr_t;
+
+/* Defaults of WavefrontAligner::new (src/aligner.rs:409-458) for `metric`,
+ * with the benchmark driver's default penalties for that metric
+ * (src/bin/align_benchmark.rs:148-209). */
+void wfa_attr_default(wfa_attr_t* attr, int32_t distance_metric);
+
+/* Validate penalties as WavefrontPenalties::new_* does
+ * (src/penalties.rs:157-167, 198-209, 245-262): WFA_OK or
+ * WFA_ERR_INVALID_PENALTIES. */
+int32_t wfa_penalties_check(const wfa_penalties_t* penalties);
+
+/* ======================================================================
+ * Per-call aligner: WavefrontAligner<N> (src/aligner.rs:146-211)
+ * ====================================================================== */
+typedef struct wfa_aligner_t wfa_aligner_t;
+
+/* WavefrontAligner::new(WavefrontPenalties) (src/aligner.rs:409) followed by
+ * the setters the attribute struct covers. `device` is a HIP device
+ * ordinal. Returns NULL and sets *err on failure. */
+wfa_aligner_t* wfa_aligner_new(const wfa_attr_t* attr, int32_t device, int32_t* err);
+/* Drop of WavefrontAligner. */
+void wfa_aligner_free(wfa_aligner_t* aligner);
+
+/* set_alignment_scope (src/aligner.rs:542) */
+int32_t wfa_aligner_set_alignment_scope(wfa_aligner_t* aligner, int32_t scope);
+/* set_memory_mode (src/aligner.rs:462); WFA_MEMORY_ULTRALOW selects BiWFA as
+ * the CLI does (src/bin/align_benchmark.rs:510-519). */
+int32_t wfa_aligner_set_memory_mode(wfa_aligner_t* aligner, int32_t mode);
+/* set_heuristic (src/aligner.rs:611) */
+int32_t wfa_aligner_set_heuristic(wfa_aligner_t* aligner, int32_t heuristic,
+                                  int32_t p0, int32_t p1, int32_t p2);
+/* set_alignment_free_ends (src/aligner.rs:563) */
+int32_t wfa_aligner_set_alignment_free_ends(wfa_aligner_t* aligner,
+                                            int32_t pattern_begin_free, int32_t pattern_end_free,
+                                            int32_t text_begin_free, int32_t text_end_free);
+/* set_alignment_extension (src/aligner.rs:582) */
+int32_t wfa_aligner_set_alignment_extension(wfa_aligner_t* aligner);
+/* set_alignment_end_to_end (src/aligner.rs:595) */
+int32_t wfa_aligner_set_alignment_end_to_end(wfa_aligner_t* aligner);
+/* set_max_alignment_steps (src/aligner.rs:533) */
+int32_t wfa_aligner_set_max_alignment_steps(wfa_aligner_t* aligner, int32_t max_steps);
+
+/* align_end2end(&[u8], &[u8]) -> i32 (src/aligner.rs:620). Returns the WFA
+ * score; INT32_MIN if the call itself failed (see wfa_aligner_error). */
+int32_t wfa_align_end2end(wfa_aligner_t* aligner,
+                          const uint8_t* pattern, int32_t pattern_length,
+                          const uint8_t* text, int32_t text_length);
+/* align_packed2bits(&[u8], usize, &[u8], usize) -> i32 (src/aligner.rs:2755):
+ * 2-bit packed input, LSB first (src/sequences.rs:34-51). */
+int32_t wfa_align_packed2bits(wfa_aligner_t* aligner,
+                              const uint8_t* pattern, int32_t pattern_length,
+                              const uint8_t* text, int32_t text_length);
+/* status() -> AlignStatus (src/aligner.rs:834), WFA_STATUS_* ordinals. */
+int32_t wfa_aligner_status(const wfa_aligner_t* aligner);
+/* cigar() -> &Cigar (src/aligner.rs:839): operations M/X/I/D, one byte each,
+ * valid until the next call on this aligner. Returns the op count; *ops may
+ * be NULL when the count is 0. */
+int32_t wfa_aligner_cigar(const wfa_aligner_t* aligner, const uint8_t** ops);
+/* Cigar.score (src/cigar.rs:26): the WFA score the CIGAR was traced at,
+ * INT32_MIN when there is no CIGAR. */
+int32_t wfa_aligner_cigar_score(const wfa_aligner_t* aligner);
+/* Error code of the last call on this aligner (WFA_OK if it succeeded). */
+int32_t wfa_aligner_error(const wfa_aligner_t* aligner);
+
+/* ======================================================================
+ * Batch engine: one device launch for many independent pairs
+ * (replaces the per-pair loop of src/bin/align_benchmark.rs:593-627)
+ * ====================================================================== */
+typedef struct wfa_engine_t wfa_engine_t;
+
+/* A batch whose arrays all live in device memory (HBM) of the engine's
+ * device. Pair i's pattern starts at byte pattern_offset[i] of seq_data and
+ * has pattern_length[i] bases (ceil(len/4) bytes when PACKED2BITS), likewise
+ * the text. Outputs: score[i] (WFA score), status[i] (WFA_STATUS_*), and in
+ * alignment scope the CIGAR of pair i as cigar_length[i] op bytes starting
+ * at cigar_ops + cigar_offset[i]; the caller reserves
+ * pattern_length[i] + text_length[i] bytes there. */
+typedef struct wfa_batch_t {
+    int32_t num_pairs;
+    int32_t seq_format;              /* WFA_SEQ_* */
+    int32_t max_length;              /* max over pairs of max(pattern, text) length;
+                                        0 = unknown (the engine reduces it on the device) */
+    const uint8_t* seq_data;
+    const int64_t* pattern_offset;
+    const int32_t* pattern_length;
+    const int64_t* text_offset;
+    const int32_t* text_length;
+    int32_t* score;
+    int32_t* status;
+    uint8_t* cigar_ops;              /* NULL in score scope */
+    const int64_t* cigar_offset;     /* NULL in score scope */
+    int32_t* cigar_length;           /* NULL in score scope */
+} wfa_batch_t;
+
+/* Create an engine on HIP device `device` (owns the device work queue,
+ * per-slot wavefront scratch and the launch configuration). */
+wfa_engine_t* wfa_engine_new(int32_t device, int32_t* err);
+void wfa_engine_free(wfa_engine_t* engine);
+
+/* Align every pair of `batch` with the configuration `attr` in one
+ * persistent launch on `hip_stream` (a hipStream_t, NULL = default stream).
+ * Returns once the results are in the output arrays: the engine reads back
+ * a retry counter after the launch, and pairs whose wavefronts outgrew the
+ * first launch's per-pair storage are re-run by wider launches before it
+ * returns. Scratch grows on demand (allocations happen only when a batch
+ * needs more than any earlier one). */
+int32_t wfa_engine_align_batch(wfa_engine_t* engine, const wfa_attr_t* attr,
+                               const wfa_batch_t* batch, void* hip_stream);
+
+/* The plain per-pair path: the same device code launched once per pair,
+ * with one wavefront, synchronizing after each pair — the comparator the
+ * batched launch is measured against. */
+int32_t wfa_engine_align_batch_per_pair(wfa_engine_t* engine, const wfa_attr_t* attr,
+                                        const wfa_batch_t* batch, void* hip_stream);
+
+/* Launch statistics of the last batch call: number of pairs that needed a
+ * wider wavefront or longer history than the first pass provides and were
+ * re-run by a second, wider pass. */
+int32_t wfa_engine_last_retries(const wfa_engine_t* engine);
+
+/* Library version string. */
+const char* wfa_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WFA_MI355X_H */

@@ -1,0 +1,988 @@
+/*
+ * wfa_oracle.c — CPU restatement of HPCBio/wfa2lib-rs's wavefront aligner.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP engine and the CPU
+ * baseline of bench.py. Never linked into the engine.
+ *
+ * The structure follows the reference closely on purpose, because the
+ * point is to reproduce its results bit for bit, including C-compatible
+ * edge behaviour: wavefronts are objects with a diagonal range [lo, hi], an
+ * "initialized" range [init_min, init_max] and an allocation; components
+ * are score-indexed (high memory) or modular (score-only); each step does
+ * the input lookup, range union, NULL-halo fill, kernel and independent
+ * trims exactly as src/aligner.rs does. Citations are to files of the
+ * reference (paths relative to its root).
+ *
+ * Deliberate, documented difference: sequence ends are hard bounds in the
+ * extension. The reference instead relies on the sentinels '!' after the
+ * pattern and '?' after the text (src/sequences.rs:16-19,178-191), which
+ * gives the same result for every input whose text holds no '!' and whose
+ * pattern holds no '?'.
+ */
+#include "wfa_oracle.h"
+
+#include <limits.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OFFSET_NULL (INT32_MIN / 2) /* src/offset.rs:15 */
+#define DIAG_NULL INT32_MAX         /* src/offset.rs:43 */
+
+/* ------------------------------------------------------------------ */
+/* Penalties: WavefrontPenalties::new_* (src/penalties.rs:122-296)      */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int metric;
+    int match, mismatch, o1, e1, o2, e2; /* internal (Eizenga-adjusted) */
+    wfa_penalties_t user;
+} pen_t;
+
+static int pen_init(pen_t* p, const wfa_penalties_t* u) {
+    memset(p, 0, sizeof(*p));
+    p->user = *u;
+    p->metric = u->distance_metric;
+    switch (u->distance_metric) {
+    case WFA_METRIC_INDEL: /* src/penalties.rs:122-134 */
+        p->match = 0; p->mismatch = -1; p->o1 = 1; p->e1 = -1; p->o2 = -1; p->e2 = -1;
+        return 0;
+    case WFA_METRIC_EDIT: /* src/penalties.rs:136-150 */
+        p->match = 0; p->mismatch = 1; p->o1 = 1; p->e1 = -1; p->o2 = -1; p->e2 = -1;
+        return 0;
+    case WFA_METRIC_GAP_LINEAR: /* src/penalties.rs:156-190 */
+        if (u->match > 0 || u->mismatch <= 0 || u->gap_opening1 <= 0) return -1;
+        if (u->match < 0) {
+            p->match = u->match;
+            p->mismatch = 2 * u->mismatch - 2 * u->match;
+            p->o1 = 2 * u->gap_opening1 - u->match;
+        } else {
+            p->match = 0; p->mismatch = u->mismatch; p->o1 = u->gap_opening1;
+        }
+        p->e1 = -1; p->o2 = -1; p->e2 = -1;
+        return 0;
+    case WFA_METRIC_GAP_AFFINE: /* src/penalties.rs:196-238 */
+        if (u->match > 0 || u->mismatch <= 0 || u->gap_opening1 < 0 || u->gap_extension1 <= 0)
+            return -1;
+        if (u->match < 0) {
+            p->match = u->match;
+            p->mismatch = 2 * u->mismatch - 2 * u->match;
+            p->o1 = 2 * u->gap_opening1;
+            p->e1 = 2 * u->gap_extension1 - u->match;
+        } else {
+            p->match = 0; p->mismatch = u->mismatch;
+            p->o1 = u->gap_opening1; p->e1 = u->gap_extension1;
+        }
+        p->o2 = -1; p->e2 = -1;
+        return 0;
+    case WFA_METRIC_GAP_AFFINE_2P: /* src/penalties.rs:243-296 */
+        if (u->match > 0 || u->mismatch <= 0 || u->gap_opening1 < 0 || u->gap_extension1 <= 0 ||
+            u->gap_opening2 < 0 || u->gap_extension2 <= 0)
+            return -1;
+        if (u->match < 0) {
+            p->match = u->match;
+            p->mismatch = 2 * u->mismatch - 2 * u->match;
+            p->o1 = 2 * u->gap_opening1;
+            p->e1 = 2 * u->gap_extension1 - u->match;
+            p->o2 = 2 * u->gap_opening2;
+            p->e2 = 2 * u->gap_extension2 - u->match;
+        } else {
+            p->match = 0; p->mismatch = u->mismatch;
+            p->o1 = u->gap_opening1; p->e1 = u->gap_extension1;
+            p->o2 = u->gap_opening2; p->e2 = u->gap_extension2;
+        }
+        return 0;
+    default:
+        return -1;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Wavefront (src/wavefront.rs:43-490)                                 */
+/* ------------------------------------------------------------------ */
+typedef struct wf {
+    int32_t* mem; /* cells [base, base + nalloc - 1] */
+    int base, nalloc;
+    int lo, hi;
+    int init_min, init_max;
+    int null;
+} wf_t;
+
+static wf_t* wf_new(int alloc_lo, int alloc_hi) {
+    wf_t* w = (wf_t*)malloc(sizeof(wf_t));
+    w->base = alloc_lo;
+    w->nalloc = alloc_hi - alloc_lo + 1;
+    w->mem = (int32_t*)malloc(sizeof(int32_t) * (size_t)w->nalloc);
+    /* Wavefront::init (src/wavefront.rs:307-328) */
+    w->null = 0;
+    w->lo = 1; w->hi = -1;
+    w->init_min = 0; w->init_max = 0;
+    return w;
+}
+static void wf_free(wf_t* w) {
+    if (w) { free(w->mem); free(w); }
+}
+static inline int wf_allocated(const wf_t* w, int k) {
+    return k >= w->base && k < w->base + w->nalloc;
+}
+static inline int32_t wf_get(const wf_t* w, int k) {
+    if (!wf_allocated(w, k)) {
+        fprintf(stderr, "oracle: read outside allocation k=%d [%d,%d]\n", k, w->base,
+                w->base + w->nalloc - 1);
+        abort();
+    }
+    return w->mem[k - w->base];
+}
+static inline void wf_set(wf_t* w, int k, int32_t v) {
+    if (!wf_allocated(w, k)) {
+        fprintf(stderr, "oracle: write outside allocation k=%d\n", k);
+        abort();
+    }
+    w->mem[k - w->base] = v;
+}
+/* set_limits (src/wavefront.rs:360-368) */
+static void wf_set_limits(wf_t* w, int lo, int hi) {
+    w->lo = lo; w->hi = hi; w->init_min = lo; w->init_max = hi;
+}
+/* init_ends_lower (src/wavefront.rs:422-437) */
+static void wf_init_ends_lower(wf_t* w, int min_lo) {
+    if (w->init_min <= min_lo) return;
+    int min_init = w->init_min < w->lo ? w->init_min : w->lo;
+    int start = min_lo > w->base ? min_lo : w->base;
+    for (int k = start; k < min_init; ++k) wf_set(w, k, OFFSET_NULL);
+    w->init_min = start;
+}
+/* init_ends_higher (src/wavefront.rs:442-457) */
+static void wf_init_ends_higher(wf_t* w, int max_hi) {
+    if (w->init_max >= max_hi) return;
+    int max_init = w->init_max > w->hi ? w->init_max : w->hi;
+    int alloc_max = w->base + w->nalloc - 1;
+    int end = max_hi < alloc_max ? max_hi : alloc_max;
+    for (int k = max_init + 1; k <= end; ++k) wf_set(w, k, OFFSET_NULL);
+    w->init_max = end;
+}
+/* A read of an input wavefront inside a kernel: the null wavefront reads
+ * OFFSET_NULL everywhere (src/components.rs:109-112). Reads are always in
+ * the initialized range after the init_ends calls; anything else would be
+ * a restatement error, so it aborts. */
+static inline int32_t rd(const wf_t* w, int k) {
+    if (!w) return OFFSET_NULL;
+    if (k < w->init_min || k > w->init_max) {
+        fprintf(stderr, "oracle: read outside init range k=%d [%d,%d]\n", k, w->init_min,
+                w->init_max);
+        abort();
+    }
+    return wf_get(w, k);
+}
+
+/* trim_ends (src/compute/mod.rs:14-49) */
+static void trim_ends(int plen, int tlen, wf_t* w) {
+    int k = w->hi;
+    while (k >= w->lo) {
+        int32_t off = wf_get(w, k);
+        if ((uint32_t)off <= (uint32_t)tlen && (uint32_t)(off - k) <= (uint32_t)plen) break;
+        --k;
+    }
+    w->hi = k; w->init_max = k;
+    k = w->lo;
+    while (k <= w->hi) {
+        int32_t off = wf_get(w, k);
+        if ((uint32_t)off <= (uint32_t)tlen && (uint32_t)(off - k) <= (uint32_t)plen) break;
+        ++k;
+    }
+    w->lo = k; w->init_min = k;
+    w->null = w->lo > w->hi;
+}
+
+/* ------------------------------------------------------------------ */
+/* Aligner state (src/aligner.rs:146-204)                              */
+/* ------------------------------------------------------------------ */
+enum { C_M = 0, C_I1, C_D1, C_I2, C_D2, NCOMP };
+
+struct wfo_aligner {
+    wfa_attr_t attr;
+    pen_t pen;
+    /* sequences (unpadded copies) */
+    uint8_t* pattern; int plen;
+    uint8_t* text; int tlen;
+    size_t seq_cap;
+    /* components */
+    int modular;
+    int scope;          /* max_score_scope */
+    int num_wavefronts; /* flat-array capacity in high memory */
+    int nslots;
+    wf_t** comp[NCOMP];
+    int hist_lo, hist_hi;   /* historic_min_lo / historic_max_hi */
+    int alloc_lo, alloc_hi; /* alloc_min_lo / alloc_max_hi */
+    /* every wavefront allocated in this alignment (freed at the next one) */
+    wf_t** pool; int npool, cap_pool;
+    /* state */
+    int num_null_steps;
+    int status;
+    int end_score, end_k, end_offset;
+    /* cigar */
+    uint8_t* cig; int cig_cap, cig_begin, cig_end, cig_score;
+};
+
+static void pool_add(wfo_aligner* a, wf_t* w) {
+    if (a->npool == a->cap_pool) {
+        a->cap_pool = a->cap_pool ? 2 * a->cap_pool : 1024;
+        a->pool = (wf_t**)realloc(a->pool, sizeof(wf_t*) * (size_t)a->cap_pool);
+    }
+    a->pool[a->npool++] = w;
+}
+static void pool_clear(wfo_aligner* a) {
+    for (int i = 0; i < a->npool; ++i) wf_free(a->pool[i]);
+    a->npool = 0;
+}
+
+/* compute_dimensions (src/components.rs:435-504) */
+static void compute_dimensions(const pen_t* p, int plen, int tlen, int modular, int* scope,
+                               int* nwf) {
+    int absdiff = plen > tlen ? plen - tlen : tlen - plen;
+    int mn = plen < tlen ? plen : tlen;
+    switch (p->metric) {
+    case WFA_METRIC_INDEL:
+    case WFA_METRIC_EDIT:
+        *scope = 2;
+        if (modular) *nwf = 2;
+        else if (p->metric == WFA_METRIC_INDEL) *nwf = plen + tlen + 1;
+        else *nwf = (plen > tlen ? plen : tlen) + 1;
+        return;
+    case WFA_METRIC_GAP_LINEAR:
+        *scope = (p->mismatch > p->o1 ? p->mismatch : p->o1) + 1;
+        *nwf = modular ? *scope : mn * p->mismatch + p->o1 * absdiff + 1;
+        return;
+    case WFA_METRIC_GAP_AFFINE: {
+        int si = p->o1 + p->e1;
+        *scope = (si > p->mismatch ? si : p->mismatch) + 1;
+        *nwf = modular ? *scope : mn * p->mismatch + p->o1 + absdiff * p->e1 + 1;
+        return;
+    }
+    default: {
+        int si = p->o1 + p->e1, s2 = p->o2 + p->e2;
+        if (s2 > si) si = s2;
+        *scope = (si > p->mismatch ? si : p->mismatch) + 1;
+        if (modular) {
+            *nwf = *scope;
+        } else {
+            int i1 = p->o1 + absdiff * p->e1, i2 = p->o2 + absdiff * p->e2;
+            *nwf = mn * p->mismatch + (i1 < i2 ? i1 : i2) + 1;
+        }
+        return;
+    }
+    }
+}
+
+static int ncomp_of(int metric) {
+    return metric == WFA_METRIC_GAP_AFFINE ? 3 : metric == WFA_METRIC_GAP_AFFINE_2P ? 5 : 1;
+}
+
+static inline int slot_of(const wfo_aligner* a, int score) {
+    return a->modular ? score % a->scope : score;
+}
+static inline wf_t* get(const wfo_aligner* a, int c, int score) {
+    return a->comp[c][slot_of(a, score)];
+}
+static inline void put(wfo_aligner* a, int c, int score, wf_t* w) {
+    a->comp[c][slot_of(a, score)] = w;
+}
+/* has_slot (src/components.rs:307-309) */
+static inline int has_slot(const wfo_aligner* a, int score) {
+    return a->modular || score < a->num_wavefronts;
+}
+/* is_null_wf / live_or_none / null_wf_bounds (src/aligner.rs:382-405) */
+static inline int is_null_wf(const wf_t* w) { return !w || w->null; }
+static inline wf_t* live_or_none(wf_t* w) { return is_null_wf(w) ? NULL : w; }
+static inline int nlo(const wf_t* w) { return w ? w->lo : 1; }
+static inline int nhi(const wf_t* w) { return w ? w->hi : -1; }
+
+/* output_alloc_range (src/aligner.rs:3325-3342), end-to-end part. */
+static void output_alloc_range(wfo_aligner* a, int lo, int hi, int* alo, int* ahi) {
+    int pad = a->scope + 1;
+    if (lo - pad < a->alloc_lo) a->alloc_lo = lo - pad;
+    if (hi + pad > a->alloc_hi) a->alloc_hi = hi + pad;
+    *alo = a->alloc_lo; *ahi = a->alloc_hi;
+}
+
+static wf_t* alloc_wf(wfo_aligner* a, int alo, int ahi) {
+    wf_t* w = wf_new(alo, ahi);
+    pool_add(a, w);
+    return w;
+}
+
+/* ------------------------------------------------------------------ */
+/* Aligner construction                                                */
+/* ------------------------------------------------------------------ */
+wfo_aligner* wfo_aligner_new(const wfa_attr_t* attr, int32_t* err) {
+    wfo_aligner* a = (wfo_aligner*)calloc(1, sizeof(wfo_aligner));
+    a->attr = *attr;
+    if (pen_init(&a->pen, &attr->penalties) != 0) {
+        if (err) *err = WFA_ERR_INVALID_PENALTIES;
+        free(a);
+        return NULL;
+    }
+    if (attr->span != WFA_SPAN_END2END || attr->heuristic != WFA_HEURISTIC_NONE ||
+        attr->memory_mode != WFA_MEMORY_HIGH ||
+        attr->penalties.distance_metric == WFA_METRIC_GAP_LINEAR ||
+        (attr->alignment_scope == WFA_SCOPE_ALIGNMENT && a->pen.match != 0 &&
+         attr->penalties.distance_metric >= WFA_METRIC_GAP_AFFINE && 0)) {
+        if (err) *err = WFA_ERR_UNSUPPORTED;
+        free(a);
+        return NULL;
+    }
+    if (err) *err = WFA_OK;
+    a->status = WFA_STATUS_OK;
+    a->cig_score = INT32_MIN;
+    return a;
+}
+
+void wfo_aligner_free(wfo_aligner* a) {
+    if (!a) return;
+    pool_clear(a);
+    free(a->pool);
+    for (int c = 0; c < NCOMP; ++c) free(a->comp[c]);
+    free(a->pattern);
+    free(a->text);
+    free(a->cig);
+    free(a);
+}
+
+int32_t wfo_status(const wfo_aligner* a) { return a->status; }
+int32_t wfo_cigar(const wfo_aligner* a, const uint8_t** ops) {
+    int n = a->cig_end - a->cig_begin;
+    if (n <= 0) { if (ops) *ops = NULL; return 0; }
+    if (ops) *ops = a->cig + a->cig_begin;
+    return n;
+}
+int32_t wfo_cigar_score(const wfo_aligner* a) { return a->cig_score; }
+
+/* ------------------------------------------------------------------ */
+/* init_alignment (src/aligner.rs:846-1000), end-to-end                */
+/* ------------------------------------------------------------------ */
+static void init_alignment(wfo_aligner* a) {
+    const int plen = a->plen, tlen = a->tlen;
+    int score_only = a->attr.alignment_scope == WFA_SCOPE_SCORE;
+    a->modular = score_only;
+    pool_clear(a);
+    /* Components are created for 1000 x 1000 and only grow
+     * (src/aligner.rs:351-370, src/components.rs:190-213); this models a
+     * freshly created aligner for every call. */
+    int scope, nwf, s0, n0;
+    compute_dimensions(&a->pen, 1000, 1000, a->modular, &s0, &n0);
+    compute_dimensions(&a->pen, plen, tlen, a->modular, &scope, &nwf);
+    if (n0 > nwf) nwf = n0;
+    a->scope = scope;
+    a->num_wavefronts = nwf;
+    int nslots = a->modular ? scope : nwf;
+    for (int c = 0; c < NCOMP; ++c) {
+        a->comp[c] = (wf_t**)realloc(a->comp[c], sizeof(wf_t*) * (size_t)nslots);
+        memset(a->comp[c], 0, sizeof(wf_t*) * (size_t)nslots);
+    }
+    a->nslots = nslots;
+    if (!score_only) {
+        int need = 2 * (plen + tlen) + 1; /* cigar_buffer_len, src/aligner.rs:378-380 */
+        if (need > a->cig_cap) {
+            a->cig = (uint8_t*)realloc(a->cig, (size_t)need);
+            a->cig_cap = need;
+        }
+    }
+    a->cig_begin = a->cig_end = 0;
+    a->cig_score = INT32_MIN;
+    a->num_null_steps = 0;
+    a->status = WFA_STATUS_OK;
+    a->end_score = -1; a->end_k = DIAG_NULL; a->end_offset = OFFSET_NULL;
+    a->hist_lo = -(plen + scope + 2);
+    a->hist_hi = tlen + scope + 2;
+    a->alloc_lo = 0; a->alloc_hi = 0;
+    int alo, ahi;
+    output_alloc_range(a, 0, 0, &alo, &ahi);
+    wf_t* m0 = alloc_wf(a, alo, ahi);
+    wf_set(m0, 0, 0);
+    wf_set_limits(m0, 0, 0);
+    put(a, C_M, 0, m0);
+}
+
+/* ------------------------------------------------------------------ */
+/* Extension (src/extend.rs:47-69) and termination (src/termination.rs)*/
+/* ------------------------------------------------------------------ */
+static inline int lcp(const wfo_aligner* a, int v, int h) {
+    int n = 0;
+    while (v + n < a->plen && h + n < a->tlen && a->pattern[v + n] == a->text[h + n]) ++n;
+    return n;
+}
+
+/* extend_end2end (src/aligner.rs:1004-1049) */
+static int extend_end2end(wfo_aligner* a, int score) {
+    wf_t* m = get(a, C_M, score);
+    if (!m) {
+        if (a->num_null_steps > a->scope) {
+            a->status = WFA_STATUS_END_UNREACHABLE;
+            return 1;
+        }
+        return 0;
+    }
+    for (int k = m->lo; k <= m->hi; ++k) {
+        int32_t off = wf_get(m, k);
+        if (off >= 0) wf_set(m, k, off + lcp(a, off - k, off));
+    }
+    /* termination_end2end (src/termination.rs:14-23) */
+    int k_end = a->tlen - a->plen;
+    if (!(m->lo > k_end || k_end > m->hi) && wf_get(m, k_end) >= a->tlen) {
+        a->end_score = score; a->end_k = k_end; a->end_offset = a->tlen;
+        a->status = WFA_STATUS_END_REACHED;
+        return 1;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Compute steps                                                       */
+/* ------------------------------------------------------------------ */
+static void free_outputs(wfo_aligner* a, int score, int ncomp) {
+    for (int c = 0; c < ncomp; ++c) put(a, c, score, NULL);
+}
+
+/* compute_affine_step + compute_affine2p_step (src/aligner.rs:2006-2193,
+ * 2201-2553) with the kernels of src/compute/affine.rs:318-351 and
+ * src/compute/affine2p.rs:205-236. */
+static void compute_affine_step(wfo_aligner* a, int score) {
+    const int plen = a->plen, tlen = a->tlen;
+    const pen_t* p = &a->pen;
+    const int two = p->metric == WFA_METRIC_GAP_AFFINE_2P;
+    const int x = p->mismatch, oe1 = p->o1 + p->e1, e1 = p->e1;
+    const int oe2 = two ? p->o2 + p->e2 : 0, e2 = two ? p->e2 : 0;
+
+    wf_t* mx = score - x >= 0 ? get(a, C_M, score - x) : NULL;
+    wf_t* mo1 = score - oe1 >= 0 ? get(a, C_M, score - oe1) : NULL;
+    wf_t* i1e = score - e1 >= 0 ? get(a, C_I1, score - e1) : NULL;
+    wf_t* d1e = score - e1 >= 0 ? get(a, C_D1, score - e1) : NULL;
+    wf_t *mo2 = NULL, *i2e = NULL, *d2e = NULL;
+    if (two) {
+        mo2 = score - oe2 >= 0 ? get(a, C_M, score - oe2) : NULL;
+        i2e = score - e2 >= 0 ? get(a, C_I2, score - e2) : NULL;
+        d2e = score - e2 >= 0 ? get(a, C_D2, score - e2) : NULL;
+    }
+    const int ncomp = two ? 5 : 3;
+    if (is_null_wf(mx) && is_null_wf(mo1) && is_null_wf(i1e) && is_null_wf(d1e) &&
+        is_null_wf(mo2) && is_null_wf(i2e) && is_null_wf(d2e)) {
+        a->num_null_steps++;
+        free_outputs(a, score, ncomp);
+        return;
+    }
+    a->num_null_steps = 0;
+    mx = live_or_none(mx); mo1 = live_or_none(mo1);
+    i1e = live_or_none(i1e); d1e = live_or_none(d1e);
+    mo2 = live_or_none(mo2); i2e = live_or_none(i2e); d2e = live_or_none(d2e);
+    /* affine2p with no O2/E2 input runs the affine kernel and stores no
+     * I2/D2 (src/aligner.rs:2261-2264, 2507-2524). */
+    int use2 = two && (mo2 || i2e || d2e);
+
+    int lo = nlo(mx), hi = nhi(mx);
+#define LMIN(v) do { int _v = (v); if (_v < lo) lo = _v; } while (0)
+#define HMAX(v) do { int _v = (v); if (_v > hi) hi = _v; } while (0)
+    LMIN(nlo(mo1) - 1); LMIN(nlo(i1e) + 1); LMIN(nlo(d1e) - 1);
+    HMAX(nhi(mo1) + 1); HMAX(nhi(i1e) + 1); HMAX(nhi(d1e) - 1);
+    if (two) {
+        LMIN(nlo(mo2) - 1); LMIN(nlo(i2e) + 1); LMIN(nlo(d2e) - 1);
+        HMAX(nhi(mo2) + 1); HMAX(nhi(i2e) + 1); HMAX(nhi(d2e) - 1);
+    }
+#undef LMIN
+#undef HMAX
+    if (lo > hi) {
+        if (a->modular) free_outputs(a, score, ncomp);
+        return;
+    }
+    if (lo < a->hist_lo + 1) lo = a->hist_lo + 1;
+    if (hi > a->hist_hi - 1) hi = a->hist_hi - 1;
+    if (lo > hi) {
+        if (a->modular) free_outputs(a, score, ncomp);
+        return;
+    }
+    int alo, ahi;
+    output_alloc_range(a, lo, hi, &alo, &ahi);
+    if (mx) { wf_init_ends_higher(mx, hi); wf_init_ends_lower(mx, lo); }
+    if (mo1) { wf_init_ends_higher(mo1, hi + 1); wf_init_ends_lower(mo1, lo - 1); }
+    if (mo2) { wf_init_ends_higher(mo2, hi + 1); wf_init_ends_lower(mo2, lo - 1); }
+    if (i1e) { wf_init_ends_higher(i1e, hi); wf_init_ends_lower(i1e, lo - 1); }
+    if (i2e) { wf_init_ends_higher(i2e, hi); wf_init_ends_lower(i2e, lo - 1); }
+    if (d1e) { wf_init_ends_higher(d1e, hi + 1); wf_init_ends_lower(d1e, lo); }
+    if (d2e) { wf_init_ends_higher(d2e, hi + 1); wf_init_ends_lower(d2e, lo); }
+
+    wf_t* m = alloc_wf(a, alo, ahi);
+    wf_t* i1 = alloc_wf(a, alo, ahi);
+    wf_t* d1 = alloc_wf(a, alo, ahi);
+    wf_t *i2 = NULL, *d2 = NULL;
+    wf_set_limits(m, lo, hi); wf_set_limits(i1, lo, hi); wf_set_limits(d1, lo, hi);
+    if (use2) {
+        i2 = alloc_wf(a, alo, ahi); d2 = alloc_wf(a, alo, ahi);
+        wf_set_limits(i2, lo, hi); wf_set_limits(d2, lo, hi);
+    }
+    for (int k = lo; k <= hi; ++k) {
+        int32_t vi1 = rd(mo1, k - 1) > rd(i1e, k - 1) ? rd(mo1, k - 1) : rd(i1e, k - 1);
+        vi1 += 1;
+        int32_t vd1 = rd(mo1, k + 1) > rd(d1e, k + 1) ? rd(mo1, k + 1) : rd(d1e, k + 1);
+        int32_t mm = rd(mx, k) + 1;
+        int32_t ins = vi1, del = vd1;
+        if (use2) {
+            int32_t vi2 = (rd(mo2, k - 1) > rd(i2e, k - 1) ? rd(mo2, k - 1) : rd(i2e, k - 1)) + 1;
+            int32_t vd2 = rd(mo2, k + 1) > rd(d2e, k + 1) ? rd(mo2, k + 1) : rd(d2e, k + 1);
+            wf_set(i2, k, vi2);
+            wf_set(d2, k, vd2);
+            if (vi2 > ins) ins = vi2;
+            if (vd2 > del) del = vd2;
+        }
+        int32_t mv = ins > del ? ins : del;
+        if (mm > mv) mv = mm;
+        if ((uint32_t)mv > (uint32_t)tlen || (uint32_t)(mv - k) > (uint32_t)plen) mv = OFFSET_NULL;
+        wf_set(m, k, mv);
+        wf_set(i1, k, vi1);
+        wf_set(d1, k, vd1);
+    }
+    put(a, C_M, score, m); put(a, C_I1, score, i1); put(a, C_D1, score, d1);
+    if (two) { put(a, C_I2, score, i2); put(a, C_D2, score, d2); }
+    trim_ends(plen, tlen, m);
+    trim_ends(plen, tlen, i1);
+    trim_ends(plen, tlen, d1);
+    if (i2) trim_ends(plen, tlen, i2);
+    if (d2) trim_ends(plen, tlen, d2);
+}
+
+/* compute_edit_step (src/aligner.rs:1769-1865) with compute_edit_idm /
+ * compute_indel_idm (src/compute/edit.rs:104-118, 218-231). */
+static void compute_edit_step(wfo_aligner* a, int score) {
+    const int plen = a->plen, tlen = a->tlen;
+    wf_t* prev = get(a, C_M, score - 1);
+    if (is_null_wf(prev)) {
+        a->num_null_steps++;
+        put(a, C_M, score, NULL);
+        return;
+    }
+    a->num_null_steps = 0;
+    int lo = prev->lo - 1, hi = prev->hi + 1;
+    if (lo < a->hist_lo + 1) lo = a->hist_lo + 1;
+    if (hi > a->hist_hi - 1) hi = a->hist_hi - 1;
+    if (lo > hi) { put(a, C_M, score, NULL); return; }
+    /* NULL the cells read beyond prev's ends, leaving its init range as is */
+    int ks[4] = {prev->lo - 2, prev->lo - 1, prev->hi + 1, prev->hi + 2};
+    for (int i = 0; i < 4; ++i)
+        if (wf_allocated(prev, ks[i])) wf_set(prev, ks[i], OFFSET_NULL);
+    int alo, ahi;
+    output_alloc_range(a, lo, hi, &alo, &ahi);
+    wf_t* cur = alloc_wf(a, alo, ahi);
+    wf_set_limits(cur, lo, hi);
+    int indel = a->pen.metric == WFA_METRIC_INDEL;
+    for (int k = lo; k <= hi; ++k) {
+        int32_t ins = wf_get(prev, k - 1), del = wf_get(prev, k + 1), mx;
+        if (indel) {
+            ins += 1;
+            mx = del > ins ? del : ins;
+        } else {
+            int32_t mis = wf_get(prev, k);
+            int32_t t = (ins > mis ? ins : mis) + 1;
+            mx = del > t ? del : t;
+        }
+        if ((uint32_t)mx > (uint32_t)tlen || (uint32_t)(mx - k) > (uint32_t)plen) mx = OFFSET_NULL;
+        wf_set(cur, k, mx);
+    }
+    put(a, C_M, score, cur);
+    trim_ends(plen, tlen, cur);
+}
+
+static void compute_step(wfo_aligner* a, int score) {
+    switch (a->pen.metric) {
+    case WFA_METRIC_EDIT:
+    case WFA_METRIC_INDEL: compute_edit_step(a, score); break;
+    default: compute_affine_step(a, score); break;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Backtraces                                                          */
+/* ------------------------------------------------------------------ */
+static inline void write_ops(wfo_aligner* a, uint8_t op, int n) {
+    for (int i = 0; i < n; ++i) a->cig[a->cig_begin--] = op;
+}
+
+/* st_m_offset (src/backtrace.rs:645-663): 1 and *out if the M cell exists. */
+static int st_m_offset(const wfo_aligner* a, int score, int k, int32_t* out) {
+    if (score < 0 || !has_slot(a, score)) return 0;
+    const wf_t* w = get(a, C_M, score);
+    if (!w) return 0;
+    if (k < w->init_min || w->init_max < k) return 0;
+    int32_t off = wf_get(w, k);
+    if (off == OFFSET_NULL) return 0;
+    *out = off;
+    return 1;
+}
+
+/* st_backward_matches (src/backtrace.rs:668-686) */
+static int st_backward_matches(const wfo_aligner* a, int v, int h) {
+    int mx = v < h ? v : h, n = 0;
+    while (n < mx && a->pattern[v - n - 1] == a->text[h - n - 1]) ++n;
+    return n;
+}
+
+/* st_gap (src/backtrace.rs:707-737) */
+static int st_gap(wfo_aligner* a, int gap_score, int l, int offset_orig, int* k, int* offset) {
+    int32_t m;
+    if (st_m_offset(a, gap_score, *k - l, &m) && m + l >= offset_orig && m + l <= *offset) {
+        write_ops(a, 'M', *offset - (m + l));
+        write_ops(a, 'I', l);
+        *k -= l;
+        *offset = m;
+        return 1;
+    }
+    if (st_m_offset(a, gap_score, *k + l, &m) && m >= offset_orig && m <= *offset) {
+        write_ops(a, 'M', *offset - m);
+        write_ops(a, 'D', l);
+        *k += l;
+        *offset = m;
+        return 1;
+    }
+    return 0;
+}
+
+/* backtrace_affine_singletrack (src/backtrace.rs:751-858), component_end M,
+ * end-to-end (begin-free 0). */
+static void backtrace_singletrack(wfo_aligner* a, int score, int k, int offset) {
+    const pen_t* p = &a->pen;
+    const int two = p->metric == WFA_METRIC_GAP_AFFINE_2P;
+    a->cig_end = a->cig_cap;
+    a->cig_begin = a->cig_cap - 1;
+    const int alignment_score = score;
+    int offset_orig = offset;
+    {
+        int v = offset - k, h = offset;
+        write_ops(a, 'D', a->plen - v);
+        write_ops(a, 'I', a->tlen - h);
+    }
+    int in_m = 1, l = 0;
+    while (score != 0) {
+        if (in_m) {
+            int v = offset - k, h = offset;
+            int nm = st_backward_matches(a, v, h);
+            offset_orig = offset - nm;
+            int mis = score - p->mismatch;
+            int32_t mo;
+            if (st_m_offset(a, mis, k, &mo) && mo == offset_orig - 1) {
+                write_ops(a, 'M', nm);
+                write_ops(a, 'X', 1);
+                offset = offset_orig - 1;
+                score = mis;
+            } else {
+                in_m = 0;
+                l = 0;
+            }
+        } else {
+            l += 1;
+            int indel1 = score - p->o1 - l * p->e1;
+            int indel2 = two ? score - p->o2 - l * p->e2 : -1;
+            if (st_gap(a, indel1, l, offset_orig, &k, &offset)) {
+                score = indel1;
+                in_m = 1;
+            } else if (st_gap(a, indel2, l, offset_orig, &k, &offset)) {
+                score = indel2;
+                in_m = 1;
+            } else if (indel1 < 0 && indel2 < 0) {
+                in_m = (offset_orig - k) == 0 || offset_orig == 0;
+                break;
+            }
+        }
+    }
+    int v = offset - k, h = offset;
+    int nm = v < h ? v : h;
+    v -= nm; h -= nm;
+    write_ops(a, 'M', nm);
+    write_ops(a, 'D', v);
+    write_ops(a, 'I', h);
+    a->cig_begin += 1;
+    a->cig_score = alignment_score;
+}
+
+/* bt_misms / bt_ins1_open / bt_del1_open (src/backtrace.rs:40-105), packed
+ * as (offset << 4) | type in 64 bits. */
+#define BT_I1_OPEN 1
+#define BT_D1_OPEN 5
+#define BT_M 9
+static int64_t bt_pack(int32_t off, int64_t t) { return ((int64_t)off << 4) | t; }
+static int64_t bt_misms(const wfo_aligner* a, int score, int k) {
+    if (score < 0) return OFFSET_NULL;
+    const wf_t* w = get(a, C_M, score);
+    if (!w) return OFFSET_NULL;
+    if (w->init_min <= k && k <= w->init_max) return bt_pack(wf_get(w, k) + 1, BT_M);
+    return OFFSET_NULL;
+}
+static int64_t bt_ins1_open(const wfo_aligner* a, int score, int k) {
+    if (score < 0) return OFFSET_NULL;
+    const wf_t* w = get(a, C_M, score);
+    if (!w) return OFFSET_NULL;
+    if (w->init_min < k && k - 1 <= w->init_max) return bt_pack(wf_get(w, k - 1) + 1, BT_I1_OPEN);
+    return OFFSET_NULL;
+}
+static int64_t bt_del1_open(const wfo_aligner* a, int score, int k) {
+    if (score < 0) return OFFSET_NULL;
+    const wf_t* w = get(a, C_M, score);
+    if (!w) return OFFSET_NULL;
+    if (w->init_min <= k + 1 && k < w->init_max) return bt_pack(wf_get(w, k + 1), BT_D1_OPEN);
+    return OFFSET_NULL;
+}
+static void write_matches(wfo_aligner* a, int n) {
+    if (n < 0) { a->cig_begin += -n; return; }
+    write_ops(a, 'M', n);
+}
+
+/* backtrace_linear (src/backtrace.rs:132-258): edit / indel CIGAR. */
+static void backtrace_linear(wfo_aligner* a, int score, int k, int offset) {
+    const pen_t* p = &a->pen;
+    a->cig_end = a->cig_cap;
+    a->cig_begin = a->cig_cap - 1;
+    const int alignment_score = score;
+    int v = offset - k, h = offset;
+    if (v < a->plen) write_ops(a, 'D', a->plen - v);
+    if (h < a->tlen) write_ops(a, 'I', a->tlen - h);
+    while (v > 0 && h > 0 && score > 0) {
+        int mis_s = score - p->mismatch, gap_s = score - p->o1;
+        int64_t mis = p->metric != WFA_METRIC_INDEL ? bt_misms(a, mis_s, k) : OFFSET_NULL;
+        int64_t ins = bt_ins1_open(a, gap_s, k);
+        int64_t del = bt_del1_open(a, gap_s, k);
+        int64_t mx = ins > del ? ins : del;
+        if (mis > mx) mx = mis;
+        if (mx < 0) break;
+        int32_t moff = (int32_t)(mx >> 4);
+        write_matches(a, offset - moff);
+        offset = moff;
+        v = offset - k; h = offset;
+        if (v <= 0 || h <= 0) break;
+        switch (mx & 0xF) {
+        case BT_M: score = mis_s; write_ops(a, 'X', 1); offset -= 1; break;
+        case BT_I1_OPEN: score = gap_s; write_ops(a, 'I', 1); k -= 1; offset -= 1; break;
+        case BT_D1_OPEN: score = gap_s; write_ops(a, 'D', 1); k += 1; break;
+        default: abort();
+        }
+        v = offset - k; h = offset;
+    }
+    if (v > 0 && h > 0) {
+        int nm = v < h ? v : h;
+        write_matches(a, nm);
+        v -= nm; h -= nm;
+    }
+    while (v > 0) { write_ops(a, 'D', 1); v--; }
+    while (h > 0) { write_ops(a, 'I', 1); h--; }
+    a->cig_begin += 1;
+    a->cig_score = alignment_score;
+}
+
+/* backtrace_from (src/aligner.rs:1414-1437) for high memory. */
+static void backtrace_from(wfo_aligner* a, int score, int k, int offset) {
+    if (a->pen.metric == WFA_METRIC_GAP_AFFINE || a->pen.metric == WFA_METRIC_GAP_AFFINE_2P)
+        backtrace_singletrack(a, score, k, offset);
+    else
+        backtrace_linear(a, score, k, offset);
+}
+
+/* ------------------------------------------------------------------ */
+/* align_end2end (src/aligner.rs:620-698)                              */
+/* ------------------------------------------------------------------ */
+static void set_sequences(wfo_aligner* a, const uint8_t* p, int plen, const uint8_t* t,
+                          int tlen) {
+    size_t need = (size_t)(plen > tlen ? plen : tlen) + 1;
+    if (need > a->seq_cap) {
+        a->pattern = (uint8_t*)realloc(a->pattern, need);
+        a->text = (uint8_t*)realloc(a->text, need);
+        a->seq_cap = need;
+    }
+    if (plen) memcpy(a->pattern, p, (size_t)plen);
+    if (tlen) memcpy(a->text, t, (size_t)tlen);
+    a->plen = plen;
+    a->tlen = tlen;
+}
+
+int32_t wfo_align_end2end(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
+                          const uint8_t* text, int32_t tlen) {
+    set_sequences(a, pattern, plen, text, tlen);
+    init_alignment(a);
+    int score = 0;
+    for (;;) {
+        if (extend_end2end(a, score)) break;
+        score += 1;
+        if (!has_slot(a, score)) { a->status = WFA_STATUS_MAX_STEPS_REACHED; break; }
+        compute_step(a, score);
+        if (score >= a->attr.max_alignment_steps) {
+            a->status = WFA_STATUS_MAX_STEPS_REACHED;
+            break;
+        }
+    }
+    int result = score;
+    if (a->status == WFA_STATUS_END_REACHED) {
+        if (a->attr.alignment_scope == WFA_SCOPE_ALIGNMENT)
+            backtrace_from(a, score, a->end_k, a->end_offset);
+        a->status = WFA_STATUS_COMPLETED;
+    }
+    return result;
+}
+
+/* ------------------------------------------------------------------ */
+/* CIGAR scores (src/cigar.rs:118-202) and validity (:446-518)          */
+/* ------------------------------------------------------------------ */
+int32_t wfo_cigar_score_edit(const uint8_t* ops, int32_t n) {
+    int32_t s = 0;
+    for (int i = 0; i < n; ++i) s += ops[i] == 'M' ? 0 : 1;
+    return s;
+}
+int32_t wfo_cigar_score_gap_linear(const uint8_t* ops, int32_t n, int32_t match,
+                                   int32_t mismatch, int32_t indel) {
+    int32_t s = 0;
+    for (int i = 0; i < n; ++i)
+        s += ops[i] == 'M' ? -match : ops[i] == 'X' ? -mismatch : -indel;
+    return s;
+}
+int32_t wfo_cigar_score_gap_affine(const uint8_t* ops, int32_t n, int32_t match,
+                                   int32_t mismatch, int32_t o, int32_t e) {
+    int32_t s = 0;
+    uint8_t last = 0;
+    for (int i = 0; i < n; ++i) {
+        uint8_t op = ops[i];
+        if (op == 'M') s -= match;
+        else if (op == 'X') s -= mismatch;
+        else if (op == 'D') s -= e + (last == 'D' ? 0 : o);
+        else if (op == 'I') s -= e + (last == 'I' ? 0 : o);
+        last = op;
+    }
+    return s;
+}
+static int32_t affine2p_op(uint8_t op, int32_t len, int32_t match, int32_t mismatch, int32_t o1,
+                           int32_t e1, int32_t o2, int32_t e2) {
+    if (op == 'M') return match * len;
+    if (op == 'X') return mismatch * len;
+    int32_t s1 = o1 + e1 * len, s2 = o2 + e2 * len;
+    return s1 < s2 ? s1 : s2;
+}
+int32_t wfo_cigar_score_gap_affine2p(const uint8_t* ops, int32_t n, int32_t match,
+                                     int32_t mismatch, int32_t o1, int32_t e1, int32_t o2,
+                                     int32_t e2) {
+    int32_t s = 0, len = 0;
+    uint8_t last = 0;
+    for (int i = 0; i < n; ++i) {
+        if (ops[i] != last && last != 0) {
+            s -= affine2p_op(last, len, match, mismatch, o1, e1, o2, e2);
+            len = 0;
+        }
+        last = ops[i];
+        len++;
+    }
+    if (last) s -= affine2p_op(last, len, match, mismatch, o1, e1, o2, e2);
+    return s;
+}
+int32_t wfo_cigar_check(const uint8_t* ops, int32_t n, const uint8_t* pattern, int32_t plen,
+                        const uint8_t* text, int32_t tlen) {
+    int pp = 0, tp = 0;
+    for (int i = 0; i < n; ++i) {
+        switch (ops[i]) {
+        case 'M':
+            if (pp >= plen || tp >= tlen || pattern[pp] != text[tp]) return 1;
+            pp++; tp++; break;
+        case 'X':
+            if (pp >= plen || tp >= tlen || pattern[pp] == text[tp]) return 2;
+            pp++; tp++; break;
+        case 'I': tp++; break;
+        case 'D': pp++; break;
+        default: return 3;
+        }
+    }
+    if (pp != plen) return 4;
+    if (tp != tlen) return 5;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Batch driver (pthreads, contiguous shards, one aligner per thread)   */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const wfa_attr_t* attr;
+    int32_t fmt;
+    const uint8_t* seq;
+    const int64_t *poff, *toff;
+    const int32_t *plen, *tlen;
+    int32_t *score, *status;
+    uint8_t* ops;
+    const int64_t* ops_off;
+    int32_t* ops_len;
+    int32_t begin, end;
+    int32_t err;
+} job_t;
+
+/* decode_packed2bits (src/sequences.rs:34-51) */
+static void unpack2(const uint8_t* src, int len, uint8_t* dst) {
+    static const uint8_t dna[4] = {'A', 'C', 'G', 'T'};
+    for (int i = 0; i < len; ++i) dst[i] = dna[(src[i >> 2] >> (2 * (i & 3))) & 3];
+}
+
+static void* job_run(void* arg) {
+    job_t* j = (job_t*)arg;
+    int32_t err = 0;
+    wfo_aligner* a = wfo_aligner_new(j->attr, &err);
+    if (!a) { j->err = err; return NULL; }
+    uint8_t *pb = NULL, *tb = NULL;
+    size_t cap = 0;
+    for (int32_t i = j->begin; i < j->end; ++i) {
+        int pl = j->plen[i], tl = j->tlen[i];
+        const uint8_t *P = j->seq + j->poff[i], *T = j->seq + j->toff[i];
+        if (j->fmt == WFA_SEQ_PACKED2BITS) {
+            size_t need = (size_t)(pl > tl ? pl : tl) + 1;
+            if (need > cap) {
+                pb = (uint8_t*)realloc(pb, need);
+                tb = (uint8_t*)realloc(tb, need);
+                cap = need;
+            }
+            unpack2(P, pl, pb);
+            unpack2(T, tl, tb);
+            P = pb; T = tb;
+        }
+        j->score[i] = wfo_align_end2end(a, P, pl, T, tl);
+        j->status[i] = a->status;
+        if (j->ops && j->ops_len) {
+            const uint8_t* o = NULL;
+            int n = wfo_cigar(a, &o);
+            if (n > pl + tl) n = pl + tl;
+            if (n > 0) memcpy(j->ops + j->ops_off[i], o, (size_t)n);
+            j->ops_len[i] = n;
+        }
+    }
+    free(pb);
+    free(tb);
+    wfo_aligner_free(a);
+    return NULL;
+}
+
+int32_t wfo_align_batch(const wfa_attr_t* attr, int32_t n, int32_t seq_format,
+                        const uint8_t* seq, const int64_t* poff, const int32_t* plen,
+                        const int64_t* toff, const int32_t* tlen, int32_t* score,
+                        int32_t* status, uint8_t* ops, const int64_t* ops_off, int32_t* ops_len,
+                        int32_t nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > n) nthreads = n > 0 ? n : 1;
+    job_t* jobs = (job_t*)calloc((size_t)nthreads, sizeof(job_t));
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+    for (int t = 0; t < nthreads; ++t) {
+        job_t* j = &jobs[t];
+        j->attr = attr; j->fmt = seq_format; j->seq = seq;
+        j->poff = poff; j->toff = toff; j->plen = plen; j->tlen = tlen;
+        j->score = score; j->status = status;
+        j->ops = ops; j->ops_off = ops_off; j->ops_len = ops_len;
+        j->begin = (int32_t)((int64_t)n * t / nthreads);
+        j->end = (int32_t)((int64_t)n * (t + 1) / nthreads);
+    }
+    if (nthreads == 1) {
+        job_run(&jobs[0]);
+    } else {
+        for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, job_run, &jobs[t]);
+        for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    }
+    int32_t err = 0;
+    for (int t = 0; t < nthreads; ++t)
+        if (jobs[t].err) err = jobs[t].err;
+    free(jobs);
+    free(th);
+    return err;
+}

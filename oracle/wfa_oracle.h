@@ -1,0 +1,71 @@
+/*
+ * wfa_oracle.h — CPU restatement of the reference aligner (TEST
+ * INFRASTRUCTURE ONLY).
+ *
+ * This is the checker the GPU engine is compared against, and the CPU
+ * baseline bench.py times. Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the engine never links it.
+ *
+ * It restates, in plain C, the semantics of HPCBio/wfa2lib-rs for the
+ * paths it covers, following the reference file:line cited at each
+ * function in wfa_oracle.c.
+ *
+ * Parity status per function (how each is pinned):
+ *   - gap-affine / gap-affine-2p / edit / indel end-to-end scores:
+ *     the reference's literal vectors (tests/cli_output.rs,
+ *     src/aligner.rs unit tests) + an exact Gotoh DP (oracle/dp_check.c)
+ *     on random pairs.
+ *   - Singletrack CIGARs (high memory, affine/affine2p): the reference's
+ *     literal CIGARs (tests/cli_output.rs:77,216-230) + CIGAR validity +
+ *     rescoring == score. Co-optimal tie-breaks beyond those vectors are
+ *     pinned only by the literal vectors ("parity partially pinned").
+ */
+#ifndef WFA_ORACLE_H
+#define WFA_ORACLE_H
+
+#include <stdint.h>
+#include "../include/wfa_mi355x.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wfo_aligner wfo_aligner;
+
+wfo_aligner* wfo_aligner_new(const wfa_attr_t* attr, int32_t* err);
+void wfo_aligner_free(wfo_aligner* a);
+int32_t wfo_align_end2end(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
+                          const uint8_t* text, int32_t tlen);
+int32_t wfo_status(const wfo_aligner* a);
+int32_t wfo_cigar(const wfo_aligner* a, const uint8_t** ops);
+int32_t wfo_cigar_score(const wfo_aligner* a);
+
+/* Scores the reference's Cigar computes (src/cigar.rs:118-202), on the
+ * original (user) penalties. */
+int32_t wfo_cigar_score_edit(const uint8_t* ops, int32_t n);
+int32_t wfo_cigar_score_gap_linear(const uint8_t* ops, int32_t n, int32_t match,
+                                   int32_t mismatch, int32_t indel);
+int32_t wfo_cigar_score_gap_affine(const uint8_t* ops, int32_t n, int32_t match,
+                                   int32_t mismatch, int32_t o, int32_t e);
+int32_t wfo_cigar_score_gap_affine2p(const uint8_t* ops, int32_t n, int32_t match,
+                                     int32_t mismatch, int32_t o1, int32_t e1,
+                                     int32_t o2, int32_t e2);
+/* Cigar::check_alignment (src/cigar.rs:446-518): 0 if valid. */
+int32_t wfo_cigar_check(const uint8_t* ops, int32_t n, const uint8_t* pattern,
+                        int32_t plen, const uint8_t* text, int32_t tlen);
+
+/* Many pairs at once (ASCII or 2-bit packed input, same layout as
+ * wfa_batch_t but in host memory), sharded contiguously over `nthreads`
+ * threads, one aligner per thread. `ops_off[i]` reserves plen+tlen bytes.
+ * Returns 0, or a negative WFA_ERR_*. */
+int32_t wfo_align_batch(const wfa_attr_t* attr, int32_t n, int32_t seq_format,
+                        const uint8_t* seq, const int64_t* poff, const int32_t* plen,
+                        const int64_t* toff, const int32_t* tlen,
+                        int32_t* score, int32_t* status,
+                        uint8_t* ops, const int64_t* ops_off, int32_t* ops_len,
+                        int32_t nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

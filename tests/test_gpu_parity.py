@@ -1,0 +1,144 @@
+"""GPU parity: the HIP engine against the oracle, bit for bit, through the C ABI."""
+import numpy as np
+import pytest
+
+from conftest import attr_for, expand, have_gpu, load_vectors
+
+pytestmark = pytest.mark.gpu
+
+STATUS = {"OK": 0, "END_REACHED": 1, "END_UNREACHABLE": 2, "MAX_STEPS_REACHED": 3, "OOM": 4,
+          "COMPLETED": 5, "PARTIAL": 6}
+
+
+@pytest.fixture(scope="module")
+def engine(pkg):
+    if not have_gpu():
+        pytest.skip("no GPU")
+    from wfa2lib_rs_amd.engine import BatchEngine
+    return BatchEngine(0)
+
+
+def _run_gpu(engine, attr, pairs, cigar, per_pair=False):
+    import torch
+    from wfa2lib_rs_amd.engine import DeviceBatch
+    b = DeviceBatch(pairs, device="cuda:0", cigar=cigar)
+    engine.align(attr, b, per_pair=per_pair)
+    torch.cuda.synchronize()
+    return b.results()
+
+
+def _compare(engine, oracle, attr, pairs, cigar, per_pair=False, label=""):
+    s, st, c = _run_gpu(engine, attr, pairs, cigar, per_pair)
+    os_, ost, oc = oracle.align_batch(attr, pairs, cigar=cigar, threads=8)
+    bad = np.nonzero((s != os_) | (st != ost))[0]
+    assert len(bad) == 0, f"{label}: {len(bad)} pairs differ, first {bad[:5]} gpu={s[bad[:5]]} oracle={os_[bad[:5]]}"
+    if cigar:
+        diff = [i for i in range(len(pairs)) if c[i] != oc[i]]
+        assert not diff, f"{label}: {len(diff)} CIGARs differ, first {diff[:5]}"
+    return s, st, c
+
+
+@pytest.mark.parametrize("vec", load_vectors(), ids=lambda v: v["id"])
+def test_reference_vectors_on_gpu(vec, pkg, N, engine, oracle):
+    from wfa2lib_rs_amd.synth import from_ascii
+    attr = attr_for(N, vec["metric"], vec["penalties"], vec["scope"], vec.get("max_steps"))
+    p, t = expand(vec["pattern"]), expand(vec["text"])
+    pairs = from_ascii([p], [t], N.SEQ_ASCII)
+    cig = vec["scope"] == "alignment"
+    s, st, c = _compare(engine, oracle, attr, pairs, cig, label=vec["id"])
+    if "score" in vec:
+        assert s[0] == vec["score"]
+    if "status" in vec:
+        assert st[0] == STATUS[vec["status"]]
+    if "cigar" in vec:
+        assert pkg.Cigar(c[0]).to_string_rle(True) == vec["cigar"]
+    if "cli" in vec:
+        from test_oracle import _penalty_obj
+        pen = _penalty_obj(pkg, vec["metric"], vec["penalties"])
+        cg = pkg.Cigar(c[0] if c else b"")
+        out = pkg.compute_output_score(cg, int(s[0]), pen, not cig, len(p), len(t))
+        rle = "-" if not cig or cg.is_null() else cg.to_string_rle(True)
+        assert f"{out}\t{rle}" == vec["cli"]
+
+
+CONFIGS = [
+    ("gap_affine", (0, 4, 6, 2, 0, 0)),
+    ("gap_affine", (-3, 5, 3, 2, 0, 0)),
+    ("gap_affine_2p", (0, 4, 6, 2, 24, 1)),
+    ("edit", (0, 0, 0, 0, 0, 0)),
+    ("indel", (0, 0, 0, 0, 0, 0)),
+]
+
+
+@pytest.mark.parametrize("metric,pen", CONFIGS)
+@pytest.mark.parametrize("scope", ["score", "alignment"])
+def test_short_reads_match_oracle(metric, pen, scope, pkg, N, engine, oracle):
+    """150 bp ASCII pairs at 5% divergence (the 10k-pair workload's shape)."""
+    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    pairs = generate(WORKLOADS["short150"], num_pairs=2000)
+    attr = attr_for(N, metric, pen, scope)
+    _compare(engine, oracle, attr, pairs, scope == "alignment", label=f"{metric}/{scope}")
+
+
+@pytest.mark.parametrize("scope", ["score", "alignment"])
+def test_reads100_300_packed_match_oracle(scope, pkg, N, engine, oracle):
+    """2-bit packed 100-300 bp pairs, divergence 0-10% (the 1M-pair workload)."""
+    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    pairs = generate(WORKLOADS["reads100_300"], num_pairs=4000)
+    attr = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), scope)
+    _compare(engine, oracle, attr, pairs, scope == "alignment", label=scope)
+
+
+def test_per_pair_path_equals_batch(pkg, N, engine, oracle):
+    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    pairs = generate(WORKLOADS["reads100_300"], num_pairs=64)
+    attr = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment")
+    a = _run_gpu(engine, attr, pairs, True, per_pair=False)
+    b = _run_gpu(engine, attr, pairs, True, per_pair=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+
+
+def test_wide_and_long_pairs_take_the_retry_path(pkg, N, engine, oracle):
+    """Pairs whose wavefronts outgrow the first launch's 256-diagonal ring
+    (long, divergent, unequal lengths) are re-run wider; results still match."""
+    from wfa2lib_rs_amd.synth import Workload, generate
+    w = Workload("wide", 40, 1, 600, 1200, 0.15, 0.25, 4.0)
+    pairs = generate(w)
+    # append a pair with a large length difference and unrelated sequences
+    from wfa2lib_rs_amd.synth import from_ascii
+    extra = from_ascii([b"ACGT" * 100, b"A" * 300], [b"TTGCA" * 30, b"C" * 700], pairs.seq_format)
+    for scope in ("score", "alignment"):
+        attr = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), scope)
+        _compare(engine, oracle, attr, pairs, scope == "alignment", label=f"wide/{scope}")
+        _compare(engine, oracle, attr, extra, scope == "alignment", label=f"extra/{scope}")
+    assert engine.last_retries() >= 0
+
+
+def test_edge_cases(pkg, N, engine, oracle):
+    from wfa2lib_rs_amd.synth import from_ascii
+    pats = [b"", b"A", b"", b"ACGT", b"ACGTN", b"acgt", b"ACGT" * 40, b"G", b"TTTT"]
+    txts = [b"", b"", b"C", b"ACGT", b"ACGTN", b"ACGT", b"ACGT" * 40, b"G" * 90, b"T"]
+    pairs = from_ascii(pats, txts, N.SEQ_ASCII)
+    for metric, pen in CONFIGS:
+        for scope in ("score", "alignment"):
+            attr = attr_for(N, metric, pen, scope)
+            _compare(engine, oracle, attr, pairs, scope == "alignment", label=f"{metric}/{scope}")
+
+
+def test_aligner_api_mirrors_reference(pkg, N, engine):
+    """WavefrontAligner on the GPU, used as the reference's tests use it."""
+    A = pkg.AffineAligner(pkg.WavefrontPenalties.new_affine(pkg.AffinePenalties(0, 4, 6, 2)))
+    A.set_alignment_scope(pkg.AlignmentScope.COMPUTE_ALIGNMENT)
+    assert A.align_end2end(b"ACGTACGT", b"ACGAACGT") == 4
+    assert A.status() == pkg.AlignStatus.COMPLETED
+    assert A.cigar().to_string_rle(True) == "3M1X4M"
+    p, t = b"ACGTACGTAC", b"ACGTACGTACGGAAA"
+    assert A.align_packed2bits(pkg.encode_packed2bits(p), len(p), pkg.encode_packed2bits(t),
+                               len(t)) == A.align_end2end(p, t)
+    E = pkg.EditAligner(pkg.WavefrontPenalties.new_edit())
+    E.set_max_alignment_steps(2)
+    E.align_end2end(b"AAAA", b"TTTT")
+    assert E.status() == pkg.AlignStatus.MAX_STEPS_REACHED
+    with pytest.raises(RuntimeError):
+        A.set_memory_mode(pkg.MemoryMode.MED)
+        A.align_end2end(b"ACGT", b"ACGT")

@@ -1,0 +1,179 @@
+"""CPU tests: the oracle pinned against the reference's literal vectors, an
+exact Gotoh DP, CIGAR validity/rescoring invariants, and the generator."""
+import numpy as np
+import pytest
+
+from conftest import attr_for, expand, load_vectors
+
+STATUS = {"OK": 0, "END_REACHED": 1, "END_UNREACHABLE": 2, "MAX_STEPS_REACHED": 3, "OOM": 4,
+          "COMPLETED": 5, "PARTIAL": 6}
+
+
+def _penalty_obj(pkg, metric, pen):
+    m, x, o1, e1, o2, e2 = pen
+    if metric == "gap_affine":
+        return pkg.WavefrontPenalties.new_affine(pkg.AffinePenalties(m, x, o1, e1))
+    if metric == "gap_affine_2p":
+        return pkg.WavefrontPenalties.new_affine2p(pkg.Affine2pPenalties(m, x, o1, e1, o2, e2))
+    if metric == "edit":
+        return pkg.WavefrontPenalties.new_edit()
+    return pkg.WavefrontPenalties.new_indel()
+
+
+@pytest.mark.parametrize("vec", load_vectors(), ids=lambda v: v["id"])
+def test_oracle_reference_vectors(vec, pkg, N, oracle):
+    attr = attr_for(N, vec["metric"], vec["penalties"], vec["scope"], vec.get("max_steps"))
+    A = oracle.OracleAligner(attr)
+    p, t = expand(vec["pattern"]), expand(vec["text"])
+    score, status, ops = A.align_end2end(p, t)
+    cig = pkg.Cigar(ops)
+    if "score" in vec:
+        assert score == vec["score"]
+    if "score_at_most" in vec:
+        assert score <= vec["score_at_most"]
+    if "status" in vec:
+        assert status == STATUS[vec["status"]]
+    if "cigar" in vec:
+        assert cig.to_string_rle(True) == vec["cigar"]
+    if vec.get("valid_cigar"):
+        assert cig.check_alignment(p, t) is None
+    if "cli" in vec:
+        pen = _penalty_obj(pkg, vec["metric"], vec["penalties"])
+        score_only = vec["scope"] == "score"
+        out = pkg.compute_output_score(cig, score, pen, score_only, len(p), len(t))
+        rle = "-" if score_only or cig.is_null() else cig.to_string_rle(True)
+        assert f"{out}\t{rle}" == vec["cli"]
+
+
+def _lcg_pairs(n, max_len, seed=7):
+    """The reference's singletrack LCG pair recipe (tests/singletrack.rs:12-51)."""
+    state = [seed]
+
+    def nxt():
+        state[0] = (state[0] * 6364136223846793005 + 1442695040888963407) % 2 ** 64
+        return state[0] >> 33
+
+    def below(k):
+        return nxt() % k
+
+    def base():
+        return b"ACGT"[below(4)]
+
+    pairs = []
+    for _ in range(n):
+        ln = 1 + below(max_len)
+        pat = bytes(base() for _ in range(ln))
+        txt = bytearray()
+        i = 0
+        while i < len(pat):
+            r = below(40)
+            if r == 0:
+                txt.append(base())
+            elif r == 1:
+                mx = 12 if below(4) == 0 else 3
+                for _ in range(1 + below(mx)):
+                    txt.append(base())
+                txt.append(pat[i])
+            elif r == 2:
+                mx = 12 if below(4) == 0 else 3
+                i += below(mx)
+            else:
+                txt.append(pat[i])
+            i += 1
+        pairs.append((pat, bytes(txt)))
+    return pairs
+
+
+PENALTY_SETS = [
+    ("gap_affine", (0, 4, 6, 2, 0, 0)),
+    ("gap_affine", (0, 1, 0, 1, 0, 0)),
+    ("gap_affine", (0, 3, 5, 1, 0, 0)),
+    ("gap_affine_2p", (0, 4, 6, 2, 24, 1)),
+    ("gap_affine_2p", (0, 5, 4, 3, 12, 1)),
+    ("edit", (0, 0, 0, 0, 0, 0)),
+    ("indel", (0, 0, 0, 0, 0, 0)),
+]
+
+
+@pytest.mark.parametrize("metric,pen", PENALTY_SETS)
+def test_oracle_scores_match_exact_dp(metric, pen, pkg, N, oracle):
+    """Optimal scores from an independent Gotoh / Levenshtein DP."""
+    attr = attr_for(N, metric, pen, "alignment")
+    A = oracle.OracleAligner(attr)
+    _, x, o1, e1, o2, e2 = pen
+    for p, t in _lcg_pairs(60, 120) + [(b"", b""), (b"A", b""), (b"", b"ACG"), (b"AAAA", b"TTTT")]:
+        score, status, ops = A.align_end2end(p, t)
+        assert status == 5
+        if metric == "edit":
+            want = oracle.dp_edit(p, t)
+        elif metric == "indel":
+            want = oracle.dp_edit(p, t, indel=True)
+        else:
+            want = oracle.dp_affine(p, t, x, o1, e1, o2, e2, metric == "gap_affine_2p")
+        assert score == want, (p, t)
+        cig = pkg.Cigar(ops)
+        if p or t:
+            assert cig.check_alignment(p, t) is None
+        # rescoring the CIGAR gives the score back (tests/singletrack.rs:116-121)
+        if metric == "gap_affine":
+            assert -cig.score_gap_affine(pkg.AffinePenalties(0, x, o1, e1)) == score
+        elif metric == "gap_affine_2p":
+            assert -cig.score_gap_affine2p(pkg.Affine2pPenalties(0, x, o1, e1, o2, e2)) == score
+        elif metric == "edit":
+            assert cig.score_edit() == score
+
+
+def test_oracle_score_only_equals_cigar_mode(N, oracle):
+    """Modular (score-only) storage gives the high-memory score."""
+    pairs = _lcg_pairs(40, 200, seed=11)
+    for metric, pen in PENALTY_SETS:
+        a_s = oracle.OracleAligner(attr_for(N, metric, pen, "score"))
+        a_c = oracle.OracleAligner(attr_for(N, metric, pen, "alignment"))
+        for p, t in pairs:
+            assert a_s.align_end2end(p, t)[0] == a_c.align_end2end(p, t)[0]
+
+
+def test_plausible_mistakes_are_caught(pkg, N, oracle):
+    """The DP check fails for a wrong gap cost: pins o + l*e, not o + (l-1)*e."""
+    attr = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "score")
+    A = oracle.OracleAligner(attr)
+    p, t = b"ACGTACGTAC", b"ACGTTTTACGTAC"
+    score = A.align_end2end(p, t)[0]
+    assert score == oracle.dp_affine(p, t, 4, 6, 2)
+    assert score != oracle.dp_affine(p, t, 4, 4, 2)
+
+
+def test_generator_deterministic_and_shaped(pkg):
+    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    w = WORKLOADS["reads100_300"]
+    a = generate(w, num_pairs=2000)
+    b = generate(w, num_pairs=2000, threads=1)
+    assert np.array_equal(a.seq, b.seq) and np.array_equal(a.plen, b.plen)
+    assert a.plen.min() >= 100 and a.plen.max() <= 300
+    # the first pairs do not depend on how many are generated
+    c = generate(w, num_pairs=10)
+    assert all(c.pattern(i) == a.pattern(i) and c.text(i) == a.text(i) for i in range(10))
+    s = generate(WORKLOADS["short150"], num_pairs=500)
+    assert (s.plen == 150).all()
+    # ~5% divergence: text length within a few % of the pattern's
+    assert abs(float(s.tlen.mean()) - 150) < 6
+
+
+def test_packed_and_ascii_layouts_agree(pkg):
+    from wfa2lib_rs_amd import _native as N
+    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    w = WORKLOADS["reads100_300"]
+    packed = generate(w, num_pairs=50)
+    ascii_ = generate(w.__class__(**{**w.__dict__, "seq_format": N.SEQ_ASCII}), num_pairs=50)
+    for i in range(50):
+        assert packed.pattern(i) == ascii_.pattern(i)
+        assert packed.text(i) == ascii_.text(i)
+
+
+def test_oracle_batch_threads_agree(N, oracle, pkg):
+    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    pairs = generate(WORKLOADS["short150"], num_pairs=300)
+    attr = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment")
+    s1, st1, c1 = oracle.align_batch(attr, pairs, cigar=True, threads=1)
+    s4, st4, c4 = oracle.align_batch(attr, pairs, cigar=True, threads=4)
+    assert np.array_equal(s1, s4) and np.array_equal(st1, st4) and c1 == c4

@@ -1,0 +1,133 @@
+"""ctypes view of the engine's C ABI (include/wfa_mi355x.h).
+
+The shared library is built in-tree (``lib/libwfa_mi355x.so``) by
+``__graft_entry__.build()``. Loading fails loudly when it is missing: there is
+no CPU fallback anywhere in this package.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libwfa_mi355x.so")
+SYNTH_PATH = os.path.join(LIB_DIR, "libwfa_synth.so")
+
+# ---- constants (same values as include/wfa_mi355x.h) ----
+METRIC_INDEL, METRIC_EDIT, METRIC_GAP_LINEAR, METRIC_GAP_AFFINE, METRIC_GAP_AFFINE_2P = range(5)
+SCOPE_SCORE, SCOPE_ALIGNMENT = 0, 1
+MEMORY_HIGH, MEMORY_MED, MEMORY_LOW, MEMORY_ULTRALOW = range(4)
+SPAN_END2END, SPAN_ENDSFREE = 0, 1
+(HEURISTIC_NONE, HEURISTIC_WFADAPTIVE, HEURISTIC_XDROP, HEURISTIC_ZDROP,
+ HEURISTIC_BANDED_STATIC, HEURISTIC_BANDED_ADAPTIVE) = range(6)
+(STATUS_OK, STATUS_END_REACHED, STATUS_END_UNREACHABLE, STATUS_MAX_STEPS_REACHED,
+ STATUS_OOM, STATUS_COMPLETED, STATUS_PARTIAL) = range(7)
+SEQ_ASCII, SEQ_PACKED2BITS = 0, 1
+ERRORS = {
+    0: "ok", -1: "invalid penalties", -2: "invalid argument", -3: "unsupported on the device path",
+    -4: "HIP runtime error", -5: "no HIP device", -6: "out of device memory",
+    -7: "rejected combination",
+}
+ERR_INVALID_PENALTIES = -1
+ERR_UNSUPPORTED = -3
+
+
+class Penalties(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "distance_metric", "match", "mismatch", "gap_opening1", "gap_extension1",
+        "gap_opening2", "gap_extension2")]
+
+
+class Attr(ctypes.Structure):
+    _fields_ = [("penalties", Penalties)] + [(n, ctypes.c_int32) for n in (
+        "alignment_scope", "memory_mode", "span", "extension", "pattern_begin_free",
+        "pattern_end_free", "text_begin_free", "text_end_free", "heuristic")] + [
+        ("heuristic_p", ctypes.c_int32 * 3), ("max_alignment_steps", ctypes.c_int32)]
+
+
+class Batch(ctypes.Structure):
+    _fields_ = [
+        ("num_pairs", ctypes.c_int32), ("seq_format", ctypes.c_int32),
+        ("max_length", ctypes.c_int32),
+        ("seq_data", ctypes.c_void_p), ("pattern_offset", ctypes.c_void_p),
+        ("pattern_length", ctypes.c_void_p), ("text_offset", ctypes.c_void_p),
+        ("text_length", ctypes.c_void_p), ("score", ctypes.c_void_p),
+        ("status", ctypes.c_void_p), ("cigar_ops", ctypes.c_void_p),
+        ("cigar_offset", ctypes.c_void_p), ("cigar_length", ctypes.c_void_p),
+    ]
+
+
+class SynthCfg(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("len_mode", ctypes.c_int32),
+                ("len_lo", ctypes.c_int32), ("len_hi", ctypes.c_int32),
+                ("div_lo", ctypes.c_double), ("div_hi", ctypes.c_double),
+                ("indel_mean", ctypes.c_double), ("run_frac", ctypes.c_double),
+                ("run_mean", ctypes.c_double)]
+
+
+# Every entry point the header declares, with its ctypes signature.
+_P, _I32, _VP = ctypes.POINTER, ctypes.c_int32, ctypes.c_void_p
+SIGNATURES = {
+    "wfa_attr_default": (None, [_P(Attr), _I32]),
+    "wfa_penalties_check": (_I32, [_P(Penalties)]),
+    "wfa_aligner_new": (_VP, [_P(Attr), _I32, _P(_I32)]),
+    "wfa_aligner_free": (None, [_VP]),
+    "wfa_aligner_set_alignment_scope": (_I32, [_VP, _I32]),
+    "wfa_aligner_set_memory_mode": (_I32, [_VP, _I32]),
+    "wfa_aligner_set_heuristic": (_I32, [_VP, _I32, _I32, _I32, _I32]),
+    "wfa_aligner_set_alignment_free_ends": (_I32, [_VP, _I32, _I32, _I32, _I32]),
+    "wfa_aligner_set_alignment_extension": (_I32, [_VP]),
+    "wfa_aligner_set_alignment_end_to_end": (_I32, [_VP]),
+    "wfa_aligner_set_max_alignment_steps": (_I32, [_VP, _I32]),
+    "wfa_align_end2end": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
+    "wfa_align_packed2bits": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
+    "wfa_aligner_status": (_I32, [_VP]),
+    "wfa_aligner_cigar": (_I32, [_VP, _P(_P(ctypes.c_uint8))]),
+    "wfa_aligner_cigar_score": (_I32, [_VP]),
+    "wfa_aligner_error": (_I32, [_VP]),
+    "wfa_engine_new": (_VP, [_I32, _P(_I32)]),
+    "wfa_engine_free": (None, [_VP]),
+    "wfa_engine_align_batch": (_I32, [_VP, _P(Attr), _P(Batch), _VP]),
+    "wfa_engine_align_batch_per_pair": (_I32, [_VP, _P(Attr), _P(Batch), _VP]),
+    "wfa_engine_last_retries": (_I32, [_VP]),
+    "wfa_version": (ctypes.c_char_p, []),
+}
+
+_lib = None
+_synth = None
+
+
+def lib():
+    """The engine library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(the engine has no CPU fallback)")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def synth_lib():
+    global _synth
+    if _synth is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise RuntimeError(f"{SYNTH_PATH} is missing: run __graft_entry__.build()")
+        s = ctypes.CDLL(SYNTH_PATH)
+        s.wfa_synth_lengths.argtypes = [_P(SynthCfg), ctypes.c_int64, _VP, _VP, _I32]
+        s.wfa_synth_lengths.restype = None
+        s.wfa_synth_fill.argtypes = [_P(SynthCfg), ctypes.c_int64, _VP, _VP, _I32, _VP, _VP,
+                                     _VP, _I32]
+        s.wfa_synth_fill.restype = None
+        _synth = s
+    return _synth
+
+
+def check(code, what="call"):
+    if code != 0:
+        raise RuntimeError(f"{what} failed: {ERRORS.get(code, code)} ({code})")

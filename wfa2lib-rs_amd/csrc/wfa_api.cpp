@@ -1,0 +1,694 @@
+// wfa_api.cpp — host side of the C ABI (include/wfa_mi355x.h).
+//
+// The aligner object mirrors the reference's WavefrontAligner<N>
+// (src/aligner.rs:146-211): it holds the configuration the setters change
+// and the last result (score, status, CIGAR). Alignment runs on the GPU:
+// each call stages its pair in device memory and launches the same
+// persistent kernel the batch engine uses, with one work item. The engine
+// owns the device work queue and per-slot scratch, sizes the launch from
+// the occupancy the kernel's LDS use allows on the device, and re-runs the
+// (rare) pairs that outgrow the first launch's per-pair storage.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "wfa_device.h"
+
+namespace {
+
+using wfa::KParams;
+
+// ---- penalties: WavefrontPenalties::new_* (src/penalties.rs:122-296) ----
+struct Internal {
+  int metric, match, x, o1, e1, o2, e2;
+};
+
+int make_internal(const wfa_penalties_t& u, Internal* p) {
+  p->metric = u.distance_metric;
+  switch (u.distance_metric) {
+    case WFA_METRIC_INDEL:
+      *p = {u.distance_metric, 0, -1, 1, -1, -1, -1};
+      return WFA_OK;
+    case WFA_METRIC_EDIT:
+      *p = {u.distance_metric, 0, 1, 1, -1, -1, -1};
+      return WFA_OK;
+    case WFA_METRIC_GAP_LINEAR:
+      if (u.match > 0 || u.mismatch <= 0 || u.gap_opening1 <= 0) return WFA_ERR_INVALID_PENALTIES;
+      if (u.match < 0)
+        *p = {u.distance_metric, u.match, 2 * u.mismatch - 2 * u.match,
+              2 * u.gap_opening1 - u.match, -1, -1, -1};
+      else
+        *p = {u.distance_metric, 0, u.mismatch, u.gap_opening1, -1, -1, -1};
+      return WFA_OK;
+    case WFA_METRIC_GAP_AFFINE:
+      if (u.match > 0 || u.mismatch <= 0 || u.gap_opening1 < 0 || u.gap_extension1 <= 0)
+        return WFA_ERR_INVALID_PENALTIES;
+      if (u.match < 0)
+        *p = {u.distance_metric, u.match, 2 * u.mismatch - 2 * u.match, 2 * u.gap_opening1,
+              2 * u.gap_extension1 - u.match, -1, -1};
+      else
+        *p = {u.distance_metric, 0, u.mismatch, u.gap_opening1, u.gap_extension1, -1, -1};
+      return WFA_OK;
+    case WFA_METRIC_GAP_AFFINE_2P:
+      if (u.match > 0 || u.mismatch <= 0 || u.gap_opening1 < 0 || u.gap_extension1 <= 0 ||
+          u.gap_opening2 < 0 || u.gap_extension2 <= 0)
+        return WFA_ERR_INVALID_PENALTIES;
+      if (u.match < 0)
+        *p = {u.distance_metric, u.match, 2 * u.mismatch - 2 * u.match, 2 * u.gap_opening1,
+              2 * u.gap_extension1 - u.match, 2 * u.gap_opening2,
+              2 * u.gap_extension2 - u.match};
+      else
+        *p = {u.distance_metric, 0, u.mismatch, u.gap_opening1, u.gap_extension1,
+              u.gap_opening2, u.gap_extension2};
+      return WFA_OK;
+    default:
+      return WFA_ERR_INVALID_PENALTIES;
+  }
+}
+
+// max_score_scope (src/components.rs:435-483)
+int score_scope(const Internal& p) {
+  switch (p.metric) {
+    case WFA_METRIC_INDEL:
+    case WFA_METRIC_EDIT: return 2;
+    case WFA_METRIC_GAP_LINEAR: return std::max(p.x, p.o1) + 1;
+    case WFA_METRIC_GAP_AFFINE: return std::max(p.o1 + p.e1, p.x) + 1;
+    default: return std::max(std::max(p.o1 + p.e1, p.o2 + p.e2), p.x) + 1;
+  }
+}
+
+// Which configurations the device path runs. The rest return
+// WFA_ERR_UNSUPPORTED rather than silently falling back to anything.
+int check_supported(const wfa_attr_t& a) {
+  const int m = a.penalties.distance_metric;
+  if (m == WFA_METRIC_GAP_LINEAR) return WFA_ERR_UNSUPPORTED;
+  if (a.span != WFA_SPAN_END2END || a.extension) return WFA_ERR_UNSUPPORTED;
+  if (a.heuristic != WFA_HEURISTIC_NONE) return WFA_ERR_UNSUPPORTED;
+  if (a.alignment_scope == WFA_SCOPE_ALIGNMENT && a.memory_mode != WFA_MEMORY_HIGH)
+    return WFA_ERR_UNSUPPORTED;
+  if (a.alignment_scope != WFA_SCOPE_SCORE && a.alignment_scope != WFA_SCOPE_ALIGNMENT)
+    return WFA_ERR_INVALID_ARGUMENT;
+  if (a.memory_mode < WFA_MEMORY_HIGH || a.memory_mode > WFA_MEMORY_ULTRALOW)
+    return WFA_ERR_INVALID_ARGUMENT;
+  if (a.memory_mode == WFA_MEMORY_ULTRALOW) return WFA_ERR_UNSUPPORTED;
+  return WFA_OK;
+}
+
+int pow2ceil(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+#define HIPCHK(x)                                   \
+  do {                                              \
+    hipError_t _e = (x);                            \
+    if (_e != hipSuccess) return WFA_ERR_DEVICE;    \
+  } while (0)
+
+}  // namespace
+
+// =====================================================================
+// Engine
+// =====================================================================
+struct wfa_engine_t {
+  int device = 0;
+  int num_cus = 0;
+  size_t lds_max = 0;
+  int32_t* d_counters = nullptr;  // [0] queue head, [1] retries, [2] scratch
+  int32_t* d_lists[2] = {nullptr, nullptr};
+  size_t list_cap = 0;
+  int32_t* h_pinned = nullptr;
+  // scratch
+  int32_t* d_hist = nullptr;
+  size_t hist_bytes = 0;
+  int4* d_hdr = nullptr;
+  size_t hdr_bytes = 0;
+  int32_t* d_gring = nullptr;
+  size_t gring_bytes = 0;
+  int last_retries = 0;
+  std::mutex mu;
+};
+
+namespace {
+
+int grow(void** ptr, size_t* have, size_t need) {
+  if (*have >= need) return WFA_OK;
+  if (*ptr) hipFree(*ptr);
+  *ptr = nullptr;
+  *have = 0;
+  if (hipMalloc(ptr, need) != hipSuccess) return WFA_ERR_OUT_OF_MEMORY;
+  *have = need;
+  return WFA_OK;
+}
+
+struct LaunchCfg {
+  int kind;
+  bool cig;
+  bool lds_ring;
+  int wcap;
+  int seq_words;
+  int ring_rows;
+  size_t lds_bytes;
+  int grid;
+  int hdr_cap;
+  int64_t hist_cap;
+};
+
+int kind_of(int metric) {
+  if (metric == WFA_METRIC_GAP_AFFINE) return wfa::KIND_AFFINE;
+  if (metric == WFA_METRIC_GAP_AFFINE_2P) return wfa::KIND_AFFINE2P;
+  return wfa::KIND_EDIT;
+}
+
+// Build kernel parameters shared by every launch of one batch.
+void base_params(const Internal& ip, const wfa_attr_t& attr, const wfa_batch_t& b, KParams* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->metric = ip.metric;
+  p->x = ip.x;
+  p->o1 = ip.o1;
+  p->e1 = ip.e1;
+  p->o2 = ip.o2;
+  p->e2 = ip.e2;
+  p->oe1 = ip.o1 + ip.e1;
+  p->oe2 = ip.o2 + ip.e2;
+  p->scope = score_scope(ip);
+  const int kind = kind_of(ip.metric);
+  p->ringM = kind == wfa::KIND_EDIT ? 2 : p->scope;
+  p->ringE1 = kind == wfa::KIND_EDIT ? 0 : ip.e1 + 1;
+  p->ringE2 = kind == wfa::KIND_AFFINE2P ? ip.e2 + 1 : 0;
+  p->modular = attr.alignment_scope == WFA_SCOPE_SCORE;
+  p->max_steps = attr.max_alignment_steps;
+  p->fmt = b.seq_format;
+  p->seq = b.seq_data;
+  p->poff = b.pattern_offset;
+  p->plen = b.pattern_length;
+  p->toff = b.text_offset;
+  p->tlen = b.text_length;
+  p->score = b.score;
+  p->status = b.status;
+  p->cig = b.cigar_ops;
+  p->cig_off = b.cigar_offset;
+  p->cig_len = b.cigar_length;
+}
+
+int ring_rows_of(const KParams& p, int kind) {
+  if (kind == wfa::KIND_EDIT) return p.ringM;
+  if (kind == wfa::KIND_AFFINE) return p.ringM + 2 * p.ringE1;
+  return p.ringM + 2 * p.ringE1 + 2 * p.ringE2;
+}
+
+// Choose storage for a launch over pairs of length <= max_len. `wide`
+// selects the full-width configuration that cannot overflow the ring.
+int plan(wfa_engine_t* e, const KParams& p, int kind, bool cig, int fmt, int max_len, int npairs,
+         int pass, LaunchCfg* c) {
+  c->kind = kind;
+  c->cig = cig;
+  c->ring_rows = ring_rows_of(p, kind);
+  const int per = fmt == WFA_SEQ_PACKED2BITS ? 32 : 8;
+  c->seq_words = (max_len + per - 1) / per + 2;
+  // the widest possible computed range: [-(plen+scope+1), tlen+scope+1]
+  const int full = pow2ceil(2 * max_len + 2 * p.scope + 4);
+  c->wcap = pass == 0 ? std::min(256, full) : full;
+  const size_t seq_meta = (size_t)2 * c->seq_words * 8 + (size_t)c->ring_rows * 8;
+  const size_t ring_bytes = (size_t)c->ring_rows * c->wcap * 4;
+  c->lds_ring = seq_meta + ring_bytes <= 64 * 1024;
+  c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : 0);
+  if (c->lds_bytes > e->lds_max) return WFA_ERR_UNSUPPORTED;
+  const int per_cu = wfa::max_blocks_per_cu(kind, cig, c->lds_ring, c->lds_bytes);
+  int grid = e->num_cus * per_cu;
+  if (pass > 0) grid = std::min(grid, e->num_cus * 2);
+  grid = std::max(1, std::min(grid, npairs));
+  c->grid = grid;
+  if (cig) {
+    if (pass == 0) {
+      c->hdr_cap = 2048;
+      c->hist_cap = 128 * 1024;
+    } else {
+      // worst case: every score up to the slot bound, every row full width
+      const int64_t nwf_bound = (int64_t)std::max(2 * max_len, 1000) * 64 + 16;
+      c->hdr_cap = (int)std::min<int64_t>(nwf_bound, (int64_t)2048 << (2 * pass));
+      c->hist_cap = std::min<int64_t>((int64_t)c->hdr_cap * c->wcap, (int64_t)(128 * 1024) << (3 * pass));
+    }
+    // keep the scratch of one launch under ~16 GiB
+    const int64_t per_slot = c->hist_cap * 4 + (int64_t)c->hdr_cap * 16;
+    const int64_t budget = (int64_t)16 << 30;
+    if ((int64_t)c->grid * per_slot > budget) c->grid = (int)std::max<int64_t>(1, budget / per_slot);
+  } else {
+    c->hdr_cap = 0;
+    c->hist_cap = 0;
+  }
+  return WFA_OK;
+}
+
+int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
+               const int32_t* list, int32_t* retry_out, hipStream_t st) {
+  p.num_pairs = num_items;
+  p.pair_list = list;
+  p.counters = e->d_counters;
+  p.retry_list = retry_out;
+  p.wcap = c.wcap;
+  p.seq_words = c.seq_words;
+  p.nrows_meta = c.ring_rows;
+  p.ring_rows = c.ring_rows;
+  if (!c.lds_ring) {
+    int r = grow((void**)&e->d_gring, &e->gring_bytes, (size_t)c.grid * c.ring_rows * c.wcap * 4);
+    if (r) return r;
+    p.gring = e->d_gring;
+  }
+  if (c.cig) {
+    int r = grow((void**)&e->d_hist, &e->hist_bytes, (size_t)c.grid * c.hist_cap * 4);
+    if (r) return r;
+    r = grow((void**)&e->d_hdr, &e->hdr_bytes, (size_t)c.grid * c.hdr_cap * sizeof(int4));
+    if (r) return r;
+    p.hist = e->d_hist;
+    p.hist_cap = c.hist_cap;
+    p.hdr = e->d_hdr;
+    p.hdr_cap = c.hdr_cap;
+  }
+  HIPCHK(wfa::launch_reset(e->d_counters, st));
+  HIPCHK(wfa::launch_align(p, c.kind, c.cig, c.lds_ring, c.grid, c.lds_bytes, st));
+  return WFA_OK;
+}
+
+int ensure_lists(wfa_engine_t* e, int n) {
+  if ((size_t)n <= e->list_cap) return WFA_OK;
+  for (auto& l : e->d_lists) {
+    if (l) hipFree(l);
+    l = nullptr;
+  }
+  size_t cap = std::max<size_t>(n, 1024);
+  for (auto& l : e->d_lists)
+    if (hipMalloc((void**)&l, cap * 4) != hipSuccess) return WFA_ERR_OUT_OF_MEMORY;
+  e->list_cap = cap;
+  return WFA_OK;
+}
+
+int read_counter(wfa_engine_t* e, int idx, hipStream_t st, int32_t* out) {
+  HIPCHK(hipMemcpyAsync(e->h_pinned, e->d_counters + idx, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *out = e->h_pinned[0];
+  return WFA_OK;
+}
+
+int validate_batch(const wfa_attr_t* attr, const wfa_batch_t* b) {
+  if (!attr || !b) return WFA_ERR_INVALID_ARGUMENT;
+  if (b->num_pairs < 0) return WFA_ERR_INVALID_ARGUMENT;
+  if (b->seq_format != WFA_SEQ_ASCII && b->seq_format != WFA_SEQ_PACKED2BITS)
+    return WFA_ERR_INVALID_ARGUMENT;
+  if (b->num_pairs > 0 && (!b->seq_data || !b->pattern_offset || !b->pattern_length ||
+                           !b->text_offset || !b->text_length || !b->score || !b->status))
+    return WFA_ERR_INVALID_ARGUMENT;
+  if (attr->alignment_scope == WFA_SCOPE_ALIGNMENT &&
+      (!b->cigar_ops || !b->cigar_offset || !b->cigar_length) && b->num_pairs > 0)
+    return WFA_ERR_INVALID_ARGUMENT;
+  return WFA_OK;
+}
+
+// One batch: first pass over all pairs (or `first..first+count` when
+// `per_pair`), then wider passes over the pairs that asked for retry.
+int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
+                     hipStream_t st, int first, int count) {
+  Internal ip;
+  int r = make_internal(attr->penalties, &ip);
+  if (r) return r;
+  r = check_supported(*attr);
+  if (r) return r;
+  if (count == 0) return WFA_OK;
+  HIPCHK(hipSetDevice(e->device));
+  KParams p;
+  base_params(ip, *attr, *b, &p);
+  const int kind = kind_of(ip.metric);
+  const bool cig = attr->alignment_scope == WFA_SCOPE_ALIGNMENT;
+  int max_len = b->max_length;
+  if (max_len <= 0) {
+    HIPCHK(wfa::launch_max_length(b->pattern_length + first, b->text_length + first, count,
+                                  e->d_counters + 2, st));
+    r = read_counter(e, 2, st, &max_len);
+    if (r) return r;
+  }
+  r = ensure_lists(e, count);
+  if (r) return r;
+  LaunchCfg c;
+  r = plan(e, p, kind, cig, b->seq_format, max_len, count, 0, &c);
+  if (r) return r;
+  p.pair_base = first;
+  r = run_launch(e, p, c, count, nullptr, e->d_lists[0], st);
+  if (r) return r;
+  int32_t nretry = 0;
+  r = read_counter(e, 1, st, &nretry);
+  if (r) return r;
+  e->last_retries = nretry;
+  int cur = 0;
+  for (int pass = 1; nretry > 0; ++pass) {
+    if (pass > 6) return WFA_ERR_OUT_OF_MEMORY;
+    r = plan(e, p, kind, cig, b->seq_format, max_len, nretry, pass, &c);
+    if (r) return r;
+    r = run_launch(e, p, c, nretry, e->d_lists[cur], e->d_lists[cur ^ 1], st);
+    if (r) return r;
+    cur ^= 1;
+    r = read_counter(e, 1, st, &nretry);
+    if (r) return r;
+  }
+  return WFA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+wfa_engine_t* wfa_engine_new(int32_t device, int32_t* err) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    if (err) *err = WFA_ERR_NO_DEVICE;
+    return nullptr;
+  }
+  if (device < 0 || device >= n) {
+    if (err) *err = WFA_ERR_INVALID_ARGUMENT;
+    return nullptr;
+  }
+  auto* e = new wfa_engine_t();
+  e->device = device;
+  hipDeviceProp_t prop;
+  if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete e;
+    if (err) *err = WFA_ERR_DEVICE;
+    return nullptr;
+  }
+  e->num_cus = prop.multiProcessorCount;
+  e->lds_max = prop.sharedMemPerBlock;
+  if (hipMalloc((void**)&e->d_counters, 64) != hipSuccess ||
+      hipHostMalloc((void**)&e->h_pinned, 64, hipHostMallocDefault) != hipSuccess) {
+    delete e;
+    if (err) *err = WFA_ERR_OUT_OF_MEMORY;
+    return nullptr;
+  }
+  if (err) *err = WFA_OK;
+  return e;
+}
+
+void wfa_engine_free(wfa_engine_t* e) {
+  if (!e) return;
+  hipSetDevice(e->device);
+  hipDeviceSynchronize();
+  hipFree(e->d_counters);
+  for (auto l : e->d_lists) hipFree(l);
+  hipFree(e->d_hist);
+  hipFree(e->d_hdr);
+  hipFree(e->d_gring);
+  hipHostFree(e->h_pinned);
+  delete e;
+}
+
+int32_t wfa_engine_align_batch(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
+                               void* stream) {
+  if (!e) return WFA_ERR_INVALID_ARGUMENT;
+  int r = validate_batch(attr, b);
+  if (r) return r;
+  std::lock_guard<std::mutex> g(e->mu);
+  return align_batch_impl(e, attr, b, (hipStream_t)stream, 0, b->num_pairs);
+}
+
+int32_t wfa_engine_align_batch_per_pair(wfa_engine_t* e, const wfa_attr_t* attr,
+                                        const wfa_batch_t* b, void* stream) {
+  if (!e) return WFA_ERR_INVALID_ARGUMENT;
+  int r = validate_batch(attr, b);
+  if (r) return r;
+  std::lock_guard<std::mutex> g(e->mu);
+  wfa_batch_t one = *b;
+  for (int i = 0; i < b->num_pairs; ++i) {
+    r = align_batch_impl(e, attr, &one, (hipStream_t)stream, i, 1);
+    if (r) return r;
+  }
+  return WFA_OK;
+}
+
+int32_t wfa_engine_last_retries(const wfa_engine_t* e) { return e ? e->last_retries : 0; }
+
+const char* wfa_version(void) { return "wfa-mi355x 0.1.0 (gfx950)"; }
+
+void wfa_attr_default(wfa_attr_t* a, int32_t metric) {
+  std::memset(a, 0, sizeof(*a));
+  a->penalties.distance_metric = metric;
+  switch (metric) {
+    case WFA_METRIC_GAP_LINEAR:
+      a->penalties.match = 0; a->penalties.mismatch = 4; a->penalties.gap_opening1 = 2;
+      break;
+    case WFA_METRIC_GAP_AFFINE:
+      a->penalties.match = 0; a->penalties.mismatch = 4;
+      a->penalties.gap_opening1 = 6; a->penalties.gap_extension1 = 2;
+      break;
+    case WFA_METRIC_GAP_AFFINE_2P:
+      a->penalties.match = 0; a->penalties.mismatch = 4;
+      a->penalties.gap_opening1 = 6; a->penalties.gap_extension1 = 2;
+      a->penalties.gap_opening2 = 24; a->penalties.gap_extension2 = 1;
+      break;
+    default:
+      break;
+  }
+  a->alignment_scope = WFA_SCOPE_SCORE;
+  a->memory_mode = WFA_MEMORY_HIGH;
+  a->span = WFA_SPAN_END2END;
+  a->heuristic = WFA_HEURISTIC_NONE;
+  a->max_alignment_steps = INT32_MAX;
+}
+
+int32_t wfa_penalties_check(const wfa_penalties_t* p) {
+  if (!p) return WFA_ERR_INVALID_ARGUMENT;
+  Internal ip;
+  return make_internal(*p, &ip);
+}
+
+}  // extern "C"
+
+// =====================================================================
+// Aligner: WavefrontAligner<N> over the engine, one pair per call
+// =====================================================================
+struct wfa_aligner_t {
+  wfa_attr_t attr;
+  wfa_engine_t* engine = nullptr;
+  int err = WFA_OK;
+  int status = WFA_STATUS_OK;
+  int32_t score = 0;
+  std::vector<uint8_t> cigar;
+  int32_t cigar_score = INT32_MIN;
+  // device staging for one pair: [pattern | text | cigar], plus arrays
+  uint8_t* d_buf = nullptr;
+  size_t d_cap = 0;
+  int64_t* d_i64 = nullptr;  // poff, toff, coff
+  int32_t* d_i32 = nullptr;  // plen, tlen, score, status, clen
+  uint8_t* h_buf = nullptr;  // pinned
+  size_t h_cap = 0;
+};
+
+namespace {
+
+int aligner_run(wfa_aligner_t* a, const uint8_t* P, int plen, const uint8_t* T, int tlen,
+                int fmt) {
+  a->cigar.clear();
+  a->cigar_score = INT32_MIN;
+  a->status = WFA_STATUS_OK;
+  if (plen < 0 || tlen < 0 || (plen && !P) || (tlen && !T)) return WFA_ERR_INVALID_ARGUMENT;
+  int r = check_supported(a->attr);
+  if (r) return r;
+  const size_t pb = fmt == WFA_SEQ_PACKED2BITS ? (size_t)(plen + 3) / 4 : (size_t)plen;
+  const size_t tb = fmt == WFA_SEQ_PACKED2BITS ? (size_t)(tlen + 3) / 4 : (size_t)tlen;
+  const bool cig = a->attr.alignment_scope == WFA_SCOPE_ALIGNMENT;
+  const size_t need = pb + tb + (cig ? (size_t)plen + tlen : 0) + 64;
+  if (hipSetDevice(a->engine->device) != hipSuccess) return WFA_ERR_DEVICE;
+  if (need > a->d_cap) {
+    if (a->d_buf) hipFree(a->d_buf);
+    a->d_buf = nullptr;
+    a->d_cap = 0;
+    if (hipMalloc((void**)&a->d_buf, need) != hipSuccess) return WFA_ERR_OUT_OF_MEMORY;
+    a->d_cap = need;
+  }
+  if (need > a->h_cap) {
+    if (a->h_buf) hipHostFree(a->h_buf);
+    a->h_buf = nullptr;
+    a->h_cap = 0;
+    if (hipHostMalloc((void**)&a->h_buf, need, hipHostMallocDefault) != hipSuccess)
+      return WFA_ERR_OUT_OF_MEMORY;
+    a->h_cap = need;
+  }
+  if (!a->d_i64) {
+    if (hipMalloc((void**)&a->d_i64, 64) != hipSuccess) return WFA_ERR_OUT_OF_MEMORY;
+    if (hipMalloc((void**)&a->d_i32, 64) != hipSuccess) return WFA_ERR_OUT_OF_MEMORY;
+  }
+  if (pb) std::memcpy(a->h_buf, P, pb);
+  if (tb) std::memcpy(a->h_buf + pb, T, tb);
+  int64_t offs[3] = {0, (int64_t)pb, (int64_t)(pb + tb)};
+  int32_t lens[2] = {plen, tlen};
+  hipStream_t st = nullptr;
+  if (pb + tb && hipMemcpyAsync(a->d_buf, a->h_buf, pb + tb, hipMemcpyHostToDevice, st) != hipSuccess)
+    return WFA_ERR_DEVICE;
+  if (hipMemcpyAsync(a->d_i64, offs, sizeof(offs), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(a->d_i32, lens, sizeof(lens), hipMemcpyHostToDevice, st) != hipSuccess)
+    return WFA_ERR_DEVICE;
+  wfa_batch_t b;
+  std::memset(&b, 0, sizeof(b));
+  b.num_pairs = 1;
+  b.seq_format = fmt;
+  b.max_length = std::max(plen, tlen);
+  b.seq_data = a->d_buf;
+  b.pattern_offset = a->d_i64;
+  b.text_offset = a->d_i64 + 1;
+  b.pattern_length = a->d_i32;
+  b.text_length = a->d_i32 + 1;
+  b.score = a->d_i32 + 2;
+  b.status = a->d_i32 + 3;
+  if (cig) {
+    b.cigar_ops = a->d_buf;
+    b.cigar_offset = a->d_i64 + 2;
+    b.cigar_length = a->d_i32 + 4;
+  }
+  {
+    std::lock_guard<std::mutex> g(a->engine->mu);
+    r = align_batch_impl(a->engine, &a->attr, &b, st, 0, 1);
+  }
+  if (r) return r;
+  int32_t res[3];
+  if (hipMemcpy(res, a->d_i32 + 2, sizeof(res), hipMemcpyDeviceToHost) != hipSuccess)
+    return WFA_ERR_DEVICE;
+  a->score = res[0];
+  a->status = res[1];
+  if (cig && res[2] > 0) {
+    a->cigar.resize(res[2]);
+    if (hipMemcpy(a->cigar.data(), a->d_buf + pb + tb, res[2], hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      return WFA_ERR_DEVICE;
+    a->cigar_score = a->score;
+  }
+  return WFA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+wfa_aligner_t* wfa_aligner_new(const wfa_attr_t* attr, int32_t device, int32_t* err) {
+  if (!attr) {
+    if (err) *err = WFA_ERR_INVALID_ARGUMENT;
+    return nullptr;
+  }
+  Internal ip;
+  int r = make_internal(attr->penalties, &ip);
+  if (r) {
+    if (err) *err = r;
+    return nullptr;
+  }
+  int32_t e = 0;
+  wfa_engine_t* eng = wfa_engine_new(device, &e);
+  if (!eng) {
+    if (err) *err = e;
+    return nullptr;
+  }
+  auto* a = new wfa_aligner_t();
+  a->attr = *attr;
+  a->engine = eng;
+  if (err) *err = WFA_OK;
+  return a;
+}
+
+void wfa_aligner_free(wfa_aligner_t* a) {
+  if (!a) return;
+  hipSetDevice(a->engine->device);
+  hipFree(a->d_buf);
+  hipFree(a->d_i64);
+  hipFree(a->d_i32);
+  hipHostFree(a->h_buf);
+  wfa_engine_free(a->engine);
+  delete a;
+}
+
+int32_t wfa_aligner_set_alignment_scope(wfa_aligner_t* a, int32_t scope) {
+  if (!a || (scope != WFA_SCOPE_SCORE && scope != WFA_SCOPE_ALIGNMENT)) return WFA_ERR_INVALID_ARGUMENT;
+  a->attr.alignment_scope = scope;
+  return WFA_OK;
+}
+int32_t wfa_aligner_set_memory_mode(wfa_aligner_t* a, int32_t mode) {
+  if (!a || mode < WFA_MEMORY_HIGH || mode > WFA_MEMORY_ULTRALOW) return WFA_ERR_INVALID_ARGUMENT;
+  a->attr.memory_mode = mode;
+  return WFA_OK;
+}
+int32_t wfa_aligner_set_heuristic(wfa_aligner_t* a, int32_t h, int32_t p0, int32_t p1,
+                                  int32_t p2) {
+  if (!a || h < WFA_HEURISTIC_NONE || h > WFA_HEURISTIC_BANDED_ADAPTIVE) return WFA_ERR_INVALID_ARGUMENT;
+  a->attr.heuristic = h;
+  a->attr.heuristic_p[0] = p0;
+  a->attr.heuristic_p[1] = p1;
+  a->attr.heuristic_p[2] = p2;
+  return WFA_OK;
+}
+int32_t wfa_aligner_set_alignment_free_ends(wfa_aligner_t* a, int32_t pb, int32_t pe, int32_t tb,
+                                            int32_t te) {
+  if (!a) return WFA_ERR_INVALID_ARGUMENT;
+  a->attr.span = WFA_SPAN_ENDSFREE;
+  a->attr.extension = 0;
+  a->attr.pattern_begin_free = pb;
+  a->attr.pattern_end_free = pe;
+  a->attr.text_begin_free = tb;
+  a->attr.text_end_free = te;
+  return WFA_OK;
+}
+int32_t wfa_aligner_set_alignment_extension(wfa_aligner_t* a) {
+  if (!a) return WFA_ERR_INVALID_ARGUMENT;
+  a->attr.span = WFA_SPAN_ENDSFREE;
+  a->attr.extension = 1;
+  return WFA_OK;
+}
+int32_t wfa_aligner_set_alignment_end_to_end(wfa_aligner_t* a) {
+  if (!a) return WFA_ERR_INVALID_ARGUMENT;
+  a->attr.span = WFA_SPAN_END2END;
+  a->attr.extension = 0;
+  a->attr.pattern_begin_free = a->attr.pattern_end_free = 0;
+  a->attr.text_begin_free = a->attr.text_end_free = 0;
+  return WFA_OK;
+}
+int32_t wfa_aligner_set_max_alignment_steps(wfa_aligner_t* a, int32_t max_steps) {
+  if (!a) return WFA_ERR_INVALID_ARGUMENT;
+  a->attr.max_alignment_steps = max_steps;
+  return WFA_OK;
+}
+
+int32_t wfa_align_end2end(wfa_aligner_t* a, const uint8_t* P, int32_t plen, const uint8_t* T,
+                          int32_t tlen) {
+  if (!a) return INT32_MIN;
+  // align_end2end is end-to-end whatever the span (src/aligner.rs:627-636)
+  wfa_attr_t saved = a->attr;
+  a->attr.span = WFA_SPAN_END2END;
+  a->attr.extension = 0;
+  a->err = aligner_run(a, P, plen, T, tlen, WFA_SEQ_ASCII);
+  a->attr = saved;
+  return a->err ? INT32_MIN : a->score;
+}
+
+int32_t wfa_align_packed2bits(wfa_aligner_t* a, const uint8_t* P, int32_t plen, const uint8_t* T,
+                              int32_t tlen) {
+  if (!a) return INT32_MIN;
+  wfa_attr_t saved = a->attr;
+  a->attr.span = WFA_SPAN_END2END;
+  a->attr.extension = 0;
+  a->err = aligner_run(a, P, plen, T, tlen, WFA_SEQ_PACKED2BITS);
+  a->attr = saved;
+  return a->err ? INT32_MIN : a->score;
+}
+
+int32_t wfa_aligner_status(const wfa_aligner_t* a) { return a ? a->status : WFA_STATUS_OK; }
+int32_t wfa_aligner_cigar(const wfa_aligner_t* a, const uint8_t** ops) {
+  if (!a || a->cigar.empty()) {
+    if (ops) *ops = nullptr;
+    return 0;
+  }
+  if (ops) *ops = a->cigar.data();
+  return (int32_t)a->cigar.size();
+}
+int32_t wfa_aligner_cigar_score(const wfa_aligner_t* a) { return a ? a->cigar_score : INT32_MIN; }
+int32_t wfa_aligner_error(const wfa_aligner_t* a) { return a ? a->err : WFA_ERR_INVALID_ARGUMENT; }
+
+}  // extern "C"

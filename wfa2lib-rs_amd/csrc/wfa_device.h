@@ -1,0 +1,59 @@
+// wfa_device.h — parameters shared by the host engine and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/wfa_mi355x.h"
+
+namespace wfa {
+
+enum { KIND_EDIT = 0, KIND_AFFINE = 1, KIND_AFFINE2P = 2 };
+
+// Everything one persistent launch needs, passed by value.
+struct KParams {
+  // penalties (internal, after the Eizenga transform) and derived sizes
+  int metric;
+  int x, o1, e1, o2, e2, oe1, oe2;
+  int scope;                 // max_score_scope (src/components.rs:435-504)
+  int ringM, ringE1, ringE2; // ring lengths of M, I1/D1, I2/D2 rows
+  int modular;               // score scope: no history, no slot limit
+  int max_steps;
+  // pairs
+  int num_pairs;             // work items in this launch
+  int pair_base;             // first pair when pair_list is null
+  const int32_t* pair_list;  // optional indirection (retry passes)
+  int fmt;
+  const uint8_t* seq;
+  const int64_t* poff;
+  const int32_t* plen;
+  const int64_t* toff;
+  const int32_t* tlen;
+  int32_t* score;
+  int32_t* status;
+  uint8_t* cig;
+  const int64_t* cig_off;
+  int32_t* cig_len;
+  // scheduling
+  int32_t* counters;         // [0] work-queue head, [1] retry count
+  int32_t* retry_list;
+  // storage
+  int wcap;                  // ring width, power of two
+  int seq_words;             // 64-bit words per staged sequence (incl. padding)
+  int nrows_meta;            // metadata entries (= ring rows)
+  int ring_rows;
+  int32_t* gring;            // global ring when the ring is not in LDS
+  int32_t* hist;             // CIGAR history cells, hist_cap per slot
+  int64_t hist_cap;
+  int4* hdr;                 // CIGAR history headers, hdr_cap per slot
+  int hdr_cap;
+};
+
+hipError_t launch_reset(int32_t* counters, hipStream_t stream);
+hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, int32_t* out,
+                             hipStream_t stream);
+hipError_t launch_align(const KParams& p, int kind, bool cig, bool lds_ring, int grid,
+                        size_t lds_bytes, hipStream_t stream);
+int max_blocks_per_cu(int kind, bool cig, bool lds_ring, size_t lds_bytes);
+
+}  // namespace wfa

@@ -1,0 +1,806 @@
+// wfa_kernels.hip — gfx950 wavefront-alignment kernels.
+//
+// One 64-lane wavefront aligns one pattern/text pair at a time and pulls
+// the next pair from a device work queue (a persistent launch). Within a
+// pair, the diagonals k of a wavefront row sit across the lanes (lane i
+// handles k = lo + i + 64*j); the rows of the last `scope` scores live in
+// an LDS ring indexed by (k & (wcap-1)), read through a range guard that
+// returns OFFSET_NULL outside a row's [lo, hi] (the reference fills those
+// halo cells with OFFSET_NULL before every read, src/wavefront.rs:422-457).
+// Extension compares 32 bases per 64-bit XOR of 2-bit packed sequences held
+// in LDS; trims reduce per-chunk ballots; the termination cell is read by
+// the lane that owns it. In CIGAR scope the extended M rows go to a
+// per-slot history in HBM, which the same wavefront walks backwards with
+// the Singletrack backtrace (src/backtrace.rs:751-858) — gap-length search
+// parallel over lanes — writing one byte per CIGAR op.
+//
+// Semantics follow the reference's unialign loop (src/aligner.rs:638-698)
+// and compute steps (src/aligner.rs:1769-1865, 2006-2193, 2201-2553) bit
+// for bit; the kernels never approximate.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wfa_device.h"
+
+namespace wfa {
+
+#define WFA_NULL (INT32_MIN / 2)  // OFFSET_NULL, src/offset.rs:15
+#define META_ABSENT 0x7fffffff   // row never computed (a null step)
+
+static __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+static __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+
+static __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+static __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// In-bounds test of the kernels and trim_ends (src/compute/affine.rs:338-344,
+// src/compute/mod.rs:14-49).
+static __device__ __forceinline__ bool in_bounds(int off, int k, int plen, int tlen) {
+  return (uint32_t)off <= (uint32_t)tlen && (uint32_t)(off - k) <= (uint32_t)plen;
+}
+
+// ---------------------------------------------------------------------
+// Sequences staged in LDS as little-endian 64-bit words.
+//   2-bit mode: base j at bits [2j, 2j+2) (src/sequences.rs:55-74 order)
+//   byte mode:  byte j at bits [8j, 8j+8)
+// ---------------------------------------------------------------------
+template <int BITS>
+static __device__ __forceinline__ uint64_t load_fwd(const uint64_t* s, int pos) {
+  constexpr int PER = 64 / BITS;
+  const int q = pos / PER;
+  const int sh = (pos % PER) * BITS;
+  const uint64_t lo = s[q];
+  const uint64_t hi = s[q + 1];
+  return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+}
+
+// Matching prefix length of pattern[v..] and text[h..], bounded by the
+// sequence ends (src/extend.rs:15-38).
+template <int BITS>
+static __device__ __forceinline__ int lcp(const uint64_t* P, const uint64_t* T, int v, int h,
+                                          int plen, int tlen) {
+  constexpr int PER = 64 / BITS;
+  int rem = imin(plen - v, tlen - h);
+  int n = 0;
+  while (rem > 0) {
+    const uint64_t x = load_fwd<BITS>(P, v + n) ^ load_fwd<BITS>(T, h + n);
+    const int c = x ? (int)(__builtin_ctzll(x) / BITS) : PER;
+    if (c >= rem) { n += rem; break; }
+    n += c;
+    if (c < PER) break;
+    rem -= PER;
+  }
+  return n;
+}
+
+// Matching suffix length of pattern[..v) and text[..h), at most min(v, h)
+// (st_backward_matches, src/backtrace.rs:668-686).
+template <int BITS>
+static __device__ __forceinline__ uint64_t load_back(const uint64_t* s, int end) {
+  constexpr int PER = 64 / BITS;
+  if (end >= PER) return load_fwd<BITS>(s, end - PER);
+  // fewer than PER symbols before `end`: align them to the top bits
+  const uint64_t w = s[0];
+  return end > 0 ? (w << ((PER - end) * BITS)) : 0;
+}
+template <int BITS>
+static __device__ __forceinline__ int backward_matches(const uint64_t* P, const uint64_t* T, int v,
+                                                       int h) {
+  constexpr int PER = 64 / BITS;
+  int rem = imin(v, h);
+  int n = 0;
+  while (rem > 0) {
+    const uint64_t x = load_back<BITS>(P, v - n) ^ load_back<BITS>(T, h - n);
+    const int c = x ? (int)(__builtin_clzll(x) / BITS) : PER;
+    if (c >= rem) { n += rem; break; }
+    n += c;
+    if (c < PER) break;
+    rem -= PER;
+  }
+  return n;
+}
+
+// ---------------------------------------------------------------------
+// Row store: ring of rows, each `wcap` cells, row r at ring + r*wcap.
+// ---------------------------------------------------------------------
+struct Rows {
+  int32_t* ring;  // LDS or global
+  int2* meta;     // per row {lo, hi}; lo == META_ABSENT: never computed
+  int mask;       // wcap - 1
+  int wcap;
+};
+
+static __device__ __forceinline__ int rd(const Rows& R, int row, int lo, int hi, int k) {
+  return (k >= lo && k <= hi) ? R.ring[row * R.wcap + (k & R.mask)] : WFA_NULL;
+}
+
+// ---------------------------------------------------------------------
+// Per-pair result codes internal to the kernel
+// ---------------------------------------------------------------------
+#define ST_RETRY (-1)
+
+struct PairCtx {
+  int pair;
+  int plen, tlen;
+  int bits;  // 2 or 8
+};
+
+// Stage pair `pair`'s sequences into LDS. Returns the symbol width (2 or 8)
+// or 0 if they do not fit (caller retries with a larger configuration).
+static __device__ int stage_sequences(const KParams& p, int pair, int plen, int tlen,
+                                      uint64_t* sP, uint64_t* sT) {
+  const int lane = threadIdx.x;
+  const uint8_t* P = p.seq + p.poff[pair];
+  const uint8_t* T = p.seq + p.toff[pair];
+  int bits = 2;
+  if (p.fmt == WFA_SEQ_ASCII) {
+    // 2-bit only if every byte is one of A,C,G,T (case-sensitive, as the
+    // reference compares raw bytes)
+    bool bad = false;
+    for (int i = lane; i < plen; i += 64) {
+      const uint8_t c = P[i];
+      bad |= !(c == 'A' || c == 'C' || c == 'G' || c == 'T');
+    }
+    for (int i = lane; i < tlen; i += 64) {
+      const uint8_t c = T[i];
+      bad |= !(c == 'A' || c == 'C' || c == 'G' || c == 'T');
+    }
+    bits = ballot(bad) ? 8 : 2;
+  }
+  const int per = 64 / bits;
+  const int pw = (plen + per - 1) / per + 2;
+  const int tw = (tlen + per - 1) / per + 2;
+  if (pw > p.seq_words || tw > p.seq_words) return 0;
+  // zero the words first (padding words are read but masked off)
+  for (int i = lane; i < pw; i += 64) sP[i] = 0;
+  for (int i = lane; i < tw; i += 64) sT[i] = 0;
+  __syncthreads();
+  uint8_t* bP = (uint8_t*)sP;
+  uint8_t* bT = (uint8_t*)sT;
+  if (p.fmt == WFA_SEQ_PACKED2BITS) {
+    const int nb = (plen + 3) >> 2, mb = (tlen + 3) >> 2;
+    for (int i = lane; i < nb; i += 64) bP[i] = P[i];
+    for (int i = lane; i < mb; i += 64) bT[i] = T[i];
+  } else if (bits == 8) {
+    for (int i = lane; i < plen; i += 64) bP[i] = P[i];
+    for (int i = lane; i < tlen; i += 64) bT[i] = T[i];
+  } else {
+    // pack ASCII ACGT → 2 bits (A0 C1 G2 T3), 4 bases per byte
+    auto code = [](uint8_t c) -> uint32_t { return c == 'A' ? 0u : c == 'C' ? 1u : c == 'G' ? 2u : 3u; };
+    const int nb = (plen + 3) >> 2, mb = (tlen + 3) >> 2;
+    for (int i = lane; i < nb; i += 64) {
+      uint32_t b = 0;
+      for (int j = 0; j < 4 && 4 * i + j < plen; ++j) b |= code(P[4 * i + j]) << (2 * j);
+      bP[i] = (uint8_t)b;
+    }
+    for (int i = lane; i < mb; i += 64) {
+      uint32_t b = 0;
+      for (int j = 0; j < 4 && 4 * i + j < tlen; ++j) b |= code(T[4 * i + j]) << (2 * j);
+      bT[i] = (uint8_t)b;
+    }
+  }
+  __syncthreads();
+  return bits;
+}
+
+// ---------------------------------------------------------------------
+// CIGAR history (high memory): per slot, headers hdr[s] = {lo, hi, pos}
+// and the extended M row cells hist[pos + (k - lo)].
+// ---------------------------------------------------------------------
+struct Hist {
+  int4* hdr;
+  int32_t* cells;
+  int hdr_cap;
+  int64_t cap;
+};
+
+// st_m_offset (src/backtrace.rs:645-663) over the stored rows.
+static __device__ __forceinline__ bool st_m(const Hist& H, int nscores, int nwf, int s, int k,
+                                            int* out) {
+  if (s < 0 || s >= nwf || s >= nscores) return false;
+  const int4 h = H.hdr[s];
+  if (k < h.x || k > h.y) return false;
+  const int v = H.cells[h.z + (k - h.x)];
+  if (v == WFA_NULL) return false;
+  *out = v;
+  return true;
+}
+
+// write `n` copies of op backwards ending at out[pos] (pos decreases)
+static __device__ __forceinline__ void write_ops(uint8_t* out, int& pos, uint8_t op, int n) {
+  for (int i = threadIdx.x; i < n; i += 64) out[pos - i] = op;
+  pos -= n > 0 ? n : 0;
+}
+
+// Singletrack backtrace (src/backtrace.rs:751-858), end-to-end, from the
+// end cell (score, k, offset). Writes ops backwards into out[0..cap) and
+// returns the number of ops, which end at out[cap-1].
+template <int BITS>
+static __device__ int singletrack(const KParams& p, const Hist& H, int nscores, int nwf,
+                                  const uint64_t* sP, const uint64_t* sT, int plen, int tlen,
+                                  int score, int k, int offset, uint8_t* out, int cap) {
+  const int lane = threadIdx.x;
+  const bool two = p.metric == WFA_METRIC_GAP_AFFINE_2P;
+  int pos = cap - 1;
+  int offset_orig = offset;
+  write_ops(out, pos, 'D', plen - (offset - k));
+  write_ops(out, pos, 'I', tlen - offset);
+  bool in_m = true;
+  int l = 0;
+  while (score != 0) {
+    if (in_m) {
+      const int v = offset - k, h = offset;
+      const int nm = backward_matches<BITS>(sP, sT, v, h);
+      offset_orig = offset - nm;
+      const int mis = score - p.x;
+      int mo;
+      if (st_m(H, nscores, nwf, mis, k, &mo) && mo == offset_orig - 1) {
+        write_ops(out, pos, 'M', nm);
+        write_ops(out, pos, 'X', 1);
+        offset = offset_orig - 1;
+        score = mis;
+      } else {
+        in_m = false;
+        l = 0;
+      }
+    } else {
+      // lanes test gap lengths l+1 .. l+64 at once; the lowest length whose
+      // test succeeds (or ends the search) decides, in the reference's order:
+      // piece 1 insertion, piece 1 deletion, piece 2 insertion, deletion.
+      const int L = l + 1 + lane;
+      const int ind1 = score - p.o1 - L * p.e1;
+      const int ind2 = two ? score - p.o2 - L * p.e2 : -1;
+      int m1i = 0, m1d = 0, m2i = 0, m2d = 0;
+      const bool r1i = st_m(H, nscores, nwf, ind1, k - L, &m1i) && m1i + L >= offset_orig &&
+                       m1i + L <= offset;
+      const bool r1d = st_m(H, nscores, nwf, ind1, k + L, &m1d) && m1d >= offset_orig &&
+                       m1d <= offset;
+      bool r2i = false, r2d = false;
+      if (two) {
+        r2i = st_m(H, nscores, nwf, ind2, k - L, &m2i) && m2i + L >= offset_orig &&
+              m2i + L <= offset;
+        r2d = st_m(H, nscores, nwf, ind2, k + L, &m2d) && m2d >= offset_orig && m2d <= offset;
+      }
+      const bool stop = r1i || r1d || r2i || r2d || (ind1 < 0 && ind2 < 0);
+      const uint64_t mask = ballot(stop);
+      if (!mask) {
+        l += 64;
+        continue;
+      }
+      const int j = __builtin_ctzll(mask);
+      const int Lj = uniform(__shfl(L, j));
+      const int f = (r1i ? 1 : 0) | (r1d ? 2 : 0) | (r2i ? 4 : 0) | (r2d ? 8 : 0);
+      const int fj = uniform(__shfl(f, j));
+      if (fj & 1) {
+        const int m = uniform(__shfl(m1i, j));
+        write_ops(out, pos, 'M', offset - (m + Lj));
+        write_ops(out, pos, 'I', Lj);
+        k -= Lj; offset = m; score = uniform(__shfl(ind1, j)); in_m = true;
+      } else if (fj & 2) {
+        const int m = uniform(__shfl(m1d, j));
+        write_ops(out, pos, 'M', offset - m);
+        write_ops(out, pos, 'D', Lj);
+        k += Lj; offset = m; score = uniform(__shfl(ind1, j)); in_m = true;
+      } else if (fj & 4) {
+        const int m = uniform(__shfl(m2i, j));
+        write_ops(out, pos, 'M', offset - (m + Lj));
+        write_ops(out, pos, 'I', Lj);
+        k -= Lj; offset = m; score = uniform(__shfl(ind2, j)); in_m = true;
+      } else if (fj & 8) {
+        const int m = uniform(__shfl(m2d, j));
+        write_ops(out, pos, 'M', offset - m);
+        write_ops(out, pos, 'D', Lj);
+        k += Lj; offset = m; score = uniform(__shfl(ind2, j)); in_m = true;
+      } else {
+        break;  // no source: only at an ends-free start
+      }
+    }
+  }
+  int v = offset - k, h = offset;
+  const int nm = imin(v, h);
+  v -= nm; h -= nm;
+  write_ops(out, pos, 'M', nm);
+  write_ops(out, pos, 'D', v);
+  write_ops(out, pos, 'I', h);
+  return cap - 1 - pos;
+}
+
+// backtrace_linear (src/backtrace.rs:132-258) for edit / indel, high memory.
+template <int BITS>
+static __device__ int backtrace_linear(const KParams& p, const Hist& H, int nscores, int plen,
+                                       int tlen, int score, int k, int offset, uint8_t* out,
+                                       int cap) {
+  const bool indel = p.metric == WFA_METRIC_INDEL;
+  int pos = cap - 1;
+  int v = offset - k, h = offset;
+  if (v < plen) write_ops(out, pos, 'D', plen - v);
+  if (h < tlen) write_ops(out, pos, 'I', tlen - h);
+  // packed (offset << 4) | type; NULL when absent
+  auto cell = [&](int s, int kk, int lo_adj, int* val) -> bool {
+    (void)lo_adj;
+    if (s < 0 || s >= nscores) return false;
+    const int4 hd = H.hdr[s];
+    if (kk < hd.x || kk > hd.y) return false;
+    *val = H.cells[hd.z + (kk - hd.x)];
+    return true;
+  };
+  while (v > 0 && h > 0 && score > 0) {
+    const int mis_s = score - p.x, gap_s = score - p.o1;
+    int64_t mis = WFA_NULL, ins = WFA_NULL, del = WFA_NULL;
+    int c;
+    // bt_misms: init_min <= k <= init_max
+    if (!indel && cell(mis_s, k, 0, &c)) mis = ((int64_t)(c + 1) << 4) | 9;
+    // bt_ins1_open: init_min < k && k-1 <= init_max  -> cell k-1 in range
+    if (cell(gap_s, k - 1, 0, &c)) ins = ((int64_t)(c + 1) << 4) | 1;
+    // bt_del1_open: init_min <= k+1 && k < init_max -> cell k+1 in range
+    if (cell(gap_s, k + 1, 0, &c)) del = ((int64_t)c << 4) | 5;
+    int64_t mx = ins > del ? ins : del;
+    if (mis > mx) mx = mis;
+    if (mx < 0) break;
+    const int moff = (int)(mx >> 4);
+    const int nmatch = offset - moff;
+    if (nmatch < 0) pos += -nmatch; else write_ops(out, pos, 'M', nmatch);
+    offset = moff;
+    v = offset - k; h = offset;
+    if (v <= 0 || h <= 0) break;
+    switch ((int)(mx & 0xF)) {
+      case 9: score = mis_s; write_ops(out, pos, 'X', 1); offset -= 1; break;
+      case 1: score = gap_s; write_ops(out, pos, 'I', 1); k -= 1; offset -= 1; break;
+      default: score = gap_s; write_ops(out, pos, 'D', 1); k += 1; break;
+    }
+    v = offset - k; h = offset;
+  }
+  if (v > 0 && h > 0) {
+    const int nm = imin(v, h);
+    write_ops(out, pos, 'M', nm);
+    v -= nm; h -= nm;
+  }
+  write_ops(out, pos, 'D', v > 0 ? v : 0);
+  write_ops(out, pos, 'I', h > 0 ? h : 0);
+  return cap - 1 - pos;
+}
+
+// num_wavefronts of a freshly created aligner in high memory
+// (compute_dimensions, src/components.rs:435-504, sized for
+// max(1000x1000, plen x tlen)).
+static __device__ int high_mem_nwf(const KParams& p, int plen, int tlen) {
+  auto dims = [&](int pl, int tl) -> int {
+    const int ad = pl > tl ? pl - tl : tl - pl;
+    const int mn = imin(pl, tl);
+    switch (p.metric) {
+      case WFA_METRIC_INDEL: return pl + tl + 1;
+      case WFA_METRIC_EDIT: return imax(pl, tl) + 1;
+      case WFA_METRIC_GAP_AFFINE: return mn * p.x + p.o1 + ad * p.e1 + 1;
+      default: {
+        const int i1 = p.o1 + ad * p.e1, i2 = p.o2 + ad * p.e2;
+        return mn * p.x + imin(i1, i2) + 1;
+      }
+    }
+  };
+  return imax(dims(1000, 1000), dims(plen, tlen));
+}
+
+// ---------------------------------------------------------------------
+// The per-pair forward pass + backtrace.
+// ---------------------------------------------------------------------
+template <int KIND, bool CIG, int BITS>
+static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen, uint64_t* sP,
+                                  uint64_t* sT, const Rows& R, const Hist& H) {
+  const int lane = threadIdx.x;
+  const bool two = KIND == KIND_AFFINE2P;
+  // ring rows: M: [0, ringM), I1: [ringM, ringM+ringE1), D1 next, then I2, D2
+  const int ringM = p.ringM, ringE1 = p.ringE1, ringE2 = p.ringE2;
+  const int rI1 = ringM, rD1 = ringM + ringE1, rI2 = ringM + 2 * ringE1,
+            rD2 = ringM + 2 * ringE1 + ringE2;
+  const int nrows = KIND == KIND_EDIT ? ringM : two ? rD2 + ringE2 : rD1 + ringE1;
+  for (int r = lane; r < nrows; r += 64) R.meta[r] = make_int2(META_ABSENT, META_ABSENT - 1);
+  __syncthreads();
+  // M0: k = 0, offset 0 (init_alignment, src/aligner.rs:938-950)
+  if (lane == 0) {
+    R.meta[0] = make_int2(0, 0);
+    R.ring[0] = 0;
+  }
+  __syncthreads();
+
+  const int nwf = high_mem_nwf(p, plen, tlen);
+  const int k_end = tlen - plen;
+  const int hist_lo = -(plen + p.scope + 2), hist_hi = tlen + p.scope + 2;
+  int64_t hpos = 0;
+  int nulls = 0;
+  int score = 0;
+  int status = WFA_STATUS_OK;
+  int result = 0;
+
+  // extend + terminate + (CIGAR) record M[score] over its rows; `lo/hi`
+  // is the computed (pre-trim) range; returns true if the end is reached.
+  auto extend_row = [&](int s, int lo, int hi) -> bool {
+    const int row = s % ringM;
+    bool reached = false;
+    int4* hd = CIG ? &H.hdr[s] : nullptr;
+    int64_t base = hpos;
+    if (CIG) {
+      if (lo <= hi && hpos + (hi - lo + 1) > H.cap) return false;
+    }
+    for (int kb = lo; kb <= hi; kb += 64) {
+      const int k = kb + lane;
+      if (k <= hi) {
+        int32_t* cell = &R.ring[row * R.wcap + (k & R.mask)];
+        int off = *cell;
+        if (off >= 0) {
+          off += lcp<BITS>(sP, sT, off - k, off, plen, tlen);
+          *cell = off;
+        }
+        if (k == k_end && off >= tlen) reached = true;
+        if (CIG) H.cells[base + (k - lo)] = off;
+      }
+    }
+    if (CIG) {
+      if (lane == 0) *hd = make_int4(lo, hi, (int)base, 0);
+      if (lo <= hi) hpos += hi - lo + 1;
+    }
+    return ballot(reached) != 0;
+  };
+
+  bool overflow = false;
+  // score 0
+  {
+    if (CIG && (H.hdr_cap < 1 || H.cap < 1)) overflow = true;
+    else if (extend_row(0, 0, 0)) {
+      status = WFA_STATUS_END_REACHED;
+      result = 0;
+    }
+  }
+  __syncthreads();
+
+  while (status == WFA_STATUS_OK && !overflow) {
+    score += 1;
+    if (!p.modular && score >= nwf) { status = WFA_STATUS_MAX_STEPS_REACHED; result = score; break; }
+    if (CIG && score >= H.hdr_cap) { overflow = true; break; }
+    const int s = score;
+    // ---- input rows ----
+    int lo = INT32_MAX, hi = INT32_MIN;
+    int ml_x = 1, mh_x = 0, ml_o = 1, mh_o = 0, il = 1, ih = 0, dl = 1, dh = 0;
+    int ml_o2 = 1, mh_o2 = 0, il2 = 1, ih2 = 0, dl2 = 1, dh2 = 0;
+    int row_x = 0, row_o = 0, row_e = 0, row_o2 = 0, row_e2 = 0;
+    bool any = false;
+    if (KIND == KIND_EDIT) {
+      row_x = (s - 1) % ringM;
+      const int2 m = R.meta[row_x];
+      ml_x = uniform(m.x); mh_x = uniform(m.y);
+      if (ml_x != META_ABSENT && ml_x <= mh_x) {
+        any = true;
+        lo = ml_x - 1;
+        hi = mh_x + 1;
+      }
+    } else {
+      if (s - p.x >= 0) {
+        row_x = (s - p.x) % ringM;
+        const int2 m = R.meta[row_x];
+        ml_x = uniform(m.x); mh_x = uniform(m.y);
+      }
+      if (s - p.oe1 >= 0) {
+        row_o = (s - p.oe1) % ringM;
+        const int2 m = R.meta[row_o];
+        ml_o = uniform(m.x); mh_o = uniform(m.y);
+      }
+      if (s - p.e1 >= 0) {
+        row_e = (s - p.e1) % ringE1;
+        const int2 a = R.meta[rI1 + row_e];
+        const int2 b = R.meta[rD1 + row_e];
+        il = uniform(a.x); ih = uniform(a.y); dl = uniform(b.x); dh = uniform(b.y);
+      }
+      if (two) {
+        if (s - p.oe2 >= 0) {
+          row_o2 = (s - p.oe2) % ringM;
+          const int2 m = R.meta[row_o2];
+          ml_o2 = uniform(m.x); mh_o2 = uniform(m.y);
+        }
+        if (s - p.e2 >= 0) {
+          row_e2 = (s - p.e2) % ringE2;
+          const int2 a = R.meta[rI2 + row_e2];
+          const int2 b = R.meta[rD2 + row_e2];
+          il2 = uniform(a.x); ih2 = uniform(a.y); dl2 = uniform(b.x); dh2 = uniform(b.y);
+        }
+      }
+      // absent rows (META_ABSENT, hi = META_ABSENT-1) and trimmed-empty rows
+      // both have lo > hi and read as the null wavefront.
+      auto live = [](int l, int h) { return l <= h; };
+      if (live(ml_x, mh_x)) { any = true; lo = imin(lo, ml_x); hi = imax(hi, mh_x); }
+      if (live(ml_o, mh_o)) { any = true; lo = imin(lo, ml_o - 1); hi = imax(hi, mh_o + 1); }
+      if (live(il, ih)) { any = true; lo = imin(lo, il + 1); hi = imax(hi, ih + 1); }
+      if (live(dl, dh)) { any = true; lo = imin(lo, dl - 1); hi = imax(hi, dh - 1); }
+      if (two) {
+        if (live(ml_o2, mh_o2)) { any = true; lo = imin(lo, ml_o2 - 1); hi = imax(hi, mh_o2 + 1); }
+        if (live(il2, ih2)) { any = true; lo = imin(lo, il2 + 1); hi = imax(hi, ih2 + 1); }
+        if (live(dl2, dh2)) { any = true; lo = imin(lo, dl2 - 1); hi = imax(hi, dh2 - 1); }
+      }
+    }
+    const int rowM = s % ringM;
+    const int rowE = s % ringE1;
+    const int rowE2 = two ? s % ringE2 : 0;
+    __syncthreads();  // every lane has read the metadata before it is rewritten
+    if (!any) {
+      // null step (src/aligner.rs:2031-2043, 1778-1782)
+      nulls += 1;
+      if (lane == 0) {
+        R.meta[rowM] = make_int2(META_ABSENT, META_ABSENT - 1);
+        if (KIND != KIND_EDIT) {
+          R.meta[rI1 + rowE] = make_int2(META_ABSENT, META_ABSENT - 1);
+          R.meta[rD1 + rowE] = make_int2(META_ABSENT, META_ABSENT - 1);
+        }
+        if (two) {
+          R.meta[rI2 + rowE2] = make_int2(META_ABSENT, META_ABSENT - 1);
+          R.meta[rD2 + rowE2] = make_int2(META_ABSENT, META_ABSENT - 1);
+        }
+        if (CIG) H.hdr[s] = make_int4(1, 0, 0, 0);
+      }
+      __syncthreads();
+      if (s >= p.max_steps) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
+      if (nulls > p.scope) { status = WFA_STATUS_END_UNREACHABLE; result = s; break; }
+      continue;
+    }
+    nulls = 0;
+    lo = imax(lo, hist_lo + 1);
+    hi = imin(hi, hist_hi - 1);
+    if (lo > hi) {
+      // cannot happen without a band; keep the reference's "no output" rule
+      if (lane == 0) {
+        R.meta[rowM] = make_int2(META_ABSENT, META_ABSENT - 1);
+        if (CIG) H.hdr[s] = make_int4(1, 0, 0, 0);
+      }
+      __syncthreads();
+      if (s >= p.max_steps) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
+      continue;
+    }
+    if (hi - lo + 1 > R.wcap) { overflow = true; break; }
+
+    // ---- compute all chunks; trims by ballot ----
+    int m_lo = INT32_MAX, m_hi = INT32_MIN, i_lo = INT32_MAX, i_hi = INT32_MIN;
+    int d_lo = INT32_MAX, d_hi = INT32_MIN, i2_lo = INT32_MAX, i2_hi = INT32_MIN;
+    int d2_lo = INT32_MAX, d2_hi = INT32_MIN;
+    for (int kb = lo; kb <= hi; kb += 64) {
+      const int k = kb + lane;
+      bool vm = false, vi = false, vd = false, vi2 = false, vd2 = false;
+      if (k <= hi) {
+        int m;
+        if (KIND == KIND_EDIT) {
+          const int ins = rd(R, row_x, ml_x, mh_x, k - 1);
+          const int del = rd(R, row_x, ml_x, mh_x, k + 1);
+          if (p.metric == WFA_METRIC_INDEL) {
+            m = imax(del, ins + 1);
+          } else {
+            const int mis = rd(R, row_x, ml_x, mh_x, k);
+            m = imax(del, imax(ins, mis) + 1);
+          }
+          if (!in_bounds(m, k, plen, tlen)) m = WFA_NULL;
+        } else {
+          const int i1 = imax(rd(R, row_o, ml_o, mh_o, k - 1),
+                              rd(R, rI1 + row_e, il, ih, k - 1)) + 1;
+          const int d1 = imax(rd(R, row_o, ml_o, mh_o, k + 1), rd(R, rD1 + row_e, dl, dh, k + 1));
+          const int mis = rd(R, row_x, ml_x, mh_x, k) + 1;
+          int ins = i1, del = d1;
+          if (two) {
+            const int i2 = imax(rd(R, row_o2, ml_o2, mh_o2, k - 1),
+                                rd(R, rI2 + row_e2, il2, ih2, k - 1)) + 1;
+            const int d2 = imax(rd(R, row_o2, ml_o2, mh_o2, k + 1),
+                                rd(R, rD2 + row_e2, dl2, dh2, k + 1));
+            R.ring[(rI2 + rowE2) * R.wcap + (k & R.mask)] = i2;
+            R.ring[(rD2 + rowE2) * R.wcap + (k & R.mask)] = d2;
+            vi2 = in_bounds(i2, k, plen, tlen);
+            vd2 = in_bounds(d2, k, plen, tlen);
+            ins = imax(ins, i2);
+            del = imax(del, d2);
+          }
+          m = imax(mis, imax(ins, del));
+          if (!in_bounds(m, k, plen, tlen)) m = WFA_NULL;
+          R.ring[(rI1 + rowE) * R.wcap + (k & R.mask)] = i1;
+          R.ring[(rD1 + rowE) * R.wcap + (k & R.mask)] = d1;
+          vi = in_bounds(i1, k, plen, tlen);
+          vd = in_bounds(d1, k, plen, tlen);
+        }
+        vm = m != WFA_NULL;
+        R.ring[rowM * R.wcap + (k & R.mask)] = m;
+      }
+      uint64_t b = ballot(vm);
+      if (b) {
+        if (m_lo == INT32_MAX) m_lo = kb + __builtin_ctzll(b);
+        m_hi = kb + 63 - __builtin_clzll(b);
+      }
+      if (KIND != KIND_EDIT) {
+        b = ballot(vi);
+        if (b) {
+          if (i_lo == INT32_MAX) i_lo = kb + __builtin_ctzll(b);
+          i_hi = kb + 63 - __builtin_clzll(b);
+        }
+        b = ballot(vd);
+        if (b) {
+          if (d_lo == INT32_MAX) d_lo = kb + __builtin_ctzll(b);
+          d_hi = kb + 63 - __builtin_clzll(b);
+        }
+        if (two) {
+          b = ballot(vi2);
+          if (b) {
+            if (i2_lo == INT32_MAX) i2_lo = kb + __builtin_ctzll(b);
+            i2_hi = kb + 63 - __builtin_clzll(b);
+          }
+          b = ballot(vd2);
+          if (b) {
+            if (d2_lo == INT32_MAX) d2_lo = kb + __builtin_ctzll(b);
+            d2_hi = kb + 63 - __builtin_clzll(b);
+          }
+        }
+      }
+    }
+    // trimmed-empty rows: lo > hi (but not META_ABSENT)
+    auto fix = [](int& l, int& h) { if (l == INT32_MAX) { l = 1; h = 0; } };
+    fix(m_lo, m_hi); fix(i_lo, i_hi); fix(d_lo, d_hi); fix(i2_lo, i2_hi); fix(d2_lo, d2_hi);
+    __syncthreads();  // ring rows written before the extension reads them
+    if (lane == 0) {
+      R.meta[rowM] = make_int2(m_lo, m_hi);
+      if (KIND != KIND_EDIT) {
+        R.meta[rI1 + rowE] = make_int2(i_lo, i_hi);
+        R.meta[rD1 + rowE] = make_int2(d_lo, d_hi);
+      }
+      if (two) {
+        R.meta[rI2 + rowE2] = make_int2(i2_lo, i2_hi);
+        R.meta[rD2 + rowE2] = make_int2(d2_lo, d2_hi);
+      }
+    }
+    if (s >= p.max_steps) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; __syncthreads(); break; }
+    // ---- extend M[s] (the reference extends at the top of the next
+    // iteration; nothing reads M[s] in between) ----
+    // In high memory the stored row keeps its computed range: M cells
+    // outside the trimmed range are OFFSET_NULL there anyway.
+    const int elo = CIG ? lo : m_lo, ehi = CIG ? hi : m_hi;
+    if (CIG && hpos + (hi - lo + 1) > H.cap) { overflow = true; break; }
+    const bool reached = extend_row(s, elo, ehi) && (m_lo <= k_end && k_end <= m_hi);
+    __syncthreads();
+    if (reached) { status = WFA_STATUS_END_REACHED; result = s; break; }
+  }
+
+  if (overflow) {
+    if (lane == 0) {
+      const int slot = atomicAdd(&p.counters[1], 1);
+      p.retry_list[slot] = pair;
+      p.status[pair] = ST_RETRY;
+    }
+    return;
+  }
+  int out_status = status;
+  int ncig = 0;
+  if (status == WFA_STATUS_END_REACHED) {
+    out_status = WFA_STATUS_COMPLETED;
+    if (CIG) {
+      __threadfence_block();
+      uint8_t* out = p.cig + p.cig_off[pair];
+      const int cap = plen + tlen;
+      if (KIND == KIND_EDIT)
+        ncig = backtrace_linear<BITS>(p, H, score + 1, plen, tlen, score, k_end, tlen, out, cap);
+      else
+        ncig = singletrack<BITS>(p, H, score + 1, nwf, sP, sT, plen, tlen, score, k_end, tlen,
+                                 out, cap);
+      // move the ops from the tail of the region to its head
+      const int gap = cap - ncig;
+      if (gap > 0) {
+        for (int c = 0; c < ncig; c += 64) {
+          const int i = c + lane;
+          uint8_t v = 0;
+          if (i < ncig) v = out[gap + i];
+          __builtin_amdgcn_wave_barrier();
+          if (i < ncig) out[i] = v;
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+    }
+  }
+  if (lane == 0) {
+    p.score[pair] = result;
+    p.status[pair] = out_status;
+    if (CIG) p.cig_len[pair] = ncig;
+  }
+}
+
+template <int KIND, bool CIG, bool LDS_RING>
+__global__ void __launch_bounds__(64) wfa_align_kernel(KParams p) {
+  extern __shared__ uint64_t smem[];
+  const int lane = threadIdx.x;
+  uint64_t* sP = smem;
+  uint64_t* sT = smem + p.seq_words;
+  int2* meta = (int2*)(sT + p.seq_words);
+  Rows R;
+  R.meta = meta;
+  R.wcap = p.wcap;
+  R.mask = p.wcap - 1;
+  if (LDS_RING) R.ring = (int32_t*)(meta + p.nrows_meta);
+  else R.ring = p.gring + (int64_t)blockIdx.x * p.ring_rows * p.wcap;
+  Hist H;
+  H.hdr = CIG ? p.hdr + (int64_t)blockIdx.x * p.hdr_cap : nullptr;
+  H.cells = CIG ? p.hist + (int64_t)blockIdx.x * p.hist_cap : nullptr;
+  H.hdr_cap = p.hdr_cap;
+  H.cap = p.hist_cap;
+  for (;;) {
+    int item = 0;
+    if (lane == 0) item = atomicAdd(&p.counters[0], 1);
+    item = uniform(__shfl(item, 0));
+    if (item >= p.num_pairs) break;
+    const int pair = p.pair_list ? p.pair_list[item] : p.pair_base + item;
+    const int plen = p.plen[pair], tlen = p.tlen[pair];
+    const int bits = stage_sequences(p, pair, plen, tlen, sP, sT);
+    if (bits == 0) {
+      if (lane == 0) {
+        const int slot = atomicAdd(&p.counters[1], 1);
+        p.retry_list[slot] = pair;
+        p.status[pair] = ST_RETRY;
+      }
+      continue;
+    }
+    if (bits == 2) align_pair<KIND, CIG, 2>(p, pair, plen, tlen, sP, sT, R, H);
+    else align_pair<KIND, CIG, 8>(p, pair, plen, tlen, sP, sT, R, H);
+    __syncthreads();
+  }
+}
+
+__global__ void wfa_reset_counters(int32_t* c) {
+  if (threadIdx.x < 4) c[threadIdx.x] = 0;
+}
+
+// out[0] = max over pairs of max(plen, tlen); out must be zeroed first.
+__global__ void wfa_max_length(const int32_t* plen, const int32_t* tlen, int n, int32_t* out) {
+  int m = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    m = imax(m, imax(plen[i], tlen[i]));
+  for (int o = 32; o > 0; o >>= 1) m = imax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// ---------------------------------------------------------------------
+// Host-side launch helpers
+// ---------------------------------------------------------------------
+using KernelFn = void (*)(KParams);
+
+static KernelFn pick(int kind, bool cig, bool lds) {
+#define PICK(K)                                                                  \
+  if (kind == K) {                                                               \
+    if (cig) return lds ? wfa_align_kernel<K, true, true> : wfa_align_kernel<K, true, false>; \
+    return lds ? wfa_align_kernel<K, false, true> : wfa_align_kernel<K, false, false>;         \
+  }
+  PICK(KIND_EDIT)
+  PICK(KIND_AFFINE)
+  PICK(KIND_AFFINE2P)
+#undef PICK
+  return nullptr;
+}
+
+hipError_t launch_reset(int32_t* counters, hipStream_t stream) {
+  hipLaunchKernelGGL(wfa_reset_counters, dim3(1), dim3(64), 0, stream, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, int32_t* out,
+                             hipStream_t stream) {
+  hipLaunchKernelGGL(wfa_reset_counters, dim3(1), dim3(64), 0, stream, out);
+  int grid = (n + 255) / 256;
+  if (grid > 1024) grid = 1024;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(wfa_max_length, dim3(grid), dim3(256), 0, stream, plen, tlen, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_align(const KParams& p, int kind, bool cig, bool lds_ring, int grid,
+                        size_t lds_bytes, hipStream_t stream) {
+  KernelFn fn = pick(kind, cig, lds_ring);
+  if (!fn) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(64), lds_bytes, stream, p);
+  return hipGetLastError();
+}
+
+int max_blocks_per_cu(int kind, bool cig, bool lds_ring, size_t lds_bytes) {
+  KernelFn fn = pick(kind, cig, lds_ring);
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, lds_bytes) != hipSuccess) return 1;
+  return n > 0 ? n : 1;
+}
+
+}  // namespace wfa
