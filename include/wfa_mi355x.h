@@ -235,6 +235,12 @@ int32_t wfa_engine_align_batch_per_pair(wfa_engine_t* engine, const wfa_attr_t* 
  * re-run by a second, wider pass. */
 int32_t wfa_engine_last_retries(const wfa_engine_t* engine);
 
+/* HIP-event timing of the last batch call, measured on the stream it ran
+ * on: the first (persistent) launch alone and all launches including the
+ * retry passes, in milliseconds. */
+int32_t wfa_engine_last_kernel_ms(const wfa_engine_t* engine, float* first_pass_ms,
+                                  float* all_passes_ms);
+
 /* Library version string. */
 const char* wfa_version(void);
 

@@ -102,17 +102,23 @@ static int pen_init(pen_t* p, const wfa_penalties_t* u) {
 /* ------------------------------------------------------------------ */
 typedef struct wf {
     int32_t* mem; /* cells [base, base + nalloc - 1] */
+    uint32_t* pc; /* piggyback: packed CIGAR per cell (med/low memory) */
+    uint32_t* pv; /* piggyback: previous backtrace block per cell */
+    int occ;      /* bt_occupancy_max */
     int base, nalloc;
     int lo, hi;
     int init_min, init_max;
     int null;
 } wf_t;
 
-static wf_t* wf_new(int alloc_lo, int alloc_hi) {
+static wf_t* wf_new(int alloc_lo, int alloc_hi, int bt) {
     wf_t* w = (wf_t*)malloc(sizeof(wf_t));
     w->base = alloc_lo;
     w->nalloc = alloc_hi - alloc_lo + 1;
     w->mem = (int32_t*)malloc(sizeof(int32_t) * (size_t)w->nalloc);
+    w->pc = bt ? (uint32_t*)calloc((size_t)w->nalloc, sizeof(uint32_t)) : NULL;
+    w->pv = bt ? (uint32_t*)calloc((size_t)w->nalloc, sizeof(uint32_t)) : NULL;
+    w->occ = 0;
     /* Wavefront::init (src/wavefront.rs:307-328) */
     w->null = 0;
     w->lo = 1; w->hi = -1;
@@ -120,7 +126,7 @@ static wf_t* wf_new(int alloc_lo, int alloc_hi) {
     return w;
 }
 static void wf_free(wf_t* w) {
-    if (w) { free(w->mem); free(w); }
+    if (w) { free(w->mem); free(w->pc); free(w->pv); free(w); }
 }
 static inline int wf_allocated(const wf_t* w, int k) {
     return k >= w->base && k < w->base + w->nalloc;
@@ -140,6 +146,10 @@ static inline void wf_set(wf_t* w, int k, int32_t v) {
     }
     w->mem[k - w->base] = v;
 }
+/* zero_bt (src/wavefront.rs:570-580): halo cells carry an empty chain */
+static void wf_zero_bt(wf_t* w, int k) {
+    if (w->pc) { w->pc[k - w->base] = 0; w->pv[k - w->base] = 0; }
+}
 /* set_limits (src/wavefront.rs:360-368) */
 static void wf_set_limits(wf_t* w, int lo, int hi) {
     w->lo = lo; w->hi = hi; w->init_min = lo; w->init_max = hi;
@@ -149,7 +159,7 @@ static void wf_init_ends_lower(wf_t* w, int min_lo) {
     if (w->init_min <= min_lo) return;
     int min_init = w->init_min < w->lo ? w->init_min : w->lo;
     int start = min_lo > w->base ? min_lo : w->base;
-    for (int k = start; k < min_init; ++k) wf_set(w, k, OFFSET_NULL);
+    for (int k = start; k < min_init; ++k) { wf_set(w, k, OFFSET_NULL); wf_zero_bt(w, k); }
     w->init_min = start;
 }
 /* init_ends_higher (src/wavefront.rs:442-457) */
@@ -158,7 +168,7 @@ static void wf_init_ends_higher(wf_t* w, int max_hi) {
     int max_init = w->init_max > w->hi ? w->init_max : w->hi;
     int alloc_max = w->base + w->nalloc - 1;
     int end = max_hi < alloc_max ? max_hi : alloc_max;
-    for (int k = max_init + 1; k <= end; ++k) wf_set(w, k, OFFSET_NULL);
+    for (int k = max_init + 1; k <= end; ++k) { wf_set(w, k, OFFSET_NULL); wf_zero_bt(w, k); }
     w->init_max = end;
 }
 /* A read of an input wavefront inside a kernel: the null wavefront reads
@@ -222,6 +232,12 @@ struct wfo_aligner {
     int end_score, end_k, end_offset;
     /* cigar */
     uint8_t* cig; int cig_cap, cig_begin, cig_end, cig_score;
+    /* piggyback backtrace (med/low memory): BacktraceBuffer
+     * (src/bt_buffer.rs:34-209) as flat arrays */
+    int piggyback, low;
+    uint32_t *blk_pc, *blk_prev; int nblk, cap_blk;
+    int32_t *init_v, *init_h; int ninit, cap_init;
+    uint32_t* packed; int cap_packed;
 };
 
 static void pool_add(wfo_aligner* a, wf_t* w) {
@@ -306,7 +322,7 @@ static void output_alloc_range(wfo_aligner* a, int lo, int hi, int* alo, int* ah
 }
 
 static wf_t* alloc_wf(wfo_aligner* a, int alo, int ahi) {
-    wf_t* w = wf_new(alo, ahi);
+    wf_t* w = wf_new(alo, ahi, a->piggyback);
     pool_add(a, w);
     return w;
 }
@@ -323,10 +339,8 @@ wfo_aligner* wfo_aligner_new(const wfa_attr_t* attr, int32_t* err) {
         return NULL;
     }
     if (attr->span != WFA_SPAN_END2END || attr->heuristic != WFA_HEURISTIC_NONE ||
-        attr->memory_mode != WFA_MEMORY_HIGH ||
-        attr->penalties.distance_metric == WFA_METRIC_GAP_LINEAR ||
-        (attr->alignment_scope == WFA_SCOPE_ALIGNMENT && a->pen.match != 0 &&
-         attr->penalties.distance_metric >= WFA_METRIC_GAP_AFFINE && 0)) {
+        attr->memory_mode == WFA_MEMORY_ULTRALOW ||
+        attr->penalties.distance_metric == WFA_METRIC_GAP_LINEAR) {
         if (err) *err = WFA_ERR_UNSUPPORTED;
         free(a);
         return NULL;
@@ -345,6 +359,11 @@ void wfo_aligner_free(wfo_aligner* a) {
     free(a->pattern);
     free(a->text);
     free(a->cig);
+    free(a->blk_pc);
+    free(a->blk_prev);
+    free(a->init_v);
+    free(a->init_h);
+    free(a->packed);
     free(a);
 }
 
@@ -358,13 +377,93 @@ int32_t wfo_cigar(const wfo_aligner* a, const uint8_t** ops) {
 int32_t wfo_cigar_score(const wfo_aligner* a) { return a->cig_score; }
 
 /* ------------------------------------------------------------------ */
+/* Backtrace buffer (src/bt_buffer.rs:113-209) and packed CIGARs       */
+/* (src/pcigar.rs)                                                      */
+/* ------------------------------------------------------------------ */
+#define BT_NULL 0xFFFFFFFFu
+#define PC_DEL 1u
+#define PC_MISMS 2u
+#define PC_INS 3u
+#define PC_FULL_MASK 0x40000000u
+#define PC_ALMOST_FULL_MASK 0x10000000u
+#define PC_HALF_FULL_MASK 0x00010000u
+
+static uint32_t store_block(wfo_aligner* a, uint32_t pcigar, uint32_t prev) {
+    if (a->nblk == a->cap_blk) {
+        a->cap_blk = a->cap_blk ? 2 * a->cap_blk : 4096;
+        a->blk_pc = (uint32_t*)realloc(a->blk_pc, sizeof(uint32_t) * (size_t)a->cap_blk);
+        a->blk_prev = (uint32_t*)realloc(a->blk_prev, sizeof(uint32_t) * (size_t)a->cap_blk);
+    }
+    a->blk_pc[a->nblk] = pcigar;
+    a->blk_prev[a->nblk] = prev;
+    return (uint32_t)a->nblk++;
+}
+/* init_block (src/bt_buffer.rs:135-139) */
+static uint32_t init_block(wfo_aligner* a, int v, int h) {
+    if (a->ninit == a->cap_init) {
+        a->cap_init = a->cap_init ? 2 * a->cap_init : 64;
+        a->init_v = (int32_t*)realloc(a->init_v, sizeof(int32_t) * (size_t)a->cap_init);
+        a->init_h = (int32_t*)realloc(a->init_h, sizeof(int32_t) * (size_t)a->cap_init);
+    }
+    a->init_v[a->ninit] = v;
+    a->init_h[a->ninit] = h;
+    return store_block(a, (uint32_t)a->ninit++, BT_NULL);
+}
+/* set_bt_init (src/aligner.rs:216-225) */
+static void set_bt_init(wfo_aligner* a, wf_t* w, int k, int v, int h) {
+    w->pc[k - w->base] = 0;
+    w->pv[k - w->base] = init_block(a, v, h);
+}
+static inline uint32_t rd_pc(const wf_t* w, int k) { return w ? w->pc[k - w->base] : 0; }
+static inline uint32_t rd_pv(const wf_t* w, int k) { return w ? w->pv[k - w->base] : 0; }
+
+/* offload_selective (src/compute/piggyback.rs:535-556) */
+static void offload_selective(wfo_aligner* a, wf_t* w, int lo, int hi, uint32_t mask) {
+    for (int k = lo; k <= hi; ++k) {
+        uint32_t pc = w->pc[k - w->base], pv = w->pv[k - w->base];
+        if (wf_get(w, k) >= 0 && pc >= mask) {
+            uint32_t idx = store_block(a, pc, pv);
+            w->pc[k - w->base] = 0;
+            w->pv[k - w->base] = idx;
+        }
+    }
+}
+/* occupancy helpers (src/compute/piggyback.rs:591-620): a missing input is
+ * the null wavefront, whose `null` flag is set */
+static inline int occ(const wf_t* w) { return w ? w->occ : 0; }
+static inline int occ_gap(const wf_t* open, const wf_t* ext) {
+    int o = open ? open->occ + 1 : 0;
+    if (ext && ext->occ + 1 > o) o = ext->occ + 1;
+    return o;
+}
+/* offload_affine_component + offload_blocks_affine (src/compute/
+ * piggyback.rs:576-590, 622-633) */
+static void offload_affine_component(wfo_aligner* a, wf_t* w, int lo, int hi) {
+    if (w->occ >= 16 - 1) {
+        if (a->low) {
+            offload_selective(a, w, lo, hi, PC_ALMOST_FULL_MASK);
+            w->occ = 16 - 2;
+        } else {
+            offload_selective(a, w, lo, hi, PC_HALF_FULL_MASK);
+            w->occ = 16 / 2;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* init_alignment (src/aligner.rs:846-1000), end-to-end                */
 /* ------------------------------------------------------------------ */
 static void init_alignment(wfo_aligner* a) {
     const int plen = a->plen, tlen = a->tlen;
     int score_only = a->attr.alignment_scope == WFA_SCOPE_SCORE;
-    a->modular = score_only;
+    /* med/low alignments: modular storage with piggyback backtrace
+     * (src/aligner.rs:855-866) */
+    a->piggyback = !score_only && a->attr.memory_mode != WFA_MEMORY_HIGH;
+    a->low = a->attr.memory_mode == WFA_MEMORY_LOW;
+    a->modular = score_only || a->piggyback;
     pool_clear(a);
+    a->nblk = 0;
+    a->ninit = 0;
     /* Components are created for 1000 x 1000 and only grow
      * (src/aligner.rs:351-370, src/components.rs:190-213); this models a
      * freshly created aligner for every call. */
@@ -400,6 +499,7 @@ static void init_alignment(wfo_aligner* a) {
     wf_t* m0 = alloc_wf(a, alo, ahi);
     wf_set(m0, 0, 0);
     wf_set_limits(m0, 0, 0);
+    if (a->piggyback) set_bt_init(a, m0, 0, 0, 0);
     put(a, C_M, 0, m0);
 }
 
@@ -518,6 +618,81 @@ static void compute_affine_step(wfo_aligner* a, int score) {
         i2 = alloc_wf(a, alo, ahi); d2 = alloc_wf(a, alo, ahi);
         wf_set_limits(i2, lo, hi); wf_set_limits(d2, lo, hi);
     }
+    if (a->piggyback) {
+        /* compute_affine_idm / compute_affine2p_idm with piggyback
+         * (src/compute/piggyback.rs:341-393, 438-528): a gap cell takes the
+         * extension if ext >= open; M takes I1, I2, D1, D2 then misms, a
+         * later source winning ties; M always pushes MISMS. */
+        for (int k = lo; k <= hi; ++k) {
+            int32_t oo, ee, ins1, del1, ins2 = 0, del2 = 0;
+            uint32_t ipc1, ipv1, dpc1, dpv1, ipc2 = 0, ipv2 = 0, dpc2 = 0, dpv2 = 0;
+            oo = rd(mo1, k - 1); ee = rd(i1e, k - 1);
+            if (ee >= oo) { ins1 = ee; ipc1 = rd_pc(i1e, k - 1); ipv1 = rd_pv(i1e, k - 1); }
+            else { ins1 = oo; ipc1 = rd_pc(mo1, k - 1); ipv1 = rd_pv(mo1, k - 1); }
+            ins1 += 1; ipc1 = (ipc1 << 2) | PC_INS;
+            oo = rd(mo1, k + 1); ee = rd(d1e, k + 1);
+            if (ee >= oo) { del1 = ee; dpc1 = rd_pc(d1e, k + 1); dpv1 = rd_pv(d1e, k + 1); }
+            else { del1 = oo; dpc1 = rd_pc(mo1, k + 1); dpv1 = rd_pv(mo1, k + 1); }
+            dpc1 = (dpc1 << 2) | PC_DEL;
+            wf_set(i1, k, ins1); i1->pc[k - i1->base] = ipc1; i1->pv[k - i1->base] = ipv1;
+            wf_set(d1, k, del1); d1->pc[k - d1->base] = dpc1; d1->pv[k - d1->base] = dpv1;
+            int32_t mis = rd(mx, k) + 1, mxv;
+            uint32_t pc, pv;
+            if (use2) {
+                oo = rd(mo2, k - 1); ee = rd(i2e, k - 1);
+                if (ee >= oo) { ins2 = ee; ipc2 = rd_pc(i2e, k - 1); ipv2 = rd_pv(i2e, k - 1); }
+                else { ins2 = oo; ipc2 = rd_pc(mo2, k - 1); ipv2 = rd_pv(mo2, k - 1); }
+                ins2 += 1; ipc2 = (ipc2 << 2) | PC_INS;
+                oo = rd(mo2, k + 1); ee = rd(d2e, k + 1);
+                if (ee >= oo) { del2 = ee; dpc2 = rd_pc(d2e, k + 1); dpv2 = rd_pv(d2e, k + 1); }
+                else { del2 = oo; dpc2 = rd_pc(mo2, k + 1); dpv2 = rd_pv(mo2, k + 1); }
+                dpc2 = (dpc2 << 2) | PC_DEL;
+                wf_set(i2, k, ins2); i2->pc[k - i2->base] = ipc2; i2->pv[k - i2->base] = ipv2;
+                wf_set(d2, k, del2); d2->pc[k - d2->base] = dpc2; d2->pv[k - d2->base] = dpv2;
+                mxv = ins1;
+                if (ins2 > mxv) mxv = ins2;
+                if (del1 > mxv) mxv = del1;
+                if (del2 > mxv) mxv = del2;
+                if (mis > mxv) mxv = mis;
+                pc = ipc1; pv = ipv1;
+                if (mxv == ins2) { pc = ipc2; pv = ipv2; }
+                if (mxv == del1) { pc = dpc1; pv = dpv1; }
+                if (mxv == del2) { pc = dpc2; pv = dpv2; }
+                if (mxv == mis) { pc = rd_pc(mx, k); pv = rd_pv(mx, k); }
+            } else {
+                mxv = mis > ins1 ? mis : ins1;
+                if (del1 > mxv) mxv = del1;
+                pc = ipc1; pv = ipv1;
+                if (mxv == del1) { pc = dpc1; pv = dpv1; }
+                if (mxv == mis) { pc = rd_pc(mx, k); pv = rd_pv(mx, k); }
+            }
+            m->pc[k - m->base] = (pc << 2) | PC_MISMS;
+            m->pv[k - m->base] = pv;
+            if ((uint32_t)mxv > (uint32_t)tlen || (uint32_t)(mxv - k) > (uint32_t)plen)
+                mxv = OFFSET_NULL;
+            wf_set(m, k, mxv);
+        }
+        /* offload_affine / offload_affine2p (src/compute/piggyback.rs:636-705) */
+        int oi1 = occ_gap(mo1, i1e), od1 = occ_gap(mo1, d1e);
+        int om = occ(mx);
+        if (oi1 > om) om = oi1;
+        if (od1 > om) om = od1;
+        int oi2 = 0, od2 = 0;
+        if (use2) {
+            oi2 = occ_gap(mo2, i2e); od2 = occ_gap(mo2, d2e);
+            if (oi2 > om) om = oi2;
+            if (od2 > om) om = od2;
+            i2->occ = oi2; d2->occ = od2;
+        }
+        i1->occ = oi1; d1->occ = od1; m->occ = om + 1;
+        offload_affine_component(a, m, lo, hi);
+        offload_affine_component(a, i1, lo, hi);
+        offload_affine_component(a, d1, lo, hi);
+        if (use2) {
+            offload_affine_component(a, i2, lo, hi);
+            offload_affine_component(a, d2, lo, hi);
+        }
+    } else
     for (int k = lo; k <= hi; ++k) {
         int32_t vi1 = rd(mo1, k - 1) > rd(i1e, k - 1) ? rd(mo1, k - 1) : rd(i1e, k - 1);
         vi1 += 1;
@@ -566,12 +741,47 @@ static void compute_edit_step(wfo_aligner* a, int score) {
     /* NULL the cells read beyond prev's ends, leaving its init range as is */
     int ks[4] = {prev->lo - 2, prev->lo - 1, prev->hi + 1, prev->hi + 2};
     for (int i = 0; i < 4; ++i)
-        if (wf_allocated(prev, ks[i])) wf_set(prev, ks[i], OFFSET_NULL);
+        if (wf_allocated(prev, ks[i])) { wf_set(prev, ks[i], OFFSET_NULL); wf_zero_bt(prev, ks[i]); }
     int alo, ahi;
     output_alloc_range(a, lo, hi, &alo, &ahi);
     wf_t* cur = alloc_wf(a, alo, ahi);
     wf_set_limits(cur, lo, hi);
     int indel = a->pen.metric == WFA_METRIC_INDEL;
+    if (a->piggyback) {
+        /* edit_kernel / indel_kernel (src/compute/piggyback.rs:195-285):
+         * later checks win: ins, then del, then misms */
+        for (int k = lo; k <= hi; ++k) {
+            int32_t ins = wf_get(prev, k - 1) + 1, del = wf_get(prev, k + 1), mx;
+            uint32_t pc = (prev->pc[k - 1 - prev->base] << 2) | PC_INS;
+            uint32_t pv = prev->pv[k - 1 - prev->base];
+            if (indel) {
+                mx = del > ins ? del : ins;
+                if (mx == del) {
+                    pc = (prev->pc[k + 1 - prev->base] << 2) | PC_DEL;
+                    pv = prev->pv[k + 1 - prev->base];
+                }
+            } else {
+                int32_t mis = wf_get(prev, k) + 1;
+                mx = ins > mis ? ins : mis;
+                if (del > mx) mx = del;
+                if (mx == del) {
+                    pc = (prev->pc[k + 1 - prev->base] << 2) | PC_DEL;
+                    pv = prev->pv[k + 1 - prev->base];
+                }
+                if (mx == mis) {
+                    pc = (prev->pc[k - prev->base] << 2) | PC_MISMS;
+                    pv = prev->pv[k - prev->base];
+                }
+            }
+            if ((uint32_t)mx > (uint32_t)tlen || (uint32_t)(mx - k) > (uint32_t)plen) mx = OFFSET_NULL;
+            wf_set(cur, k, mx);
+            cur->pc[k - cur->base] = pc;
+            cur->pv[k - cur->base] = pv;
+        }
+        /* offload every 16 scores (src/aligner.rs:1849-1852) */
+        if (score % 16 == 0)
+            offload_selective(a, cur, lo, hi, a->low ? PC_FULL_MASK : PC_HALF_FULL_MASK);
+    } else
     for (int k = lo; k <= hi; ++k) {
         int32_t ins = wf_get(prev, k - 1), del = wf_get(prev, k + 1), mx;
         if (indel) {
@@ -773,8 +983,70 @@ static void backtrace_linear(wfo_aligner* a, int score, int k, int offset) {
     a->cig_score = alignment_score;
 }
 
-/* backtrace_from (src/aligner.rs:1414-1437) for high memory. */
+/* backtrace_piggyback (src/aligner.rs:1443-1481) -> traceback_from
+ * (src/bt_buffer.rs:198-209) -> unpack_piggyback (src/aligner.rs:232-311). */
+static void backtrace_piggyback(wfo_aligner* a, int score, int k, int offset) {
+    const wf_t* m = get(a, C_M, score);
+    if (!m || !wf_allocated(m, k)) { a->cig_begin = a->cig_end = 0; return; }
+    uint32_t pcigar = m->pc[k - m->base], prev = m->pv[k - m->base];
+    /* chain: collect pcigars end-first until the initial block */
+    int np = 0;
+    while (prev != BT_NULL) {
+        if (np == a->cap_packed) {
+            a->cap_packed = a->cap_packed ? 2 * a->cap_packed : 256;
+            a->packed = (uint32_t*)realloc(a->packed, sizeof(uint32_t) * (size_t)a->cap_packed);
+        }
+        a->packed[np++] = pcigar;
+        pcigar = a->blk_pc[prev];
+        prev = a->blk_prev[prev];
+    }
+    int bv = a->init_v[pcigar], bh = a->init_h[pcigar];
+    int ev = offset - k, eh = offset;
+    const int affine = a->pen.metric == WFA_METRIC_GAP_AFFINE ||
+                       a->pen.metric == WFA_METRIC_GAP_AFFINE_2P;
+    uint8_t* ops = a->cig;
+    int n = 0, v = bv, h = bh;
+    for (int i = 0; i < h; ++i) ops[n++] = 'I';
+    for (int i = 0; i < v; ++i) ops[n++] = 'D';
+    int in_m = 1;
+    for (int b = np - 1; b >= 0; --b) {
+        uint32_t pc = a->packed[b];
+        int length = 16;
+        if (pc < PC_FULL_MASK) {
+            int free_slots = pc ? __builtin_clz(pc) / 2 : 16;
+            length -= free_slots;
+            pc = free_slots >= 16 ? 0 : pc << (2 * free_slots);
+        }
+        for (int j = 0; j < length; ++j) {
+            if (in_m || !affine) {
+                while (v < a->plen && h < a->tlen && a->pattern[v] == a->text[h]) {
+                    ops[n++] = 'M'; v++; h++;
+                }
+            }
+            uint32_t op = pc >> 30;
+            pc <<= 2;
+            uint8_t ch; int dv, dh, mm;
+            if (op == PC_DEL) { ch = 'D'; dv = 1; dh = 0; mm = 0; }
+            else if (op == PC_MISMS) { ch = 'X'; dv = 1; dh = 1; mm = 1; }
+            else if (op == PC_INS) { ch = 'I'; dv = 0; dh = 1; mm = 0; }
+            else { ch = '?'; dv = 0; dh = 0; mm = 1; }
+            if (affine && !in_m && ch == 'X') { in_m = 1; continue; }
+            ops[n++] = ch; v += dv; h += dh; in_m = mm;
+        }
+    }
+    int nm = (ev - v) < (eh - h) ? (ev - v) : (eh - h);
+    for (int i = 0; i < nm; ++i) ops[n++] = 'M';
+    v += nm; h += nm;
+    while (h < a->tlen) { ops[n++] = 'I'; h++; }
+    while (v < a->plen) { ops[n++] = 'D'; v++; }
+    a->cig_begin = 0;
+    a->cig_end = n;
+    a->cig_score = score;
+}
+
+/* backtrace_from (src/aligner.rs:1414-1437). */
 static void backtrace_from(wfo_aligner* a, int score, int k, int offset) {
+    if (a->piggyback) { backtrace_piggyback(a, score, k, offset); return; }
     if (a->pen.metric == WFA_METRIC_GAP_AFFINE || a->pen.metric == WFA_METRIC_GAP_AFFINE_2P)
         backtrace_singletrack(a, score, k, offset);
     else

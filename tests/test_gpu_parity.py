@@ -139,6 +139,32 @@ def test_aligner_api_mirrors_reference(pkg, N, engine):
     E.set_max_alignment_steps(2)
     E.align_end2end(b"AAAA", b"TTTT")
     assert E.status() == pkg.AlignStatus.MAX_STEPS_REACHED
+    A.set_memory_mode(pkg.MemoryMode.MED)
+    assert A.align_end2end(b"ACGTACGT", b"ACGAACGT") == 4
+    assert A.cigar().to_string_rle(True) == "3M1X4M"
     with pytest.raises(RuntimeError):
-        A.set_memory_mode(pkg.MemoryMode.MED)
+        A.set_memory_mode(pkg.MemoryMode.ULTRALOW)
         A.align_end2end(b"ACGT", b"ACGT")
+
+
+@pytest.mark.parametrize("metric,pen", CONFIGS)
+@pytest.mark.parametrize("memory", ["med", "low"])
+def test_med_low_memory_match_oracle(metric, pen, memory, pkg, N, engine, oracle):
+    """Piggyback CIGARs (medium/low memory) on 150 bp pairs."""
+    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    pairs = generate(WORKLOADS["short150"], num_pairs=1500)
+    attr = attr_for(N, metric, pen, "alignment")
+    attr.memory_mode = N.MEMORY_MED if memory == "med" else N.MEMORY_LOW
+    _compare(engine, oracle, attr, pairs, True, label=f"{metric}/{memory}")
+
+
+def test_long_reads_affine2p_med_match_oracle(pkg, N, engine, oracle):
+    """1-10 kbp pairs, gap-affine 2-piece (0,4,6,2,24,1), medium memory: the
+    long-read workload's configuration on a sample."""
+    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    pairs = generate(WORKLOADS["long1k_10k"], num_pairs=48)
+    attr = attr_for(N, "gap_affine_2p", (0, 4, 6, 2, 24, 1), "alignment")
+    attr.memory_mode = N.MEMORY_MED
+    _compare(engine, oracle, attr, pairs, True, label="long/2p/med")
+    attr_s = attr_for(N, "gap_affine_2p", (0, 4, 6, 2, 24, 1), "score")
+    _compare(engine, oracle, attr_s, pairs, False, label="long/2p/score")

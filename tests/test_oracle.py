@@ -177,3 +177,41 @@ def test_oracle_batch_threads_agree(N, oracle, pkg):
     s1, st1, c1 = oracle.align_batch(attr, pairs, cigar=True, threads=1)
     s4, st4, c4 = oracle.align_batch(attr, pairs, cigar=True, threads=4)
     assert np.array_equal(s1, s4) and np.array_equal(st1, st4) and c1 == c4
+
+
+@pytest.mark.parametrize("metric,pen", PENALTY_SETS)
+def test_oracle_med_low_memory(metric, pen, pkg, N, oracle):
+    """Piggyback backtrace (med/low): same score as high memory, a valid CIGAR
+    that rescores to it, and med == low (offloading never changes the op
+    chain, only where it is stored)."""
+    a_h = oracle.OracleAligner(attr_for(N, metric, pen, "alignment"))
+    am = attr_for(N, metric, pen, "alignment")
+    am.memory_mode = N.MEMORY_MED
+    al = attr_for(N, metric, pen, "alignment")
+    al.memory_mode = N.MEMORY_LOW
+    a_m, a_l = oracle.OracleAligner(am), oracle.OracleAligner(al)
+    _, x, o1, e1, o2, e2 = pen
+    for p, t in _lcg_pairs(80, 250, seed=5) + [(b"", b""), (b"ACGT", b""), (b"", b"GG")]:
+        sh = a_h.align_end2end(p, t)[0]
+        sm, stm, cm = a_m.align_end2end(p, t)
+        sl, stl, cl = a_l.align_end2end(p, t)
+        assert sm == sh == sl and stm == stl == 5
+        assert cm == cl
+        cig = pkg.Cigar(cm)
+        if p or t:
+            assert cig.check_alignment(p, t) is None
+        if metric == "gap_affine":
+            assert -cig.score_gap_affine(pkg.AffinePenalties(0, x, o1, e1)) == sm
+        elif metric == "gap_affine_2p":
+            assert -cig.score_gap_affine2p(pkg.Affine2pPenalties(0, x, o1, e1, o2, e2)) == sm
+        elif metric == "edit":
+            assert cig.score_edit() == sm
+
+
+def test_oracle_indel_literal_med_low(pkg, N, oracle):
+    """src/aligner.rs:4481-4493: 3000 and 1500I1500D in every memory mode."""
+    for mem in (N.MEMORY_HIGH, N.MEMORY_MED, N.MEMORY_LOW):
+        a = attr_for(N, "indel", (0,) * 6, "alignment")
+        a.memory_mode = mem
+        s, st, ops = oracle.OracleAligner(a).align_end2end(b"A" * 1500, b"C" * 1500)
+        assert (s, st, pkg.Cigar(ops).to_string_rle(True)) == (3000, 5, "1500I1500D")

@@ -89,6 +89,7 @@ SIGNATURES = {
     "wfa_engine_align_batch": (_I32, [_VP, _P(Attr), _P(Batch), _VP]),
     "wfa_engine_align_batch_per_pair": (_I32, [_VP, _P(Attr), _P(Batch), _VP]),
     "wfa_engine_last_retries": (_I32, [_VP]),
+    "wfa_engine_last_kernel_ms": (_I32, [_VP, _P(ctypes.c_float), _P(ctypes.c_float)]),
     "wfa_version": (ctypes.c_char_p, []),
 }
 

@@ -89,8 +89,6 @@ int check_supported(const wfa_attr_t& a) {
   if (m == WFA_METRIC_GAP_LINEAR) return WFA_ERR_UNSUPPORTED;
   if (a.span != WFA_SPAN_END2END || a.extension) return WFA_ERR_UNSUPPORTED;
   if (a.heuristic != WFA_HEURISTIC_NONE) return WFA_ERR_UNSUPPORTED;
-  if (a.alignment_scope == WFA_SCOPE_ALIGNMENT && a.memory_mode != WFA_MEMORY_HIGH)
-    return WFA_ERR_UNSUPPORTED;
   if (a.alignment_scope != WFA_SCOPE_SCORE && a.alignment_scope != WFA_SCOPE_ALIGNMENT)
     return WFA_ERR_INVALID_ARGUMENT;
   if (a.memory_mode < WFA_MEMORY_HIGH || a.memory_mode > WFA_MEMORY_ULTRALOW)
@@ -131,7 +129,11 @@ struct wfa_engine_t {
   size_t hdr_bytes = 0;
   int32_t* d_gring = nullptr;
   size_t gring_bytes = 0;
+  uint8_t* d_chain = nullptr;
+  size_t chain_bytes = 0;
   int last_retries = 0;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  float first_ms = 0.f, all_ms = 0.f;
   std::mutex mu;
 };
 
@@ -149,6 +151,7 @@ int grow(void** ptr, size_t* have, size_t need) {
 
 struct LaunchCfg {
   int kind;
+  int mode;  // 0 score only, 1 high memory CIGAR, 2 medium/low memory CIGAR
   bool cig;
   bool lds_ring;
   int wcap;
@@ -158,6 +161,7 @@ struct LaunchCfg {
   int grid;
   int hdr_cap;
   int64_t hist_cap;
+  int chain_cap;
 };
 
 int kind_of(int metric) {
@@ -182,7 +186,9 @@ void base_params(const Internal& ip, const wfa_attr_t& attr, const wfa_batch_t& 
   p->ringM = kind == wfa::KIND_EDIT ? 2 : p->scope;
   p->ringE1 = kind == wfa::KIND_EDIT ? 0 : ip.e1 + 1;
   p->ringE2 = kind == wfa::KIND_AFFINE2P ? ip.e2 + 1 : 0;
-  p->modular = attr.alignment_scope == WFA_SCOPE_SCORE;
+  // score-only and medium/low memory keep only the wavefronts in scope
+  // (src/aligner.rs:855-866): no score-indexed slot limit
+  p->modular = attr.alignment_scope == WFA_SCOPE_SCORE || attr.memory_mode != WFA_MEMORY_HIGH;
   p->max_steps = attr.max_alignment_steps;
   p->fmt = b.seq_format;
   p->seq = b.seq_data;
@@ -205,9 +211,11 @@ int ring_rows_of(const KParams& p, int kind) {
 
 // Choose storage for a launch over pairs of length <= max_len. `wide`
 // selects the full-width configuration that cannot overflow the ring.
-int plan(wfa_engine_t* e, const KParams& p, int kind, bool cig, int fmt, int max_len, int npairs,
+int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max_len, int npairs,
          int pass, LaunchCfg* c) {
+  const bool cig = mode != 0;
   c->kind = kind;
+  c->mode = mode;
   c->cig = cig;
   c->ring_rows = ring_rows_of(p, kind);
   const int per = fmt == WFA_SEQ_PACKED2BITS ? 32 : 8;
@@ -220,7 +228,7 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, bool cig, int fmt, int max
   c->lds_ring = seq_meta + ring_bytes <= 64 * 1024;
   c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : 0);
   if (c->lds_bytes > e->lds_max) return WFA_ERR_UNSUPPORTED;
-  const int per_cu = wfa::max_blocks_per_cu(kind, cig, c->lds_ring, c->lds_bytes);
+  const int per_cu = wfa::max_blocks_per_cu(kind, mode, c->lds_ring, c->lds_bytes);
   int grid = e->num_cus * per_cu;
   if (pass > 0) grid = std::min(grid, e->num_cus * 2);
   grid = std::max(1, std::min(grid, npairs));
@@ -235,14 +243,20 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, bool cig, int fmt, int max
       c->hdr_cap = (int)std::min<int64_t>(nwf_bound, (int64_t)2048 << (2 * pass));
       c->hist_cap = std::min<int64_t>((int64_t)c->hdr_cap * c->wcap, (int64_t)(128 * 1024) << (3 * pass));
     }
+    // medium/low memory keeps one flag byte per cell instead of an offset
+    if (mode == 2) c->hist_cap *= 4;
     // keep the scratch of one launch under ~16 GiB
-    const int64_t per_slot = c->hist_cap * 4 + (int64_t)c->hdr_cap * 16;
+    const int64_t per_slot = c->hist_cap * (mode == 2 ? 1 : 4) + (int64_t)c->hdr_cap * 16 +
+                             (mode == 2 ? 4 * (int64_t)max_len + 64 : 0);
     const int64_t budget = (int64_t)16 << 30;
     if ((int64_t)c->grid * per_slot > budget) c->grid = (int)std::max<int64_t>(1, budget / per_slot);
   } else {
     c->hdr_cap = 0;
     c->hist_cap = 0;
   }
+  // op chain of one pair: at most every base consumed by a gap op plus one
+  // gap-closing mismatch per gap
+  c->chain_cap = 4 * max_len + 64;
   return WFA_OK;
 }
 
@@ -262,7 +276,8 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
     p.gring = e->d_gring;
   }
   if (c.cig) {
-    int r = grow((void**)&e->d_hist, &e->hist_bytes, (size_t)c.grid * c.hist_cap * 4);
+    const size_t cell = c.mode == 2 ? 1 : 4;
+    int r = grow((void**)&e->d_hist, &e->hist_bytes, (size_t)c.grid * c.hist_cap * cell);
     if (r) return r;
     r = grow((void**)&e->d_hdr, &e->hdr_bytes, (size_t)c.grid * c.hdr_cap * sizeof(int4));
     if (r) return r;
@@ -270,9 +285,15 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
     p.hist_cap = c.hist_cap;
     p.hdr = e->d_hdr;
     p.hdr_cap = c.hdr_cap;
+    if (c.mode == 2) {
+      p.chain_cap = c.chain_cap;
+      r = grow((void**)&e->d_chain, &e->chain_bytes, (size_t)c.grid * c.chain_cap);
+      if (r) return r;
+      p.chain = e->d_chain;
+    }
   }
   HIPCHK(wfa::launch_reset(e->d_counters, st));
-  HIPCHK(wfa::launch_align(p, c.kind, c.cig, c.lds_ring, c.grid, c.lds_bytes, st));
+  HIPCHK(wfa::launch_align(p, c.kind, c.mode, c.lds_ring, c.grid, c.lds_bytes, st));
   return WFA_OK;
 }
 
@@ -325,6 +346,7 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
   base_params(ip, *attr, *b, &p);
   const int kind = kind_of(ip.metric);
   const bool cig = attr->alignment_scope == WFA_SCOPE_ALIGNMENT;
+  const int mode = !cig ? 0 : attr->memory_mode == WFA_MEMORY_HIGH ? 1 : 2;
   int max_len = b->max_length;
   if (max_len <= 0) {
     HIPCHK(wfa::launch_max_length(b->pattern_length + first, b->text_length + first, count,
@@ -335,11 +357,13 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
   r = ensure_lists(e, count);
   if (r) return r;
   LaunchCfg c;
-  r = plan(e, p, kind, cig, b->seq_format, max_len, count, 0, &c);
+  r = plan(e, p, kind, mode, b->seq_format, max_len, count, 0, &c);
   if (r) return r;
   p.pair_base = first;
+  HIPCHK(hipEventRecord(e->ev[0], st));
   r = run_launch(e, p, c, count, nullptr, e->d_lists[0], st);
   if (r) return r;
+  HIPCHK(hipEventRecord(e->ev[1], st));
   int32_t nretry = 0;
   r = read_counter(e, 1, st, &nretry);
   if (r) return r;
@@ -347,7 +371,7 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
   int cur = 0;
   for (int pass = 1; nretry > 0; ++pass) {
     if (pass > 6) return WFA_ERR_OUT_OF_MEMORY;
-    r = plan(e, p, kind, cig, b->seq_format, max_len, nretry, pass, &c);
+    r = plan(e, p, kind, mode, b->seq_format, max_len, nretry, pass, &c);
     if (r) return r;
     r = run_launch(e, p, c, nretry, e->d_lists[cur], e->d_lists[cur ^ 1], st);
     if (r) return r;
@@ -355,6 +379,10 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
     r = read_counter(e, 1, st, &nretry);
     if (r) return r;
   }
+  HIPCHK(hipEventRecord(e->ev[2], st));
+  HIPCHK(hipEventSynchronize(e->ev[2]));
+  HIPCHK(hipEventElapsedTime(&e->first_ms, e->ev[0], e->ev[1]));
+  HIPCHK(hipEventElapsedTime(&e->all_ms, e->ev[0], e->ev[2]));
   return WFA_OK;
 }
 
@@ -382,6 +410,13 @@ wfa_engine_t* wfa_engine_new(int32_t device, int32_t* err) {
   }
   e->num_cus = prop.multiProcessorCount;
   e->lds_max = prop.sharedMemPerBlock;
+  for (auto& ev : e->ev) {
+    if (hipEventCreate(&ev) != hipSuccess) {
+      delete e;
+      if (err) *err = WFA_ERR_DEVICE;
+      return nullptr;
+    }
+  }
   if (hipMalloc((void**)&e->d_counters, 64) != hipSuccess ||
       hipHostMalloc((void**)&e->h_pinned, 64, hipHostMallocDefault) != hipSuccess) {
     delete e;
@@ -401,7 +436,9 @@ void wfa_engine_free(wfa_engine_t* e) {
   hipFree(e->d_hist);
   hipFree(e->d_hdr);
   hipFree(e->d_gring);
+  hipFree(e->d_chain);
   hipHostFree(e->h_pinned);
+  for (auto ev : e->ev) hipEventDestroy(ev);
   delete e;
 }
 
@@ -429,6 +466,13 @@ int32_t wfa_engine_align_batch_per_pair(wfa_engine_t* e, const wfa_attr_t* attr,
 }
 
 int32_t wfa_engine_last_retries(const wfa_engine_t* e) { return e ? e->last_retries : 0; }
+
+int32_t wfa_engine_last_kernel_ms(const wfa_engine_t* e, float* first_ms, float* all_ms) {
+  if (!e) return WFA_ERR_INVALID_ARGUMENT;
+  if (first_ms) *first_ms = e->first_ms;
+  if (all_ms) *all_ms = e->all_ms;
+  return WFA_OK;
+}
 
 const char* wfa_version(void) { return "wfa-mi355x 0.1.0 (gfx950)"; }
 

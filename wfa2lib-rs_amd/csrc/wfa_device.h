@@ -47,13 +47,16 @@ struct KParams {
   int64_t hist_cap;
   int4* hdr;                 // CIGAR history headers, hdr_cap per slot
   int hdr_cap;
+  uint8_t* chain;            // medium/low memory op-chain scratch, chain_cap per slot
+  int chain_cap;
 };
 
 hipError_t launch_reset(int32_t* counters, hipStream_t stream);
 hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, int32_t* out,
                              hipStream_t stream);
-hipError_t launch_align(const KParams& p, int kind, bool cig, bool lds_ring, int grid,
+// mode: 0 score only, 1 high memory CIGAR, 2 medium/low memory CIGAR
+hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, int grid,
                         size_t lds_bytes, hipStream_t stream);
-int max_blocks_per_cu(int kind, bool cig, bool lds_ring, size_t lds_bytes);
+int max_blocks_per_cu(int kind, int mode, bool lds_ring, size_t lds_bytes);
 
 }  // namespace wfa

@@ -191,10 +191,21 @@ static __device__ int stage_sequences(const KParams& p, int pair, int plen, int 
 // ---------------------------------------------------------------------
 struct Hist {
   int4* hdr;
-  int32_t* cells;
+  int32_t* cells;   // high memory: extended M offsets
+  uint8_t* bytes;   // medium/low memory: winner flags per (score, k)
+  uint8_t* chain;   // medium/low memory: op chain scratch
+  int chain_cap;
   int hdr_cap;
   int64_t cap;
 };
+
+// Winner flags of one (score, k) in medium/low memory. Bits 0-2: the
+// source of M (0 I1, 1 I2, 2 D1, 3 D2, 4 mismatch; for edit 0 ins, 1 del,
+// 2 mismatch); bits 3-6: I1, D1, I2, D2 took their extension input.
+#define FL_I1_EXT 8
+#define FL_D1_EXT 16
+#define FL_I2_EXT 32
+#define FL_D2_EXT 64
 
 // st_m_offset (src/backtrace.rs:645-663) over the stored rows.
 static __device__ __forceinline__ bool st_m(const Hist& H, int nscores, int nwf, int s, int k,
@@ -210,7 +221,8 @@ static __device__ __forceinline__ bool st_m(const Hist& H, int nscores, int nwf,
 
 // write `n` copies of op backwards ending at out[pos] (pos decreases)
 static __device__ __forceinline__ void write_ops(uint8_t* out, int& pos, uint8_t op, int n) {
-  for (int i = threadIdx.x; i < n; i += 64) out[pos - i] = op;
+  for (int i = threadIdx.x; i < n; i += 64)
+    if (pos - i >= 0) out[pos - i] = op;  // never below the pair's region
   pos -= n > 0 ? n : 0;
 }
 
@@ -362,6 +374,96 @@ static __device__ int backtrace_linear(const KParams& p, const Hist& H, int nsco
   return cap - 1 - pos;
 }
 
+// Medium/low memory: the chain of operations the reference's piggyback
+// kernels push (src/compute/piggyback.rs:195-528), recovered by following
+// the stored winner flags from the end cell back to M at score 0, then
+// unpacked exactly as unpack_piggyback (src/aligner.rs:232-311).
+template <int KIND, int BITS>
+static __device__ int piggyback_backtrace(const KParams& p, const Hist& H, int nscores,
+                                          const uint64_t* sP, const uint64_t* sT, int plen,
+                                          int tlen, int score, int k, int offset, uint8_t* out,
+                                          int cap) {
+  const int lane = threadIdx.x;
+  enum { CM = 0, CI1, CD1, CI2, CD2 };
+  int comp = CM, s = score, kk = k;
+  int pos = H.chain_cap - 1;
+  bool bad = false;
+  for (;;) {
+    if (comp == CM && s == 0) break;
+    if (s < 0 || s >= nscores || pos < 0) { bad = true; break; }
+    const int4 hd = H.hdr[s];
+    if (kk < hd.x || kk > hd.y) { bad = true; break; }
+    const int f = H.bytes[hd.z + (kk - hd.x)];
+    uint8_t op;
+    if (KIND == KIND_EDIT) {
+      const int w = f & 7;
+      op = w == 0 ? 'I' : w == 1 ? 'D' : 'X';
+      kk += w == 0 ? -1 : w == 1 ? 1 : 0;
+      s -= 1;
+    } else if (comp == CM) {
+      op = 'X';
+      const int w = f & 7;
+      if (w == 4) s -= p.x;
+      else comp = w == 0 ? CI1 : w == 1 ? CI2 : w == 2 ? CD1 : CD2;
+    } else if (comp == CI1 || comp == CI2) {
+      op = 'I';
+      const bool ext = f & (comp == CI1 ? FL_I1_EXT : FL_I2_EXT);
+      if (ext) s -= comp == CI1 ? p.e1 : p.e2;
+      else { s -= comp == CI1 ? p.oe1 : p.oe2; comp = CM; }
+      kk -= 1;
+    } else {
+      op = 'D';
+      const bool ext = f & (comp == CD1 ? FL_D1_EXT : FL_D2_EXT);
+      if (ext) s -= comp == CD1 ? p.e1 : p.e2;
+      else { s -= comp == CD1 ? p.oe1 : p.oe2; comp = CM; }
+      kk += 1;
+    }
+    if (lane == 0) H.chain[pos] = op;
+    --pos;
+  }
+  if (bad) return -1;
+  __threadfence_block();
+  const int nchain = H.chain_cap - 1 - pos;
+  const uint8_t* ch = H.chain + pos + 1;
+  const bool affine = KIND != KIND_EDIT;
+  // unpack: begin at (0, 0); re-extend matches before each op (affine: only
+  // while in M); drop the X that closes a gap; pad to the end cell and the
+  // sequence ends
+  int n = 0, v = 0, h = 0;
+  bool in_m = true;
+  auto put_run = [&](uint8_t c, int len) {
+    for (int i = lane; i < len; i += 64)
+      if (n + i < cap) out[n + i] = c;
+    n += len > 0 ? len : 0;
+  };
+  for (int b = 0; b < nchain; b += 64) {
+    const int c = (b + lane < nchain) ? ch[b + lane] : 0;
+    const int cnt = imin(64, nchain - b);
+    for (int j = 0; j < cnt; ++j) {
+      const int op = uniform(__shfl(c, j));
+      if (in_m || !affine) {
+        const int m = lcp<BITS>(sP, sT, v, h, plen, tlen);
+        put_run('M', m);
+        v += m;
+        h += m;
+      }
+      if (affine && !in_m && op == 'X') { in_m = true; continue; }
+      put_run((uint8_t)op, 1);
+      if (op == 'D') { v += 1; in_m = false; }
+      else if (op == 'I') { h += 1; in_m = false; }
+      else { v += 1; h += 1; in_m = true; }
+    }
+  }
+  const int ev = offset - k, eh = offset;
+  const int nm = imin(ev - v, eh - h);
+  put_run('M', nm);
+  v += nm;
+  h += nm;
+  put_run('I', tlen - h);
+  put_run('D', plen - v);
+  return n;
+}
+
 // num_wavefronts of a freshly created aligner in high memory
 // (compute_dimensions, src/components.rs:435-504, sized for
 // max(1000x1000, plen x tlen)).
@@ -385,9 +487,13 @@ static __device__ int high_mem_nwf(const KParams& p, int plen, int tlen) {
 // ---------------------------------------------------------------------
 // The per-pair forward pass + backtrace.
 // ---------------------------------------------------------------------
-template <int KIND, bool CIG, int BITS>
+template <int KIND, int MODE, int BITS>
 static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen, uint64_t* sP,
                                   uint64_t* sT, const Rows& R, const Hist& H) {
+  // MODE 0: score only; 1: high memory (M rows kept); 2: medium/low memory
+  // (winner flags kept)
+  constexpr bool CIG = MODE != 0;
+  constexpr bool MED = MODE == 2;
   const int lane = threadIdx.x;
   const bool two = KIND == KIND_AFFINE2P;
   // ring rows: M: [0, ringM), I1: [ringM, ringM+ringE1), D1 next, then I2, D2
@@ -418,11 +524,8 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   auto extend_row = [&](int s, int lo, int hi) -> bool {
     const int row = s % ringM;
     bool reached = false;
-    int4* hd = CIG ? &H.hdr[s] : nullptr;
+    int4* hd = MODE == 1 ? &H.hdr[s] : nullptr;
     int64_t base = hpos;
-    if (CIG) {
-      if (lo <= hi && hpos + (hi - lo + 1) > H.cap) return false;
-    }
     for (int kb = lo; kb <= hi; kb += 64) {
       const int k = kb + lane;
       if (k <= hi) {
@@ -433,10 +536,10 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           *cell = off;
         }
         if (k == k_end && off >= tlen) reached = true;
-        if (CIG) H.cells[base + (k - lo)] = off;
+        if (MODE == 1) H.cells[base + (k - lo)] = off;
       }
     }
-    if (CIG) {
+    if (MODE == 1) {
       if (lane == 0) *hd = make_int4(lo, hi, (int)base, 0);
       if (lo <= hi) hpos += hi - lo + 1;
     }
@@ -447,7 +550,12 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   // score 0
   {
     if (CIG && (H.hdr_cap < 1 || H.cap < 1)) overflow = true;
-    else if (extend_row(0, 0, 0)) {
+    else if (MED && lane == 0) {
+      H.hdr[0] = make_int4(0, 0, 0, 0);
+      H.bytes[0] = 0;
+    }
+    if (MED) hpos = 1;
+    if (!overflow && extend_row(0, 0, 0)) {
       status = WFA_STATUS_END_REACHED;
       result = 0;
     }
@@ -555,6 +663,8 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       continue;
     }
     if (hi - lo + 1 > R.wcap) { overflow = true; break; }
+    if (MED && hpos + (hi - lo + 1) > H.cap) { overflow = true; break; }
+    const int64_t fbase = hpos;
 
     // ---- compute all chunks; trims by ballot ----
     int m_lo = INT32_MAX, m_hi = INT32_MIN, i_lo = INT32_MAX, i_hi = INT32_MIN;
@@ -565,27 +675,40 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       bool vm = false, vi = false, vd = false, vi2 = false, vd2 = false;
       if (k <= hi) {
         int m;
+        int fl = 0;
         if (KIND == KIND_EDIT) {
           const int ins = rd(R, row_x, ml_x, mh_x, k - 1);
           const int del = rd(R, row_x, ml_x, mh_x, k + 1);
           if (p.metric == WFA_METRIC_INDEL) {
             m = imax(del, ins + 1);
+            if (MED) fl = m == del ? 1 : 0;
           } else {
             const int mis = rd(R, row_x, ml_x, mh_x, k);
             m = imax(del, imax(ins, mis) + 1);
+            // piggyback priority (src/compute/piggyback.rs:218-224):
+            // mismatch, then deletion, then insertion
+            if (MED) fl = m == mis + 1 ? 2 : m == del ? 1 : 0;
           }
           if (!in_bounds(m, k, plen, tlen)) m = WFA_NULL;
         } else {
-          const int i1 = imax(rd(R, row_o, ml_o, mh_o, k - 1),
-                              rd(R, rI1 + row_e, il, ih, k - 1)) + 1;
-          const int d1 = imax(rd(R, row_o, ml_o, mh_o, k + 1), rd(R, rD1 + row_e, dl, dh, k + 1));
+          const int io = rd(R, row_o, ml_o, mh_o, k - 1), ie = rd(R, rI1 + row_e, il, ih, k - 1);
+          const int dop = rd(R, row_o, ml_o, mh_o, k + 1), de = rd(R, rD1 + row_e, dl, dh, k + 1);
+          const int i1 = imax(io, ie) + 1;
+          const int d1 = imax(dop, de);
           const int mis = rd(R, row_x, ml_x, mh_x, k) + 1;
+          // a gap cell takes its extension input on ties
+          // (src/compute/piggyback.rs:146-160)
+          if (MED) fl |= (ie >= io ? FL_I1_EXT : 0) | (de >= dop ? FL_D1_EXT : 0);
           int ins = i1, del = d1;
+          int i2 = WFA_NULL, d2 = WFA_NULL;
           if (two) {
-            const int i2 = imax(rd(R, row_o2, ml_o2, mh_o2, k - 1),
-                                rd(R, rI2 + row_e2, il2, ih2, k - 1)) + 1;
-            const int d2 = imax(rd(R, row_o2, ml_o2, mh_o2, k + 1),
-                                rd(R, rD2 + row_e2, dl2, dh2, k + 1));
+            const int io2 = rd(R, row_o2, ml_o2, mh_o2, k - 1);
+            const int ie2 = rd(R, rI2 + row_e2, il2, ih2, k - 1);
+            const int do2 = rd(R, row_o2, ml_o2, mh_o2, k + 1);
+            const int de2 = rd(R, rD2 + row_e2, dl2, dh2, k + 1);
+            i2 = imax(io2, ie2) + 1;
+            d2 = imax(do2, de2);
+            if (MED) fl |= (ie2 >= io2 ? FL_I2_EXT : 0) | (de2 >= do2 ? FL_D2_EXT : 0);
             R.ring[(rI2 + rowE2) * R.wcap + (k & R.mask)] = i2;
             R.ring[(rD2 + rowE2) * R.wcap + (k & R.mask)] = d2;
             vi2 = in_bounds(i2, k, plen, tlen);
@@ -594,6 +717,16 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
             del = imax(del, d2);
           }
           m = imax(mis, imax(ins, del));
+          if (MED) {
+            // M's source, a later one winning ties (src/compute/piggyback.rs
+            // 383-391, 515-525): I1, I2, D1, D2, then the mismatch
+            int w = 0;
+            if (two && m == i2) w = 1;
+            if (m == d1) w = 2;
+            if (two && m == d2) w = 3;
+            if (m == mis) w = 4;
+            fl |= w;
+          }
           if (!in_bounds(m, k, plen, tlen)) m = WFA_NULL;
           R.ring[(rI1 + rowE) * R.wcap + (k & R.mask)] = i1;
           R.ring[(rD1 + rowE) * R.wcap + (k & R.mask)] = d1;
@@ -602,6 +735,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         }
         vm = m != WFA_NULL;
         R.ring[rowM * R.wcap + (k & R.mask)] = m;
+        if (MED) H.bytes[fbase + (k - lo)] = (uint8_t)fl;
       }
       uint64_t b = ballot(vm);
       if (b) {
@@ -637,6 +771,10 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     auto fix = [](int& l, int& h) { if (l == INT32_MAX) { l = 1; h = 0; } };
     fix(m_lo, m_hi); fix(i_lo, i_hi); fix(d_lo, d_hi); fix(i2_lo, i2_hi); fix(d2_lo, d2_hi);
     __syncthreads();  // ring rows written before the extension reads them
+    if (MED) {
+      if (lane == 0) H.hdr[s] = make_int4(lo, hi, (int)fbase, 0);
+      hpos += hi - lo + 1;
+    }
     if (lane == 0) {
       R.meta[rowM] = make_int2(m_lo, m_hi);
       if (KIND != KIND_EDIT) {
@@ -653,8 +791,8 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     // iteration; nothing reads M[s] in between) ----
     // In high memory the stored row keeps its computed range: M cells
     // outside the trimmed range are OFFSET_NULL there anyway.
-    const int elo = CIG ? lo : m_lo, ehi = CIG ? hi : m_hi;
-    if (CIG && hpos + (hi - lo + 1) > H.cap) { overflow = true; break; }
+    const int elo = MODE == 1 ? lo : m_lo, ehi = MODE == 1 ? hi : m_hi;
+    if (MODE == 1 && hpos + (hi - lo + 1) > H.cap) { overflow = true; break; }
     const bool reached = extend_row(s, elo, ehi) && (m_lo <= k_end && k_end <= m_hi);
     __syncthreads();
     if (reached) { status = WFA_STATUS_END_REACHED; result = s; break; }
@@ -676,13 +814,22 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       __threadfence_block();
       uint8_t* out = p.cig + p.cig_off[pair];
       const int cap = plen + tlen;
-      if (KIND == KIND_EDIT)
+      if (MED)
+        ncig = piggyback_backtrace<KIND, BITS>(p, H, score + 1, sP, sT, plen, tlen, score, k_end,
+                                               tlen, out, cap);
+      else if (KIND == KIND_EDIT)
         ncig = backtrace_linear<BITS>(p, H, score + 1, plen, tlen, score, k_end, tlen, out, cap);
       else
         ncig = singletrack<BITS>(p, H, score + 1, nwf, sP, sT, plen, tlen, score, k_end, tlen,
                                  out, cap);
+      if (ncig < 0) {
+        // the stored flags do not lead back to the start: never expected;
+        // report the pair as unattainable rather than a wrong CIGAR
+        out_status = WFA_STATUS_END_UNREACHABLE;
+        ncig = 0;
+      }
       // move the ops from the tail of the region to its head
-      const int gap = cap - ncig;
+      const int gap = MED ? 0 : cap - ncig;
       if (gap > 0) {
         for (int c = 0; c < ncig; c += 64) {
           const int i = c + lane;
@@ -702,8 +849,9 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   }
 }
 
-template <int KIND, bool CIG, bool LDS_RING>
+template <int KIND, int MODE, bool LDS_RING>
 __global__ void __launch_bounds__(64) wfa_align_kernel(KParams p) {
+  constexpr bool CIG = MODE != 0;
   extern __shared__ uint64_t smem[];
   const int lane = threadIdx.x;
   uint64_t* sP = smem;
@@ -717,7 +865,10 @@ __global__ void __launch_bounds__(64) wfa_align_kernel(KParams p) {
   else R.ring = p.gring + (int64_t)blockIdx.x * p.ring_rows * p.wcap;
   Hist H;
   H.hdr = CIG ? p.hdr + (int64_t)blockIdx.x * p.hdr_cap : nullptr;
-  H.cells = CIG ? p.hist + (int64_t)blockIdx.x * p.hist_cap : nullptr;
+  H.cells = MODE == 1 ? p.hist + (int64_t)blockIdx.x * p.hist_cap : nullptr;
+  H.bytes = MODE == 2 ? (uint8_t*)p.hist + (int64_t)blockIdx.x * p.hist_cap : nullptr;
+  H.chain = MODE == 2 ? p.chain + (int64_t)blockIdx.x * p.chain_cap : nullptr;
+  H.chain_cap = p.chain_cap;
   H.hdr_cap = p.hdr_cap;
   H.cap = p.hist_cap;
   for (;;) {
@@ -736,8 +887,8 @@ __global__ void __launch_bounds__(64) wfa_align_kernel(KParams p) {
       }
       continue;
     }
-    if (bits == 2) align_pair<KIND, CIG, 2>(p, pair, plen, tlen, sP, sT, R, H);
-    else align_pair<KIND, CIG, 8>(p, pair, plen, tlen, sP, sT, R, H);
+    if (bits == 2) align_pair<KIND, MODE, 2>(p, pair, plen, tlen, sP, sT, R, H);
+    else align_pair<KIND, MODE, 8>(p, pair, plen, tlen, sP, sT, R, H);
     __syncthreads();
   }
 }
@@ -760,11 +911,12 @@ __global__ void wfa_max_length(const int32_t* plen, const int32_t* tlen, int n, 
 // ---------------------------------------------------------------------
 using KernelFn = void (*)(KParams);
 
-static KernelFn pick(int kind, bool cig, bool lds) {
-#define PICK(K)                                                                  \
-  if (kind == K) {                                                               \
-    if (cig) return lds ? wfa_align_kernel<K, true, true> : wfa_align_kernel<K, true, false>; \
-    return lds ? wfa_align_kernel<K, false, true> : wfa_align_kernel<K, false, false>;         \
+static KernelFn pick(int kind, int mode, bool lds) {
+#define PICK(K)                                                                    \
+  if (kind == K) {                                                                 \
+    if (mode == 1) return lds ? wfa_align_kernel<K, 1, true> : wfa_align_kernel<K, 1, false>; \
+    if (mode == 2) return lds ? wfa_align_kernel<K, 2, true> : wfa_align_kernel<K, 2, false>; \
+    return lds ? wfa_align_kernel<K, 0, true> : wfa_align_kernel<K, 0, false>;     \
   }
   PICK(KIND_EDIT)
   PICK(KIND_AFFINE)
@@ -788,16 +940,16 @@ hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, in
   return hipGetLastError();
 }
 
-hipError_t launch_align(const KParams& p, int kind, bool cig, bool lds_ring, int grid,
+hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, int grid,
                         size_t lds_bytes, hipStream_t stream) {
-  KernelFn fn = pick(kind, cig, lds_ring);
+  KernelFn fn = pick(kind, mode, lds_ring);
   if (!fn) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fn, dim3(grid), dim3(64), lds_bytes, stream, p);
   return hipGetLastError();
 }
 
-int max_blocks_per_cu(int kind, bool cig, bool lds_ring, size_t lds_bytes) {
-  KernelFn fn = pick(kind, cig, lds_ring);
+int max_blocks_per_cu(int kind, int mode, bool lds_ring, size_t lds_bytes) {
+  KernelFn fn = pick(kind, mode, lds_ring);
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, lds_bytes) != hipSuccess) return 1;
   return n > 0 ? n : 1;

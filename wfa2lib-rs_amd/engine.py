@@ -105,3 +105,9 @@ class BatchEngine:
 
     def last_retries(self):
         return N.lib().wfa_engine_last_retries(self._h)
+
+    def last_kernel_ms(self):
+        """(first persistent launch, all launches) of the last batch, HIP events."""
+        a, b = ctypes.c_float(0), ctypes.c_float(0)
+        N.check(N.lib().wfa_engine_last_kernel_ms(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
