@@ -33,7 +33,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--plain-sample", type=int, default=2000)
-    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-sample", type=int, default=500_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--parity-sample", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")

@@ -32,6 +32,8 @@ def lib(attr_type):
         l.wfo_aligner_free.argtypes = [VP]
         l.wfo_align_end2end.restype = I32
         l.wfo_align_end2end.argtypes = [VP, ctypes.c_char_p, I32, ctypes.c_char_p, I32]
+        l.wfo_align_endsfree.restype = I32
+        l.wfo_align_endsfree.argtypes = [VP, ctypes.c_char_p, I32, ctypes.c_char_p, I32]
         l.wfo_status.restype = I32
         l.wfo_status.argtypes = [VP]
         l.wfo_cigar.restype = I32
@@ -60,6 +62,7 @@ class OracleAligner:
         self.L = lib(type(attr))
         err = ctypes.c_int32(0)
         self.h = self.L.wfo_aligner_new(ctypes.byref(attr), ctypes.byref(err))
+        self.endsfree = attr.span == 1
         if not self.h:
             raise ValueError(f"oracle rejected the configuration ({err.value})")
 
@@ -68,8 +71,18 @@ class OracleAligner:
             self.L.wfo_aligner_free(self.h)
             self.h = None
 
+    def align(self, pattern: bytes, text: bytes):
+        """align_endsfree when the attribute's span is ends-free, else end-to-end."""
+        if self.endsfree:
+            return self._finish(self.L.wfo_align_endsfree(self.h, pattern, len(pattern), text,
+                                                          len(text)))
+        return self.align_end2end(pattern, text)
+
     def align_end2end(self, pattern: bytes, text: bytes):
-        s = self.L.wfo_align_end2end(self.h, pattern, len(pattern), text, len(text))
+        return self._finish(self.L.wfo_align_end2end(self.h, pattern, len(pattern), text,
+                                                     len(text)))
+
+    def _finish(self, s):
         st = self.L.wfo_status(self.h)
         ptr = ctypes.POINTER(ctypes.c_uint8)()
         n = self.L.wfo_cigar(self.h, ctypes.byref(ptr))

@@ -235,6 +235,8 @@ struct wfo_aligner {
     /* piggyback backtrace (med/low memory): BacktraceBuffer
      * (src/bt_buffer.rs:34-209) as flat arrays */
     int piggyback, low;
+    /* span (src/aligner.rs:163-170): ends-free limits of this alignment */
+    int endsfree, pb_free, pe_free, tb_free, te_free;
     uint32_t *blk_pc, *blk_prev; int nblk, cap_blk;
     int32_t *init_v, *init_h; int ninit, cap_init;
     uint32_t* packed; int cap_packed;
@@ -338,7 +340,8 @@ wfo_aligner* wfo_aligner_new(const wfa_attr_t* attr, int32_t* err) {
         free(a);
         return NULL;
     }
-    if (attr->span != WFA_SPAN_END2END || attr->heuristic != WFA_HEURISTIC_NONE ||
+    if ((attr->span == WFA_SPAN_ENDSFREE && a->pen.match != 0) ||
+        attr->heuristic != WFA_HEURISTIC_NONE ||
         attr->memory_mode == WFA_MEMORY_ULTRALOW ||
         attr->penalties.distance_metric == WFA_METRIC_GAP_LINEAR) {
         if (err) *err = WFA_ERR_UNSUPPORTED;
@@ -493,13 +496,29 @@ static void init_alignment(wfo_aligner* a) {
     a->end_score = -1; a->end_k = DIAG_NULL; a->end_offset = OFFSET_NULL;
     a->hist_lo = -(plen + scope + 2);
     a->hist_hi = tlen + scope + 2;
-    a->alloc_lo = 0; a->alloc_hi = 0;
     int alo, ahi;
-    output_alloc_range(a, 0, 0, &alo, &ahi);
+    int lo0 = 0, hi0 = 0;
+    if (a->endsfree && a->pen.match == 0) { lo0 = -a->pb_free; hi0 = a->tb_free; }
+    a->alloc_lo = 0; a->alloc_hi = 0;
+    output_alloc_range(a, lo0, hi0, &alo, &ahi);
     wf_t* m0 = alloc_wf(a, alo, ahi);
     wf_set(m0, 0, 0);
     wf_set_limits(m0, 0, 0);
     if (a->piggyback) set_bt_init(a, m0, 0, 0, 0);
+    /* ends-free with match == 0: the begin-free cells (h,0) and (0,v) are
+     * reachable at score 0 (src/aligner.rs:966-998) */
+    if (a->endsfree && a->pen.match == 0) {
+        for (int h = 1; h <= a->tb_free; ++h) {
+            wf_set(m0, h, h);
+            if (a->piggyback) set_bt_init(a, m0, h, 0, h);
+        }
+        for (int v = 1; v <= a->pb_free; ++v) {
+            wf_set(m0, -v, 0);
+            if (a->piggyback) set_bt_init(a, m0, -v, v, 0);
+        }
+        m0->lo = -a->pb_free; m0->hi = a->tb_free;
+        m0->init_min = -a->pb_free; m0->init_max = a->tb_free;
+    }
     put(a, C_M, 0, m0);
 }
 
@@ -532,6 +551,41 @@ static int extend_end2end(wfo_aligner* a, int score) {
         a->end_score = score; a->end_k = k_end; a->end_offset = a->tlen;
         a->status = WFA_STATUS_END_REACHED;
         return 1;
+    }
+    return 0;
+}
+
+/* termination_endsfree (src/termination.rs:31-59) */
+static inline int term_endsfree(const wfo_aligner* a, int k, int offset) {
+    int h = offset, v = offset - k;
+    if (h >= a->tlen && a->plen - v <= a->pe_free) return 1;
+    if (v >= a->plen && a->tlen - h <= a->te_free) return 1;
+    return 0;
+}
+
+/* extend_endsfree (src/aligner.rs:1053-1102) with
+ * extend_matches_packed_endsfree (src/extend.rs:112-170): the first
+ * diagonal in ascending k that reaches a free end wins. */
+static int extend_endsfree(wfo_aligner* a, int score) {
+    wf_t* m = get(a, C_M, score);
+    if (!m) {
+        if (a->num_null_steps > a->scope) {
+            a->status = WFA_STATUS_END_UNREACHABLE;
+            return 1;
+        }
+        return 0;
+    }
+    for (int k = m->lo; k <= m->hi; ++k) {
+        int32_t off = wf_get(m, k);
+        if (off != OFFSET_NULL) {
+            off += lcp(a, off - k, off);
+            wf_set(m, k, off);
+            if (term_endsfree(a, k, off)) {
+                a->end_score = score; a->end_k = k; a->end_offset = off;
+                a->status = WFA_STATUS_END_REACHED;
+                return 1;
+            }
+        }
     }
     return 0;
 }
@@ -1070,9 +1124,140 @@ static void set_sequences(wfo_aligner* a, const uint8_t* p, int plen, const uint
     a->tlen = tlen;
 }
 
+/* maxtrim_gap_affine / _affine2p (src/cigar.rs:272-419) over ops[b, e):
+ * the maximal-scoring prefix; returns the new end (b when cleared) and sets
+ * *trimmed. */
+static int maxtrim(const wfo_aligner* a, const uint8_t* ops, int b, int e, int* trimmed) {
+    const wfa_penalties_t* u = &a->pen.user;
+    *trimmed = 0;
+    if (b >= e) return e;
+    int mscore = u->match != 0 ? u->match : -1;
+    int max_score = 0, max_off = b, score = 0;
+    if (a->pen.metric == WFA_METRIC_GAP_AFFINE_2P) {
+        uint8_t last = 0;
+        int len = 0;
+        for (int i = b; i < e; ++i) {
+            if (ops[i] != last && last != 0) {
+                int d;
+                if (last == 'M') d = mscore * len;
+                else if (last == 'X') d = u->mismatch * len;
+                else {
+                    int s1 = u->gap_opening1 + u->gap_extension1 * len;
+                    int s2 = u->gap_opening2 + u->gap_extension2 * len;
+                    d = s1 < s2 ? s1 : s2;
+                }
+                score -= d;
+                len = 0;
+                if (max_score < score) { max_score = score; max_off = i - 1; }
+            }
+            last = ops[i];
+            len++;
+        }
+        int d;
+        if (last == 'M') d = mscore * len;
+        else if (last == 'X') d = u->mismatch * len;
+        else {
+            int s1 = u->gap_opening1 + u->gap_extension1 * len;
+            int s2 = u->gap_opening2 + u->gap_extension2 * len;
+            d = s1 < s2 ? s1 : s2;
+        }
+        score -= d;
+        if (max_score < score) { max_score = score; max_off = e - 1; }
+    } else {
+        uint8_t last = 0;
+        for (int i = b; i < e; ++i) {
+            uint8_t op = ops[i];
+            if (op == 'M') score -= mscore;
+            else if (op == 'X') score -= u->mismatch;
+            else if (a->pen.metric == WFA_METRIC_GAP_LINEAR) score -= u->gap_opening1;
+            else score -= u->gap_extension1 + (last == op ? 0 : u->gap_opening1);
+            last = op;
+            if (max_score < score) { max_score = score; max_off = i; }
+        }
+    }
+    *trimmed = max_off != e - 1;
+    if (max_score == 0) return b; /* cleared */
+    return max_off + 1;
+}
+
+/* maxtrim_cigar (src/aligner.rs:1545-1583) */
+static int maxtrim_cigar(wfo_aligner* a) {
+    if (a->pen.metric == WFA_METRIC_EDIT || a->pen.metric == WFA_METRIC_INDEL) return 0;
+    int begin_v = 0, begin_h = 0;
+    if (a->endsfree) {
+        while (begin_h < a->tb_free && a->cig_begin < a->cig_end && a->cig[a->cig_begin] == 'I') {
+            a->cig_begin++; begin_h++;
+        }
+        if (begin_h == 0)
+            while (begin_v < a->pb_free && a->cig_begin < a->cig_end &&
+                   a->cig[a->cig_begin] == 'D') {
+                a->cig_begin++; begin_v++;
+            }
+    }
+    if (a->cig_begin >= a->cig_end) {
+        a->cig_begin = a->cig_end = 0;
+        a->cig_score = INT32_MIN;
+        return 1;
+    }
+    int trimmed;
+    int ne = maxtrim(a, a->cig, a->cig_begin, a->cig_end, &trimmed);
+    if (ne == a->cig_begin) {
+        /* Cigar::clear */
+        a->cig_begin = a->cig_end = 0;
+        a->cig_score = INT32_MIN;
+        return trimmed;
+    }
+    a->cig_end = ne;
+    a->cig_begin -= begin_v + begin_h;
+    return trimmed;
+}
+
+int32_t wfo_align_endsfree(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
+                           const uint8_t* text, int32_t tlen) {
+    set_sequences(a, pattern, plen, text, tlen);
+    a->endsfree = 1;
+    if (a->attr.extension) {
+        a->pb_free = 0; a->pe_free = plen; a->tb_free = 0; a->te_free = tlen;
+    } else {
+        a->pb_free = a->attr.pattern_begin_free; a->pe_free = a->attr.pattern_end_free;
+        a->tb_free = a->attr.text_begin_free; a->te_free = a->attr.text_end_free;
+    }
+    init_alignment(a);
+    int score = 0;
+    for (;;) {
+        if (extend_endsfree(a, score)) break;
+        score += 1;
+        if (!has_slot(a, score)) { a->status = WFA_STATUS_MAX_STEPS_REACHED; break; }
+        compute_step(a, score);
+        if (score >= a->attr.max_alignment_steps) {
+            a->status = WFA_STATUS_MAX_STEPS_REACHED;
+            break;
+        }
+    }
+    int result = score;
+    if (a->status == WFA_STATUS_END_REACHED || a->status == WFA_STATUS_END_UNREACHABLE) {
+        int unreachable = a->status == WFA_STATUS_END_UNREACHABLE;
+        if (a->attr.alignment_scope == WFA_SCOPE_ALIGNMENT &&
+            (a->status == WFA_STATUS_END_REACHED || a->end_offset != OFFSET_NULL)) {
+            backtrace_from(a, score, a->end_k, a->end_offset);
+            if (a->attr.extension || unreachable) {
+                int trimmed = maxtrim_cigar(a);
+                a->status = (trimmed || unreachable) ? WFA_STATUS_PARTIAL : WFA_STATUS_COMPLETED;
+            } else {
+                a->status = WFA_STATUS_COMPLETED;
+            }
+        } else {
+            a->status = unreachable ? WFA_STATUS_PARTIAL : WFA_STATUS_COMPLETED;
+        }
+    }
+    return result;
+}
+
 int32_t wfo_align_end2end(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
                           const uint8_t* text, int32_t tlen) {
     set_sequences(a, pattern, plen, text, tlen);
+    a->endsfree = 0;
+    a->pb_free = a->pe_free = a->tb_free = a->te_free = 0;
     init_alignment(a);
     int score = 0;
     for (;;) {
@@ -1211,7 +1396,8 @@ static void* job_run(void* arg) {
             unpack2(T, tl, tb);
             P = pb; T = tb;
         }
-        j->score[i] = wfo_align_end2end(a, P, pl, T, tl);
+        j->score[i] = j->attr->span == WFA_SPAN_ENDSFREE ? wfo_align_endsfree(a, P, pl, T, tl)
+                                                          : wfo_align_end2end(a, P, pl, T, tl);
         j->status[i] = a->status;
         if (j->ops && j->ops_len) {
             const uint8_t* o = NULL;

@@ -36,6 +36,10 @@ wfo_aligner* wfo_aligner_new(const wfa_attr_t* attr, int32_t* err);
 void wfo_aligner_free(wfo_aligner* a);
 int32_t wfo_align_end2end(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
                           const uint8_t* text, int32_t tlen);
+/* align_endsfree (src/aligner.rs:707-831): the attribute's free ends, or
+ * extension when attr->extension is set */
+int32_t wfo_align_endsfree(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
+                           const uint8_t* text, int32_t tlen);
 int32_t wfo_status(const wfo_aligner* a);
 int32_t wfo_cigar(const wfo_aligner* a, const uint8_t** ops);
 int32_t wfo_cigar_score(const wfo_aligner* a);

@@ -49,7 +49,7 @@ def expand(seq):
     return seq.encode()
 
 
-def attr_for(N, metric, penalties, scope, max_steps=None):
+def attr_for(N, metric, penalties, scope, max_steps=None, span=None):
     a = N.Attr()
     N.lib().wfa_attr_default(ctypes.byref(a), METRICS[metric] if isinstance(metric, str) else metric)
     m, x, o1, e1, o2, e2 = penalties
@@ -60,6 +60,12 @@ def attr_for(N, metric, penalties, scope, max_steps=None):
     a.alignment_scope = 1 if scope in ("alignment", 1) else 0
     if max_steps is not None:
         a.max_alignment_steps = max_steps
+    if span == "extension":
+        a.span, a.extension = N.SPAN_ENDSFREE, 1
+    elif span is not None:
+        a.span = N.SPAN_ENDSFREE
+        (a.pattern_begin_free, a.pattern_end_free, a.text_begin_free,
+         a.text_end_free) = span
     return a
 
 

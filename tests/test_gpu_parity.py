@@ -41,7 +41,8 @@ def _compare(engine, oracle, attr, pairs, cigar, per_pair=False, label=""):
 @pytest.mark.parametrize("vec", load_vectors(), ids=lambda v: v["id"])
 def test_reference_vectors_on_gpu(vec, pkg, N, engine, oracle):
     from wfa2lib_rs_amd.synth import from_ascii
-    attr = attr_for(N, vec["metric"], vec["penalties"], vec["scope"], vec.get("max_steps"))
+    attr = attr_for(N, vec["metric"], vec["penalties"], vec["scope"], vec.get("max_steps"),
+                    vec.get("span"))
     p, t = expand(vec["pattern"]), expand(vec["text"])
     pairs = from_ascii([p], [t], N.SEQ_ASCII)
     cig = vec["scope"] == "alignment"
@@ -168,3 +169,20 @@ def test_long_reads_affine2p_med_match_oracle(pkg, N, engine, oracle):
     _compare(engine, oracle, attr, pairs, True, label="long/2p/med")
     attr_s = attr_for(N, "gap_affine_2p", (0, 4, 6, 2, 24, 1), "score")
     _compare(engine, oracle, attr_s, pairs, False, label="long/2p/score")
+
+
+SPANS = [("extension", "extension"), ("endsfree_5_5", [5, 5, 5, 5]), ("endsfree_txt", [0, 0, 0, 20]),
+         ("begin_free", [15, 0, 15, 0])]
+
+
+@pytest.mark.parametrize("name,span", SPANS)
+@pytest.mark.parametrize("metric,pen", [c for c in CONFIGS if c[1][0] == 0])
+@pytest.mark.parametrize("memory", ["high", "med"])
+def test_endsfree_spans_match_oracle(name, span, metric, pen, memory, pkg, N, engine, oracle):
+    """Ends-free and extension spans, score and CIGAR, high and medium memory."""
+    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    pairs = generate(WORKLOADS["short150"], num_pairs=600)
+    for scope in ("score", "alignment"):
+        attr = attr_for(N, metric, pen, scope, span=span)
+        attr.memory_mode = N.MEMORY_HIGH if memory == "high" else N.MEMORY_MED
+        _compare(engine, oracle, attr, pairs, scope == "alignment", label=f"{name}/{metric}/{scope}")

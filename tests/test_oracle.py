@@ -22,10 +22,11 @@ def _penalty_obj(pkg, metric, pen):
 
 @pytest.mark.parametrize("vec", load_vectors(), ids=lambda v: v["id"])
 def test_oracle_reference_vectors(vec, pkg, N, oracle):
-    attr = attr_for(N, vec["metric"], vec["penalties"], vec["scope"], vec.get("max_steps"))
+    attr = attr_for(N, vec["metric"], vec["penalties"], vec["scope"], vec.get("max_steps"),
+                    vec.get("span"))
     A = oracle.OracleAligner(attr)
     p, t = expand(vec["pattern"]), expand(vec["text"])
-    score, status, ops = A.align_end2end(p, t)
+    score, status, ops = A.align(p, t)
     cig = pkg.Cigar(ops)
     if "score" in vec:
         assert score == vec["score"]

@@ -87,7 +87,14 @@ int score_scope(const Internal& p) {
 int check_supported(const wfa_attr_t& a) {
   const int m = a.penalties.distance_metric;
   if (m == WFA_METRIC_GAP_LINEAR) return WFA_ERR_UNSUPPORTED;
-  if (a.span != WFA_SPAN_END2END || a.extension) return WFA_ERR_UNSUPPORTED;
+  if (a.span != WFA_SPAN_END2END && a.span != WFA_SPAN_ENDSFREE) return WFA_ERR_INVALID_ARGUMENT;
+  // ends-free with a match reward adds begin-free cells during compute
+  // (src/aligner.rs:3369-3389): not on the device path yet
+  if (a.span == WFA_SPAN_ENDSFREE && a.penalties.match != 0) return WFA_ERR_UNSUPPORTED;
+  if (a.span == WFA_SPAN_ENDSFREE && !a.extension &&
+      (a.pattern_begin_free < 0 || a.pattern_end_free < 0 || a.text_begin_free < 0 ||
+       a.text_end_free < 0))
+    return WFA_ERR_INVALID_ARGUMENT;
   if (a.heuristic != WFA_HEURISTIC_NONE) return WFA_ERR_UNSUPPORTED;
   if (a.alignment_scope != WFA_SCOPE_SCORE && a.alignment_scope != WFA_SCOPE_ALIGNMENT)
     return WFA_ERR_INVALID_ARGUMENT;
@@ -190,6 +197,18 @@ void base_params(const Internal& ip, const wfa_attr_t& attr, const wfa_batch_t& 
   // (src/aligner.rs:855-866): no score-indexed slot limit
   p->modular = attr.alignment_scope == WFA_SCOPE_SCORE || attr.memory_mode != WFA_MEMORY_HIGH;
   p->max_steps = attr.max_alignment_steps;
+  p->endsfree = attr.span == WFA_SPAN_ENDSFREE;
+  p->extension = p->endsfree && attr.extension;
+  p->pb_free = attr.pattern_begin_free;
+  p->pe_free = attr.pattern_end_free;
+  p->tb_free = attr.text_begin_free;
+  p->te_free = attr.text_end_free;
+  p->u_match = attr.penalties.match;
+  p->u_x = attr.penalties.mismatch;
+  p->u_o1 = attr.penalties.gap_opening1;
+  p->u_e1 = attr.penalties.gap_extension1;
+  p->u_o2 = attr.penalties.gap_opening2;
+  p->u_e2 = attr.penalties.gap_extension2;
   p->fmt = b.seq_format;
   p->seq = b.seq_data;
   p->poff = b.pattern_offset;

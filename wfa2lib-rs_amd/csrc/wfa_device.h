@@ -19,6 +19,11 @@ struct KParams {
   int ringM, ringE1, ringE2; // ring lengths of M, I1/D1, I2/D2 rows
   int modular;               // score scope: no history, no slot limit
   int max_steps;
+  // span (ends-free with match == 0; extension)
+  int endsfree, extension;
+  int pb_free, pe_free, tb_free, te_free;
+  // user penalties (max-trim of extension alignments scores with them)
+  int u_match, u_x, u_o1, u_e1, u_o2, u_e2;
   // pairs
   int num_pairs;             // work items in this launch
   int pair_base;             // first pair when pair_list is null

@@ -374,6 +374,62 @@ static __device__ int backtrace_linear(const KParams& p, const Hist& H, int nsco
   return cap - 1 - pos;
 }
 
+// maxtrim_cigar (src/aligner.rs:1545-1583) with Cigar::maxtrim_gap_affine
+// / _affine2p (src/cigar.rs:272-419) on the user penalties: keep the
+// maximal-scoring prefix, a free leading gap skipped and put back; an
+// alignment whose best prefix scores 0 is cleared. Returns the new length.
+static __device__ int maxtrim_cigar(const KParams& p, const uint8_t* ops, int n, int pb_free,
+                                    int tb_free, bool* trimmed) {
+  const int lane = threadIdx.x;
+  // leading free gap
+  int b = 0;
+  int bh = 0, bv = 0;
+  while (bh < tb_free && b < n && ops[b] == 'I') { ++b; ++bh; }
+  if (bh == 0)
+    while (bv < pb_free && b < n && ops[b] == 'D') { ++b; ++bv; }
+  *trimmed = false;
+  if (b >= n) { *trimmed = true; return 0; }
+  const int mscore = p.u_match != 0 ? p.u_match : -1;
+  const bool two = p.metric == WFA_METRIC_GAP_AFFINE_2P;
+  int max_score = 0, max_off = b, score = 0, last = 0, len = 0;
+  auto run_cost = [&](int op, int l) -> int {
+    if (op == 'M') return mscore * l;
+    if (op == 'X') return p.u_x * l;
+    const int s1 = p.u_o1 + p.u_e1 * l, s2 = p.u_o2 + p.u_e2 * l;
+    return s1 < s2 ? s1 : s2;
+  };
+  for (int c0 = b; c0 < n; c0 += 64) {
+    const int mine = c0 + lane < n ? ops[c0 + lane] : 0;
+    const int cnt = imin(64, n - c0);
+    for (int j = 0; j < cnt; ++j) {
+      const int op = uniform(__shfl(mine, j));
+      const int i = c0 + j;
+      if (two) {
+        if (op != last && last != 0) {
+          score -= run_cost(last, len);
+          len = 0;
+          if (max_score < score) { max_score = score; max_off = i - 1; }
+        }
+        last = op;
+        ++len;
+      } else {
+        if (op == 'M') score -= mscore;
+        else if (op == 'X') score -= p.u_x;
+        else score -= p.u_e1 + (last == op ? 0 : p.u_o1);
+        last = op;
+        if (max_score < score) { max_score = score; max_off = i; }
+      }
+    }
+  }
+  if (two) {
+    score -= run_cost(last, len);
+    if (max_score < score) { max_score = score; max_off = n - 1; }
+  }
+  *trimmed = max_off != n - 1;
+  if (max_score == 0) return 0;  // Cigar::clear
+  return max_off + 1;            // the skipped leading gap stays in front
+}
+
 // Medium/low memory: the chain of operations the reference's piggyback
 // kernels push (src/compute/piggyback.rs:195-528), recovered by following
 // the stored winner flags from the end cell back to M at score 0, then
@@ -423,19 +479,24 @@ static __device__ int piggyback_backtrace(const KParams& p, const Hist& H, int n
   }
   if (bad) return -1;
   __threadfence_block();
+  // the chain starts at the M cell of score 0 it reached: (0,0) or an
+  // ends-free seed (h,0) at k = h, (0,v) at k = -v
+  const int bv = kk < 0 ? -kk : 0, bh = kk > 0 ? kk : 0;
   const int nchain = H.chain_cap - 1 - pos;
   const uint8_t* ch = H.chain + pos + 1;
   const bool affine = KIND != KIND_EDIT;
   // unpack: begin at (0, 0); re-extend matches before each op (affine: only
   // while in M); drop the X that closes a gap; pad to the end cell and the
   // sequence ends
-  int n = 0, v = 0, h = 0;
+  int n = 0, v = bv, h = bh;
   bool in_m = true;
   auto put_run = [&](uint8_t c, int len) {
     for (int i = lane; i < len; i += 64)
       if (n + i < cap) out[n + i] = c;
     n += len > 0 ? len : 0;
   };
+  put_run('I', bh);
+  put_run('D', bv);
   for (int b = 0; b < nchain; b += 64) {
     const int c = (b + lane < nchain) ? ch[b + lane] : 0;
     const int cnt = imin(64, nchain - b);
@@ -502,16 +563,28 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
             rD2 = ringM + 2 * ringE1 + ringE2;
   const int nrows = KIND == KIND_EDIT ? ringM : two ? rD2 + ringE2 : rD1 + ringE1;
   for (int r = lane; r < nrows; r += 64) R.meta[r] = make_int2(META_ABSENT, META_ABSENT - 1);
-  __syncthreads();
-  // M0: k = 0, offset 0 (init_alignment, src/aligner.rs:938-950)
-  if (lane == 0) {
-    R.meta[0] = make_int2(0, 0);
-    R.ring[0] = 0;
+  // ends-free limits of this pair (extension: free ends up to the full
+  // lengths, no begin gaps; src/aligner.rs:713-718)
+  const bool ef = p.endsfree;
+  const int pb_free = ef && !p.extension ? p.pb_free : 0;
+  const int tb_free = ef && !p.extension ? p.tb_free : 0;
+  const int pe_free = !ef ? 0 : p.extension ? plen : p.pe_free;
+  const int te_free = !ef ? 0 : p.extension ? tlen : p.te_free;
+  // M0: k = 0, offset 0, plus (ends-free, match == 0) the begin-free cells
+  // (h, 0) at k = h and (0, v) at k = -v (init_alignment,
+  // src/aligner.rs:938-998)
+  const int lo0 = -pb_free, hi0 = tb_free;
+  bool overflow = hi0 - lo0 + 1 > R.wcap;
+  if (!overflow) {
+    for (int k = lo0 + lane; k <= hi0; k += 64) R.ring[k & R.mask] = k > 0 ? k : 0;
   }
+  __syncthreads();
+  if (lane == 0) R.meta[0] = make_int2(lo0, hi0);
   __syncthreads();
 
   const int nwf = high_mem_nwf(p, plen, tlen);
   const int k_end = tlen - plen;
+  int end_k = k_end, end_off = tlen;
   const int hist_lo = -(plen + p.scope + 2), hist_hi = tlen + p.scope + 2;
   int64_t hpos = 0;
   int nulls = 0;
@@ -524,10 +597,13 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   auto extend_row = [&](int s, int lo, int hi) -> bool {
     const int row = s % ringM;
     bool reached = false;
+    uint64_t hit = 0;  // ends-free: first chunk with a terminating diagonal
+    int hit_kb = 0;
     int4* hd = MODE == 1 ? &H.hdr[s] : nullptr;
     int64_t base = hpos;
     for (int kb = lo; kb <= hi; kb += 64) {
       const int k = kb + lane;
+      bool term = false;
       if (k <= hi) {
         int32_t* cell = &R.ring[row * R.wcap + (k & R.mask)];
         int off = *cell;
@@ -535,10 +611,28 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           off += lcp<BITS>(sP, sT, off - k, off, plen, tlen);
           *cell = off;
         }
-        if (k == k_end && off >= tlen) reached = true;
+        if (!ef) {
+          if (k == k_end && off >= tlen) reached = true;
+        } else {
+          // termination_endsfree (src/termination.rs:31-59)
+          const int v = off - k;
+          term = off >= 0 &&
+                 ((off >= tlen && plen - v <= pe_free) || (v >= plen && tlen - off <= te_free));
+        }
         if (MODE == 1) H.cells[base + (k - lo)] = off;
       }
+      if (ef && !hit) {
+        // the lowest terminating diagonal wins (src/extend.rs:133-166)
+        hit = ballot(term);
+        hit_kb = kb;
+        if (hit) {
+          const int j = __builtin_ctzll(hit);
+          end_k = hit_kb + j;
+          end_off = uniform(__shfl(R.ring[row * R.wcap + ((hit_kb + lane) & R.mask)], j));
+        }
+      }
     }
+    if (ef) reached = hit != 0;
     if (MODE == 1) {
       if (lane == 0) *hd = make_int4(lo, hi, (int)base, 0);
       if (lo <= hi) hpos += hi - lo + 1;
@@ -546,16 +640,12 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     return ballot(reached) != 0;
   };
 
-  bool overflow = false;
   // score 0
   {
-    if (CIG && (H.hdr_cap < 1 || H.cap < 1)) overflow = true;
-    else if (MED && lane == 0) {
-      H.hdr[0] = make_int4(0, 0, 0, 0);
-      H.bytes[0] = 0;
-    }
-    if (MED) hpos = 1;
-    if (!overflow && extend_row(0, 0, 0)) {
+    if (CIG && (H.hdr_cap < 1 || H.cap < hi0 - lo0 + 1)) overflow = true;
+    else if (MED && lane == 0) H.hdr[0] = make_int4(lo0, hi0, 0, 0);
+    if (MED) hpos = hi0 - lo0 + 1;
+    if (!overflow && extend_row(0, lo0, hi0)) {
       status = WFA_STATUS_END_REACHED;
       result = 0;
     }
@@ -793,7 +883,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     // outside the trimmed range are OFFSET_NULL there anyway.
     const int elo = MODE == 1 ? lo : m_lo, ehi = MODE == 1 ? hi : m_hi;
     if (MODE == 1 && hpos + (hi - lo + 1) > H.cap) { overflow = true; break; }
-    const bool reached = extend_row(s, elo, ehi) && (m_lo <= k_end && k_end <= m_hi);
+    const bool reached = extend_row(s, elo, ehi) && (ef || (m_lo <= k_end && k_end <= m_hi));
     __syncthreads();
     if (reached) { status = WFA_STATUS_END_REACHED; result = s; break; }
   }
@@ -815,12 +905,13 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       uint8_t* out = p.cig + p.cig_off[pair];
       const int cap = plen + tlen;
       if (MED)
-        ncig = piggyback_backtrace<KIND, BITS>(p, H, score + 1, sP, sT, plen, tlen, score, k_end,
-                                               tlen, out, cap);
+        ncig = piggyback_backtrace<KIND, BITS>(p, H, score + 1, sP, sT, plen, tlen, score, end_k,
+                                               end_off, out, cap);
       else if (KIND == KIND_EDIT)
-        ncig = backtrace_linear<BITS>(p, H, score + 1, plen, tlen, score, k_end, tlen, out, cap);
+        ncig = backtrace_linear<BITS>(p, H, score + 1, plen, tlen, score, end_k, end_off, out,
+                                      cap);
       else
-        ncig = singletrack<BITS>(p, H, score + 1, nwf, sP, sT, plen, tlen, score, k_end, tlen,
+        ncig = singletrack<BITS>(p, H, score + 1, nwf, sP, sT, plen, tlen, score, end_k, end_off,
                                  out, cap);
       if (ncig < 0) {
         // the stored flags do not lead back to the start: never expected;
@@ -839,6 +930,12 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           if (i < ncig) out[i] = v;
           __builtin_amdgcn_wave_barrier();
         }
+      }
+      if (ef && p.extension && KIND != KIND_EDIT && ncig > 0) {
+        __threadfence_block();
+        bool trimmed = false;
+        ncig = maxtrim_cigar(p, out, ncig, pb_free, tb_free, &trimmed);
+        if (trimmed) out_status = WFA_STATUS_PARTIAL;
       }
     }
   }
