@@ -37,7 +37,19 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--parity-sample", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: every box here has one MI355X, and pairs are "
               f"independent, so N={args.gpus} would only shard the batch (see DESIGN.md).",

@@ -235,6 +235,9 @@ struct wfo_aligner {
     /* piggyback backtrace (med/low memory): BacktraceBuffer
      * (src/bt_buffer.rs:34-209) as flat arrays */
     int piggyback, low;
+    /* heuristic state (src/heuristic.rs:60-143) */
+    int h_steps_wait, h_max_sw_score, h_max_sw_k;
+    int32_t* dist; int cap_dist;
     /* span (src/aligner.rs:163-170): ends-free limits of this alignment */
     int endsfree, pb_free, pe_free, tb_free, te_free;
     uint32_t *blk_pc, *blk_prev; int nblk, cap_blk;
@@ -341,7 +344,7 @@ wfo_aligner* wfo_aligner_new(const wfa_attr_t* attr, int32_t* err) {
         return NULL;
     }
     if ((attr->span == WFA_SPAN_ENDSFREE && a->pen.match != 0) ||
-        attr->heuristic != WFA_HEURISTIC_NONE ||
+        attr->heuristic == WFA_HEURISTIC_ZDROP ||
         attr->memory_mode == WFA_MEMORY_ULTRALOW ||
         attr->penalties.distance_metric == WFA_METRIC_GAP_LINEAR) {
         if (err) *err = WFA_ERR_UNSUPPORTED;
@@ -367,6 +370,7 @@ void wfo_aligner_free(wfo_aligner* a) {
     free(a->init_v);
     free(a->init_h);
     free(a->packed);
+    free(a->dist);
     free(a);
 }
 
@@ -453,6 +457,8 @@ static void offload_affine_component(wfo_aligner* a, wf_t* w, int lo, int hi) {
     }
 }
 
+static void heuristic_reset(wfo_aligner* a);
+
 /* ------------------------------------------------------------------ */
 /* init_alignment (src/aligner.rs:846-1000), end-to-end                */
 /* ------------------------------------------------------------------ */
@@ -493,6 +499,7 @@ static void init_alignment(wfo_aligner* a) {
     a->cig_score = INT32_MIN;
     a->num_null_steps = 0;
     a->status = WFA_STATUS_OK;
+    heuristic_reset(a);
     a->end_score = -1; a->end_k = DIAG_NULL; a->end_offset = OFFSET_NULL;
     a->hist_lo = -(plen + scope + 2);
     a->hist_hi = tlen + scope + 2;
@@ -531,6 +538,158 @@ static inline int lcp(const wfo_aligner* a, int v, int h) {
     return n;
 }
 
+/* ------------------------------------------------------------------ */
+/* Heuristics (src/heuristic.rs, src/aligner.rs:1644-1766)              */
+/* ------------------------------------------------------------------ */
+static void heuristic_reset(wfo_aligner* a) {
+    int h = a->attr.heuristic;
+    a->h_steps_wait = 0;
+    if (h == WFA_HEURISTIC_WFADAPTIVE || h == WFA_HEURISTIC_BANDED_ADAPTIVE)
+        a->h_steps_wait = a->attr.heuristic_p[2];
+    else if (h == WFA_HEURISTIC_XDROP || h == WFA_HEURISTIC_ZDROP)
+        a->h_steps_wait = a->attr.heuristic_p[1];
+    a->h_max_sw_score = 0;
+    a->h_max_sw_k = DIAG_NULL;
+}
+/* wf_distance_end2end (src/heuristic.rs:150-160) */
+static inline int32_t wf_distance(int32_t off, int k, int plen, int tlen) {
+    if (off < 0) return INT32_MAX;
+    int lv = plen - (off - k), lh = tlen - off;
+    return lv > lh ? lv : lh;
+}
+static int32_t* dist_at(wfo_aligner* a, const wf_t* w) {
+    /* scratch indexed by k - w->base over the allocation */
+    if (w->nalloc > a->cap_dist) {
+        a->cap_dist = w->nalloc * 2;
+        a->dist = (int32_t*)realloc(a->dist, sizeof(int32_t) * (size_t)a->cap_dist);
+    }
+    return a->dist;
+}
+/* heuristic_cutoff (src/heuristic.rs:167-255) without Z-drop; narrows
+ * m->lo/hi only (cut cells keep their offsets and stay readable). */
+static void heuristic_cutoff(wfo_aligner* a, wf_t* wf, int score) {
+    if (wf->lo > wf->hi) return;
+    a->h_steps_wait -= 1;
+    const int plen = a->plen, tlen = a->tlen;
+    const int* hp = a->attr.heuristic_p;
+    switch (a->attr.heuristic) {
+    case WFA_HEURISTIC_WFADAPTIVE: { /* src/heuristic.rs:258-346 */
+        if (a->h_steps_wait > 0) return;
+        int lo = wf->lo, hi = wf->hi;
+        if (hi - lo + 1 < hp[0]) return;
+        int32_t* d = dist_at(a, wf);
+        int min_d = plen > tlen ? plen : tlen;
+        for (int k = lo; k <= hi; ++k) {
+            int32_t dd = wf_distance(wf_get(wf, k), k, plen, tlen);
+            d[k - wf->base] = dd;
+            if (dd < min_d) min_d = dd;
+        }
+        int ak = tlen - plen;
+        int top = ak < wf->hi ? ak : wf->hi;
+        int lo_r = wf->lo;
+        for (int k = wf->lo; k < top; ++k) {
+            if ((int64_t)d[k - wf->base] - min_d <= hp[1]) break;
+            lo_r++;
+        }
+        wf->lo = lo_r;
+        int bottom = ak > wf->lo ? ak : wf->lo;
+        int hi_r = wf->hi;
+        for (int k = wf->hi; k > bottom; --k) {
+            if ((int64_t)d[k - wf->base] - min_d <= hp[1]) break;
+            hi_r--;
+        }
+        wf->hi = hi_r;
+        a->h_steps_wait = hp[2];
+        return;
+    }
+    case WFA_HEURISTIC_BANDED_STATIC: /* src/heuristic.rs:400-407 */
+        if (wf->lo < hp[0]) wf->lo = hp[0];
+        if (wf->hi > hp[1]) wf->hi = hp[1];
+        return;
+    case WFA_HEURISTIC_BANDED_ADAPTIVE: { /* src/heuristic.rs:355-394 */
+        if (a->h_steps_wait > 0) return;
+        int lo = wf->lo, hi = wf->hi, len = hi - lo + 1;
+        if (len < 4) return;
+        int maxlen = hp[1] - hp[0] + 1;
+        if (len > maxlen) {
+            int leeway = (len - maxlen) / 2, q = len / 4;
+            int32_t d0 = wf_distance(wf_get(wf, lo), lo, plen, tlen);
+            int32_t d1 = wf_distance(wf_get(wf, lo + q), lo + q, plen, tlen);
+            int32_t d2 = wf_distance(wf_get(wf, lo + 2 * q), lo + 2 * q, plen, tlen);
+            int32_t d3 = wf_distance(wf_get(wf, hi), hi, plen, tlen);
+            int nlo = lo;
+            if (d0 > d3) nlo += leeway;
+            if (d1 > d2) nlo += leeway;
+            wf->lo = nlo > lo ? nlo : lo;
+            int nhi = nlo + maxlen - 1;
+            wf->hi = nhi < hi ? nhi : hi;
+        }
+        a->h_steps_wait = hp[2];
+        return;
+    }
+    case WFA_HEURISTIC_XDROP: { /* src/heuristic.rs:410-448, 491-525 */
+        if (a->h_steps_wait > 0) return;
+        const int swg_match = a->pen.match != 0 ? -a->pen.match : 1;
+        const int bfree = a->pen.match == 0 && a->endsfree;
+        const int pbf = bfree ? a->pb_free : 0, tbf = bfree ? a->tb_free : 0;
+        int32_t* sw = dist_at(a, wf);
+        int cmax = INT32_MIN, cmax_k = 0;
+        for (int k = wf->lo; k <= wf->hi; ++k) {
+            int32_t off = wf_get(wf, k);
+            if (off < 0) continue;
+            int fv = k < 0 ? ((-k) < pbf ? -k : pbf) : 0;
+            int fh = k > 0 ? (k < tbf ? k : tbf) : 0;
+            int v = off - k - fv, h = off - fh;
+            int sws = (swg_match * (v + h) - score) / 2;
+            sw[k - wf->base] = sws;
+            if (cmax < sws) { cmax = sws; cmax_k = k; }
+        }
+        if (a->h_max_sw_k != DIAG_NULL) {
+            int k = wf->lo;
+            while (k <= wf->hi &&
+                   (wf_get(wf, k) < 0 || a->h_max_sw_score - sw[k - wf->base] >= hp[0]))
+                k++;
+            wf->lo = k;
+            k = wf->hi;
+            while (k >= wf->lo &&
+                   (wf_get(wf, k) < 0 || a->h_max_sw_score - sw[k - wf->base] >= hp[0]))
+                k--;
+            wf->hi = k;
+            if (cmax > a->h_max_sw_score) { a->h_max_sw_score = cmax; a->h_max_sw_k = cmax_k; }
+        } else {
+            a->h_max_sw_score = cmax;
+            a->h_max_sw_k = cmax_k;
+        }
+        a->h_steps_wait = hp[1];
+        return;
+    }
+    default:
+        return;
+    }
+}
+
+/* apply_heuristic_cutoff (src/aligner.rs:1644-1766), without Z-drop. */
+static void apply_heuristic_cutoff(wfo_aligner* a, int score, wf_t* wf) {
+    if (a->attr.heuristic == WFA_HEURISTIC_BANDED_STATIC &&
+        wf->lo >= a->attr.heuristic_p[0] && wf->hi <= a->attr.heuristic_p[1])
+        return;
+    int lo_base = wf->lo, hi_base = wf->hi;
+    heuristic_cutoff(a, wf, score);
+    if (lo_base > hi_base || (wf->lo == lo_base && wf->hi == hi_base)) return;
+    if (wf->lo > wf->hi) {
+        wf->null = 1;
+        wf->init_min = wf->lo; wf->init_max = wf->hi; /* clear_init_range */
+    }
+    int nc = a->pen.metric == WFA_METRIC_GAP_AFFINE ? 3 : a->pen.metric == WFA_METRIC_GAP_AFFINE_2P ? 5 : 1;
+    for (int c = 1; c < nc; ++c) {
+        wf_t* d = get(a, c, score);
+        if (!d) continue;
+        if (wf->lo > d->lo) d->lo = wf->lo;
+        if (wf->hi < d->hi) d->hi = wf->hi;
+        if (d->lo > d->hi) { d->null = 1; d->init_min = d->lo; d->init_max = d->hi; }
+    }
+}
+
 /* extend_end2end (src/aligner.rs:1004-1049) */
 static int extend_end2end(wfo_aligner* a, int score) {
     wf_t* m = get(a, C_M, score);
@@ -552,6 +711,7 @@ static int extend_end2end(wfo_aligner* a, int score) {
         a->status = WFA_STATUS_END_REACHED;
         return 1;
     }
+    if (a->attr.heuristic != WFA_HEURISTIC_NONE) apply_heuristic_cutoff(a, score, m);
     return 0;
 }
 
@@ -587,6 +747,7 @@ static int extend_endsfree(wfo_aligner* a, int score) {
             }
         }
     }
+    if (a->attr.heuristic != WFA_HEURISTIC_NONE) apply_heuristic_cutoff(a, score, m);
     return 0;
 }
 
@@ -643,6 +804,12 @@ static void compute_affine_step(wfo_aligner* a, int score) {
     }
 #undef LMIN
 #undef HMAX
+    /* clamp_to_band (src/aligner.rs:3308-3316) */
+    if (a->attr.heuristic == WFA_HEURISTIC_BANDED_STATIC ||
+        a->attr.heuristic == WFA_HEURISTIC_BANDED_ADAPTIVE) {
+        if (lo < a->attr.heuristic_p[0]) lo = a->attr.heuristic_p[0];
+        if (hi > a->attr.heuristic_p[1]) hi = a->attr.heuristic_p[1];
+    }
     if (lo > hi) {
         if (a->modular) free_outputs(a, score, ncomp);
         return;
@@ -1212,6 +1379,38 @@ static int maxtrim_cigar(wfo_aligner* a) {
     return trimmed;
 }
 
+/* rescore_heuristic_cigar (src/aligner.rs:1605-1640) with cigar_wfa_score
+ * (src/aligner.rs:2700-2719): with a heuristic, the gap-affine/2p CIGAR's
+ * own score (free end gaps excluded) becomes the returned score. */
+static int rescore_heuristic_cigar(wfo_aligner* a, int score, int end_v, int end_h) {
+    if (a->attr.heuristic == WFA_HEURISTIC_NONE ||
+        !(a->pen.metric == WFA_METRIC_GAP_AFFINE || a->pen.metric == WFA_METRIC_GAP_AFFINE_2P))
+        return score;
+    int b = a->cig_begin, e = a->cig_end;
+    if (a->endsfree) {
+        int trailing = (a->plen - end_v) + (a->tlen - end_h);
+        e = e - trailing < 0 ? 0 : e - trailing;
+        uint8_t op = b < a->cig_end ? a->cig[b] : 0;
+        int fr = op == 'I' ? a->tb_free : op == 'D' ? a->pb_free : 0;
+        int len = 0;
+        while (len < fr && b < e && a->cig[b] == op) { b++; len++; }
+    }
+    if (e < b) e = b;
+    if (e == b) { a->cig_score = 0; return 0; }
+    const wfa_penalties_t* u = &a->pen.user;
+    int v = 0, h = 0;
+    for (int i = b; i < e; ++i) { v += a->cig[i] != 'I'; h += a->cig[i] != 'D'; }
+    int classic = a->pen.metric == WFA_METRIC_GAP_AFFINE
+                      ? wfo_cigar_score_gap_affine(a->cig + b, e - b, u->match, u->mismatch,
+                                                   u->gap_opening1, u->gap_extension1)
+                      : wfo_cigar_score_gap_affine2p(a->cig + b, e - b, u->match, u->mismatch,
+                                                     u->gap_opening1, u->gap_extension1,
+                                                     u->gap_opening2, u->gap_extension2);
+    int r = a->pen.match < 0 ? -a->pen.match * (v + h) - 2 * classic : -classic;
+    a->cig_score = r;
+    return r;
+}
+
 int32_t wfo_align_endsfree(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
                            const uint8_t* text, int32_t tlen) {
     set_sequences(a, pattern, plen, text, tlen);
@@ -1244,6 +1443,8 @@ int32_t wfo_align_endsfree(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
                 int trimmed = maxtrim_cigar(a);
                 a->status = (trimmed || unreachable) ? WFA_STATUS_PARTIAL : WFA_STATUS_COMPLETED;
             } else {
+                result = rescore_heuristic_cigar(a, score, a->end_offset - a->end_k,
+                                                 a->end_offset);
                 a->status = WFA_STATUS_COMPLETED;
             }
         } else {
@@ -1272,8 +1473,10 @@ int32_t wfo_align_end2end(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
     }
     int result = score;
     if (a->status == WFA_STATUS_END_REACHED) {
-        if (a->attr.alignment_scope == WFA_SCOPE_ALIGNMENT)
+        if (a->attr.alignment_scope == WFA_SCOPE_ALIGNMENT) {
             backtrace_from(a, score, a->end_k, a->end_offset);
+            result = rescore_heuristic_cigar(a, score, a->plen, a->tlen);
+        }
         a->status = WFA_STATUS_COMPLETED;
     }
     return result;

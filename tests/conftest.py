@@ -49,6 +49,24 @@ def expand(seq):
     return seq.encode()
 
 
+HEURISTICS = {"wfadaptive": 1, "xdrop": 2, "zdrop": 3, "banded_static": 4, "banded_adaptive": 5}
+MEMORY = {"high": 0, "med": 1, "low": 2, "ultralow": 3}
+
+
+def attr_for_vec(N, vec):
+    """The aligner configuration a golden vector describes."""
+    a = attr_for(N, vec["metric"], vec["penalties"], vec["scope"], vec.get("max_steps"),
+                 vec.get("span"))
+    if "heuristic" in vec:
+        h = vec["heuristic"]
+        a.heuristic = HEURISTICS[h[0]]
+        for i, v in enumerate(h[1:]):
+            a.heuristic_p[i] = v
+    if "memory" in vec:
+        a.memory_mode = MEMORY[vec["memory"]]
+    return a
+
+
 def attr_for(N, metric, penalties, scope, max_steps=None, span=None):
     a = N.Attr()
     N.lib().wfa_attr_default(ctypes.byref(a), METRICS[metric] if isinstance(metric, str) else metric)

@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from conftest import attr_for, expand, have_gpu, load_vectors
+from conftest import attr_for, attr_for_vec, expand, have_gpu, load_vectors
 
 pytestmark = pytest.mark.gpu
 
@@ -41,8 +41,7 @@ def _compare(engine, oracle, attr, pairs, cigar, per_pair=False, label=""):
 @pytest.mark.parametrize("vec", load_vectors(), ids=lambda v: v["id"])
 def test_reference_vectors_on_gpu(vec, pkg, N, engine, oracle):
     from wfa2lib_rs_amd.synth import from_ascii
-    attr = attr_for(N, vec["metric"], vec["penalties"], vec["scope"], vec.get("max_steps"),
-                    vec.get("span"))
+    attr = attr_for_vec(N, vec)
     p, t = expand(vec["pattern"]), expand(vec["text"])
     pairs = from_ascii([p], [t], N.SEQ_ASCII)
     cig = vec["scope"] == "alignment"
@@ -186,3 +185,47 @@ def test_endsfree_spans_match_oracle(name, span, metric, pen, memory, pkg, N, en
         attr = attr_for(N, metric, pen, scope, span=span)
         attr.memory_mode = N.MEMORY_HIGH if memory == "high" else N.MEMORY_MED
         _compare(engine, oracle, attr, pairs, scope == "alignment", label=f"{name}/{metric}/{scope}")
+
+
+# (heuristic, p0, p1, p2): the serving-mix settings and tighter ones that cut
+# on most steps
+HEURISTIC_CASES = [
+    ("wfadaptive_10_50_1", (1, 10, 50, 1)),
+    ("wfadaptive_4_6_2", (1, 4, 6, 2)),
+    ("xdrop_200_1", (2, 200, 1, 0)),
+    ("xdrop_15_1", (2, 15, 1, 0)),
+    ("xdrop_8_3", (2, 8, 3, 0)),
+    ("banded_static_m12_9", (4, -12, 9, 0)),
+    ("banded_adaptive_m50_50_1", (5, -50, 50, 1)),
+    ("banded_adaptive_m6_7_1", (5, -6, 7, 1)),
+]
+HEUR_SPANS = [("end2end", None), ("extension", "extension"), ("endsfree_5_5", [5, 5, 5, 5])]
+
+
+def _heuristic_pairs():
+    from wfa2lib_rs_amd.synth import Workload, generate
+    import wfa2lib_rs_amd._native as Nn
+    w = Workload("heuristic_mix", 300, 1, 40, 400, 0.0, 0.30, 3.0, run_frac=0.1, run_mean=15.0,
+                 seq_format=Nn.SEQ_ASCII, seed=11)
+    return generate(w)
+
+
+@pytest.mark.parametrize("hname,h", HEURISTIC_CASES)
+@pytest.mark.parametrize("sname,span", HEUR_SPANS)
+@pytest.mark.parametrize("metric,pen", CONFIGS)
+@pytest.mark.parametrize("memory", ["high", "med"])
+def test_heuristics_match_oracle(hname, h, sname, span, metric, pen, memory, pkg, N, engine,
+                                 oracle):
+    """WF-adaptive, X-drop, banded static/adaptive: cut-offs, band clamp,
+    statuses and (gap-affine/2p) CIGAR rescoring, against the oracle."""
+    if span is not None and pen[0] != 0:
+        pytest.skip("ends-free on the device needs match == 0")
+    pairs = _heuristic_pairs()
+    for scope in ("score", "alignment"):
+        attr = attr_for(N, metric, pen, scope, span=span)
+        attr.memory_mode = N.MEMORY_HIGH if memory == "high" else N.MEMORY_MED
+        attr.heuristic = h[0]
+        for i in range(3):
+            attr.heuristic_p[i] = h[i + 1]
+        _compare(engine, oracle, attr, pairs, scope == "alignment",
+                 label=f"{hname}/{sname}/{metric}/{memory}/{scope}")

@@ -3,7 +3,7 @@ exact Gotoh DP, CIGAR validity/rescoring invariants, and the generator."""
 import numpy as np
 import pytest
 
-from conftest import attr_for, expand, load_vectors
+from conftest import attr_for, attr_for_vec, expand, load_vectors
 
 STATUS = {"OK": 0, "END_REACHED": 1, "END_UNREACHABLE": 2, "MAX_STEPS_REACHED": 3, "OOM": 4,
           "COMPLETED": 5, "PARTIAL": 6}
@@ -22,8 +22,7 @@ def _penalty_obj(pkg, metric, pen):
 
 @pytest.mark.parametrize("vec", load_vectors(), ids=lambda v: v["id"])
 def test_oracle_reference_vectors(vec, pkg, N, oracle):
-    attr = attr_for(N, vec["metric"], vec["penalties"], vec["scope"], vec.get("max_steps"),
-                    vec.get("span"))
+    attr = attr_for_vec(N, vec)
     A = oracle.OracleAligner(attr)
     p, t = expand(vec["pattern"]), expand(vec["text"])
     score, status, ops = A.align(p, t)

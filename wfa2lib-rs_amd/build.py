@@ -33,11 +33,34 @@ def _run(cmd):
 
 
 def build_engine():
+    """Compile each .hip translation unit to an object in parallel (the
+    per-kind kernel sets are the slow part), then link with the host API."""
     os.makedirs(LIB, exist_ok=True)
-    srcs = [os.path.join(CSRC, f) for f in ("wfa_kernels.hip", "wfa_api.cpp")]
+    obj_dir = os.path.join(LIB, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    hips = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    procs = []
+    for f in hips:
+        obj = os.path.join(obj_dir, f[:-4] + ".o")
+        cmd = [HIPCC] + compile_flags + ["-c", "-o", obj, os.path.join(CSRC, f)]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((cmd, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                 stderr=subprocess.STDOUT, text=True)))
+    objs, failed = [], []
+    for cmd, obj, pr in procs:
+        out, _ = pr.communicate()
+        out = "\n".join(l for l in out.splitlines() if "packed-fp32-ops" not in l)
+        if out.strip():
+            print(out, flush=True)
+        if pr.returncode != 0:
+            failed.append(" ".join(cmd))
+        objs.append(obj)
+    if failed:
+        raise RuntimeError("build step failed: " + "; ".join(failed))
     out = os.path.join(LIB, "libwfa_mi355x.so")
     tmp = out + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + srcs)
+    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + objs + [os.path.join(CSRC, "wfa_api.cpp")])
     os.replace(tmp, out)
     return out
 

@@ -1,4 +1,5 @@
-// wfa_kernels.hip — gfx950 wavefront-alignment kernels.
+// wfa_align.h — gfx950 wavefront-alignment kernels (included by one
+// translation unit per distance-metric kind, which instantiates them).
 //
 // One 64-lane wavefront aligns one pattern/text pair at a time and pulls
 // the next pair from a device work queue (a persistent launch). Within a
@@ -18,8 +19,12 @@
 // and compute steps (src/aligner.rs:1769-1865, 2006-2193, 2201-2553) bit
 // for bit; the kernels never approximate.
 
+#pragma once
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "wfa_device.h"
 
@@ -27,6 +32,9 @@ namespace wfa {
 
 #define WFA_NULL (INT32_MIN / 2)  // OFFSET_NULL, src/offset.rs:15
 #define META_ABSENT 0x7fffffff   // row never computed (a null step)
+static __device__ __forceinline__ int4 meta_absent() {
+  return make_int4(META_ABSENT, META_ABSENT - 1, META_ABSENT, META_ABSENT - 1);
+}
 
 static __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 static __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
@@ -34,6 +42,17 @@ static __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b;
 static __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 static __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+static __device__ __forceinline__ int wave_min(int v) {
+  for (int d = 32; d >= 1; d >>= 1) v = imin(v, __shfl_xor(v, d));
+  return v;
+}
+static __device__ __forceinline__ int64_t wave_max64(int64_t v) {
+  for (int d = 32; d >= 1; d >>= 1) {
+    const int64_t o = __shfl_xor(v, d);
+    v = v > o ? v : o;
+  }
+  return v;
+}
 
 // In-bounds test of the kernels and trim_ends (src/compute/affine.rs:338-344,
 // src/compute/mod.rs:14-49).
@@ -105,15 +124,32 @@ static __device__ __forceinline__ int backward_matches(const uint64_t* P, const 
 // ---------------------------------------------------------------------
 // Row store: ring of rows, each `wcap` cells, row r at ring + r*wcap.
 // ---------------------------------------------------------------------
+// Offsets are held as RT: int32, or int16 for pairs short enough. Only
+// the sign of an offset derived from OFFSET_NULL ever matters (a valid
+// offset, >= 0, beats it in every maximum; M is re-bounded to NULL; the
+// extension skips it), so int16 keeps valid offsets exactly and saturates
+// the NULL-derived ones at -16384, where adding the few +1s of later steps
+// keeps them negative.
+template <typename RT>
 struct Rows {
-  int32_t* ring;  // LDS or global
-  int2* meta;     // per row {lo, hi}; lo == META_ABSENT: never computed
+  RT* ring;       // LDS or global
+  // per row {lo, hi, init_min, init_max}: the limits the next steps use and
+  // the initialized range reads see (wider than [lo, hi] after a heuristic
+  // cut, whose cells stay readable, src/wavefront.rs:459-462);
+  // lo == META_ABSENT: never computed
+  int4* meta;
   int mask;       // wcap - 1
   int wcap;
+  __device__ __forceinline__ int at(int idx) const { return (int)ring[idx]; }
+  __device__ __forceinline__ void set(int idx, int v) const {
+    if constexpr (sizeof(RT) == 2) ring[idx] = (RT)imax(v, -16384);
+    else ring[idx] = v;
+  }
 };
 
-static __device__ __forceinline__ int rd(const Rows& R, int row, int lo, int hi, int k) {
-  return (k >= lo && k <= hi) ? R.ring[row * R.wcap + (k & R.mask)] : WFA_NULL;
+template <typename RT>
+static __device__ __forceinline__ int rd(const Rows<RT>& R, int row, int lo, int hi, int k) {
+  return (k >= lo && k <= hi) ? R.at(row * R.wcap + (k & R.mask)) : WFA_NULL;
 }
 
 // ---------------------------------------------------------------------
@@ -430,6 +466,71 @@ static __device__ int maxtrim_cigar(const KParams& p, const uint8_t* ops, int n,
   return max_off + 1;            // the skipped leading gap stays in front
 }
 
+static __device__ __forceinline__ int wave_sum(int v) {
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+// rescore_heuristic_cigar (src/aligner.rs:1605-1640) with cigar_wfa_score
+// (src/aligner.rs:2700-2719): with a heuristic the returned score of a
+// gap-affine/2p alignment is its CIGAR's score on the user penalties
+// (ends-free: a leading free gap and the trailing free ends not counted),
+// in the internal scale. `ops` holds the n ops at the head of the region.
+static __device__ int rescore_cigar(const KParams& p, const uint8_t* ops, int n, bool ef,
+                                    int pb_free, int tb_free, int trailing) {
+  const int lane = threadIdx.x;
+  int b = 0, e = n;
+  if (ef) {
+    e = imax(0, e - trailing);
+    const int op = n > 0 ? ops[0] : 0;
+    const int fr = op == 'I' ? tb_free : op == 'D' ? pb_free : 0;
+    while (b < fr && b < e && ops[b] == op) ++b;
+  }
+  if (e <= b) return 0;
+  int classic = 0, vh = 0;
+  if (p.metric == WFA_METRIC_GAP_AFFINE) {
+    // Cigar::score_gap_affine (src/cigar.rs:139-169): each op on its own,
+    // a gap op opening unless it follows the same op
+    for (int i = b + lane; i < e; i += 64) {
+      const int op = ops[i];
+      const int last = i > b ? ops[i - 1] : 0;
+      classic -= op == 'M' ? p.u_match
+               : op == 'X' ? p.u_x
+                           : p.u_e1 + (last == op ? 0 : p.u_o1);
+      vh += (op != 'I') + (op != 'D');
+    }
+    classic = wave_sum(classic);
+    vh = wave_sum(vh);
+  } else {
+    // Cigar::score_gap_affine2p (src/cigar.rs:171-202): runs, each gap run
+    // priced by the cheaper of the two gap models
+    auto run_cost = [&](int op, int l) -> int {
+      if (op == 'M') return p.u_match * l;
+      if (op == 'X') return p.u_x * l;
+      const int s1 = p.u_o1 + p.u_e1 * l, s2 = p.u_o2 + p.u_e2 * l;
+      return s1 < s2 ? s1 : s2;
+    };
+    int last = 0, len = 0;
+    for (int c0 = b; c0 < e; c0 += 64) {
+      const int mine = c0 + lane < e ? ops[c0 + lane] : 0;
+      vh += c0 + lane < e ? (mine != 'I') + (mine != 'D') : 0;
+      const int cnt = imin(64, e - c0);
+      for (int j = 0; j < cnt; ++j) {
+        const int op = uniform(__shfl(mine, j));
+        if (op != last && last != 0) {
+          classic -= run_cost(last, len);
+          len = 0;
+        }
+        last = op;
+        ++len;
+      }
+    }
+    classic -= run_cost(last, len);
+    vh = wave_sum(vh);
+  }
+  return p.u_match < 0 ? -p.u_match * vh - 2 * classic : -classic;
+}
+
 // Medium/low memory: the chain of operations the reference's piggyback
 // kernels push (src/compute/piggyback.rs:195-528), recovered by following
 // the stored winner flags from the end cell back to M at score 0, then
@@ -548,9 +649,9 @@ static __device__ int high_mem_nwf(const KParams& p, int plen, int tlen) {
 // ---------------------------------------------------------------------
 // The per-pair forward pass + backtrace.
 // ---------------------------------------------------------------------
-template <int KIND, int MODE, int BITS>
+template <int KIND, int MODE, int BITS, typename RT, bool HEUR>
 static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen, uint64_t* sP,
-                                  uint64_t* sT, const Rows& R, const Hist& H) {
+                                  uint64_t* sT, const Rows<RT>& R, const Hist& H) {
   // MODE 0: score only; 1: high memory (M rows kept); 2: medium/low memory
   // (winner flags kept)
   constexpr bool CIG = MODE != 0;
@@ -562,7 +663,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   const int rI1 = ringM, rD1 = ringM + ringE1, rI2 = ringM + 2 * ringE1,
             rD2 = ringM + 2 * ringE1 + ringE2;
   const int nrows = KIND == KIND_EDIT ? ringM : two ? rD2 + ringE2 : rD1 + ringE1;
-  for (int r = lane; r < nrows; r += 64) R.meta[r] = make_int2(META_ABSENT, META_ABSENT - 1);
+  for (int r = lane; r < nrows; r += 64) R.meta[r] = meta_absent();
   // ends-free limits of this pair (extension: free ends up to the full
   // lengths, no begin gaps; src/aligner.rs:713-718)
   const bool ef = p.endsfree;
@@ -576,10 +677,10 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   const int lo0 = -pb_free, hi0 = tb_free;
   bool overflow = hi0 - lo0 + 1 > R.wcap;
   if (!overflow) {
-    for (int k = lo0 + lane; k <= hi0; k += 64) R.ring[k & R.mask] = k > 0 ? k : 0;
+    for (int k = lo0 + lane; k <= hi0; k += 64) R.set(k & R.mask, k > 0 ? k : 0);
   }
   __syncthreads();
-  if (lane == 0) R.meta[0] = make_int2(lo0, hi0);
+  if (lane == 0) R.meta[0] = make_int4(lo0, hi0, lo0, hi0);
   __syncthreads();
 
   const int nwf = high_mem_nwf(p, plen, tlen);
@@ -605,11 +706,11 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       const int k = kb + lane;
       bool term = false;
       if (k <= hi) {
-        int32_t* cell = &R.ring[row * R.wcap + (k & R.mask)];
-        int off = *cell;
+        const int cell = row * R.wcap + (k & R.mask);
+        int off = R.at(cell);
         if (off >= 0) {
           off += lcp<BITS>(sP, sT, off - k, off, plen, tlen);
-          *cell = off;
+          R.set(cell, off);
         }
         if (!ef) {
           if (k == k_end && off >= tlen) reached = true;
@@ -619,7 +720,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           term = off >= 0 &&
                  ((off >= tlen && plen - v <= pe_free) || (v >= plen && tlen - off <= te_free));
         }
-        if (MODE == 1) H.cells[base + (k - lo)] = off;
+        if (MODE == 1) H.cells[base + (k - lo)] = off < 0 ? WFA_NULL : off;
       }
       if (ef && !hit) {
         // the lowest terminating diagonal wins (src/extend.rs:133-166)
@@ -628,7 +729,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         if (hit) {
           const int j = __builtin_ctzll(hit);
           end_k = hit_kb + j;
-          end_off = uniform(__shfl(R.ring[row * R.wcap + ((hit_kb + lane) & R.mask)], j));
+          end_off = uniform(__shfl(R.at(row * R.wcap + ((hit_kb + lane) & R.mask)), j));
         }
       }
     }
@@ -640,6 +741,157 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     return ballot(reached) != 0;
   };
 
+  // ---- heuristic cut-off of M[s] after its extension (apply_heuristic_cutoff,
+  // src/aligner.rs:1644-1766; heuristic_cutoff, src/heuristic.rs:167-525).
+  // The cut narrows M's limits; its cells stay readable within the
+  // initialized range (an emptied M reads as the null wavefront) and the
+  // gap components of the same score take M's new limits.
+  const int heur = HEUR ? p.heuristic : WFA_HEURISTIC_NONE;
+  int steps_wait = heur == WFA_HEURISTIC_WFADAPTIVE || heur == WFA_HEURISTIC_BANDED_ADAPTIVE
+                       ? p.hp2
+                       : heur == WFA_HEURISTIC_XDROP ? p.hp1 : 0;
+  int max_sw = 0;
+  bool have_max = false;
+  auto cutoff = [&](int s) {
+    const int rowM = s % ringM;
+    const int4 mm = R.meta[rowM];
+    const int lo = uniform(mm.x), hi = uniform(mm.y);
+    if (heur == WFA_HEURISTIC_BANDED_STATIC && lo >= p.hp0 && hi <= p.hp1) return;
+    if (lo > hi) return;
+    steps_wait -= 1;
+    int nlo = lo, nhi = hi;
+    auto cell = [&](int k) { return R.at(rowM * R.wcap + (k & R.mask)); };
+    // wf_distance_end2end (src/heuristic.rs:150-160)
+    auto dist = [&](int off, int k) -> int {
+      return off < 0 ? INT32_MAX : imax(plen - (off - k), tlen - off);
+    };
+    if (heur == WFA_HEURISTIC_WFADAPTIVE) {
+      // src/heuristic.rs:258-346
+      if (steps_wait > 0 || hi - lo + 1 < p.hp0) return;
+      int md = imax(plen, tlen);
+      for (int kb = lo; kb <= hi; kb += 64) {
+        const int k = kb + lane;
+        if (k <= hi) md = imin(md, dist(cell(k), k));
+      }
+      md = wave_min(md);
+      int fg = INT32_MAX, lg = INT32_MIN;
+      for (int kb = lo; kb <= hi; kb += 64) {
+        const int k = kb + lane;
+        const bool good = k <= hi && (int64_t)dist(cell(k), k) - md <= (int64_t)p.hp1;
+        const uint64_t b = ballot(good);
+        if (b) {
+          if (fg == INT32_MAX) fg = kb + __builtin_ctzll(b);
+          lg = kb + 63 - __builtin_clzll(b);
+        }
+      }
+      // the low end moves up to the first close diagonal, stopping at the
+      // end diagonal; then the high end down, stopping there or at the new lo
+      const int ak = tlen - plen;
+      const int top = imin(ak, hi);
+      nlo = top <= lo ? lo : imin(fg, top);
+      const int bottom = imax(ak, nlo);
+      nhi = hi <= bottom ? hi : imax(lg, bottom);
+      steps_wait = p.hp2;
+    } else if (heur == WFA_HEURISTIC_BANDED_STATIC) {
+      // src/heuristic.rs:400-407
+      nlo = imax(lo, p.hp0);
+      nhi = imin(hi, p.hp1);
+    } else if (heur == WFA_HEURISTIC_BANDED_ADAPTIVE) {
+      // src/heuristic.rs:355-394
+      if (steps_wait > 0) return;
+      const int len = hi - lo + 1;
+      if (len < 4) return;
+      const int maxlen = p.hp1 - p.hp0 + 1;
+      if (len > maxlen) {
+        const int leeway = (len - maxlen) / 2, q = len / 4;
+        const int d0 = dist(uniform(cell(lo)), lo);
+        const int d1 = dist(uniform(cell(lo + q)), lo + q);
+        const int d2 = dist(uniform(cell(lo + 2 * q)), lo + 2 * q);
+        const int d3 = dist(uniform(cell(hi)), hi);
+        int nl = lo;
+        if (d0 > d3) nl += leeway;
+        if (d1 > d2) nl += leeway;
+        nlo = imax(nl, lo);
+        nhi = imin(nl + maxlen - 1, hi);
+      }
+      steps_wait = p.hp2;
+    } else if (heur == WFA_HEURISTIC_XDROP) {
+      // src/heuristic.rs:410-448, 491-525: the Smith-Waterman-like score of
+      // each cell (free begin gaps not counted), cells more than `xdrop`
+      // below the best seen so far cut from both ends
+      if (steps_wait > 0) return;
+      const int swg = p.u_match < 0 ? -p.u_match : 1;
+      int64_t best = INT64_MIN;  // (sw << 32) | (~biased k): max sw, first k
+      int fg = INT32_MAX, lg = INT32_MIN;
+      for (int kb = lo; kb <= hi; kb += 64) {
+        const int k = kb + lane;
+        bool good = false;
+        if (k <= hi) {
+          const int off = cell(k);
+          if (off >= 0) {
+            const int fv = k < 0 ? imin(-k, pb_free) : 0;
+            const int fh = k > 0 ? imin(k, tb_free) : 0;
+            const int sw = (swg * ((off - k - fv) + (off - fh)) - s) / 2;
+            const int64_t key = ((int64_t)sw << 32) | (uint32_t)(0x7fffffffu - (uint32_t)k);
+            best = best > key ? best : key;
+            good = have_max && max_sw - sw < p.hp0;
+          }
+        }
+        const uint64_t b = ballot(good);
+        if (b) {
+          if (fg == INT32_MAX) fg = kb + __builtin_ctzll(b);
+          lg = kb + 63 - __builtin_clzll(b);
+        }
+      }
+      best = wave_max64(best);
+      const int cmax = (int)(best >> 32);
+      if (have_max) {
+        nlo = fg != INT32_MAX ? fg : hi + 1;
+        nhi = lg != INT32_MIN ? lg : hi;
+        if (cmax > max_sw) max_sw = cmax;
+      } else {
+        max_sw = cmax;
+        have_max = true;
+      }
+      steps_wait = p.hp1;
+    }
+    if (nlo == lo && nhi == hi) return;
+    const bool empty = nlo > nhi;
+    __syncthreads();  // every lane has read M's cells and metadata
+    if (lane == 0) {
+      // an emptied M is the null wavefront with an empty initialized range
+      R.meta[rowM] = empty ? make_int4(nlo, nhi, 1, 0) : make_int4(nlo, nhi, mm.z, mm.w);
+      if (KIND != KIND_EDIT) {
+        const int rowE = s % ringE1;
+        auto equate = [&](int row) {
+          const int4 g = R.meta[row];
+          if (g.x > g.y) return;
+          const int l = imax(g.x, nlo), h = imin(g.y, nhi);
+          R.meta[row] = l > h ? make_int4(l, h, 1, 0) : make_int4(l, h, g.z, g.w);
+        };
+        equate(rI1 + rowE);
+        equate(rD1 + rowE);
+        if (two) {
+          const int rowE2 = s % ringE2;
+          equate(rI2 + rowE2);
+          equate(rD2 + rowE2);
+        }
+      }
+      if (MODE == 1 && empty) H.hdr[s] = make_int4(1, 0, 0, 0);
+    }
+    if (MODE == 1 && KIND == KIND_EDIT && !empty) {
+      // the next edit step nulls the two cells beyond each end of M[s] in
+      // place (src/aligner.rs:1796-1806); inside the initialized range the
+      // backtrace sees them
+      const int4 hd = H.hdr[s];
+      if (lane < 4) {
+        const int k = lane < 2 ? nlo - 2 + lane : nhi + lane - 1;
+        if (k >= hd.x && k <= hd.y) H.cells[hd.z + (k - hd.x)] = WFA_NULL;
+      }
+    }
+    __syncthreads();
+  };
+
   // score 0
   {
     if (CIG && (H.hdr_cap < 1 || H.cap < hi0 - lo0 + 1)) overflow = true;
@@ -649,6 +901,8 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       status = WFA_STATUS_END_REACHED;
       result = 0;
     }
+    __syncthreads();
+    if (!overflow && status == WFA_STATUS_OK && heur != WFA_HEURISTIC_NONE) cutoff(0);
   }
   __syncthreads();
 
@@ -656,17 +910,29 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     score += 1;
     if (!p.modular && score >= nwf) { status = WFA_STATUS_MAX_STEPS_REACHED; result = score; break; }
     if (CIG && score >= H.hdr_cap) { overflow = true; break; }
+    // int16 rows hold offsets up to tlen + score: leave long runs to int32
+    if (sizeof(RT) == 2 && score > 16000) { overflow = true; break; }
     const int s = score;
     // ---- input rows ----
     int lo = INT32_MAX, hi = INT32_MIN;
+    // limits (ml/mh, il/ih, ...) and initialized ranges (gx..., the read guards)
     int ml_x = 1, mh_x = 0, ml_o = 1, mh_o = 0, il = 1, ih = 0, dl = 1, dh = 0;
     int ml_o2 = 1, mh_o2 = 0, il2 = 1, ih2 = 0, dl2 = 1, dh2 = 0;
+    int gl_x = 1, gh_x = 0, gl_o = 1, gh_o = 0, gil = 1, gih = 0, gdl = 1, gdh = 0;
+    int gl_o2 = 1, gh_o2 = 0, gil2 = 1, gih2 = 0, gdl2 = 1, gdh2 = 0;
     int row_x = 0, row_o = 0, row_e = 0, row_o2 = 0, row_e2 = 0;
     bool any = false;
+    auto load = [&](int row, int& l, int& h, int& gl, int& gh) {
+      const int4 m = R.meta[row];
+      l = uniform(m.x); h = uniform(m.y);
+      gl = uniform(m.z); gh = uniform(m.w);
+    };
     if (KIND == KIND_EDIT) {
       row_x = (s - 1) % ringM;
-      const int2 m = R.meta[row_x];
-      ml_x = uniform(m.x); mh_x = uniform(m.y);
+      load(row_x, ml_x, mh_x, gl_x, gh_x);
+      // the edit step nulls prev's cells next to [lo, hi] before reading
+      // them (src/aligner.rs:1795-1808): reads outside [lo, hi] are NULL
+      gl_x = ml_x; gh_x = mh_x;
       if (ml_x != META_ABSENT && ml_x <= mh_x) {
         any = true;
         lo = ml_x - 1;
@@ -675,44 +941,60 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     } else {
       if (s - p.x >= 0) {
         row_x = (s - p.x) % ringM;
-        const int2 m = R.meta[row_x];
-        ml_x = uniform(m.x); mh_x = uniform(m.y);
+        load(row_x, ml_x, mh_x, gl_x, gh_x);
       }
       if (s - p.oe1 >= 0) {
         row_o = (s - p.oe1) % ringM;
-        const int2 m = R.meta[row_o];
-        ml_o = uniform(m.x); mh_o = uniform(m.y);
+        load(row_o, ml_o, mh_o, gl_o, gh_o);
       }
       if (s - p.e1 >= 0) {
         row_e = (s - p.e1) % ringE1;
-        const int2 a = R.meta[rI1 + row_e];
-        const int2 b = R.meta[rD1 + row_e];
-        il = uniform(a.x); ih = uniform(a.y); dl = uniform(b.x); dh = uniform(b.y);
+        load(rI1 + row_e, il, ih, gil, gih);
+        load(rD1 + row_e, dl, dh, gdl, gdh);
       }
       if (two) {
         if (s - p.oe2 >= 0) {
           row_o2 = (s - p.oe2) % ringM;
-          const int2 m = R.meta[row_o2];
-          ml_o2 = uniform(m.x); mh_o2 = uniform(m.y);
+          load(row_o2, ml_o2, mh_o2, gl_o2, gh_o2);
         }
         if (s - p.e2 >= 0) {
           row_e2 = (s - p.e2) % ringE2;
-          const int2 a = R.meta[rI2 + row_e2];
-          const int2 b = R.meta[rD2 + row_e2];
-          il2 = uniform(a.x); ih2 = uniform(a.y); dl2 = uniform(b.x); dh2 = uniform(b.y);
+          load(rI2 + row_e2, il2, ih2, gil2, gih2);
+          load(rD2 + row_e2, dl2, dh2, gdl2, gdh2);
         }
       }
-      // absent rows (META_ABSENT, hi = META_ABSENT-1) and trimmed-empty rows
-      // both have lo > hi and read as the null wavefront.
-      auto live = [](int l, int h) { return l <= h; };
-      if (live(ml_x, mh_x)) { any = true; lo = imin(lo, ml_x); hi = imax(hi, mh_x); }
-      if (live(ml_o, mh_o)) { any = true; lo = imin(lo, ml_o - 1); hi = imax(hi, mh_o + 1); }
-      if (live(il, ih)) { any = true; lo = imin(lo, il + 1); hi = imax(hi, ih + 1); }
-      if (live(dl, dh)) { any = true; lo = imin(lo, dl - 1); hi = imax(hi, dh - 1); }
+      // an input whose limits are empty is the null wavefront; so are its reads
+      auto ng = [](int l, int h, int& gl, int& gh) { if (l > h) { gl = 1; gh = 0; } };
+      ng(ml_x, mh_x, gl_x, gh_x); ng(ml_o, mh_o, gl_o, gh_o); ng(il, ih, gil, gih);
+      ng(dl, dh, gdl, gdh); ng(ml_o2, mh_o2, gl_o2, gh_o2); ng(il2, ih2, gil2, gih2);
+      ng(dl2, dh2, gdl2, gdh2);
+      // Absent rows (META_ABSENT, hi = META_ABSENT-1), missing inputs and
+      // trimmed- or cut-empty rows all have lo > hi and read as the null
+      // wavefront, whose bounds (1, -1) still enter the range union
+      // (src/aligner.rs:2045-2062, 2265-2289). Without a heuristic the extra
+      // cells only see NULL and are trimmed away, so those kernels leave the
+      // null bounds out; after a cut they can read the cut cells.
+      auto bound = [&](int l, int h, int dlo, int dhi) {
+        if (l <= h) any = true;
+        else if (!HEUR) return;
+        else { l = 1; h = -1; }
+        lo = imin(lo, l + dlo);
+        hi = imax(hi, h + dhi);
+      };
+      bound(ml_x, mh_x, 0, 0);
+      bound(ml_o, mh_o, -1, 1);
+      bound(il, ih, 1, 1);
+      bound(dl, dh, -1, -1);
       if (two) {
-        if (live(ml_o2, mh_o2)) { any = true; lo = imin(lo, ml_o2 - 1); hi = imax(hi, mh_o2 + 1); }
-        if (live(il2, ih2)) { any = true; lo = imin(lo, il2 + 1); hi = imax(hi, ih2 + 1); }
-        if (live(dl2, dh2)) { any = true; lo = imin(lo, dl2 - 1); hi = imax(hi, dh2 - 1); }
+        bound(ml_o2, mh_o2, -1, 1);
+        bound(il2, ih2, 1, 1);
+        bound(dl2, dh2, -1, -1);
+      }
+      // clamp_to_band (src/aligner.rs:3308-3316); an emptied range takes
+      // the no-output path below
+      if (heur == WFA_HEURISTIC_BANDED_STATIC || heur == WFA_HEURISTIC_BANDED_ADAPTIVE) {
+        lo = imax(lo, p.hp0);
+        hi = imin(hi, p.hp1);
       }
     }
     const int rowM = s % ringM;
@@ -723,14 +1005,14 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       // null step (src/aligner.rs:2031-2043, 1778-1782)
       nulls += 1;
       if (lane == 0) {
-        R.meta[rowM] = make_int2(META_ABSENT, META_ABSENT - 1);
+        R.meta[rowM] = meta_absent();
         if (KIND != KIND_EDIT) {
-          R.meta[rI1 + rowE] = make_int2(META_ABSENT, META_ABSENT - 1);
-          R.meta[rD1 + rowE] = make_int2(META_ABSENT, META_ABSENT - 1);
+          R.meta[rI1 + rowE] = meta_absent();
+          R.meta[rD1 + rowE] = meta_absent();
         }
         if (two) {
-          R.meta[rI2 + rowE2] = make_int2(META_ABSENT, META_ABSENT - 1);
-          R.meta[rD2 + rowE2] = make_int2(META_ABSENT, META_ABSENT - 1);
+          R.meta[rI2 + rowE2] = meta_absent();
+          R.meta[rD2 + rowE2] = meta_absent();
         }
         if (CIG) H.hdr[s] = make_int4(1, 0, 0, 0);
       }
@@ -743,9 +1025,18 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     lo = imax(lo, hist_lo + 1);
     hi = imin(hi, hist_hi - 1);
     if (lo > hi) {
-      // cannot happen without a band; keep the reference's "no output" rule
+      // a band (or the historic limits) left nothing to compute: no outputs
+      // at this score (src/aligner.rs:2058-2081)
       if (lane == 0) {
-        R.meta[rowM] = make_int2(META_ABSENT, META_ABSENT - 1);
+        R.meta[rowM] = meta_absent();
+        if (KIND != KIND_EDIT) {
+          R.meta[rI1 + rowE] = meta_absent();
+          R.meta[rD1 + rowE] = meta_absent();
+        }
+        if (two) {
+          R.meta[rI2 + rowE2] = meta_absent();
+          R.meta[rD2 + rowE2] = meta_absent();
+        }
         if (CIG) H.hdr[s] = make_int4(1, 0, 0, 0);
       }
       __syncthreads();
@@ -767,13 +1058,13 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         int m;
         int fl = 0;
         if (KIND == KIND_EDIT) {
-          const int ins = rd(R, row_x, ml_x, mh_x, k - 1);
-          const int del = rd(R, row_x, ml_x, mh_x, k + 1);
+          const int ins = rd(R, row_x, gl_x, gh_x, k - 1);
+          const int del = rd(R, row_x, gl_x, gh_x, k + 1);
           if (p.metric == WFA_METRIC_INDEL) {
             m = imax(del, ins + 1);
             if (MED) fl = m == del ? 1 : 0;
           } else {
-            const int mis = rd(R, row_x, ml_x, mh_x, k);
+            const int mis = rd(R, row_x, gl_x, gh_x, k);
             m = imax(del, imax(ins, mis) + 1);
             // piggyback priority (src/compute/piggyback.rs:218-224):
             // mismatch, then deletion, then insertion
@@ -781,26 +1072,26 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           }
           if (!in_bounds(m, k, plen, tlen)) m = WFA_NULL;
         } else {
-          const int io = rd(R, row_o, ml_o, mh_o, k - 1), ie = rd(R, rI1 + row_e, il, ih, k - 1);
-          const int dop = rd(R, row_o, ml_o, mh_o, k + 1), de = rd(R, rD1 + row_e, dl, dh, k + 1);
+          const int io = rd(R, row_o, gl_o, gh_o, k - 1), ie = rd(R, rI1 + row_e, gil, gih, k - 1);
+          const int dop = rd(R, row_o, gl_o, gh_o, k + 1), de = rd(R, rD1 + row_e, gdl, gdh, k + 1);
           const int i1 = imax(io, ie) + 1;
           const int d1 = imax(dop, de);
-          const int mis = rd(R, row_x, ml_x, mh_x, k) + 1;
+          const int mis = rd(R, row_x, gl_x, gh_x, k) + 1;
           // a gap cell takes its extension input on ties
           // (src/compute/piggyback.rs:146-160)
           if (MED) fl |= (ie >= io ? FL_I1_EXT : 0) | (de >= dop ? FL_D1_EXT : 0);
           int ins = i1, del = d1;
           int i2 = WFA_NULL, d2 = WFA_NULL;
           if (two) {
-            const int io2 = rd(R, row_o2, ml_o2, mh_o2, k - 1);
-            const int ie2 = rd(R, rI2 + row_e2, il2, ih2, k - 1);
-            const int do2 = rd(R, row_o2, ml_o2, mh_o2, k + 1);
-            const int de2 = rd(R, rD2 + row_e2, dl2, dh2, k + 1);
+            const int io2 = rd(R, row_o2, gl_o2, gh_o2, k - 1);
+            const int ie2 = rd(R, rI2 + row_e2, gil2, gih2, k - 1);
+            const int do2 = rd(R, row_o2, gl_o2, gh_o2, k + 1);
+            const int de2 = rd(R, rD2 + row_e2, gdl2, gdh2, k + 1);
             i2 = imax(io2, ie2) + 1;
             d2 = imax(do2, de2);
             if (MED) fl |= (ie2 >= io2 ? FL_I2_EXT : 0) | (de2 >= do2 ? FL_D2_EXT : 0);
-            R.ring[(rI2 + rowE2) * R.wcap + (k & R.mask)] = i2;
-            R.ring[(rD2 + rowE2) * R.wcap + (k & R.mask)] = d2;
+            R.set((rI2 + rowE2) * R.wcap + (k & R.mask), i2);
+            R.set((rD2 + rowE2) * R.wcap + (k & R.mask), d2);
             vi2 = in_bounds(i2, k, plen, tlen);
             vd2 = in_bounds(d2, k, plen, tlen);
             ins = imax(ins, i2);
@@ -818,13 +1109,13 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
             fl |= w;
           }
           if (!in_bounds(m, k, plen, tlen)) m = WFA_NULL;
-          R.ring[(rI1 + rowE) * R.wcap + (k & R.mask)] = i1;
-          R.ring[(rD1 + rowE) * R.wcap + (k & R.mask)] = d1;
+          R.set((rI1 + rowE) * R.wcap + (k & R.mask), i1);
+          R.set((rD1 + rowE) * R.wcap + (k & R.mask), d1);
           vi = in_bounds(i1, k, plen, tlen);
           vd = in_bounds(d1, k, plen, tlen);
         }
         vm = m != WFA_NULL;
-        R.ring[rowM * R.wcap + (k & R.mask)] = m;
+        R.set(rowM * R.wcap + (k & R.mask), m);
         if (MED) H.bytes[fbase + (k - lo)] = (uint8_t)fl;
       }
       uint64_t b = ballot(vm);
@@ -866,14 +1157,14 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       hpos += hi - lo + 1;
     }
     if (lane == 0) {
-      R.meta[rowM] = make_int2(m_lo, m_hi);
+      R.meta[rowM] = make_int4(m_lo, m_hi, m_lo, m_hi);
       if (KIND != KIND_EDIT) {
-        R.meta[rI1 + rowE] = make_int2(i_lo, i_hi);
-        R.meta[rD1 + rowE] = make_int2(d_lo, d_hi);
+        R.meta[rI1 + rowE] = make_int4(i_lo, i_hi, i_lo, i_hi);
+        R.meta[rD1 + rowE] = make_int4(d_lo, d_hi, d_lo, d_hi);
       }
       if (two) {
-        R.meta[rI2 + rowE2] = make_int2(i2_lo, i2_hi);
-        R.meta[rD2 + rowE2] = make_int2(d2_lo, d2_hi);
+        R.meta[rI2 + rowE2] = make_int4(i2_lo, i2_hi, i2_lo, i2_hi);
+        R.meta[rD2 + rowE2] = make_int4(d2_lo, d2_hi, d2_lo, d2_hi);
       }
     }
     if (s >= p.max_steps) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; __syncthreads(); break; }
@@ -886,6 +1177,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     const bool reached = extend_row(s, elo, ehi) && (ef || (m_lo <= k_end && k_end <= m_hi));
     __syncthreads();
     if (reached) { status = WFA_STATUS_END_REACHED; result = s; break; }
+    if (heur != WFA_HEURISTIC_NONE) cutoff(s);
   }
 
   if (overflow) {
@@ -936,8 +1228,16 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         bool trimmed = false;
         ncig = maxtrim_cigar(p, out, ncig, pb_free, tb_free, &trimmed);
         if (trimmed) out_status = WFA_STATUS_PARTIAL;
+      } else if (heur != WFA_HEURISTIC_NONE && KIND != KIND_EDIT) {
+        __threadfence_block();
+        const int trailing = ef ? (plen - (end_off - end_k)) + (tlen - end_off) : 0;
+        result = rescore_cigar(p, out, ncig, ef, pb_free, tb_free, trailing);
       }
     }
+  } else if (status == WFA_STATUS_END_UNREACHABLE && ef) {
+    // an ends-free alignment that ran out of wavefront is partial
+    // (src/aligner.rs:819-825)
+    out_status = WFA_STATUS_PARTIAL;
   }
   if (lane == 0) {
     p.score[pair] = result;
@@ -946,20 +1246,21 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   }
 }
 
-template <int KIND, int MODE, bool LDS_RING>
-__global__ void __launch_bounds__(64) wfa_align_kernel(KParams p) {
+template <int KIND, int MODE, bool LDS_RING, bool I16, bool PACKED, bool HEUR>
+__global__ void __launch_bounds__(64, 5) wfa_align_kernel(KParams p) {
+  using RT = typename std::conditional<I16, int16_t, int32_t>::type;
   constexpr bool CIG = MODE != 0;
   extern __shared__ uint64_t smem[];
   const int lane = threadIdx.x;
   uint64_t* sP = smem;
   uint64_t* sT = smem + p.seq_words;
-  int2* meta = (int2*)(sT + p.seq_words);
-  Rows R;
+  int4* meta = (int4*)(sT + p.seq_words);
+  Rows<RT> R;
   R.meta = meta;
   R.wcap = p.wcap;
   R.mask = p.wcap - 1;
-  if (LDS_RING) R.ring = (int32_t*)(meta + p.nrows_meta);
-  else R.ring = p.gring + (int64_t)blockIdx.x * p.ring_rows * p.wcap;
+  if constexpr (LDS_RING) R.ring = (RT*)(meta + p.nrows_meta);
+  else R.ring = (RT*)(p.gring + (int64_t)blockIdx.x * p.ring_rows * p.wcap);
   Hist H;
   H.hdr = CIG ? p.hdr + (int64_t)blockIdx.x * p.hdr_cap : nullptr;
   H.cells = MODE == 1 ? p.hist + (int64_t)blockIdx.x * p.hist_cap : nullptr;
@@ -984,72 +1285,38 @@ __global__ void __launch_bounds__(64) wfa_align_kernel(KParams p) {
       }
       continue;
     }
-    if (bits == 2) align_pair<KIND, MODE, 2>(p, pair, plen, tlen, sP, sT, R, H);
-    else align_pair<KIND, MODE, 8>(p, pair, plen, tlen, sP, sT, R, H);
+    // 2-bit packed input never needs the byte-wise path
+    if (PACKED || bits == 2)
+      align_pair<KIND, MODE, 2, RT, HEUR>(p, pair, plen, tlen, sP, sT, R, H);
+    else
+      align_pair<KIND, MODE, 8, RT, HEUR>(p, pair, plen, tlen, sP, sT, R, H);
     __syncthreads();
   }
 }
 
-__global__ void wfa_reset_counters(int32_t* c) {
-  if (threadIdx.x < 4) c[threadIdx.x] = 0;
-}
-
-// out[0] = max over pairs of max(plen, tlen); out must be zeroed first.
-__global__ void wfa_max_length(const int32_t* plen, const int32_t* tlen, int n, int32_t* out) {
-  int m = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    m = imax(m, imax(plen[i], tlen[i]));
-  for (int o = 32; o > 0; o >>= 1) m = imax(m, __shfl_xor(m, o));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
-}
-
 // ---------------------------------------------------------------------
-// Host-side launch helpers
+// Instantiation: every storage/format/heuristic variant of one kind
 // ---------------------------------------------------------------------
-using KernelFn = void (*)(KParams);
-
-static KernelFn pick(int kind, int mode, bool lds) {
-#define PICK(K)                                                                    \
-  if (kind == K) {                                                                 \
-    if (mode == 1) return lds ? wfa_align_kernel<K, 1, true> : wfa_align_kernel<K, 1, false>; \
-    if (mode == 2) return lds ? wfa_align_kernel<K, 2, true> : wfa_align_kernel<K, 2, false>; \
-    return lds ? wfa_align_kernel<K, 0, true> : wfa_align_kernel<K, 0, false>;     \
-  }
-  PICK(KIND_EDIT)
-  PICK(KIND_AFFINE)
-  PICK(KIND_AFFINE2P)
-#undef PICK
-  return nullptr;
+template <int K, int M, bool HEUR>
+static KernelFn pick_kmh(bool lds, bool i16, bool packed) {
+  if (!lds) return packed ? wfa_align_kernel<K, M, false, false, true, HEUR>
+                          : wfa_align_kernel<K, M, false, false, false, HEUR>;
+  if (i16) return packed ? wfa_align_kernel<K, M, true, true, true, HEUR>
+                         : wfa_align_kernel<K, M, true, true, false, HEUR>;
+  return packed ? wfa_align_kernel<K, M, true, false, true, HEUR>
+                : wfa_align_kernel<K, M, true, false, false, HEUR>;
 }
 
-hipError_t launch_reset(int32_t* counters, hipStream_t stream) {
-  hipLaunchKernelGGL(wfa_reset_counters, dim3(1), dim3(64), 0, stream, counters);
-  return hipGetLastError();
+template <int K, int M>
+static KernelFn pick_km(bool lds, bool i16, bool packed, bool heur) {
+  return heur ? pick_kmh<K, M, true>(lds, i16, packed) : pick_kmh<K, M, false>(lds, i16, packed);
 }
 
-hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, int32_t* out,
-                             hipStream_t stream) {
-  hipLaunchKernelGGL(wfa_reset_counters, dim3(1), dim3(64), 0, stream, out);
-  int grid = (n + 255) / 256;
-  if (grid > 1024) grid = 1024;
-  if (grid < 1) grid = 1;
-  hipLaunchKernelGGL(wfa_max_length, dim3(grid), dim3(256), 0, stream, plen, tlen, n, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, int grid,
-                        size_t lds_bytes, hipStream_t stream) {
-  KernelFn fn = pick(kind, mode, lds_ring);
-  if (!fn) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(fn, dim3(grid), dim3(64), lds_bytes, stream, p);
-  return hipGetLastError();
-}
-
-int max_blocks_per_cu(int kind, int mode, bool lds_ring, size_t lds_bytes) {
-  KernelFn fn = pick(kind, mode, lds_ring);
-  int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, lds_bytes) != hipSuccess) return 1;
-  return n > 0 ? n : 1;
+template <int K>
+static KernelFn pick_kind(int mode, bool lds, bool i16, bool packed, bool heur) {
+  if (mode == 1) return pick_km<K, 1>(lds, i16, packed, heur);
+  if (mode == 2) return pick_km<K, 2>(lds, i16, packed, heur);
+  return pick_km<K, 0>(lds, i16, packed, heur);
 }
 
 }  // namespace wfa

@@ -95,7 +95,11 @@ int check_supported(const wfa_attr_t& a) {
       (a.pattern_begin_free < 0 || a.pattern_end_free < 0 || a.text_begin_free < 0 ||
        a.text_end_free < 0))
     return WFA_ERR_INVALID_ARGUMENT;
-  if (a.heuristic != WFA_HEURISTIC_NONE) return WFA_ERR_UNSUPPORTED;
+  // Z-drop ends alignments early through its own backtrace path: not on
+  // the device path
+  if (a.heuristic < WFA_HEURISTIC_NONE || a.heuristic > WFA_HEURISTIC_BANDED_ADAPTIVE)
+    return WFA_ERR_INVALID_ARGUMENT;
+  if (a.heuristic == WFA_HEURISTIC_ZDROP) return WFA_ERR_UNSUPPORTED;
   if (a.alignment_scope != WFA_SCOPE_SCORE && a.alignment_scope != WFA_SCOPE_ALIGNMENT)
     return WFA_ERR_INVALID_ARGUMENT;
   if (a.memory_mode < WFA_MEMORY_HIGH || a.memory_mode > WFA_MEMORY_ULTRALOW)
@@ -161,6 +165,7 @@ struct LaunchCfg {
   int mode;  // 0 score only, 1 high memory CIGAR, 2 medium/low memory CIGAR
   bool cig;
   bool lds_ring;
+  bool i16;
   int wcap;
   int seq_words;
   int ring_rows;
@@ -209,6 +214,10 @@ void base_params(const Internal& ip, const wfa_attr_t& attr, const wfa_batch_t& 
   p->u_e1 = attr.penalties.gap_extension1;
   p->u_o2 = attr.penalties.gap_opening2;
   p->u_e2 = attr.penalties.gap_extension2;
+  p->heuristic = attr.heuristic;
+  p->hp0 = attr.heuristic_p[0];
+  p->hp1 = attr.heuristic_p[1];
+  p->hp2 = attr.heuristic_p[2];
   p->fmt = b.seq_format;
   p->seq = b.seq_data;
   p->poff = b.pattern_offset;
@@ -238,16 +247,23 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   c->cig = cig;
   c->ring_rows = ring_rows_of(p, kind);
   const int per = fmt == WFA_SEQ_PACKED2BITS ? 32 : 8;
-  c->seq_words = (max_len + per - 1) / per + 2;
+  // even, so the per-row int4 metadata after the two sequences is 16-byte aligned
+  c->seq_words = ((max_len + per - 1) / per + 2 + 1) & ~1;
   // the widest possible computed range: [-(plen+scope+1), tlen+scope+1]
   const int full = pow2ceil(2 * max_len + 2 * p.scope + 4);
   c->wcap = pass == 0 ? std::min(256, full) : full;
-  const size_t seq_meta = (size_t)2 * c->seq_words * 8 + (size_t)c->ring_rows * 8;
-  const size_t ring_bytes = (size_t)c->ring_rows * c->wcap * 4;
+  const size_t seq_meta = (size_t)2 * c->seq_words * 8 + (size_t)c->ring_rows * 16;
+  // 16-bit offsets for pairs of at most 8000 bases (the kernel re-runs any
+  // pair whose score passes 16000 with 32-bit offsets)
+  c->i16 = max_len <= 8000 && pass < 2;
+  const size_t ring_bytes = (size_t)c->ring_rows * c->wcap * (c->i16 ? 2 : 4);
   c->lds_ring = seq_meta + ring_bytes <= 64 * 1024;
+  if (!c->lds_ring) c->i16 = false;
   c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : 0);
   if (c->lds_bytes > e->lds_max) return WFA_ERR_UNSUPPORTED;
-  const int per_cu = wfa::max_blocks_per_cu(kind, mode, c->lds_ring, c->lds_bytes);
+  const int per_cu =
+      wfa::max_blocks_per_cu(kind, mode, c->lds_ring, c->i16, fmt == WFA_SEQ_PACKED2BITS,
+                             p.heuristic != WFA_HEURISTIC_NONE, c->lds_bytes);
   int grid = e->num_cus * per_cu;
   if (pass > 0) grid = std::min(grid, e->num_cus * 2);
   grid = std::max(1, std::min(grid, npairs));
@@ -312,7 +328,7 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
     }
   }
   HIPCHK(wfa::launch_reset(e->d_counters, st));
-  HIPCHK(wfa::launch_align(p, c.kind, c.mode, c.lds_ring, c.grid, c.lds_bytes, st));
+  HIPCHK(wfa::launch_align(p, c.kind, c.mode, c.lds_ring, c.i16, c.grid, c.lds_bytes, st));
   return WFA_OK;
 }
 

@@ -22,8 +22,11 @@ struct KParams {
   // span (ends-free with match == 0; extension)
   int endsfree, extension;
   int pb_free, pe_free, tb_free, te_free;
-  // user penalties (max-trim of extension alignments scores with them)
+  // user penalties (max-trim of extension alignments and the heuristic
+  // rescoring score with them)
   int u_match, u_x, u_o1, u_e1, u_o2, u_e2;
+  // heuristic (WFA_HEURISTIC_*) and its three parameters
+  int heuristic, hp0, hp1, hp2;
   // pairs
   int num_pairs;             // work items in this launch
   int pair_base;             // first pair when pair_list is null
@@ -56,12 +59,21 @@ struct KParams {
   int chain_cap;
 };
 
+using KernelFn = void (*)(KParams);
+// the alignment kernel for one configuration, from the kind's kernel set
+// (mode: 0 score only, 1 high memory CIGAR, 2 medium/low memory CIGAR)
+KernelFn pick_edit(int mode, bool lds, bool i16, bool packed, bool heur);
+KernelFn pick_affine(int mode, bool lds, bool i16, bool packed, bool heur);
+KernelFn pick_affine2p(int mode, bool lds, bool i16, bool packed, bool heur);
+
 hipError_t launch_reset(int32_t* counters, hipStream_t stream);
 hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, int32_t* out,
                              hipStream_t stream);
 // mode: 0 score only, 1 high memory CIGAR, 2 medium/low memory CIGAR
-hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, int grid,
+// i16: ring offsets held in 16 bits (pairs of at most 8000 bases)
+hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, bool i16, int grid,
                         size_t lds_bytes, hipStream_t stream);
-int max_blocks_per_cu(int kind, int mode, bool lds_ring, size_t lds_bytes);
+int max_blocks_per_cu(int kind, int mode, bool lds_ring, bool i16, bool packed, bool heur,
+                      size_t lds_bytes);
 
 }  // namespace wfa

@@ -1,0 +1,11 @@
+// wfa_kernels_affine2p.hip — the affine2p kernel set (one translation unit per
+// kind so the sets compile in parallel).
+#include "wfa_align.h"
+
+namespace wfa {
+
+KernelFn pick_affine2p(int mode, bool lds, bool i16, bool packed, bool heur) {
+  return pick_kind<KIND_AFFINE2P>(mode, lds, i16, packed, heur);
+}
+
+}  // namespace wfa
