@@ -34,6 +34,10 @@ def lib(attr_type):
         l.wfo_align_end2end.argtypes = [VP, ctypes.c_char_p, I32, ctypes.c_char_p, I32]
         l.wfo_align_endsfree.restype = I32
         l.wfo_align_endsfree.argtypes = [VP, ctypes.c_char_p, I32, ctypes.c_char_p, I32]
+        for name in ("wfo_align_biwfa", "wfo_align_biwfa_score"):
+            f = getattr(l, name)
+            f.restype = I32
+            f.argtypes = [VP, ctypes.c_char_p, I32, ctypes.c_char_p, I32]
         l.wfo_status.restype = I32
         l.wfo_status.argtypes = [VP]
         l.wfo_cigar.restype = I32
@@ -63,6 +67,8 @@ class OracleAligner:
         err = ctypes.c_int32(0)
         self.h = self.L.wfo_aligner_new(ctypes.byref(attr), ctypes.byref(err))
         self.endsfree = attr.span == 1
+        self.biwfa = attr.memory_mode == 3
+        self.score_only = attr.alignment_scope == 0
         if not self.h:
             raise ValueError(f"oracle rejected the configuration ({err.value})")
 
@@ -72,7 +78,11 @@ class OracleAligner:
             self.h = None
 
     def align(self, pattern: bytes, text: bytes):
-        """align_endsfree when the attribute's span is ends-free, else end-to-end."""
+        """align_biwfa / align_biwfa_score in ultralow memory, align_endsfree
+        when the attribute's span is ends-free, else end-to-end."""
+        if self.biwfa:
+            f = self.L.wfo_align_biwfa_score if self.score_only else self.L.wfo_align_biwfa
+            return self._finish(f(self.h, pattern, len(pattern), text, len(text)))
         if self.endsfree:
             return self._finish(self.L.wfo_align_endsfree(self.h, pattern, len(pattern), text,
                                                           len(text)))
@@ -81,6 +91,10 @@ class OracleAligner:
     def align_end2end(self, pattern: bytes, text: bytes):
         return self._finish(self.L.wfo_align_end2end(self.h, pattern, len(pattern), text,
                                                      len(text)))
+
+    def cigar_score(self):
+        """The CIGAR's score field (Cigar.score) after the last alignment."""
+        return self.L.wfo_cigar_score(self.h)
 
     def _finish(self, s):
         st = self.L.wfo_status(self.h)

@@ -238,6 +238,9 @@ struct wfo_aligner {
     /* heuristic state (src/heuristic.rs:60-143) */
     int h_steps_wait, h_max_sw_score, h_max_sw_k;
     int32_t* dist; int cap_dist;
+    /* BiWFA sub-aligners (forward, reverse: score-only; base: CIGAR),
+     * created on first use (src/aligner.rs:2721-2740) */
+    struct wfo_aligner *bi_fwd, *bi_rev, *bi_base;
     /* span (src/aligner.rs:163-170): ends-free limits of this alignment */
     int endsfree, pb_free, pe_free, tb_free, te_free;
     uint32_t *blk_pc, *blk_prev; int nblk, cap_blk;
@@ -345,7 +348,8 @@ wfo_aligner* wfo_aligner_new(const wfa_attr_t* attr, int32_t* err) {
     }
     if ((attr->span == WFA_SPAN_ENDSFREE && a->pen.match != 0) ||
         attr->heuristic == WFA_HEURISTIC_ZDROP ||
-        attr->memory_mode == WFA_MEMORY_ULTRALOW ||
+        (attr->memory_mode == WFA_MEMORY_ULTRALOW &&
+         (attr->span != WFA_SPAN_END2END || attr->heuristic == WFA_HEURISTIC_XDROP)) ||
         attr->penalties.distance_metric == WFA_METRIC_GAP_LINEAR) {
         if (err) *err = WFA_ERR_UNSUPPORTED;
         free(a);
@@ -371,6 +375,9 @@ void wfo_aligner_free(wfo_aligner* a) {
     free(a->init_h);
     free(a->packed);
     free(a->dist);
+    wfo_aligner_free(a->bi_fwd);
+    wfo_aligner_free(a->bi_rev);
+    wfo_aligner_free(a->bi_base);
     free(a);
 }
 
@@ -462,7 +469,15 @@ static void heuristic_reset(wfo_aligner* a);
 /* ------------------------------------------------------------------ */
 /* init_alignment (src/aligner.rs:846-1000), end-to-end                */
 /* ------------------------------------------------------------------ */
+static void init_alignment_component(wfo_aligner* a, int comp);
 static void init_alignment(wfo_aligner* a) {
+    init_alignment_component(a, C_M);
+    a->end_score = -1; a->end_k = DIAG_NULL; a->end_offset = OFFSET_NULL;
+}
+/* init_alignment_component (src/aligner.rs:3205-3266): as init_alignment,
+ * with the score-0 cell (k = 0, offset 0) placed in component `comp`; the
+ * end position starts as (0, 0, 0) there. */
+static void init_alignment_component(wfo_aligner* a, int comp) {
     const int plen = a->plen, tlen = a->tlen;
     int score_only = a->attr.alignment_scope == WFA_SCOPE_SCORE;
     /* med/low alignments: modular storage with piggyback backtrace
@@ -500,7 +515,7 @@ static void init_alignment(wfo_aligner* a) {
     a->num_null_steps = 0;
     a->status = WFA_STATUS_OK;
     heuristic_reset(a);
-    a->end_score = -1; a->end_k = DIAG_NULL; a->end_offset = OFFSET_NULL;
+    a->end_score = 0; a->end_k = 0; a->end_offset = 0;
     a->hist_lo = -(plen + scope + 2);
     a->hist_hi = tlen + scope + 2;
     int alo, ahi;
@@ -526,7 +541,7 @@ static void init_alignment(wfo_aligner* a) {
         m0->lo = -a->pb_free; m0->hi = a->tb_free;
         m0->init_min = -a->pb_free; m0->init_max = a->tb_free;
     }
-    put(a, C_M, 0, m0);
+    put(a, comp, 0, m0);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1483,6 +1498,517 @@ int32_t wfo_align_end2end(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
 }
 
 /* ------------------------------------------------------------------ */
+/* BiWFA (src/aligner.rs:2563-3302, src/bialign.rs:1-520)              */
+/* ------------------------------------------------------------------ */
+/* BiStatus (src/bialign.rs) */
+enum { BI_OK = 0, BI_END_REACHED, BI_UNREACHABLE, BI_MAX_STEPS, BI_UNATTAINABLE };
+/* BIALIGN_* thresholds (src/bialign.rs:511-520) */
+#define BI_FALLBACK_MIN_SCORE 250
+#define BI_FALLBACK_MIN_LENGTH 100
+#define BI_RECOVERY_MIN_SCORE 500
+
+/* BiAlignBreakpoint (src/bialign.rs:20-57) */
+typedef struct {
+    int score, score_forward, score_reverse;
+    int k_forward, k_reverse, offset_forward, offset_reverse;
+    int component;
+} bi_bp_t;
+
+static void bp_init(bi_bp_t* bp) {
+    memset(bp, 0, sizeof(*bp));
+    bp->score = INT32_MAX;
+    bp->component = C_M;
+}
+
+/* set_sequences with init_ascii(.., reverse = true) (src/sequences.rs:153-206) */
+static void set_sequences_reversed(wfo_aligner* a, const uint8_t* p, int plen, const uint8_t* t,
+                                   int tlen) {
+    set_sequences(a, p, plen, t, tlen);
+    for (int i = 0; i < plen; ++i) a->pattern[i] = p[plen - 1 - i];
+    for (int i = 0; i < tlen; ++i) a->text[i] = t[tlen - 1 - i];
+}
+
+/* heuristic_band / with_band: the banded heuristics' (min_k, max_k) */
+static int has_band(const wfa_attr_t* at) {
+    return at->heuristic == WFA_HEURISTIC_BANDED_STATIC ||
+           at->heuristic == WFA_HEURISTIC_BANDED_ADAPTIVE;
+}
+
+/* new_biwfa_sub (src/aligner.rs:2768-2800) and take_biwfa_subs
+ * (:2721-2740): forward/reverse run score-only with modular storage; the
+ * base aligner computes CIGARs with full storage. */
+static wfo_aligner* bi_sub(wfo_aligner* a, wfo_aligner** slot, int alignment) {
+    if (!*slot) {
+        wfa_attr_t at = a->attr;
+        at.alignment_scope = alignment ? WFA_SCOPE_ALIGNMENT : WFA_SCOPE_SCORE;
+        at.memory_mode = WFA_MEMORY_HIGH;
+        at.span = WFA_SPAN_END2END;
+        at.extension = 0;
+        at.max_alignment_steps = INT32_MAX;
+        at.heuristic = WFA_HEURISTIC_NONE;
+        int32_t err = 0;
+        *slot = wfo_aligner_new(&at, &err);
+    }
+    return *slot;
+}
+static void bi_set_heuristic(wfo_aligner* sub, const wfa_attr_t* top, int min_k, int max_k) {
+    sub->attr.heuristic = top->heuristic;
+    sub->attr.heuristic_p[0] = top->heuristic_p[0];
+    sub->attr.heuristic_p[1] = top->heuristic_p[1];
+    sub->attr.heuristic_p[2] = top->heuristic_p[2];
+    if (has_band(top)) {
+        sub->attr.heuristic_p[0] = min_k;
+        sub->attr.heuristic_p[1] = max_k;
+    }
+}
+
+/* check_component_termination (src/aligner.rs:3269-3302) */
+static int check_component_termination(wfo_aligner* a, int score, int comp) {
+    const wf_t* w = get(a, comp, score);
+    if (!w) return 0;
+    int k_end = a->tlen - a->plen;
+    if (k_end >= w->lo && k_end <= w->hi && wf_get(w, k_end) >= a->tlen) {
+        a->end_score = score; a->end_k = k_end; a->end_offset = a->tlen;
+        a->status = WFA_STATUS_END_REACHED;
+        return 1;
+    }
+    return 0;
+}
+
+/* extend_biwfa_step (src/aligner.rs:3516-3554) with
+ * extend_matches_packed_end2end[_max] (src/extend.rs:47-106): extend M,
+ * then the end check on `comp_end`, then the heuristic. *max_ak is the
+ * largest antidiagonal 2*offset - k of an extended cell (0 if none). */
+static int extend_biwfa_step(wfo_aligner* a, int score, int comp_end, int* max_ak) {
+    *max_ak = 0;
+    wf_t* m = get(a, C_M, score);
+    if (!m) {
+        if (a->num_null_steps > a->scope) {
+            a->status = WFA_STATUS_END_UNREACHABLE;
+            return 1;
+        }
+        return 0;
+    }
+    int mx = 0;
+    for (int k = m->lo; k <= m->hi; ++k) {
+        int32_t off = wf_get(m, k);
+        if (off >= 0) {
+            off += lcp(a, off - k, off);
+            wf_set(m, k, off);
+            if (2 * off - k > mx) mx = 2 * off - k;
+        }
+    }
+    if (check_component_termination(a, score, comp_end)) return 1;
+    if (a->attr.heuristic != WFA_HEURISTIC_NONE) apply_heuristic_cutoff(a, score, m);
+    *max_ak = mx;
+    return 0;
+}
+
+static inline const wf_t* valid_wf(const wf_t* w) { return w && !w->null ? w : NULL; }
+static inline int k_inverse(int k, int plen, int tlen) { return (tlen - plen) - k; }
+
+/* breakpoint_m2m / breakpoint_indel2indel (src/bialign.rs:79-227) with
+ * scan_overlap (:244-295): the first diagonal, ascending in wf_0's k, where
+ * both directions' text positions add up to the text length. An indel
+ * breakpoint also needs the forward cell inside the sequences. */
+static void bi_breakpoint_scan(const wf_t* w0, const wf_t* w1, int plen, int tlen, int s0,
+                               int s1, int gap_open, int comp, int fwd, bi_bp_t* bp) {
+    if (s0 + s1 - gap_open >= bp->score) return;
+    int lo1 = k_inverse(w1->hi, plen, tlen), hi1 = k_inverse(w1->lo, plen, tlen);
+    if (hi1 < w0->lo || w0->hi < lo1) return;
+    int lo = w0->lo > lo1 ? w0->lo : lo1, hi = w0->hi < hi1 ? w0->hi : hi1;
+    for (int k0 = lo; k0 <= hi; ++k0) {
+        int k1 = k_inverse(k0, plen, tlen);
+        int32_t o0 = wf_get(w0, k0), o1 = wf_get(w1, k1);
+        if (o0 + o1 < tlen) continue;
+        if (comp != C_M) {
+            int ck = fwd ? k0 : k1, ch = fwd ? o0 : o1;
+            if (ch - ck > plen || ch > tlen) continue;
+        }
+        if (fwd) {
+            bp->score_forward = s0; bp->score_reverse = s1;
+            bp->k_forward = k0; bp->k_reverse = k1;
+            bp->offset_forward = o0; bp->offset_reverse = o1;
+        } else {
+            bp->score_forward = s1; bp->score_reverse = s0;
+            bp->k_forward = k1; bp->k_reverse = k0;
+            bp->offset_forward = o1; bp->offset_reverse = o0;
+        }
+        bp->score = s0 + s1 - gap_open;
+        bp->component = comp;
+        return;
+    }
+}
+
+/* bialign_overlap (src/bialign.rs:362-508): a0 at s0 against a1 at the last
+ * `scope` scores from s1 down: D2, I2, D1, I1, then M. */
+static void bi_overlap(const wfo_aligner* a0, const wfo_aligner* a1, int s0, int s1, int plen,
+                       int tlen, int fwd, bi_bp_t* bp) {
+    const pen_t* p = &a0->pen;
+    const int affine = p->metric == WFA_METRIC_GAP_AFFINE || p->metric == WFA_METRIC_GAP_AFFINE_2P;
+    const int two = p->metric == WFA_METRIC_GAP_AFFINE_2P;
+    const wf_t* m0 = valid_wf(get(a0, C_M, s0));
+    if (!m0) return;
+    const wf_t* d1 = affine ? valid_wf(get(a0, C_D1, s0)) : NULL;
+    const wf_t* i1 = affine ? valid_wf(get(a0, C_I1, s0)) : NULL;
+    const wf_t* d2 = two ? valid_wf(get(a0, C_D2, s0)) : NULL;
+    const wf_t* i2 = two ? valid_wf(get(a0, C_I2, s0)) : NULL;
+    for (int i = 0; i < a1->scope; ++i) {
+        int si = s1 - i;
+        if (si < 0) break;
+        if (two && s0 + si - p->o2 < bp->score) {
+            const wf_t* w;
+            if (d2 && (w = valid_wf(get(a1, C_D2, si))))
+                bi_breakpoint_scan(d2, w, plen, tlen, s0, si, p->o2, C_D2, fwd, bp);
+            if (i2 && (w = valid_wf(get(a1, C_I2, si))))
+                bi_breakpoint_scan(i2, w, plen, tlen, s0, si, p->o2, C_I2, fwd, bp);
+        }
+        if (affine && s0 + si - p->o1 < bp->score) {
+            const wf_t* w;
+            if (d1 && (w = valid_wf(get(a1, C_D1, si))))
+                bi_breakpoint_scan(d1, w, plen, tlen, s0, si, p->o1, C_D1, fwd, bp);
+            if (i1 && (w = valid_wf(get(a1, C_I1, si))))
+                bi_breakpoint_scan(i1, w, plen, tlen, s0, si, p->o1, C_I1, fwd, bp);
+        }
+        if (s0 + si >= bp->score) continue;
+        const wf_t* w = valid_wf(get(a1, C_M, si));
+        if (w) bi_breakpoint_scan(m0, w, plen, tlen, s0, si, 0, C_M, fwd, bp);
+    }
+}
+
+/* bialign_find_breakpoint (src/aligner.rs:2958-3120). Returns a BI_*
+ * status; *end_score is the end score of BI_END_REACHED. */
+static int bi_find_breakpoint(const pen_t* pen, int max_steps, wfo_aligner* fwd,
+                              wfo_aligner* rev, const uint8_t* P, int plen, const uint8_t* T,
+                              int tlen, int cbeg, int cend, bi_bp_t* bp, int* end_score) {
+    set_sequences(fwd, P, plen, T, tlen);
+    init_alignment_component(fwd, cbeg);
+    set_sequences_reversed(rev, P, plen, T, tlen);
+    init_alignment_component(rev, cend);
+    const int fwd_end = cend, rev_end = cbeg;
+#define QUIT(A) \
+    do { \
+        if ((A)->status == WFA_STATUS_END_REACHED) { *end_score = (A)->end_score; return BI_END_REACHED; } \
+        return BI_UNREACHABLE; \
+    } while (0)
+    int gap_open = pen->metric == WFA_METRIC_GAP_AFFINE ? pen->o1
+                   : pen->metric == WFA_METRIC_GAP_AFFINE_2P ? (pen->o1 > pen->o2 ? pen->o1 : pen->o2)
+                                                             : 0;
+    const int max_ad = plen + tlen - 1;
+    int s_fwd = 0, s_rev = 0, last_fwd = 0, ak;
+    int fwd_max, rev_max;
+    if (extend_biwfa_step(fwd, 0, fwd_end, &fwd_max)) QUIT(fwd);
+    if (extend_biwfa_step(rev, 0, rev_end, &rev_max)) QUIT(rev);
+    /* phase 1: alternate until the antidiagonals meet */
+    while (fwd_max + rev_max < max_ad) {
+        s_fwd += 1;
+        compute_step(fwd, s_fwd);
+        int done = extend_biwfa_step(fwd, s_fwd, fwd_end, &ak);
+        if (ak > fwd_max) fwd_max = ak;
+        last_fwd = 1;
+        if (done) QUIT(fwd);
+        if (fwd_max + rev_max >= max_ad) break;
+        s_rev += 1;
+        compute_step(rev, s_rev);
+        done = extend_biwfa_step(rev, s_rev, rev_end, &ak);
+        if (ak > rev_max) rev_max = ak;
+        last_fwd = 0;
+        if (done) QUIT(rev);
+        if (s_fwd + s_rev >= max_steps) return BI_MAX_STEPS;
+    }
+    /* phase 2: overlap checks, then one step of the other direction */
+    const int scope = fwd->scope;
+    for (;;) {
+        if (last_fwd) {
+            int min_rev = s_rev > scope - 1 ? s_rev - (scope - 1) : 0;
+            if (s_fwd + min_rev - gap_open >= bp->score) break;
+            bi_overlap(fwd, rev, s_fwd, s_rev, plen, tlen, 1, bp);
+            s_rev += 1;
+            compute_step(rev, s_rev);
+            if (extend_biwfa_step(rev, s_rev, rev_end, &ak)) QUIT(rev);
+        }
+        int min_fwd = s_fwd > scope - 1 ? s_fwd - (scope - 1) : 0;
+        if (min_fwd + s_rev - gap_open >= bp->score) break;
+        bi_overlap(rev, fwd, s_rev, s_fwd, plen, tlen, 0, bp);
+        s_fwd += 1;
+        compute_step(fwd, s_fwd);
+        if (extend_biwfa_step(fwd, s_fwd, fwd_end, &ak)) QUIT(fwd);
+        if (s_fwd + s_rev >= max_steps) return BI_MAX_STEPS;
+        last_fwd = 1;
+    }
+#undef QUIT
+    return BI_OK;
+}
+
+/* bt_* extend lookups (src/backtrace.rs:260-449): a gap component's cell
+ * continued by one more gap op; M opens read M (bt_ins1_open etc.). */
+#define BT_I1_EXT 2
+#define BT_I2_OPEN 3
+#define BT_I2_EXT 4
+#define BT_D1_EXT 6
+#define BT_D2_OPEN 7
+#define BT_D2_EXT 8
+static int64_t bt_ins(const wfo_aligner* a, int comp, int score, int k, int64_t type) {
+    if (score < 0) return OFFSET_NULL;
+    const wf_t* w = get(a, comp, score);
+    if (!w) return OFFSET_NULL;
+    if (w->init_min < k && k - 1 <= w->init_max) return bt_pack(wf_get(w, k - 1) + 1, type);
+    return OFFSET_NULL;
+}
+static int64_t bt_del(const wfo_aligner* a, int comp, int score, int k, int64_t type) {
+    if (score < 0) return OFFSET_NULL;
+    const wf_t* w = get(a, comp, score);
+    if (!w) return OFFSET_NULL;
+    if (w->init_min <= k + 1 && k < w->init_max) return bt_pack(wf_get(w, k + 1), type);
+    return OFFSET_NULL;
+}
+static inline int64_t max64(int64_t x, int64_t y) { return x > y ? x : y; }
+
+/* backtrace_affine_c (src/backtrace.rs:492-635): gap-affine / 2-piece
+ * CIGAR between components, every predecessor read from its source
+ * wavefront, the larger packed (offset << 4 | type) winning. */
+static void backtrace_affine_c(wfo_aligner* a, int score, int k, int offset, int comp_end) {
+    const pen_t* p = &a->pen;
+    const int two = p->metric == WFA_METRIC_GAP_AFFINE_2P;
+    a->cig_end = a->cig_cap;
+    a->cig_begin = a->cig_cap - 1;
+    const int alignment_score = score;
+    int matrix = comp_end;
+    int v = offset - k, h = offset;
+    if (comp_end == C_M) {
+        if (a->plen - v > 0) write_ops(a, 'D', a->plen - v);
+        if (a->tlen - h > 0) write_ops(a, 'I', a->tlen - h);
+    }
+    while (v > 0 && h > 0 && score > 0) {
+        const int mis = score - p->mismatch;
+        const int go1 = score - p->o1 - p->e1, go2 = score - p->o2 - p->e2;
+        const int ge1 = score - p->e1, ge2 = score - p->e2;
+        int64_t mx;
+        switch (matrix) {
+        case C_M: {
+            int64_t misms = bt_misms(a, mis, k);
+            int64_t ins1 = max64(bt_ins(a, C_M, go1, k, BT_I1_OPEN), bt_ins(a, C_I1, ge1, k, BT_I1_EXT));
+            int64_t del1 = max64(bt_del(a, C_M, go1, k, BT_D1_OPEN), bt_del(a, C_D1, ge1, k, BT_D1_EXT));
+            if (two) {
+                int64_t ins2 = max64(bt_ins(a, C_M, go2, k, BT_I2_OPEN), bt_ins(a, C_I2, ge2, k, BT_I2_EXT));
+                int64_t del2 = max64(bt_del(a, C_M, go2, k, BT_D2_OPEN), bt_del(a, C_D2, ge2, k, BT_D2_EXT));
+                mx = max64(misms, max64(max64(ins1, ins2), max64(del1, del2)));
+            } else {
+                mx = max64(misms, max64(ins1, del1));
+            }
+            break;
+        }
+        case C_I1: mx = max64(bt_ins(a, C_M, go1, k, BT_I1_OPEN), bt_ins(a, C_I1, ge1, k, BT_I1_EXT)); break;
+        case C_D1: mx = max64(bt_del(a, C_M, go1, k, BT_D1_OPEN), bt_del(a, C_D1, ge1, k, BT_D1_EXT)); break;
+        case C_I2: mx = max64(bt_ins(a, C_M, go2, k, BT_I2_OPEN), bt_ins(a, C_I2, ge2, k, BT_I2_EXT)); break;
+        default: mx = max64(bt_del(a, C_M, go2, k, BT_D2_OPEN), bt_del(a, C_D2, ge2, k, BT_D2_EXT)); break;
+        }
+        if (mx < 0) break;
+        if (matrix == C_M) {
+            int32_t moff = (int32_t)(mx >> 4);
+            write_matches(a, offset - moff);
+            offset = moff;
+            v = offset - k; h = offset;
+            if (v <= 0 || h <= 0) break;
+        }
+        int64_t t = mx & 0xF;
+        switch (t) {
+        case BT_M: score = mis; matrix = C_M; break;
+        case BT_I1_OPEN: case BT_D1_OPEN: score = go1; matrix = C_M; break;
+        case BT_I2_OPEN: case BT_D2_OPEN: score = go2; matrix = C_M; break;
+        case BT_I1_EXT: score = ge1; matrix = C_I1; break;
+        case BT_I2_EXT: score = ge2; matrix = C_I2; break;
+        case BT_D1_EXT: score = ge1; matrix = C_D1; break;
+        case BT_D2_EXT: score = ge2; matrix = C_D2; break;
+        default: abort();
+        }
+        if (t == BT_M) { write_ops(a, 'X', 1); offset -= 1; }
+        else if (t == BT_I1_OPEN || t == BT_I1_EXT || t == BT_I2_OPEN || t == BT_I2_EXT) {
+            write_ops(a, 'I', 1); k -= 1; offset -= 1;
+        } else { write_ops(a, 'D', 1); k += 1; }
+        v = offset - k; h = offset;
+    }
+    if (matrix == C_M) {
+        if (v > 0 && h > 0) {
+            int nm = v < h ? v : h;
+            write_matches(a, nm);
+            v -= nm; h -= nm;
+        }
+        if (v > 0) write_ops(a, 'D', v);
+        if (h > 0) write_ops(a, 'I', h);
+    }
+    a->cig_begin += 1;
+    a->cig_score = alignment_score;
+}
+
+/* backtrace_components (src/aligner.rs:1489-1536) */
+static void backtrace_components(wfo_aligner* a, int score, int k, int offset, int comp_end) {
+    if (a->pen.metric == WFA_METRIC_GAP_AFFINE || a->pen.metric == WFA_METRIC_GAP_AFFINE_2P)
+        backtrace_affine_c(a, score, k, offset, comp_end);
+    else
+        backtrace_linear(a, score, k, offset);
+}
+
+/* Cigar append (src/cigar.rs:80-111) into the top aligner's forward CIGAR */
+static void cig_append(wfo_aligner* a, const uint8_t* ops, int n) {
+    if (n > 0) memcpy(a->cig + a->cig_end, ops, (size_t)n);
+    if (n > 0) a->cig_end += n;
+}
+static void cig_append_run(wfo_aligner* a, uint8_t op, int n) {
+    for (int i = 0; i < n; ++i) a->cig[a->cig_end++] = op;
+}
+
+/* bialign_base (src/aligner.rs:3127-3202): unidirectional WFA from
+ * component cbeg to cend on the sub-problem, its CIGAR appended whether or
+ * not it completed. *score_out is the base score on success. */
+static int bi_base(wfo_aligner* top, const uint8_t* P, const uint8_t* T, int pb, int pe, int tb,
+                   int te, int cbeg, int cend, int* score_out) {
+    wfo_aligner* b = bi_sub(top, &top->bi_base, 1);
+    bi_set_heuristic(b, &top->attr, top->attr.heuristic_p[0] - (tb - pb),
+                     top->attr.heuristic_p[1] - (tb - pb));
+    set_sequences(b, P + pb, pe - pb, T + tb, te - tb);
+    b->endsfree = 0;
+    b->pb_free = b->pe_free = b->tb_free = b->te_free = 0;
+    init_alignment_component(b, cbeg);
+    int score = 0, ak;
+    for (;;) {
+        if (extend_biwfa_step(b, score, cend, &ak)) break;
+        score += 1;
+        if (!has_slot(b, score)) { b->status = WFA_STATUS_MAX_STEPS_REACHED; break; }
+        compute_step(b, score);
+        if (score >= b->attr.max_alignment_steps) { b->status = WFA_STATUS_MAX_STEPS_REACHED; break; }
+    }
+    int r;
+    if (b->status == WFA_STATUS_END_REACHED) {
+        backtrace_components(b, score, b->end_k, b->end_offset, cend);
+        *score_out = score;
+        r = BI_OK;
+    } else if (b->status == WFA_STATUS_END_UNREACHABLE && b->end_offset != OFFSET_NULL) {
+        int from = b->end_score >= 0 ? b->end_score : score;
+        backtrace_components(b, from, b->end_k, b->end_offset, cend);
+        maxtrim_cigar(b);
+        r = BI_UNATTAINABLE;
+    } else {
+        b->cig_begin = b->cig_end = 0;
+        r = BI_UNATTAINABLE;
+    }
+    cig_append(top, b->cig + b->cig_begin, b->cig_end - b->cig_begin);
+    return r;
+}
+
+/* bialign_recursive (src/aligner.rs:2809-2950) */
+static int bi_recursive(wfo_aligner* top, const uint8_t* P, const uint8_t* T, int pb, int pe,
+                        int tb, int te, int cbeg, int cend, int score_remaining, int* score_out) {
+    const int plen = pe - pb, tlen = te - tb;
+    *score_out = 0;
+    if (tlen == 0) { cig_append_run(top, 'D', plen); return BI_OK; }
+    if (plen == 0) { cig_append_run(top, 'I', tlen); return BI_OK; }
+    if (score_remaining <= BI_FALLBACK_MIN_SCORE)
+        return bi_base(top, P, T, pb, pe, tb, te, cbeg, cend, score_out);
+    wfo_aligner* fwd = bi_sub(top, &top->bi_fwd, 0);
+    wfo_aligner* rev = bi_sub(top, &top->bi_rev, 0);
+    /* wavefront_bialign_set_subsidiary_band: the band in each direction's
+     * own diagonals */
+    const int shift = tb - pb, diag = tlen - plen;
+    const int fmin = top->attr.heuristic_p[0] - shift, fmax = top->attr.heuristic_p[1] - shift;
+    bi_set_heuristic(fwd, &top->attr, fmin, fmax);
+    bi_set_heuristic(rev, &top->attr, diag - fmax, diag - fmin);
+    bi_bp_t bp;
+    bp_init(&bp);
+    int end_score = 0;
+    int st = bi_find_breakpoint(&top->pen, top->attr.max_alignment_steps, fwd, rev, P + pb, plen,
+                                T + tb, tlen, cbeg, cend, &bp, &end_score);
+    if (st == BI_END_REACHED) {
+        if (end_score <= BI_RECOVERY_MIN_SCORE)
+            return bi_base(top, P, T, pb, pe, tb, te, cbeg, cend, score_out);
+        return BI_UNREACHABLE;
+    }
+    if (st != BI_OK) return st;
+    const int bp_h = bp.offset_forward, bp_v = bp.offset_forward - bp.k_forward;
+    int s;
+    st = bi_recursive(top, P, T, pb, pb + bp_v, tb, tb + bp_h, cbeg, bp.component,
+                      bp.score_forward, &s);
+    if (st != BI_OK) return st;
+    st = bi_recursive(top, P, T, pb + bp_v, pe, tb + bp_h, te, bp.component, cend,
+                      bp.score_reverse, &s);
+    if (st != BI_OK) return st;
+    *score_out = bp.score;
+    return BI_OK;
+}
+
+/* cigar_wfa_score (src/aligner.rs:2700-2719) of the forward CIGAR */
+static int cigar_wfa_score_all(const wfo_aligner* a, int plen, int tlen) {
+    const wfa_penalties_t* u = &a->pen.user;
+    const uint8_t* ops = a->cig + a->cig_begin;
+    int n = a->cig_end - a->cig_begin;
+    int classic;
+    switch (a->pen.metric) {
+    case WFA_METRIC_INDEL:
+    case WFA_METRIC_EDIT: return wfo_cigar_score_edit(ops, n);
+    case WFA_METRIC_GAP_AFFINE:
+        classic = wfo_cigar_score_gap_affine(ops, n, u->match, u->mismatch, u->gap_opening1,
+                                             u->gap_extension1);
+        break;
+    default:
+        classic = wfo_cigar_score_gap_affine2p(ops, n, u->match, u->mismatch, u->gap_opening1,
+                                               u->gap_extension1, u->gap_opening2,
+                                               u->gap_extension2);
+        break;
+    }
+    return a->pen.match < 0 ? -a->pen.match * (plen + tlen) - 2 * classic : -classic;
+}
+
+/* align_biwfa (src/aligner.rs:2563-2642) */
+int32_t wfo_align_biwfa(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
+                        const uint8_t* text, int32_t tlen) {
+    set_sequences(a, pattern, plen, text, tlen);
+    int need = 2 * (plen + tlen) + 1;
+    if (need > a->cig_cap) {
+        a->cig = (uint8_t*)realloc(a->cig, (size_t)need);
+        a->cig_cap = need;
+    }
+    a->cig_begin = a->cig_end = 0;
+    a->cig_score = INT32_MIN;
+    a->status = WFA_STATUS_OK;
+    int score = 0;
+    int st = bi_recursive(a, pattern, text, 0, plen, 0, tlen, C_M, C_M,
+                          (plen > tlen ? plen : tlen) <= BI_FALLBACK_MIN_LENGTH ? 0 : INT32_MAX,
+                          &score);
+    if (st == BI_OK) {
+        int heur = a->attr.heuristic != WFA_HEURISTIC_NONE && a->cig_begin < a->cig_end;
+        if (plen == 0 || tlen == 0 || heur) score = cigar_wfa_score_all(a, plen, tlen);
+        a->cig_score = score;
+        a->status = WFA_STATUS_COMPLETED;
+        return score;
+    }
+    a->status = st == BI_MAX_STEPS ? WFA_STATUS_MAX_STEPS_REACHED : WFA_STATUS_END_UNREACHABLE;
+    return 0;
+}
+
+/* align_biwfa_score (src/aligner.rs:2650-2690) */
+int32_t wfo_align_biwfa_score(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
+                              const uint8_t* text, int32_t tlen) {
+    set_sequences(a, pattern, plen, text, tlen);
+    a->cig_begin = a->cig_end = 0;
+    a->cig_score = INT32_MIN;
+    wfo_aligner* fwd = bi_sub(a, &a->bi_fwd, 0);
+    wfo_aligner* rev = bi_sub(a, &a->bi_rev, 0);
+    const int diag = tlen - plen;
+    bi_set_heuristic(fwd, &a->attr, a->attr.heuristic_p[0], a->attr.heuristic_p[1]);
+    bi_set_heuristic(rev, &a->attr, diag - a->attr.heuristic_p[1], diag - a->attr.heuristic_p[0]);
+    bi_bp_t bp;
+    bp_init(&bp);
+    int end_score = 0;
+    int st = bi_find_breakpoint(&a->pen, a->attr.max_alignment_steps, fwd, rev, pattern, plen,
+                                text, tlen, C_M, C_M, &bp, &end_score);
+    int score = st == BI_OK ? bp.score : st == BI_END_REACHED ? end_score : 0;
+    a->status = (st == BI_OK || st == BI_END_REACHED) ? WFA_STATUS_COMPLETED
+                : st == BI_MAX_STEPS ? WFA_STATUS_MAX_STEPS_REACHED
+                                     : WFA_STATUS_END_UNREACHABLE;
+    return score;
+}
+
+/* ------------------------------------------------------------------ */
 /* CIGAR scores (src/cigar.rs:118-202) and validity (:446-518)          */
 /* ------------------------------------------------------------------ */
 int32_t wfo_cigar_score_edit(const uint8_t* ops, int32_t n) {
@@ -1599,8 +2125,14 @@ static void* job_run(void* arg) {
             unpack2(T, tl, tb);
             P = pb; T = tb;
         }
-        j->score[i] = j->attr->span == WFA_SPAN_ENDSFREE ? wfo_align_endsfree(a, P, pl, T, tl)
-                                                          : wfo_align_end2end(a, P, pl, T, tl);
+        if (j->attr->memory_mode == WFA_MEMORY_ULTRALOW)
+            j->score[i] = j->attr->alignment_scope == WFA_SCOPE_ALIGNMENT
+                              ? wfo_align_biwfa(a, P, pl, T, tl)
+                              : wfo_align_biwfa_score(a, P, pl, T, tl);
+        else if (j->attr->span == WFA_SPAN_ENDSFREE)
+            j->score[i] = wfo_align_endsfree(a, P, pl, T, tl);
+        else
+            j->score[i] = wfo_align_end2end(a, P, pl, T, tl);
         j->status[i] = a->status;
         if (j->ops && j->ops_len) {
             const uint8_t* o = NULL;

@@ -40,6 +40,13 @@ int32_t wfo_align_end2end(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
  * extension when attr->extension is set */
 int32_t wfo_align_endsfree(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
                            const uint8_t* text, int32_t tlen);
+/* BiWFA (src/aligner.rs:2563-2690): CIGAR (align_biwfa) and score only
+ * (align_biwfa_score); memory_mode ULTRALOW selects them in
+ * wfo_align_batch */
+int32_t wfo_align_biwfa(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
+                        const uint8_t* text, int32_t tlen);
+int32_t wfo_align_biwfa_score(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
+                              const uint8_t* text, int32_t tlen);
 int32_t wfo_status(const wfo_aligner* a);
 int32_t wfo_cigar(const wfo_aligner* a, const uint8_t** ops);
 int32_t wfo_cigar_score(const wfo_aligner* a);
