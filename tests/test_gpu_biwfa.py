@@ -1,0 +1,84 @@
+"""GPU BiWFA (ultralow memory) against the oracle, bit for bit: scores,
+statuses and CIGARs of split alignments, base cases, heuristics and the
+wide-row retry path."""
+import numpy as np
+import pytest
+
+from conftest import attr_for, have_gpu
+from test_gpu_parity import _compare
+from test_oracle_biwfa import BIWFA_PENALTIES, _recursive_pairs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def engine(pkg):
+    if not have_gpu():
+        pytest.skip("no GPU")
+    from wfa2lib_rs_amd.engine import BatchEngine
+    return BatchEngine(0)
+
+
+def _attr(N, metric, pen, scope, heuristic=None):
+    attr = attr_for(N, metric, pen, scope)
+    attr.memory_mode = N.MEMORY_ULTRALOW
+    if heuristic:
+        attr.heuristic = heuristic[0]
+        for i, v in enumerate(heuristic[1:]):
+            attr.heuristic_p[i] = v
+    return attr
+
+
+def _pairs(n, seed, fmt=None):
+    from wfa2lib_rs_amd.synth import from_ascii
+    import wfa2lib_rs_amd._native as Nn
+    ps = _recursive_pairs(n, seed)
+    # short pairs (top-level fallback) and empty sequences ride along
+    ps += [(b"ACGTTGCA" * 10, b"ACGTTGGA" * 11), (b"", b"ACGT"), (b"GATTACA", b""), (b"", b""),
+           (b"A" * 300, b"A" * 300)]
+    return from_ascii([p for p, _ in ps], [t for _, t in ps], fmt or Nn.SEQ_ASCII)
+
+
+@pytest.mark.parametrize("metric,pen", BIWFA_PENALTIES)
+@pytest.mark.parametrize("scope", ["score", "alignment"])
+def test_biwfa_matches_oracle(metric, pen, scope, pkg, N, engine, oracle):
+    pairs = _pairs(24, seed=5)
+    _compare(engine, oracle, _attr(N, metric, pen, scope), pairs, scope == "alignment",
+             label=f"biwfa/{metric}/{scope}")
+
+
+def test_biwfa_packed_input(pkg, N, engine, oracle):
+    pairs = _pairs(16, seed=9, fmt=N.SEQ_PACKED2BITS)
+    _compare(engine, oracle, _attr(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment"), pairs,
+             True, label="biwfa/packed")
+
+
+@pytest.mark.parametrize("h", [(1, 10, 50, 1), (1, 4, 6, 2), (4, -40, 5, 0), (4, -3, 3, 0),
+                               (5, -50, 50, 1), (5, -5, 5, 1)],
+                         ids=["wfadaptive_10_50_1", "wfadaptive_4_6_2", "banded_m40_5",
+                              "banded_m3_3", "badaptive_m50_50_1", "badaptive_m5_5_1"])
+@pytest.mark.parametrize("metric,pen", [BIWFA_PENALTIES[0], BIWFA_PENALTIES[2],
+                                        BIWFA_PENALTIES[4]])
+def test_biwfa_heuristics_match_oracle(h, metric, pen, pkg, N, engine, oracle):
+    pairs = _pairs(12, seed=13)
+    for scope in ("score", "alignment"):
+        _compare(engine, oracle, _attr(N, metric, pen, scope, h), pairs, scope == "alignment",
+                 label=f"biwfa/{metric}/{h}/{scope}")
+
+
+def test_biwfa_wide_rows_take_the_retry_path(pkg, N, engine, oracle):
+    """Unrelated 40 kbp sequences: rows wider than the first launch's ring."""
+    from wfa2lib_rs_amd.synth import from_ascii
+    rng = np.random.default_rng(3)
+    acgt = np.frombuffer(b"ACGT", dtype=np.uint8)
+    ps = [(rng.choice(acgt, 40000).tobytes(), rng.choice(acgt, 38000).tobytes())
+          for _ in range(2)]
+    pairs = from_ascii([p for p, _ in ps], [t for _, t in ps], N.SEQ_ASCII)
+    _compare(engine, oracle, _attr(N, "edit", (0,) * 6, "alignment"), pairs, True,
+             label="biwfa/wide")
+
+
+def test_biwfa_per_pair_equals_batch(pkg, N, engine, oracle):
+    pairs = _pairs(6, seed=21)
+    attr = _attr(N, "gap_affine_2p", (0, 4, 6, 2, 24, 1), "alignment")
+    _compare(engine, oracle, attr, pairs, True, per_pair=True, label="biwfa/per_pair")

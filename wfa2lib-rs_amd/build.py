@@ -60,7 +60,8 @@ def build_engine():
         raise RuntimeError("build step failed: " + "; ".join(failed))
     out = os.path.join(LIB, "libwfa_mi355x.so")
     tmp = out + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + objs + [os.path.join(CSRC, "wfa_api.cpp")])
+    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + objs +
+         [os.path.join(CSRC, f) for f in ("wfa_api.cpp", "wfa_biwfa.cpp")])
     os.replace(tmp, out)
     return out
 

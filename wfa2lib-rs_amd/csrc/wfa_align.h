@@ -46,6 +46,10 @@ static __device__ __forceinline__ int wave_min(int v) {
   for (int d = 32; d >= 1; d >>= 1) v = imin(v, __shfl_xor(v, d));
   return v;
 }
+static __device__ __forceinline__ int wave_max(int v) {
+  for (int d = 32; d >= 1; d >>= 1) v = imax(v, __shfl_xor(v, d));
+  return v;
+}
 static __device__ __forceinline__ int64_t wave_max64(int64_t v) {
   for (int d = 32; d >= 1; d >>= 1) {
     const int64_t o = __shfl_xor(v, d);
@@ -389,7 +393,8 @@ static __device__ int backtrace_linear(const KParams& p, const Hist& H, int nsco
     if (mx < 0) break;
     const int moff = (int)(mx >> 4);
     const int nmatch = offset - moff;
-    if (nmatch < 0) pos += -nmatch; else write_ops(out, pos, 'M', nmatch);
+    if (nmatch < 0) pos = imin(pos - nmatch, cap - 1);
+    else write_ops(out, pos, 'M', nmatch);
     offset = moff;
     v = offset - k; h = offset;
     if (v <= 0 || h <= 0) break;
@@ -647,6 +652,138 @@ static __device__ int high_mem_nwf(const KParams& p, int plen, int tlen) {
 }
 
 // ---------------------------------------------------------------------
+// Heuristic cut-off decision (heuristic_cutoff, src/heuristic.rs:167-525,
+// without Z-drop) for an extended M row with limits [lo, hi] at score s:
+// the new limits in *nlo / *nhi. Returns false when the strategy does not
+// cut at this step. `band_lo/hi` are the banded strategies' (min_k, max_k)
+// in this aligner's diagonals.
+// ---------------------------------------------------------------------
+struct HeurState {
+  int steps_wait, max_sw;
+  bool have_max;
+  // HeuristicState::reset (src/heuristic.rs:122-143)
+  __device__ void reset(const KParams& p, int heur) {
+    steps_wait = heur == WFA_HEURISTIC_WFADAPTIVE || heur == WFA_HEURISTIC_BANDED_ADAPTIVE
+                     ? p.hp2
+                     : heur == WFA_HEURISTIC_XDROP ? p.hp1 : 0;
+    max_sw = 0;
+    have_max = false;
+  }
+};
+
+template <typename RT>
+static __device__ bool heuristic_limits(const KParams& p, int heur, int band_lo, int band_hi,
+                                        HeurState& hs, const Rows<RT>& R, int rowM, int lo,
+                                        int hi, int s, int plen, int tlen, int pb_free,
+                                        int tb_free, int* out_lo, int* out_hi) {
+  const int lane = threadIdx.x;
+  // banded-static inside its band changes nothing and skips the step
+  // counter (src/aligner.rs:1645-1654)
+  if (heur == WFA_HEURISTIC_BANDED_STATIC && lo >= band_lo && hi <= band_hi) return false;
+  if (lo > hi) return false;
+  hs.steps_wait -= 1;
+  int nlo = lo, nhi = hi;
+  auto cell = [&](int k) { return R.at(rowM * R.wcap + (k & R.mask)); };
+  // wf_distance_end2end (src/heuristic.rs:150-160)
+  auto dist = [&](int off, int k) -> int {
+    return off < 0 ? INT32_MAX : imax(plen - (off - k), tlen - off);
+  };
+  if (heur == WFA_HEURISTIC_WFADAPTIVE) {
+    // src/heuristic.rs:258-346
+    if (hs.steps_wait > 0 || hi - lo + 1 < p.hp0) return false;
+    int md = imax(plen, tlen);
+    for (int kb = lo; kb <= hi; kb += 64) {
+      const int k = kb + lane;
+      if (k <= hi) md = imin(md, dist(cell(k), k));
+    }
+    md = wave_min(md);
+    int fg = INT32_MAX, lg = INT32_MIN;
+    for (int kb = lo; kb <= hi; kb += 64) {
+      const int k = kb + lane;
+      const bool good = k <= hi && (int64_t)dist(cell(k), k) - md <= (int64_t)p.hp1;
+      const uint64_t b = ballot(good);
+      if (b) {
+        if (fg == INT32_MAX) fg = kb + __builtin_ctzll(b);
+        lg = kb + 63 - __builtin_clzll(b);
+      }
+    }
+    // the low end moves up to the first close diagonal, stopping at the
+    // end diagonal; then the high end down, stopping there or at the new lo
+    const int ak = tlen - plen;
+    const int top = imin(ak, hi);
+    nlo = top <= lo ? lo : imin(fg, top);
+    const int bottom = imax(ak, nlo);
+    nhi = hi <= bottom ? hi : imax(lg, bottom);
+    hs.steps_wait = p.hp2;
+  } else if (heur == WFA_HEURISTIC_BANDED_STATIC) {
+    // src/heuristic.rs:400-407
+    nlo = imax(lo, band_lo);
+    nhi = imin(hi, band_hi);
+  } else if (heur == WFA_HEURISTIC_BANDED_ADAPTIVE) {
+    // src/heuristic.rs:355-394
+    if (hs.steps_wait > 0) return false;
+    const int len = hi - lo + 1;
+    if (len < 4) return false;
+    const int maxlen = band_hi - band_lo + 1;
+    if (len > maxlen) {
+      const int leeway = (len - maxlen) / 2, q = len / 4;
+      const int d0 = dist(uniform(cell(lo)), lo);
+      const int d1 = dist(uniform(cell(lo + q)), lo + q);
+      const int d2 = dist(uniform(cell(lo + 2 * q)), lo + 2 * q);
+      const int d3 = dist(uniform(cell(hi)), hi);
+      int nl = lo;
+      if (d0 > d3) nl += leeway;
+      if (d1 > d2) nl += leeway;
+      nlo = imax(nl, lo);
+      nhi = imin(nl + maxlen - 1, hi);
+    }
+    hs.steps_wait = p.hp2;
+  } else if (heur == WFA_HEURISTIC_XDROP) {
+    // src/heuristic.rs:410-448, 491-525: the Smith-Waterman-like score of
+    // each cell (free begin gaps not counted), cells more than `xdrop`
+    // below the best seen so far cut from both ends
+    if (hs.steps_wait > 0) return false;
+    const int swg = p.u_match < 0 ? -p.u_match : 1;
+    int64_t best = INT64_MIN;  // (sw << 32) | (~biased k): max sw, first k
+    int fg = INT32_MAX, lg = INT32_MIN;
+    for (int kb = lo; kb <= hi; kb += 64) {
+      const int k = kb + lane;
+      bool good = false;
+      if (k <= hi) {
+        const int off = cell(k);
+        if (off >= 0) {
+          const int fv = k < 0 ? imin(-k, pb_free) : 0;
+          const int fh = k > 0 ? imin(k, tb_free) : 0;
+          const int sw = (swg * ((off - k - fv) + (off - fh)) - s) / 2;
+          const int64_t key = ((int64_t)sw << 32) | (uint32_t)(0x7fffffffu - (uint32_t)k);
+          best = best > key ? best : key;
+          good = hs.have_max && hs.max_sw - sw < p.hp0;
+        }
+      }
+      const uint64_t b = ballot(good);
+      if (b) {
+        if (fg == INT32_MAX) fg = kb + __builtin_ctzll(b);
+        lg = kb + 63 - __builtin_clzll(b);
+      }
+    }
+    best = wave_max64(best);
+    const int cmax = (int)(best >> 32);
+    if (hs.have_max) {
+      nlo = fg != INT32_MAX ? fg : hi + 1;
+      nhi = lg != INT32_MIN ? lg : hi;
+      if (cmax > hs.max_sw) hs.max_sw = cmax;
+    } else {
+      hs.max_sw = cmax;
+      hs.have_max = true;
+    }
+    hs.steps_wait = p.hp1;
+  }
+  *out_lo = nlo;
+  *out_hi = nhi;
+  return true;
+}
+
+// ---------------------------------------------------------------------
 // The per-pair forward pass + backtrace.
 // ---------------------------------------------------------------------
 template <int KIND, int MODE, int BITS, typename RT, bool HEUR>
@@ -747,114 +884,16 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   // initialized range (an emptied M reads as the null wavefront) and the
   // gap components of the same score take M's new limits.
   const int heur = HEUR ? p.heuristic : WFA_HEURISTIC_NONE;
-  int steps_wait = heur == WFA_HEURISTIC_WFADAPTIVE || heur == WFA_HEURISTIC_BANDED_ADAPTIVE
-                       ? p.hp2
-                       : heur == WFA_HEURISTIC_XDROP ? p.hp1 : 0;
-  int max_sw = 0;
-  bool have_max = false;
+  HeurState hs;
+  hs.reset(p, heur);
   auto cutoff = [&](int s) {
     const int rowM = s % ringM;
     const int4 mm = R.meta[rowM];
     const int lo = uniform(mm.x), hi = uniform(mm.y);
-    if (heur == WFA_HEURISTIC_BANDED_STATIC && lo >= p.hp0 && hi <= p.hp1) return;
-    if (lo > hi) return;
-    steps_wait -= 1;
     int nlo = lo, nhi = hi;
-    auto cell = [&](int k) { return R.at(rowM * R.wcap + (k & R.mask)); };
-    // wf_distance_end2end (src/heuristic.rs:150-160)
-    auto dist = [&](int off, int k) -> int {
-      return off < 0 ? INT32_MAX : imax(plen - (off - k), tlen - off);
-    };
-    if (heur == WFA_HEURISTIC_WFADAPTIVE) {
-      // src/heuristic.rs:258-346
-      if (steps_wait > 0 || hi - lo + 1 < p.hp0) return;
-      int md = imax(plen, tlen);
-      for (int kb = lo; kb <= hi; kb += 64) {
-        const int k = kb + lane;
-        if (k <= hi) md = imin(md, dist(cell(k), k));
-      }
-      md = wave_min(md);
-      int fg = INT32_MAX, lg = INT32_MIN;
-      for (int kb = lo; kb <= hi; kb += 64) {
-        const int k = kb + lane;
-        const bool good = k <= hi && (int64_t)dist(cell(k), k) - md <= (int64_t)p.hp1;
-        const uint64_t b = ballot(good);
-        if (b) {
-          if (fg == INT32_MAX) fg = kb + __builtin_ctzll(b);
-          lg = kb + 63 - __builtin_clzll(b);
-        }
-      }
-      // the low end moves up to the first close diagonal, stopping at the
-      // end diagonal; then the high end down, stopping there or at the new lo
-      const int ak = tlen - plen;
-      const int top = imin(ak, hi);
-      nlo = top <= lo ? lo : imin(fg, top);
-      const int bottom = imax(ak, nlo);
-      nhi = hi <= bottom ? hi : imax(lg, bottom);
-      steps_wait = p.hp2;
-    } else if (heur == WFA_HEURISTIC_BANDED_STATIC) {
-      // src/heuristic.rs:400-407
-      nlo = imax(lo, p.hp0);
-      nhi = imin(hi, p.hp1);
-    } else if (heur == WFA_HEURISTIC_BANDED_ADAPTIVE) {
-      // src/heuristic.rs:355-394
-      if (steps_wait > 0) return;
-      const int len = hi - lo + 1;
-      if (len < 4) return;
-      const int maxlen = p.hp1 - p.hp0 + 1;
-      if (len > maxlen) {
-        const int leeway = (len - maxlen) / 2, q = len / 4;
-        const int d0 = dist(uniform(cell(lo)), lo);
-        const int d1 = dist(uniform(cell(lo + q)), lo + q);
-        const int d2 = dist(uniform(cell(lo + 2 * q)), lo + 2 * q);
-        const int d3 = dist(uniform(cell(hi)), hi);
-        int nl = lo;
-        if (d0 > d3) nl += leeway;
-        if (d1 > d2) nl += leeway;
-        nlo = imax(nl, lo);
-        nhi = imin(nl + maxlen - 1, hi);
-      }
-      steps_wait = p.hp2;
-    } else if (heur == WFA_HEURISTIC_XDROP) {
-      // src/heuristic.rs:410-448, 491-525: the Smith-Waterman-like score of
-      // each cell (free begin gaps not counted), cells more than `xdrop`
-      // below the best seen so far cut from both ends
-      if (steps_wait > 0) return;
-      const int swg = p.u_match < 0 ? -p.u_match : 1;
-      int64_t best = INT64_MIN;  // (sw << 32) | (~biased k): max sw, first k
-      int fg = INT32_MAX, lg = INT32_MIN;
-      for (int kb = lo; kb <= hi; kb += 64) {
-        const int k = kb + lane;
-        bool good = false;
-        if (k <= hi) {
-          const int off = cell(k);
-          if (off >= 0) {
-            const int fv = k < 0 ? imin(-k, pb_free) : 0;
-            const int fh = k > 0 ? imin(k, tb_free) : 0;
-            const int sw = (swg * ((off - k - fv) + (off - fh)) - s) / 2;
-            const int64_t key = ((int64_t)sw << 32) | (uint32_t)(0x7fffffffu - (uint32_t)k);
-            best = best > key ? best : key;
-            good = have_max && max_sw - sw < p.hp0;
-          }
-        }
-        const uint64_t b = ballot(good);
-        if (b) {
-          if (fg == INT32_MAX) fg = kb + __builtin_ctzll(b);
-          lg = kb + 63 - __builtin_clzll(b);
-        }
-      }
-      best = wave_max64(best);
-      const int cmax = (int)(best >> 32);
-      if (have_max) {
-        nlo = fg != INT32_MAX ? fg : hi + 1;
-        nhi = lg != INT32_MIN ? lg : hi;
-        if (cmax > max_sw) max_sw = cmax;
-      } else {
-        max_sw = cmax;
-        have_max = true;
-      }
-      steps_wait = p.hp1;
-    }
+    if (!heuristic_limits(p, heur, p.hp0, p.hp1, hs, R, rowM, lo, hi, s, plen, tlen, pb_free,
+                          tb_free, &nlo, &nhi))
+      return;
     if (nlo == lo && nhi == hi) return;
     const bool empty = nlo > nhi;
     __syncthreads();  // every lane has read M's cells and metadata

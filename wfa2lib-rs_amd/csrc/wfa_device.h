@@ -59,7 +59,75 @@ struct KParams {
   int chain_cap;
 };
 
+// ---------------------------------------------------------------------
+// BiWFA (ultralow memory)
+// ---------------------------------------------------------------------
+// wavefront components (ComponentType, src/bialign.rs:10-17)
+enum { C_M = 0, C_I1 = 1, C_D1 = 2, C_I2 = 3, C_D2 = 4 };
+
+// Outcome of a breakpoint search or a base case (BiStatus, src/bialign.rs)
+enum { BI_OK = 0, BI_END_REACHED = 1, BI_UNREACHABLE = 2, BI_MAX_STEPS = 3,
+       BI_UNATTAINABLE = 4, BI_RETRY = 5 };
+
+// One sub-problem: pattern [pb, pe) x text [tb, te) of pair `pair`,
+// starting in component cbeg and ending in cend (0 M, 1 I1, 2 D1, 3 I2, 4 D2)
+struct BiJob {
+  int pair, pb, pe, tb, te, cbeg, cend, pad;
+};
+
+struct BiResult {
+  int status;      // BI_*
+  int score;       // breakpoint score (BI_OK)
+  int score_fwd, score_rev;
+  int k_fwd, off_fwd;  // breakpoint cell in forward coordinates
+  int comp;        // breakpoint component
+  int end_score;   // BI_END_REACHED: the end score
+};
+
+struct BiParams {
+  KParams k;                 // penalties, heuristic, max steps, sequences
+  const BiJob* jobs;
+  int njobs;
+  int32_t* counters;         // [0] work-queue head
+  int seq_words;             // LDS words per staged sequence
+  int wcap;                  // ring width, power of two
+  int32_t* ring;             // rings: per slot, (2 directions or 1) x NC x scope rows
+  // breakpoint search
+  BiResult* results;
+  // base cases: full storage and fragments
+  int4* hdr;                 // per slot, hdr_cap scores x NC components
+  int hdr_cap;
+  int32_t* hist;             // per slot, hist_cap cells
+  int64_t hist_cap;
+  uint8_t* frag;             // CIGAR fragment of job i at frag + frag_off[i]
+  const int64_t* frag_off;
+  int32_t* frag_len;
+  int32_t* frag_status;      // BI_OK, BI_UNATTAINABLE or BI_RETRY
+  int32_t* frag_score;
+};
+
+// A piece of an assembled BiWFA CIGAR: `len` bytes from frag + src, or
+// (src < 0) `len` copies of `op`, to cig + dst
+struct BiPiece {
+  int64_t dst, src;
+  int32_t len, op;
+};
+
 using KernelFn = void (*)(KParams);
+using KernelFn2 = void (*)(BiParams);
+// which: 0 breakpoint search, 1 base case
+KernelFn2 pick_biwfa_edit(int which, bool heur);
+KernelFn2 pick_biwfa_affine(int which, bool heur);
+KernelFn2 pick_biwfa_affine2p(int which, bool heur);
+hipError_t launch_biwfa(int kind, int which, bool heur, const BiParams& bp, int grid,
+                        size_t lds_bytes, hipStream_t stream);
+int biwfa_blocks_per_cu(int kind, int which, bool heur, size_t lds_bytes);
+hipError_t launch_biwfa_gather(const BiPiece* pieces, int n, const uint8_t* frag, uint8_t* cig,
+                               hipStream_t stream);
+// BiWFA CIGAR score (cigar_wfa_score, src/aligner.rs:2700-2719) of pairs
+// pairs[0..n): into p.score
+hipError_t launch_biwfa_rescore(const KParams& p, const int32_t* pairs, int n,
+                                hipStream_t stream);
 // the alignment kernel for one configuration, from the kind's kernel set
 // (mode: 0 score only, 1 high memory CIGAR, 2 medium/low memory CIGAR)
 KernelFn pick_edit(int mode, bool lds, bool i16, bool packed, bool heur);

@@ -1,0 +1,230 @@
+// wfa_engine.h — host-side internals shared by the C ABI (wfa_api.cpp)
+// and the BiWFA driver (wfa_biwfa.cpp): penalties, the supported
+// configurations, the engine object and launch-parameter helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <mutex>
+
+#include "wfa_device.h"
+
+namespace wfahost {
+
+using wfa::KParams;
+
+// ---- penalties: WavefrontPenalties::new_* (src/penalties.rs:122-296) ----
+struct Internal {
+  int metric, match, x, o1, e1, o2, e2;
+};
+
+inline int make_internal(const wfa_penalties_t& u, Internal* p) {
+  p->metric = u.distance_metric;
+  switch (u.distance_metric) {
+    case WFA_METRIC_INDEL:
+      *p = {u.distance_metric, 0, -1, 1, -1, -1, -1};
+      return WFA_OK;
+    case WFA_METRIC_EDIT:
+      *p = {u.distance_metric, 0, 1, 1, -1, -1, -1};
+      return WFA_OK;
+    case WFA_METRIC_GAP_LINEAR:
+      if (u.match > 0 || u.mismatch <= 0 || u.gap_opening1 <= 0) return WFA_ERR_INVALID_PENALTIES;
+      if (u.match < 0)
+        *p = {u.distance_metric, u.match, 2 * u.mismatch - 2 * u.match,
+              2 * u.gap_opening1 - u.match, -1, -1, -1};
+      else
+        *p = {u.distance_metric, 0, u.mismatch, u.gap_opening1, -1, -1, -1};
+      return WFA_OK;
+    case WFA_METRIC_GAP_AFFINE:
+      if (u.match > 0 || u.mismatch <= 0 || u.gap_opening1 < 0 || u.gap_extension1 <= 0)
+        return WFA_ERR_INVALID_PENALTIES;
+      if (u.match < 0)
+        *p = {u.distance_metric, u.match, 2 * u.mismatch - 2 * u.match, 2 * u.gap_opening1,
+              2 * u.gap_extension1 - u.match, -1, -1};
+      else
+        *p = {u.distance_metric, 0, u.mismatch, u.gap_opening1, u.gap_extension1, -1, -1};
+      return WFA_OK;
+    case WFA_METRIC_GAP_AFFINE_2P:
+      if (u.match > 0 || u.mismatch <= 0 || u.gap_opening1 < 0 || u.gap_extension1 <= 0 ||
+          u.gap_opening2 < 0 || u.gap_extension2 <= 0)
+        return WFA_ERR_INVALID_PENALTIES;
+      if (u.match < 0)
+        *p = {u.distance_metric, u.match, 2 * u.mismatch - 2 * u.match, 2 * u.gap_opening1,
+              2 * u.gap_extension1 - u.match, 2 * u.gap_opening2,
+              2 * u.gap_extension2 - u.match};
+      else
+        *p = {u.distance_metric, 0, u.mismatch, u.gap_opening1, u.gap_extension1,
+              u.gap_opening2, u.gap_extension2};
+      return WFA_OK;
+    default:
+      return WFA_ERR_INVALID_PENALTIES;
+  }
+}
+
+// max_score_scope (src/components.rs:435-483)
+inline int score_scope(const Internal& p) {
+  switch (p.metric) {
+    case WFA_METRIC_INDEL:
+    case WFA_METRIC_EDIT: return 2;
+    case WFA_METRIC_GAP_LINEAR: return std::max(p.x, p.o1) + 1;
+    case WFA_METRIC_GAP_AFFINE: return std::max(p.o1 + p.e1, p.x) + 1;
+    default: return std::max(std::max(p.o1 + p.e1, p.o2 + p.e2), p.x) + 1;
+  }
+}
+
+// Which configurations the device path runs. The rest return
+// WFA_ERR_UNSUPPORTED rather than silently falling back to anything.
+inline int check_supported(const wfa_attr_t& a) {
+  const int m = a.penalties.distance_metric;
+  if (m == WFA_METRIC_GAP_LINEAR) return WFA_ERR_UNSUPPORTED;
+  if (a.span != WFA_SPAN_END2END && a.span != WFA_SPAN_ENDSFREE) return WFA_ERR_INVALID_ARGUMENT;
+  // ends-free with a match reward adds begin-free cells during compute
+  // (src/aligner.rs:3369-3389): not on the device path yet
+  if (a.span == WFA_SPAN_ENDSFREE && a.penalties.match != 0) return WFA_ERR_UNSUPPORTED;
+  if (a.span == WFA_SPAN_ENDSFREE && !a.extension &&
+      (a.pattern_begin_free < 0 || a.pattern_end_free < 0 || a.text_begin_free < 0 ||
+       a.text_end_free < 0))
+    return WFA_ERR_INVALID_ARGUMENT;
+  // Z-drop ends alignments early through its own backtrace path: not on
+  // the device path
+  if (a.heuristic < WFA_HEURISTIC_NONE || a.heuristic > WFA_HEURISTIC_BANDED_ADAPTIVE)
+    return WFA_ERR_INVALID_ARGUMENT;
+  if (a.heuristic == WFA_HEURISTIC_ZDROP) return WFA_ERR_UNSUPPORTED;
+  if (a.alignment_scope != WFA_SCOPE_SCORE && a.alignment_scope != WFA_SCOPE_ALIGNMENT)
+    return WFA_ERR_INVALID_ARGUMENT;
+  if (a.memory_mode < WFA_MEMORY_HIGH || a.memory_mode > WFA_MEMORY_ULTRALOW)
+    return WFA_ERR_INVALID_ARGUMENT;
+  // BiWFA: end-to-end only, and no X-drop / Z-drop, as the reference's
+  // driver checks (src/bin/align_benchmark.rs:529-560)
+  if (a.memory_mode == WFA_MEMORY_ULTRALOW &&
+      (a.span != WFA_SPAN_END2END || a.heuristic == WFA_HEURISTIC_XDROP))
+    return WFA_ERR_UNSUPPORTED;
+  return WFA_OK;
+}
+
+inline int pow2ceil(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+#define HIPCHK(x)                                   \
+  do {                                              \
+    hipError_t _e = (x);                            \
+    if (_e != hipSuccess) return WFA_ERR_DEVICE;    \
+  } while (0)
+
+}  // namespace wfahost
+
+// =====================================================================
+// Engine
+// =====================================================================
+struct wfa_engine_t {
+  int device = 0;
+  int num_cus = 0;
+  size_t lds_max = 0;
+  int32_t* d_counters = nullptr;  // [0] queue head, [1] retries, [2] scratch
+  int32_t* d_lists[2] = {nullptr, nullptr};
+  size_t list_cap = 0;
+  int32_t* h_pinned = nullptr;
+  // scratch
+  int32_t* d_hist = nullptr;
+  size_t hist_bytes = 0;
+  int4* d_hdr = nullptr;
+  size_t hdr_bytes = 0;
+  int32_t* d_gring = nullptr;
+  size_t gring_bytes = 0;
+  uint8_t* d_chain = nullptr;
+  size_t chain_bytes = 0;
+  int last_retries = 0;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  float first_ms = 0.f, all_ms = 0.f;
+  // BiWFA scratch (wfa_biwfa.cpp), grown on demand
+  static constexpr int kBiBuffers = 12;
+  void* bi_buf[kBiBuffers] = {};
+  size_t bi_cap[kBiBuffers] = {};
+  std::mutex mu;
+};
+
+namespace wfahost {
+
+inline int grow(void** ptr, size_t* have, size_t need) {
+  if (*have >= need) return WFA_OK;
+  if (*ptr) hipFree(*ptr);
+  *ptr = nullptr;
+  *have = 0;
+  if (hipMalloc(ptr, need) != hipSuccess) return WFA_ERR_OUT_OF_MEMORY;
+  *have = need;
+  return WFA_OK;
+}
+
+inline int kind_of(int metric) {
+  if (metric == WFA_METRIC_GAP_AFFINE) return wfa::KIND_AFFINE;
+  if (metric == WFA_METRIC_GAP_AFFINE_2P) return wfa::KIND_AFFINE2P;
+  return wfa::KIND_EDIT;
+}
+
+// Build kernel parameters shared by every launch of one batch.
+inline void base_params(const Internal& ip, const wfa_attr_t& attr, const wfa_batch_t& b, KParams* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->metric = ip.metric;
+  p->x = ip.x;
+  p->o1 = ip.o1;
+  p->e1 = ip.e1;
+  p->o2 = ip.o2;
+  p->e2 = ip.e2;
+  p->oe1 = ip.o1 + ip.e1;
+  p->oe2 = ip.o2 + ip.e2;
+  p->scope = score_scope(ip);
+  const int kind = kind_of(ip.metric);
+  p->ringM = kind == wfa::KIND_EDIT ? 2 : p->scope;
+  p->ringE1 = kind == wfa::KIND_EDIT ? 0 : ip.e1 + 1;
+  p->ringE2 = kind == wfa::KIND_AFFINE2P ? ip.e2 + 1 : 0;
+  // score-only and medium/low memory keep only the wavefronts in scope
+  // (src/aligner.rs:855-866): no score-indexed slot limit
+  p->modular = attr.alignment_scope == WFA_SCOPE_SCORE || attr.memory_mode != WFA_MEMORY_HIGH;
+  p->max_steps = attr.max_alignment_steps;
+  p->endsfree = attr.span == WFA_SPAN_ENDSFREE;
+  p->extension = p->endsfree && attr.extension;
+  p->pb_free = attr.pattern_begin_free;
+  p->pe_free = attr.pattern_end_free;
+  p->tb_free = attr.text_begin_free;
+  p->te_free = attr.text_end_free;
+  p->u_match = attr.penalties.match;
+  p->u_x = attr.penalties.mismatch;
+  p->u_o1 = attr.penalties.gap_opening1;
+  p->u_e1 = attr.penalties.gap_extension1;
+  p->u_o2 = attr.penalties.gap_opening2;
+  p->u_e2 = attr.penalties.gap_extension2;
+  p->heuristic = attr.heuristic;
+  p->hp0 = attr.heuristic_p[0];
+  p->hp1 = attr.heuristic_p[1];
+  p->hp2 = attr.heuristic_p[2];
+  p->fmt = b.seq_format;
+  p->seq = b.seq_data;
+  p->poff = b.pattern_offset;
+  p->plen = b.pattern_length;
+  p->toff = b.text_offset;
+  p->tlen = b.text_length;
+  p->score = b.score;
+  p->status = b.status;
+  p->cig = b.cigar_ops;
+  p->cig_off = b.cigar_offset;
+  p->cig_len = b.cigar_length;
+}
+
+// BiWFA batches (memory mode ULTRALOW): wfa_biwfa.cpp
+int biwfa_align_batch(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
+                      hipStream_t st, int first, int count);
+
+inline int read_counter(wfa_engine_t* e, int idx, hipStream_t st, int32_t* out) {
+  HIPCHK(hipMemcpyAsync(e->h_pinned, e->d_counters + idx, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *out = e->h_pinned[0];
+  return WFA_OK;
+}
+
+}  // namespace wfahost

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "wfa_align.h"
 #include "wfa_device.h"
 
 namespace wfa {
@@ -19,6 +20,33 @@ __global__ void wfa_max_length(const int32_t* plen, const int32_t* tlen, int n, 
     m = max(m, max(plen[i], tlen[i]));
   for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
   if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// BiWFA: copy the CIGAR pieces (fragments or gap runs) into place
+__global__ void biwfa_gather(const BiPiece* pieces, int n, const uint8_t* frag, uint8_t* cig) {
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const BiPiece pc = pieces[i];
+    for (int j = threadIdx.x; j < pc.len; j += blockDim.x)
+      cig[pc.dst + j] = pc.src >= 0 ? frag[pc.src + j] : (uint8_t)pc.op;
+  }
+}
+
+// BiWFA: cigar_wfa_score (src/aligner.rs:2700-2719) of whole CIGARs
+__global__ void __launch_bounds__(64) biwfa_rescore(KParams p, const int32_t* pairs, int n) {
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int pair = pairs[i];
+    const uint8_t* ops = p.cig + p.cig_off[pair];
+    const int len = p.cig_len[pair];
+    int score;
+    if (p.metric == WFA_METRIC_EDIT || p.metric == WFA_METRIC_INDEL) {
+      int c = 0;
+      for (int j = threadIdx.x; j < len; j += 64) c += ops[j] != 'M';
+      score = wave_sum(c);
+    } else {
+      score = rescore_cigar(p, ops, len, false, 0, 0, 0);
+    }
+    if (threadIdx.x == 0) p.score[pair] = score;
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -54,6 +82,45 @@ hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, boo
                      p.heuristic != WFA_HEURISTIC_NONE);
   if (!fn) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fn, dim3(grid), dim3(64), lds_bytes, stream, p);
+  return hipGetLastError();
+}
+
+static KernelFn2 pick_biwfa(int kind, int which, bool heur) {
+  switch (kind) {
+    case KIND_EDIT: return pick_biwfa_edit(which, heur);
+    case KIND_AFFINE: return pick_biwfa_affine(which, heur);
+    case KIND_AFFINE2P: return pick_biwfa_affine2p(which, heur);
+    default: return nullptr;
+  }
+}
+
+hipError_t launch_biwfa(int kind, int which, bool heur, const BiParams& bp, int grid,
+                        size_t lds_bytes, hipStream_t stream) {
+  KernelFn2 fn = pick_biwfa(kind, which, heur);
+  if (!fn) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(64), lds_bytes, stream, bp);
+  return hipGetLastError();
+}
+
+int biwfa_blocks_per_cu(int kind, int which, bool heur, size_t lds_bytes) {
+  KernelFn2 fn = pick_biwfa(kind, which, heur);
+  int n = 0;
+  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, lds_bytes) != hipSuccess)
+    return 1;
+  return n > 0 ? n : 1;
+}
+
+hipError_t launch_biwfa_gather(const BiPiece* pieces, int n, const uint8_t* frag, uint8_t* cig,
+                               hipStream_t stream) {
+  const int grid = std::min(n, 4096);
+  hipLaunchKernelGGL(biwfa_gather, dim3(grid), dim3(256), 0, stream, pieces, n, frag, cig);
+  return hipGetLastError();
+}
+
+hipError_t launch_biwfa_rescore(const KParams& p, const int32_t* pairs, int n,
+                                hipStream_t stream) {
+  const int grid = std::min(n, 4096);
+  hipLaunchKernelGGL(biwfa_rescore, dim3(grid), dim3(64), 0, stream, p, pairs, n);
   return hipGetLastError();
 }
 
