@@ -167,6 +167,22 @@ int32_t wfa_align_end2end(wfa_aligner_t* aligner,
 int32_t wfa_align_packed2bits(wfa_aligner_t* aligner,
                               const uint8_t* pattern, int32_t pattern_length,
                               const uint8_t* text, int32_t text_length);
+/* align_endsfree(&[u8], &[u8]) -> i32 (src/aligner.rs:707): ends-free with
+ * the free ends of wfa_aligner_set_alignment_free_ends, or extension after
+ * wfa_aligner_set_alignment_extension (all ends zero otherwise). */
+int32_t wfa_align_endsfree(wfa_aligner_t* aligner,
+                           const uint8_t* pattern, int32_t pattern_length,
+                           const uint8_t* text, int32_t text_length);
+/* align_biwfa(&[u8], &[u8]) -> i32 (src/aligner.rs:2563): BiWFA, CIGAR in
+ * O(score) memory, end-to-end. */
+int32_t wfa_align_biwfa(wfa_aligner_t* aligner,
+                        const uint8_t* pattern, int32_t pattern_length,
+                        const uint8_t* text, int32_t text_length);
+/* align_biwfa_score(&[u8], &[u8]) -> i32 (src/aligner.rs:2650): the score
+ * of the first breakpoint (or of the end, reached first), no CIGAR. */
+int32_t wfa_align_biwfa_score(wfa_aligner_t* aligner,
+                              const uint8_t* pattern, int32_t pattern_length,
+                              const uint8_t* text, int32_t text_length);
 /* status() -> AlignStatus (src/aligner.rs:834), WFA_STATUS_* ordinals. */
 int32_t wfa_aligner_status(const wfa_aligner_t* aligner);
 /* cigar() -> &Cigar (src/aligner.rs:839): operations M/X/I/D, one byte each,

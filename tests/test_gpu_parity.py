@@ -142,8 +142,22 @@ def test_aligner_api_mirrors_reference(pkg, N, engine):
     A.set_memory_mode(pkg.MemoryMode.MED)
     assert A.align_end2end(b"ACGTACGT", b"ACGAACGT") == 4
     assert A.cigar().to_string_rle(True) == "3M1X4M"
+    B = pkg.AffineAligner(pkg.WavefrontPenalties.new_affine(pkg.AffinePenalties(0, 4, 6, 2)))
+    assert B.align_biwfa(b"ACGTACGT", b"ACGAACGT") == 4
+    assert B.cigar().to_string_rle(True) == "3M1X4M"
+    assert B.align_biwfa_score(b"ACGTACGT", b"ACGAACGT") == 4
+    B.set_alignment_scope(pkg.AlignmentScope.COMPUTE_ALIGNMENT)
+    B.set_alignment_extension()
+    assert B.align_endsfree(b"ACGTACGTTTTT", b"ACGTACGTGGGG") == 14  # the oracle's score
+    assert B.status() == pkg.AlignStatus.PARTIAL
+    assert B.cigar().to_string_rle(True) == "8M"
+    # ultralow memory is BiWFA (the driver's --wfa-memory=ultralow)
+    A.set_memory_mode(pkg.MemoryMode.ULTRALOW)
+    assert A.align_end2end(b"ACGTACGT", b"ACGAACGT") == 4
+    assert A.cigar().to_string_rle(True) == "3M1X4M"
+    # heuristic rejected with BiWFA as the driver does
+    A.set_heuristic(pkg.HeuristicStrategy.xdrop(20, 1))
     with pytest.raises(RuntimeError):
-        A.set_memory_mode(pkg.MemoryMode.ULTRALOW)
         A.align_end2end(b"ACGT", b"ACGT")
 
 

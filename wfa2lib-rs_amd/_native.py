@@ -80,6 +80,9 @@ SIGNATURES = {
     "wfa_aligner_set_max_alignment_steps": (_I32, [_VP, _I32]),
     "wfa_align_end2end": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
     "wfa_align_packed2bits": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
+    "wfa_align_endsfree": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
+    "wfa_align_biwfa": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
+    "wfa_align_biwfa_score": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
     "wfa_aligner_status": (_I32, [_VP]),
     "wfa_aligner_cigar": (_I32, [_VP, _P(_P(ctypes.c_uint8))]),
     "wfa_aligner_cigar_score": (_I32, [_VP]),

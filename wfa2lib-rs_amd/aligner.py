@@ -343,6 +343,25 @@ class WavefrontAligner:
         return self._finish(N.lib().wfa_align_end2end(self._h, pattern, len(pattern), text,
                                                       len(text)))
 
+    def align_endsfree(self, pattern: bytes, text: bytes) -> int:
+        """align_endsfree (src/aligner.rs:707): the configured free ends or
+        extension."""
+        pattern, text = bytes(pattern), bytes(text)
+        return self._finish(N.lib().wfa_align_endsfree(self._h, pattern, len(pattern), text,
+                                                       len(text)))
+
+    def align_biwfa(self, pattern: bytes, text: bytes) -> int:
+        """align_biwfa (src/aligner.rs:2563): BiWFA with its CIGAR."""
+        pattern, text = bytes(pattern), bytes(text)
+        return self._finish(N.lib().wfa_align_biwfa(self._h, pattern, len(pattern), text,
+                                                    len(text)))
+
+    def align_biwfa_score(self, pattern: bytes, text: bytes) -> int:
+        """align_biwfa_score (src/aligner.rs:2650)."""
+        pattern, text = bytes(pattern), bytes(text)
+        return self._finish(N.lib().wfa_align_biwfa_score(self._h, pattern, len(pattern), text,
+                                                          len(text)))
+
     def align_packed2bits(self, pattern: bytes, pattern_length: int, text: bytes,
                           text_length: int) -> int:
         return self._finish(N.lib().wfa_align_packed2bits(

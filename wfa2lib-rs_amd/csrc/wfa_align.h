@@ -1313,8 +1313,9 @@ __global__ void __launch_bounds__(64, 5) wfa_align_kernel(KParams p) {
     if (lane == 0) item = atomicAdd(&p.counters[0], 1);
     item = uniform(__shfl(item, 0));
     if (item >= p.num_pairs) break;
-    const int pair = p.pair_list ? p.pair_list[item] : p.pair_base + item;
-    const int plen = p.plen[pair], tlen = p.tlen[pair];
+    // pair-level values are uniform: keep them in scalar registers
+    const int pair = uniform(p.pair_list ? p.pair_list[item] : p.pair_base + item);
+    const int plen = uniform(p.plen[pair]), tlen = uniform(p.tlen[pair]);
     const int bits = stage_sequences(p, pair, plen, tlen, sP, sT);
     if (bits == 0) {
       if (lane == 0) {

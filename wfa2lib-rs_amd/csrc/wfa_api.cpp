@@ -31,6 +31,7 @@ struct LaunchCfg {
   bool cig;
   bool lds_ring;
   bool i16;
+  bool window;  // the short-read window kernel
   int wcap;
   int seq_words;
   int ring_rows;
@@ -72,9 +73,25 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   if (!c->lds_ring) c->i16 = false;
   c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : 0);
   if (c->lds_bytes > e->lds_max) return WFA_ERR_UNSUPPORTED;
-  const int per_cu =
-      wfa::max_blocks_per_cu(kind, mode, c->lds_ring, c->i16, fmt == WFA_SEQ_PACKED2BITS,
-                             p.heuristic != WFA_HEURISTIC_NONE, c->lds_bytes);
+  // First pass of gap-affine end-to-end score-only alignments without a
+  // heuristic: the window kernel; pairs that leave its window come back for
+  // the general kernel. (With CIGARs the general kernel measures faster.)
+  c->window = pass == 0 && kind == wfa::KIND_AFFINE && mode == 0 &&
+              p.heuristic == WFA_HEURISTIC_NONE && !p.endsfree && max_len <= 8000;
+  int per_cu;
+  if (c->window) {
+    c->seq_words = ((max_len + 31) / 32 + 2 + 1) & ~1;
+    c->lds_bytes = (size_t)2 * c->seq_words * 8 + wfa::window_lds_bytes_host(c->ring_rows);
+    if (c->lds_bytes > e->lds_max) c->window = false;
+  }
+  if (c->window) {
+    per_cu = wfa::window_blocks_per_cu(mode, fmt == WFA_SEQ_PACKED2BITS, c->lds_bytes);
+  } else {
+    c->seq_words = ((max_len + per - 1) / per + 2 + 1) & ~1;
+    c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : 0);
+    per_cu = wfa::max_blocks_per_cu(kind, mode, c->lds_ring, c->i16, fmt == WFA_SEQ_PACKED2BITS,
+                                    p.heuristic != WFA_HEURISTIC_NONE, c->lds_bytes);
+  }
   int grid = e->num_cus * per_cu;
   if (pass > 0) grid = std::min(grid, e->num_cus * 2);
   grid = std::max(1, std::min(grid, npairs));
@@ -139,7 +156,10 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
     }
   }
   HIPCHK(wfa::launch_reset(e->d_counters, st));
-  HIPCHK(wfa::launch_align(p, c.kind, c.mode, c.lds_ring, c.i16, c.grid, c.lds_bytes, st));
+  if (c.window)
+    HIPCHK(wfa::launch_window(p, c.mode, c.grid, c.lds_bytes, st));
+  else
+    HIPCHK(wfa::launch_align(p, c.kind, c.mode, c.lds_ring, c.i16, c.grid, c.lds_bytes, st));
   return WFA_OK;
 }
 
@@ -552,6 +572,47 @@ int32_t wfa_align_end2end(wfa_aligner_t* a, const uint8_t* P, int32_t plen, cons
   a->err = aligner_run(a, P, plen, T, tlen, WFA_SEQ_ASCII);
   a->attr = saved;
   return a->err ? INT32_MIN : a->score;
+}
+
+int32_t wfa_align_endsfree(wfa_aligner_t* a, const uint8_t* P, int32_t plen, const uint8_t* T,
+                           int32_t tlen) {
+  if (!a) return INT32_MIN;
+  // align_endsfree runs the ends-free loop whatever the span; an aligner
+  // never given free ends has them all zero (src/aligner.rs:707-718)
+  wfa_attr_t saved = a->attr;
+  if (a->attr.span != WFA_SPAN_ENDSFREE) {
+    a->attr.span = WFA_SPAN_ENDSFREE;
+    a->attr.extension = 0;
+    a->attr.pattern_begin_free = a->attr.pattern_end_free = 0;
+    a->attr.text_begin_free = a->attr.text_end_free = 0;
+  }
+  a->err = aligner_run(a, P, plen, T, tlen, WFA_SEQ_ASCII);
+  a->attr = saved;
+  return a->err ? INT32_MIN : a->score;
+}
+
+// BiWFA through the batch path: ultralow memory, end-to-end
+static int32_t biwfa_call(wfa_aligner_t* a, const uint8_t* P, int32_t plen, const uint8_t* T,
+                          int32_t tlen, int32_t scope) {
+  if (!a) return INT32_MIN;
+  wfa_attr_t saved = a->attr;
+  a->attr.memory_mode = WFA_MEMORY_ULTRALOW;
+  a->attr.alignment_scope = scope;
+  a->attr.span = WFA_SPAN_END2END;
+  a->attr.extension = 0;
+  a->err = aligner_run(a, P, plen, T, tlen, WFA_SEQ_ASCII);
+  a->attr = saved;
+  return a->err ? INT32_MIN : a->score;
+}
+
+int32_t wfa_align_biwfa(wfa_aligner_t* a, const uint8_t* P, int32_t plen, const uint8_t* T,
+                        int32_t tlen) {
+  return biwfa_call(a, P, plen, T, tlen, WFA_SCOPE_ALIGNMENT);
+}
+
+int32_t wfa_align_biwfa_score(wfa_aligner_t* a, const uint8_t* P, int32_t plen,
+                              const uint8_t* T, int32_t tlen) {
+  return biwfa_call(a, P, plen, T, tlen, WFA_SCOPE_SCORE);
 }
 
 int32_t wfa_align_packed2bits(wfa_aligner_t* a, const uint8_t* P, int32_t plen, const uint8_t* T,

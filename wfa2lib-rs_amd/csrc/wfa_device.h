@@ -133,6 +133,10 @@ hipError_t launch_biwfa_rescore(const KParams& p, const int32_t* pairs, int n,
 KernelFn pick_edit(int mode, bool lds, bool i16, bool packed, bool heur);
 KernelFn pick_affine(int mode, bool lds, bool i16, bool packed, bool heur);
 KernelFn pick_affine2p(int mode, bool lds, bool i16, bool packed, bool heur);
+// the short-read window kernel (gap-affine, end-to-end, no heuristic;
+// mode 0 or 1)
+KernelFn pick_affine_window(int mode, bool packed);
+size_t window_lds_bytes_host(int ring_rows);
 
 hipError_t launch_reset(int32_t* counters, hipStream_t stream);
 hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, int32_t* out,
@@ -143,5 +147,7 @@ hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, boo
                         size_t lds_bytes, hipStream_t stream);
 int max_blocks_per_cu(int kind, int mode, bool lds_ring, bool i16, bool packed, bool heur,
                       size_t lds_bytes);
+hipError_t launch_window(const KParams& p, int mode, int grid, size_t lds_bytes, hipStream_t stream);
+int window_blocks_per_cu(int mode, bool packed, size_t lds_bytes);
 
 }  // namespace wfa

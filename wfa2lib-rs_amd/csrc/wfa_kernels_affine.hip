@@ -2,12 +2,20 @@
 // kind so the sets compile in parallel).
 #include "wfa_align.h"
 #include "wfa_biwfa.h"
+#include "wfa_window.h"
 
 namespace wfa {
 
 KernelFn pick_affine(int mode, bool lds, bool i16, bool packed, bool heur) {
   return pick_kind<KIND_AFFINE>(mode, lds, i16, packed, heur);
 }
+
+KernelFn pick_affine_window(int mode, bool packed) {
+  if (mode == 1) return packed ? wfa_window_kernel<1, true> : wfa_window_kernel<1, false>;
+  return packed ? wfa_window_kernel<0, true> : wfa_window_kernel<0, false>;
+}
+
+size_t window_lds_bytes_host(int ring_rows) { return window_lds_bytes(ring_rows); }
 
 KernelFn2 pick_biwfa_affine(int which, bool heur) {
   return pick_biwfa_kind<KIND_AFFINE>(which, heur);

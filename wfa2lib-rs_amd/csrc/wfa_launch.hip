@@ -124,6 +124,20 @@ hipError_t launch_biwfa_rescore(const KParams& p, const int32_t* pairs, int n,
   return hipGetLastError();
 }
 
+hipError_t launch_window(const KParams& p, int mode, int grid, size_t lds_bytes,
+                         hipStream_t stream) {
+  KernelFn fn = pick_affine_window(mode, p.fmt == WFA_SEQ_PACKED2BITS);
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(64), lds_bytes, stream, p);
+  return hipGetLastError();
+}
+
+int window_blocks_per_cu(int mode, bool packed, size_t lds_bytes) {
+  KernelFn fn = pick_affine_window(mode, packed);
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, lds_bytes) != hipSuccess) return 1;
+  return n > 0 ? n : 1;
+}
+
 int max_blocks_per_cu(int kind, int mode, bool lds_ring, bool i16, bool packed, bool heur,
                       size_t lds_bytes) {
   KernelFn fn = pick(kind, mode, lds_ring, i16, packed, heur);
