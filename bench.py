@@ -1,19 +1,26 @@
 #!/usr/bin/env python3
-"""Benchmark: batched gap-affine WFA with bit-exact CIGARs on one MI355X.
+"""Benchmark: batched wavefront alignment with bit-exact CIGARs on one MI355X.
 
-Workload (one "step" = one pass over the whole batch): 1,000,000 pairs,
-gap-affine (match 0, mismatch 4, open 6, extend 2), pattern lengths uniform
-100-300 bp of random ACGT, per-pair divergence uniform 0-10% split equally
-among substitution, insertion and deletion (indel lengths geometric, mean
-2), 2-bit packed input, end-to-end, full CIGAR in high-memory mode. Inputs
-are resident in HBM before timing starts. The same run reports the
-score-only pass, the plain one-pair-per-launch GPU path on a sample, the
-CPU oracle on the host cores, the roofline figures of the dominant kernel
-and a parity check of a sample against the oracle.
+The default workload (one "step" = one pass over the whole batch) is
+reads100_300: 1,000,000 pairs, gap-affine (match 0, mismatch 4, open 6,
+extend 2), pattern lengths uniform 100-300 bp of random ACGT, per-pair
+divergence uniform 0-10% split equally among substitution, insertion and
+deletion (indel lengths geometric, mean 2), 2-bit packed input, end-to-end,
+full CIGAR in high-memory mode. Inputs are resident in HBM before timing
+starts. The same run reports the score-only pass, the plain
+one-pair-per-launch GPU path on a sample, the CPU oracle on the host cores,
+the roofline figures of the dominant kernel and a parity check of a sample
+against the oracle.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--pairs N]
+--workload selects the other configurations: short150 (10k x 150 bp ASCII,
+gap-affine, CIGAR high), long1k_10k (100k x 1-10 kbp, gap-affine 2-piece
+(0,4,6,2,24,1), CIGAR medium memory) and ultralong50k_100k (1k x 50-100 kbp,
+BiWFA, edit or --metric gap_affine).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--pairs N]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -25,17 +32,43 @@ sys.path.insert(0, ROOT)
 METRIC = "aligned pairs/sec (gap-affine, 150bp + 1-10kbp, bit-exact CIGAR) + p99 latency"
 HBM_PEAK_GBS = 8000.0
 
+# memory modes: 0 high, 1 medium, 3 ultralow (BiWFA)
+WORKLOADS = {
+    "reads100_300": dict(metric="gap_affine", pen=(0, 4, 6, 2, 0, 0), memory=0, pairs=1_000_000,
+                         steps=5, cpu_sample=500_000, parity=4000, plain=2000, latency=200,
+                         desc="{n} pairs gap-affine (0,4,6,2), 100-300 bp, divergence 0-10%, "
+                              "2-bit packed, end-to-end, CIGAR high memory, one persistent "
+                              "launch per step"),
+    "short150": dict(metric="gap_affine", pen=(0, 4, 6, 2, 0, 0), memory=0, pairs=10_000,
+                     steps=10, cpu_sample=10_000, parity=10_000, plain=1000, latency=200,
+                     desc="{n} pairs gap-affine (0,4,6,2), 150 bp, divergence 5%, ASCII, "
+                          "end-to-end, CIGAR high memory"),
+    "long1k_10k": dict(metric="gap_affine_2p", pen=(0, 4, 6, 2, 24, 1), memory=1,
+                       pairs=100_000, steps=2, cpu_sample=256, parity=256, plain=20, latency=20,
+                       desc="{n} pairs gap-affine 2-piece (0,4,6,2,24,1), 1-10 kbp log-uniform, "
+                            "divergence 1-10%, 2-bit packed, end-to-end, CIGAR medium memory"),
+    "ultralong50k_100k": dict(metric="edit", pen=(0, 0, 0, 0, 0, 0), memory=3, pairs=1000,
+                              steps=1, warmup=0, cpu_sample=2, parity=2, plain=0, latency=0,
+                              desc="{n} pairs {metric}, 50-100 kbp, divergence 5-15% with 20% "
+                                   "indel runs, 2-bit packed, BiWFA (ultralow memory) CIGAR"),
+}
+METRIC_PEN = {"edit": (0, 0, 0, 0, 0, 0), "gap_affine": (0, 4, 6, 2, 0, 0),
+              "gap_affine_2p": (0, 4, 6, 2, 24, 1)}
+
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--pairs", type=int, default=1_000_000)
-    ap.add_argument("--plain-sample", type=int, default=2000)
-    ap.add_argument("--cpu-sample", type=int, default=500_000)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="reads100_300", choices=sorted(WORKLOADS))
+    ap.add_argument("--metric", default=None, choices=sorted(METRIC_PEN),
+                    help="override the workload's distance metric")
+    ap.add_argument("--pairs", type=int, default=None)
+    ap.add_argument("--plain-sample", type=int, default=None)
+    ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--parity-sample", type=int, default=4000)
+    ap.add_argument("--parity-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (calibrates FETCH_SIZE/WRITE_SIZE)")
@@ -62,84 +95,105 @@ def main():
     import __graft_entry__ as entry
     pkg = entry.load_package()
     from wfa2lib_rs_amd import _native as N
-    from wfa2lib_rs_amd.engine import BatchEngine, DeviceBatch, make_attr
-    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    from wfa2lib_rs_amd.engine import BatchEngine, DeviceBatch
+    from wfa2lib_rs_amd.synth import HostPairs, WORKLOADS as GEN, generate
 
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
-    w = WORKLOADS["reads100_300"]
+    cfg = dict(WORKLOADS[args.workload])
+    if args.metric:
+        cfg["metric"] = args.metric
+        cfg["pen"] = METRIC_PEN[args.metric]
+    steps = args.steps if args.steps is not None else cfg["steps"]
+    warmup = args.warmup if args.warmup is not None else cfg.get("warmup", 1)
+    npairs = args.pairs if args.pairs is not None else cfg["pairs"]
     t0 = time.time()
-    pairs = generate(w, num_pairs=args.pairs)
-    log(f"generated {len(pairs)} pairs in {time.time() - t0:.1f}s")
-    pen = pkg.WavefrontPenalties.new_affine(pkg.AffinePenalties(0, 4, 6, 2))
-    attr_cig = make_attr(pen, scope=N.SCOPE_ALIGNMENT)
-    attr_score = make_attr(pen, scope=N.SCOPE_SCORE)
+    pairs = generate(GEN[args.workload], num_pairs=npairs)
+    log(f"{args.workload}: generated {len(pairs)} pairs in {time.time() - t0:.1f}s")
+
+    def attr_of(scope):
+        a = N.Attr()
+        N.lib().wfa_attr_default(ctypes.byref(a),
+                                 {"edit": 1, "gap_affine": 3, "gap_affine_2p": 4}[cfg["metric"]])
+        m, x, o1, e1, o2, e2 = cfg["pen"]
+        if cfg["metric"] != "edit":
+            a.penalties.match, a.penalties.mismatch = m, x
+            a.penalties.gap_opening1, a.penalties.gap_extension1 = o1, e1
+            a.penalties.gap_opening2, a.penalties.gap_extension2 = o2, e2
+        a.alignment_scope = scope
+        a.memory_mode = cfg["memory"]
+        return a
+
+    attr_cig = attr_of(N.SCOPE_ALIGNMENT)
+    attr_score = attr_of(N.SCOPE_SCORE)
     eng = BatchEngine(0)
     batch = DeviceBatch(pairs, device=dev, cigar=True)
-    stream = torch.cuda.current_stream()
 
-    def timed(attr, steps, warmup):
+    def timed(attr, n_steps, warmup):
         for _ in range(warmup):
             eng.align(attr, batch)
         torch.cuda.synchronize()
         kms = []
         t = time.perf_counter()
-        for _ in range(steps):
+        for _ in range(n_steps):
             eng.align(attr, batch)
             kms.append(eng.last_kernel_ms())
         torch.cuda.synchronize()
-        return (time.perf_counter() - t) / steps, kms
+        return (time.perf_counter() - t) / n_steps, kms
 
     # ---- CIGAR mode: the headline ----
-    sec, kms = timed(attr_cig, args.steps, args.warmup)
+    sec, kms = timed(attr_cig, steps, warmup)
     retries = eng.last_retries()
     pps = len(pairs) / sec
-    log(f"CIGAR: {sec * 1e3:.1f} ms/step, {pps:,.0f} pairs/s, retries {retries}")
-    score, status, _ = batch.score.cpu().numpy()[:len(pairs)], batch.status.cpu().numpy()[:len(pairs)], None
+    log(f"CIGAR: {sec * 1e3:.1f} ms/step, {pps:,.1f} pairs/s, retries {retries}")
+    score = batch.score.cpu().numpy()[:len(pairs)]
+    status = batch.status.cpu().numpy()[:len(pairs)]
     cig_len = batch.cig_len.cpu().numpy()[:len(pairs)].astype(np.int64)
     assert (status == N.STATUS_COMPLETED).all(), "not every pair completed"
 
-    # algorithmic bytes of one launch: packed input + lengths + score/status + CIGAR ops
-    in_bytes = ((pairs.plen.astype(np.int64) + 3) // 4 + (pairs.tlen.astype(np.int64) + 3) // 4).sum()
+    # algorithmic bytes of one launch: input sequences + lengths + score/status + CIGAR ops
+    if pairs.seq_format == N.SEQ_PACKED2BITS:
+        in_bytes = ((pairs.plen.astype(np.int64) + 3) // 4 + (pairs.tlen.astype(np.int64) + 3) // 4).sum()
+    else:
+        in_bytes = (pairs.plen.astype(np.int64) + pairs.tlen.astype(np.int64)).sum()
     alg_bytes = int(in_bytes + 8 * len(pairs) + 5 * len(pairs) + cig_len.sum())
     first_ms = float(np.mean([k[0] for k in kms]))
     all_ms = float(np.mean([k[1] for k in kms]))
     achieved = alg_bytes / (first_ms * 1e-3) / 1e9
 
     # ---- score-only ----
-    sec_s, kms_s = timed(attr_score, max(1, args.steps // 2), 1)
+    sec_s, _ = timed(attr_score, max(1, steps // 2), min(warmup, 1))
     pps_score = len(pairs) / sec_s
-    log(f"score-only: {sec_s * 1e3:.1f} ms/step, {pps_score:,.0f} pairs/s")
+    log(f"score-only: {sec_s * 1e3:.1f} ms/step, {pps_score:,.1f} pairs/s")
 
-    # ---- plain per-pair path on a sample ----
-    sub = pairs.subset(range(args.plain_sample))
-    sub_batch = DeviceBatch(sub, device=dev, cigar=True)
-    eng.align(attr_cig, sub_batch, per_pair=True)  # warm
-    torch.cuda.synchronize()
-    lat = []
-    t = time.perf_counter()
-    from wfa2lib_rs_amd.synth import from_ascii
-    for i in range(min(200, args.plain_sample)):
-        one = DeviceBatch(pairs.subset([i]), device=dev, cigar=True)
+    # ---- plain per-pair path on a sample, and single-pair latency ----
+    n_plain = min(args.plain_sample if args.plain_sample is not None else cfg["plain"], len(pairs))
+    lat, plain_pps = [], None
+    if n_plain > 0:
+        sub_batch = DeviceBatch(pairs.subset(range(n_plain)), device=dev, cigar=True)
+        eng.align(attr_cig, DeviceBatch(pairs.subset([0]), device=dev, cigar=True), per_pair=True)
         torch.cuda.synchronize()
-        a = time.perf_counter()
-        eng.align(attr_cig, one, per_pair=True)
+        for i in range(min(cfg["latency"], n_plain)):
+            one = DeviceBatch(pairs.subset([i]), device=dev, cigar=True)
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            eng.align(attr_cig, one, per_pair=True)
+            torch.cuda.synchronize()
+            lat.append((time.perf_counter() - a) * 1e3)
+        t = time.perf_counter()
+        eng.align(attr_cig, sub_batch, per_pair=True)
         torch.cuda.synchronize()
-        lat.append((time.perf_counter() - a) * 1e3)
-    t = time.perf_counter()
-    eng.align(attr_cig, sub_batch, per_pair=True)
-    torch.cuda.synchronize()
-    plain_pps = args.plain_sample / (time.perf_counter() - t)
-    log(f"plain per-pair path: {plain_pps:,.0f} pairs/s; batched/plain = {pps / plain_pps:.1f}x")
+        plain_pps = n_plain / (time.perf_counter() - t)
+        log(f"plain per-pair path: {plain_pps:,.1f} pairs/s; batched/plain = {pps / plain_pps:.1f}x")
 
     # ---- parity on a sample vs the oracle ----
     o = entry.load_oracle()
-    ps = min(args.parity_sample, len(pairs))
-    psub = pairs.subset(range(ps))
-    os_, ost, ocig = o.align_batch(attr_cig, psub, cigar=True, threads=16)
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    ps = min(args.parity_sample or cfg["parity"], len(pairs))
+    os_, ost, ocig = o.align_batch(attr_cig, pairs.subset(range(ps)), cigar=True, threads=threads)
     ops = batch.cig.cpu().numpy()
     off = batch.cig_off.cpu().numpy()
     same = 0
@@ -151,54 +205,50 @@ def main():
     # ---- CPU baseline: the oracle on the host cores, bounded sample ----
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cs = min(args.cpu_sample, len(pairs))
-        csub_idx = range(cs)
-        import numpy as _np
-        # contiguous prefix of the same workload
-        from wfa2lib_rs_amd.synth import HostPairs
-        end = int(pairs.toff[cs - 1] + (pairs.tlen[cs - 1] + 3) // 4)
+        cs = min(args.cpu_sample or cfg["cpu_sample"], len(pairs))
+        end = int(pairs.toff[cs - 1] + ((pairs.tlen[cs - 1] + 3) // 4
+                                         if pairs.seq_format == N.SEQ_PACKED2BITS
+                                         else pairs.tlen[cs - 1]))
         csub = HostPairs(pairs.seq[:end], pairs.poff[:cs], pairs.plen[:cs], pairs.toff[:cs],
                          pairs.tlen[:cs], pairs.seq_format)
+        th = min(threads, cs)
         t = time.perf_counter()
-        o.align_batch(attr_cig, csub, cigar=True, threads=threads)
+        o.align_batch(attr_cig, csub, cigar=True, threads=th)
         cpu_sec = time.perf_counter() - t
-        cpu = {"value": round(cs / cpu_sec, 1), "unit": "pairs/s", "cores": threads,
-               "kind": "port",
-               "sample": f"first {cs} pairs of the same 1M-pair workload, CIGAR, "
-                         f"{threads} threads (one oracle aligner per thread, contiguous shards), "
+        cpu = {"value": round(cs / cpu_sec, 3), "unit": "pairs/s", "cores": th, "kind": "port",
+               "sample": f"first {cs} pairs of the same {len(pairs)}-pair workload, CIGAR, "
+                         f"{th} threads (one oracle aligner per thread, contiguous shards), "
                          f"{cpu_sec:.2f} s wall"}
-        log(f"cpu oracle: {cpu['value']:,.0f} pairs/s on {threads} threads")
+        log(f"cpu oracle: {cpu['value']:,.3f} pairs/s on {th} threads")
 
     traffic = None
-    tp = os.path.join(ROOT, "profiles", "traffic_reads100_300_cigar.json")
+    tp = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_cigar.json")
     if os.path.exists(tp):
         with open(tp) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
+    dtype = "int32"  # offsets and scores; short-read rows are stored in 16 bits
     out = {
         "metric": METRIC,
-        "value": round(pps, 1),
+        "value": round(pps, 3),
         "unit": "pairs/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": round(sec * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32",
+        "dtype": dtype,
         "data": "synthetic",
         "config": {
-            "workload": f"reads100_300: {len(pairs)} pairs gap-affine (0,4,6,2), 100-300 bp, "
-                        "divergence 0-10%, 2-bit packed, end-to-end, CIGAR high memory, "
-                        "one persistent launch per step",
+            "workload": f"{args.workload}: " + cfg["desc"].format(n=len(pairs), metric=cfg["metric"]),
             "pairs": len(pairs),
-            "score_only_pairs_per_s": round(pps_score, 1),
-            "plain_per_pair_pairs_per_s": round(plain_pps, 1),
-            "batched_over_plain": round(pps / plain_pps, 2),
-            "plain_latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
-            "plain_latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+            "score_only_pairs_per_s": round(pps_score, 3),
+            "plain_per_pair_pairs_per_s": round(plain_pps, 3) if plain_pps else None,
+            "batched_over_plain": round(pps / plain_pps, 2) if plain_pps else None,
+            "plain_latency_ms_p50": round(float(np.percentile(lat, 50)), 3) if lat else None,
+            "plain_latency_ms_p99": round(float(np.percentile(lat, 99)), 3) if lat else None,
             "batch_latency_ms": round(sec * 1e3, 3),
             "retried_pairs": int(retries),
             "parity_sample": f"{same}/{ps}",

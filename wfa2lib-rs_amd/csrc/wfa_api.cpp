@@ -63,7 +63,11 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   c->seq_words = ((max_len + per - 1) / per + 2 + 1) & ~1;
   // the widest possible computed range: [-(plen+scope+1), tlen+scope+1]
   const int full = pow2ceil(2 * max_len + 2 * p.scope + 4);
-  c->wcap = pass == 0 ? std::min(256, full) : full;
+  // Short reads start narrow and re-run the few pairs that outgrow it; long
+  // reads (whose wavefronts widen by up to two diagonals per score) start at
+  // full width.
+  const bool long_reads = max_len > 1024;
+  c->wcap = pass == 0 && !long_reads ? std::min(256, full) : full;
   const size_t seq_meta = (size_t)2 * c->seq_words * 8 + (size_t)c->ring_rows * 16;
   // 16-bit offsets for pairs of at most 8000 bases (the kernel re-runs any
   // pair whose score passes 16000 with 32-bit offsets)
@@ -93,30 +97,36 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
                                     p.heuristic != WFA_HEURISTIC_NONE, c->lds_bytes);
   }
   int grid = e->num_cus * per_cu;
-  if (pass > 0) grid = std::min(grid, e->num_cus * 2);
   grid = std::max(1, std::min(grid, npairs));
   c->grid = grid;
+  int64_t per_slot = c->lds_ring ? 0 : (int64_t)c->ring_rows * c->wcap * 4;
   if (cig) {
-    if (pass == 0) {
+    // worst case: every score up to the slot bound, every row full width
+    const int64_t nwf_bound = (int64_t)std::max(2 * max_len, 1000) * 64 + 16;
+    if (pass == 0 && !long_reads) {
       c->hdr_cap = 2048;
       c->hist_cap = 128 * 1024;
     } else {
-      // worst case: every score up to the slot bound, every row full width
-      const int64_t nwf_bound = (int64_t)std::max(2 * max_len, 1000) * 64 + 16;
-      c->hdr_cap = (int)std::min<int64_t>(nwf_bound, (int64_t)2048 << (2 * pass));
-      c->hist_cap = std::min<int64_t>((int64_t)c->hdr_cap * c->wcap, (int64_t)(128 * 1024) << (3 * pass));
+      // long reads: scores up to twice the longest sequence and 48 MiB of
+      // history per slot first, x4 per later pass
+      const int g = long_reads ? pass : pass - 1;
+      c->hdr_cap = (int)std::min<int64_t>(nwf_bound, (int64_t)std::max(2048, 2 * max_len) << (2 * g));
+      c->hist_cap = std::min<int64_t>((int64_t)c->hdr_cap * c->wcap,
+                                      (long_reads ? (int64_t)12 << 20 : (int64_t)128 << 10) << (2 * g));
     }
-    // medium/low memory keeps one flag byte per cell instead of an offset
+    // medium/low memory keeps one flag byte per cell instead of an offset:
+    // the same bytes hold four times the cells
     if (mode == 2) c->hist_cap *= 4;
-    // keep the scratch of one launch under ~16 GiB
-    const int64_t per_slot = c->hist_cap * (mode == 2 ? 1 : 4) + (int64_t)c->hdr_cap * 16 +
-                             (mode == 2 ? 4 * (int64_t)max_len + 64 : 0);
-    const int64_t budget = (int64_t)16 << 30;
-    if ((int64_t)c->grid * per_slot > budget) c->grid = (int)std::max<int64_t>(1, budget / per_slot);
+    per_slot += c->hist_cap * (mode == 2 ? 1 : 4) + (int64_t)c->hdr_cap * 16 +
+                (mode == 2 ? 4 * (int64_t)max_len + 64 : 0);
   } else {
     c->hdr_cap = 0;
     c->hist_cap = 0;
   }
+  // keep the scratch of one launch under 48 GiB (of the 288 GB of HBM)
+  const int64_t budget = (int64_t)48 << 30;
+  if (per_slot > 0 && (int64_t)c->grid * per_slot > budget)
+    c->grid = (int)std::max<int64_t>(1, budget / per_slot);
   // op chain of one pair: at most every base consumed by a gap op plus one
   // gap-closing mismatch per gap
   c->chain_cap = 4 * max_len + 64;
