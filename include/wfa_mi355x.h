@@ -257,6 +257,68 @@ int32_t wfa_engine_last_retries(const wfa_engine_t* engine);
 int32_t wfa_engine_last_kernel_ms(const wfa_engine_t* engine, float* first_pass_ms,
                                   float* all_passes_ms);
 
+/* ======================================================================
+ * Serving loop: continuous batching of independent requests
+ * (replaces one reference aligner per request, align_end2end /
+ * align_endsfree / align_biwfa per call, src/aligner.rs:620, 707, 2563)
+ * ====================================================================== */
+typedef struct wfa_server_t wfa_server_t;
+
+typedef struct wfa_server_config_t {
+    int32_t device;              /* HIP device ordinal */
+    int32_t num_engines;         /* launches that may run at once, each on its own
+                                    stream with its own scratch (1..16, default 4) */
+    int32_t max_batch_pairs;     /* most requests one launch takes (default 65536) */
+    int64_t memory_budget;       /* predicted device bytes of the work in flight
+                                    (default 32 GiB); also split between the
+                                    engines as their scratch budget */
+} wfa_server_config_t;
+
+/* One request: its own configuration and sequences (host memory, copied by
+ * wfa_server_submit). The span, heuristic, memory mode and scope of `attr`
+ * act as the reference's setters would on a fresh aligner. */
+typedef struct wfa_server_request_t {
+    wfa_attr_t attr;
+    int32_t seq_format;          /* WFA_SEQ_* */
+    int32_t pattern_length;      /* bases */
+    int32_t text_length;
+    const uint8_t* pattern;
+    const uint8_t* text;
+    uint64_t tag;                /* the caller's id, returned with the result */
+} wfa_server_request_t;
+
+typedef struct wfa_server_result_t {
+    uint64_t tag;
+    int32_t error;               /* WFA_OK, or the WFA_ERR_* of the launch that ran it */
+    int32_t score;               /* WFA score, as the reference's align call returns */
+    int32_t status;              /* WFA_STATUS_* */
+    int32_t cigar_length;        /* alignment scope: op count */
+    const uint8_t* cigar_ops;    /* M/X/I/D bytes, valid until the next poll */
+    double latency_us;           /* from submission to the result being ready */
+} wfa_server_result_t;
+
+/* Defaults: device 0, 4 engines, 65536 requests per launch, 32 GiB. */
+void wfa_server_config_default(wfa_server_config_t* config);
+/* Start the engines and their worker threads; NULL and *err on failure. */
+wfa_server_t* wfa_server_new(const wfa_server_config_t* config, int32_t* err);
+/* Queue one request (validated as the aligner validates its configuration:
+ * WFA_ERR_INVALID_PENALTIES, WFA_ERR_UNSUPPORTED, WFA_ERR_INVALID_ARGUMENT).
+ * Requests are admitted to the device by class (configuration, format,
+ * length bucket) in arrival order while their predicted wavefront memory
+ * fits the budget; results do not depend on the batch they ran in. */
+int32_t wfa_server_submit(wfa_server_t* server, const wfa_server_request_t* request);
+/* Up to `max_results` finished requests, in completion order. Waits up to
+ * `timeout_us` (0: no wait, negative: until a result is ready or nothing is
+ * queued or running). Returns the count, or a negative WFA_ERR_*. */
+int32_t wfa_server_poll(wfa_server_t* server, wfa_server_result_t* results,
+                        int32_t max_results, int32_t timeout_us);
+/* Requests queued, requests running, launches made and requests launched. */
+int32_t wfa_server_stats(wfa_server_t* server, int64_t* queued, int64_t* running,
+                         int64_t* launches, int64_t* launched_pairs);
+/* Stop the workers (after the launches that run finish), drop queued
+ * requests and release every engine. */
+void wfa_server_free(wfa_server_t* server);
+
 /* Library version string. */
 const char* wfa_version(void);
 

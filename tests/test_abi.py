@@ -85,3 +85,41 @@ def test_cigar_helpers(pkg):
     assert c.score_gap_affine(pkg.AffinePenalties(0, 4, 6, 2)) == -4
     assert pkg.Cigar(b"MMIIM").score_gap_affine(pkg.AffinePenalties(0, 4, 6, 2)) == -10
     assert pkg.encode_packed2bits(b"ACGTTGCA") == bytes([0b11100100, 0b00011011])
+
+
+def test_server_refuses_without_device(N):
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a device is present")
+    except Exception:
+        pass
+    cfg = N.ServerConfig()
+    N.lib().wfa_server_config_default(ctypes.byref(cfg))
+    assert (cfg.num_engines, cfg.max_batch_pairs, cfg.memory_budget) == (4, 65536, 32 << 30)
+    err = ctypes.c_int32(0)
+    assert not N.lib().wfa_server_new(ctypes.byref(cfg), ctypes.byref(err))
+    assert err.value in (-5, -4)
+    cfg.num_engines = 0
+    assert not N.lib().wfa_server_new(ctypes.byref(cfg), ctypes.byref(err))
+    assert err.value == -2
+
+
+def test_serving_mix_shape(pkg):
+    """The serving mix: deterministic, lengths/spans/heuristics in their
+    stated shares, arrivals increasing."""
+    from wfa2lib_rs_amd.synth import SERVING_HEURISTICS, serving_mix
+    lengths = ((0.7, 0, 150, 150), (0.3, 2, 1000, 2000))
+    a = serving_mix(400, 1000.0, seed=1, lengths=lengths)
+    b = serving_mix(400, 1000.0, seed=1, lengths=lengths)
+    assert [(r.pattern, r.text, r.span, r.heuristic) for r in a] == \
+        [(r.pattern, r.text, r.span, r.heuristic) for r in b]
+    short = sum(len(r.pattern) == 150 for r in a)
+    assert 240 <= short <= 320
+    assert all(1000 <= len(r.pattern) <= 2000 for r in a if len(r.pattern) != 150)
+    spans = [r.span for r in a]
+    assert 150 <= spans.count("end2end") <= 250 and spans.count("extension") > 60
+    assert set(r.heuristic for r in a) == set(SERVING_HEURISTICS)
+    t = [r.arrival_s for r in a]
+    assert all(x < y for x, y in zip(t, t[1:]))
+    assert 0.25 < t[-1] < 0.6  # 400 arrivals at 1000/s

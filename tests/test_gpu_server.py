@@ -1,0 +1,160 @@
+"""Serving loop on the GPU (continuous batching): every request's result is
+bit-exact with the oracle and identical to what the same request gets when
+it is aligned alone, whatever launch it shared; requests submitted while
+others run are admitted and finish."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import attr_for, have_gpu
+
+pytestmark = pytest.mark.gpu
+
+# a small serving mix: 150 bp reads and 1-3 kbp reads
+SMALL_MIX = ((0.85, 0, 150, 150), (0.15, 2, 1000, 3000))
+
+
+@pytest.fixture(scope="module")
+def server(pkg):
+    if not have_gpu():
+        pytest.skip("no GPU")
+    from wfa2lib_rs_amd.server import AlignmentServer
+    s = AlignmentServer(device=0, num_engines=4, memory_budget=8 << 30)
+    yield s
+    s.close()
+
+
+def _oracle(oracle, reqs, attrs):
+    """Oracle results per request, grouped by configuration."""
+    from wfa2lib_rs_amd.synth import from_ascii
+    groups = {}
+    for i, a in enumerate(attrs):
+        groups.setdefault(bytes(a), []).append(i)
+    out = {}
+    for idx in groups.values():
+        a = attrs[idx[0]]
+        pairs = from_ascii([reqs[i][0] for i in idx], [reqs[i][1] for i in idx])
+        sc, st, cg = oracle.align_batch(a, pairs, cigar=a.alignment_scope == 1, threads=8)
+        for j, i in enumerate(idx):
+            out[i] = (int(sc[j]), int(st[j]), cg[j] if cg is not None else b"")
+    return out
+
+
+def _alone(engine, reqs, attrs, which):
+    """The batch engine on a one-pair launch per request."""
+    from wfa2lib_rs_amd.engine import DeviceBatch
+    from wfa2lib_rs_amd.synth import from_ascii
+    import torch
+    out = {}
+    for i in which:
+        pairs = from_ascii([reqs[i][0]], [reqs[i][1]])
+        cig = attrs[i].alignment_scope == 1
+        b = DeviceBatch(pairs, device="cuda:0", cigar=cig)
+        engine.align(attrs[i], b)
+        torch.cuda.synchronize()
+        s, st, c = b.results()
+        out[i] = (int(s[0]), int(st[0]), c[0] if c else b"")
+    return out
+
+
+def _check(results, expect, label):
+    bad = [(i, results[i], expect[i]) for i in expect if results[i] != expect[i]]
+    assert not bad, f"{label}: {len(bad)} mismatches, first {bad[0]}"
+
+
+def test_serving_mix_matches_oracle_and_alone(pkg, N, server, oracle):
+    from wfa2lib_rs_amd.engine import BatchEngine
+    from wfa2lib_rs_amd.server import request_attr
+    from wfa2lib_rs_amd.synth import serving_mix
+    mix = serving_mix(240, 1e9, seed=3, lengths=SMALL_MIX)
+    reqs = [(r.pattern, r.text) for r in mix]
+    attrs = [request_attr(r.span, r.heuristic, len(r.pattern), len(r.text)) for r in mix]
+    # score-only, medium-memory and BiWFA requests join the mix
+    for i in range(0, len(attrs), 7):
+        attrs[i].alignment_scope = N.SCOPE_SCORE
+    for i in range(3, len(attrs), 11):
+        attrs[i].memory_mode = N.MEMORY_MED
+    for i in range(5, len(attrs), 13):
+        if attrs[i].heuristic != N.HEURISTIC_XDROP:
+            attrs[i].memory_mode = N.MEMORY_ULTRALOW
+            attrs[i].span, attrs[i].extension = N.SPAN_END2END, 0
+            attrs[i].pattern_begin_free = attrs[i].pattern_end_free = 0
+            attrs[i].text_begin_free = attrs[i].text_end_free = 0
+    # first half at once, then the rest while the first half runs
+    half = len(reqs) // 2
+    for i in range(half):
+        server.submit(attrs[i], reqs[i][0], reqs[i][1], tag=i)
+    got = server.poll(timeout_us=0)
+    for i in range(half, len(reqs)):
+        server.submit(attrs[i], reqs[i][0], reqs[i][1], tag=i)
+    got += server.drain(len(reqs) - len(got), timeout_s=300)
+    assert sorted(r.tag for r in got) == list(range(len(reqs)))
+    assert all(r.error == 0 for r in got), [r for r in got if r.error][:3]
+    assert all(r.latency_us > 0 for r in got)
+    results = {r.tag: (r.score, r.status, r.cigar) for r in got}
+    _check(results, _oracle(oracle, reqs, attrs), "server vs oracle")
+    eng = BatchEngine(0)
+    _check(results, _alone(eng, reqs, attrs, range(0, len(reqs), 5)), "server vs alone")
+    st = server.stats()
+    assert st["queued"] == 0 and st["running"] == 0
+    assert st["launched_pairs"] >= len(reqs)
+
+
+def test_serving_same_request_in_every_mix(pkg, N, server):
+    """One request, submitted alone and then inside two different mixes,
+    gets the same result each time."""
+    from wfa2lib_rs_amd.server import request_attr
+    from wfa2lib_rs_amd.synth import serving_mix
+    probe = serving_mix(1, 1.0, seed=11, lengths=((1.0, 0, 150, 150),))[0]
+    pa = request_attr("end2end", ("xdrop", 200, 1), 150, 150)
+    seen = []
+    for mix_seed in (None, 4, 5):
+        n = 0
+        if mix_seed is not None:
+            for j, r in enumerate(serving_mix(80, 1e9, seed=mix_seed, lengths=SMALL_MIX)):
+                server.submit(request_attr(r.span, r.heuristic, len(r.pattern), len(r.text)),
+                              r.pattern, r.text, tag=1000 + j)
+                n += 1
+        server.submit(pa, probe.pattern, probe.text, tag=7)
+        got = server.drain(n + 1, timeout_s=300)
+        mine = [r for r in got if r.tag == 7]
+        assert len(mine) == 1 and mine[0].error == 0
+        seen.append((mine[0].score, mine[0].status, mine[0].cigar))
+    assert seen[0] == seen[1] == seen[2]
+
+
+def test_serving_rejects_invalid_requests(pkg, N, server):
+    a = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment")
+    bad = attr_for(N, "gap_affine", (1, 4, 6, 2, 0, 0), "alignment")  # match > 0
+    with pytest.raises(RuntimeError, match="invalid penalties"):
+        server.submit(bad, b"ACGT", b"ACGT", tag=1)
+    z = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment")
+    z.heuristic = N.HEURISTIC_ZDROP
+    with pytest.raises(RuntimeError, match="unsupported"):
+        server.submit(z, b"ACGT", b"ACGT", tag=2)
+    # an empty pair is served
+    server.submit(a, b"", b"", tag=3)
+    got = server.drain(1, timeout_s=60)
+    assert got[0].tag == 3 and got[0].error == 0 and got[0].score == 0
+
+
+def test_serving_poisson_stream(pkg, N, server, oracle):
+    """Requests arriving over time (Poisson) are admitted while earlier ones
+    run; all finish and match the oracle."""
+    from wfa2lib_rs_amd.server import request_attr
+    from wfa2lib_rs_amd.synth import serving_mix
+    mix = serving_mix(300, 3000.0, seed=9, lengths=SMALL_MIX)
+    attrs = [request_attr(r.span, r.heuristic, len(r.pattern), len(r.text)) for r in mix]
+    t0 = time.perf_counter()
+    got = []
+    for i, r in enumerate(mix):
+        while time.perf_counter() - t0 < r.arrival_s:
+            got += server.poll(timeout_us=0)
+        server.submit(attrs[i], r.pattern, r.text, tag=i)
+    got += server.drain(len(mix) - len(got), timeout_s=300)
+    assert sorted(x.tag for x in got) == list(range(len(mix)))
+    results = {x.tag: (x.score, x.status, x.cigar) for x in got}
+    _check(results, _oracle(oracle, [(r.pattern, r.text) for r in mix], attrs), "poisson")
+    lat = np.array([x.latency_us for x in got])
+    assert np.isfinite(lat).all() and (lat > 0).all()

@@ -56,6 +56,24 @@ class Batch(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("num_engines", ctypes.c_int32),
+                ("max_batch_pairs", ctypes.c_int32), ("memory_budget", ctypes.c_int64)]
+
+
+class ServerRequest(ctypes.Structure):
+    _fields_ = [("attr", Attr), ("seq_format", ctypes.c_int32),
+                ("pattern_length", ctypes.c_int32), ("text_length", ctypes.c_int32),
+                ("pattern", ctypes.c_char_p), ("text", ctypes.c_char_p),
+                ("tag", ctypes.c_uint64)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("error", ctypes.c_int32), ("score", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("cigar_length", ctypes.c_int32),
+                ("cigar_ops", ctypes.POINTER(ctypes.c_uint8)), ("latency_us", ctypes.c_double)]
+
+
 class SynthCfg(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("len_mode", ctypes.c_int32),
                 ("len_lo", ctypes.c_int32), ("len_hi", ctypes.c_int32),
@@ -93,6 +111,12 @@ SIGNATURES = {
     "wfa_engine_align_batch_per_pair": (_I32, [_VP, _P(Attr), _P(Batch), _VP]),
     "wfa_engine_last_retries": (_I32, [_VP]),
     "wfa_engine_last_kernel_ms": (_I32, [_VP, _P(ctypes.c_float), _P(ctypes.c_float)]),
+    "wfa_server_config_default": (None, [_P(ServerConfig)]),
+    "wfa_server_new": (_VP, [_P(ServerConfig), _P(_I32)]),
+    "wfa_server_submit": (_I32, [_VP, _P(ServerRequest)]),
+    "wfa_server_poll": (_I32, [_VP, _P(ServerResult), _I32, _I32]),
+    "wfa_server_stats": (_I32, [_VP] + [_P(ctypes.c_int64)] * 4),
+    "wfa_server_free": (None, [_VP]),
     "wfa_version": (ctypes.c_char_p, []),
 }
 

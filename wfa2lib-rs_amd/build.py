@@ -61,7 +61,7 @@ def build_engine():
     out = os.path.join(LIB, "libwfa_mi355x.so")
     tmp = out + ".tmp"
     _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + objs +
-         [os.path.join(CSRC, f) for f in ("wfa_api.cpp", "wfa_biwfa.cpp")])
+         [os.path.join(CSRC, f) for f in ("wfa_api.cpp", "wfa_biwfa.cpp", "wfa_server.cpp")])
     os.replace(tmp, out)
     return out
 

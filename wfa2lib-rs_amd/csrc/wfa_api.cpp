@@ -123,8 +123,9 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
     c->hdr_cap = 0;
     c->hist_cap = 0;
   }
-  // keep the scratch of one launch under 48 GiB (of the 288 GB of HBM)
-  const int64_t budget = (int64_t)48 << 30;
+  // keep the scratch of one launch within the engine's budget (48 GiB of
+  // the 288 GB of HBM unless a serving loop splits it between engines)
+  const int64_t budget = e->scratch_budget;
   if (per_slot > 0 && (int64_t)c->grid * per_slot > budget)
     c->grid = (int)std::max<int64_t>(1, budget / per_slot);
   // op chain of one pair: at most every base consumed by a gap op plus one

@@ -140,6 +140,8 @@ struct wfa_engine_t {
   uint8_t* d_chain = nullptr;
   size_t chain_bytes = 0;
   int last_retries = 0;
+  // device scratch one launch may take (wavefront rings, CIGAR history)
+  int64_t scratch_budget = (int64_t)48 << 30;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   float first_ms = 0.f, all_ms = 0.f;
   // BiWFA scratch (wfa_biwfa.cpp), grown on demand

@@ -139,3 +139,48 @@ def from_ascii(patterns, texts, fmt=N.SEQ_ASCII) -> HostPairs:
                 q = padded.reshape(-1, 4)
                 seq[off:off + nb] = q[:, 0] | (q[:, 1] << 2) | (q[:, 2] << 4) | (q[:, 3] << 6)
     return HostPairs(seq, poff, plen, toff, tlen, fmt)
+
+
+# ---- serving mix: Poisson arrivals of mixed lengths, spans and heuristics ----
+# heuristics as the reference's tests parameterize them: wf-adaptive
+# (10, 50, 1), X-drop (200, 1), banded-adaptive (-50, 50, 1)
+SERVING_HEURISTICS = (("none",), ("wfadaptive", 10, 50, 1), ("xdrop", 200, 1),
+                      ("banded_adaptive", -50, 50, 1))
+# (share, length mode, shortest, longest): 70% 150 bp, 25% 1-10 kbp,
+# 5% 10-100 kbp (log-uniform)
+SERVING_LENGTHS = ((0.70, 0, 150, 150), (0.25, 2, 1000, 10_000), (0.05, 2, 10_000, 100_000))
+
+
+@dataclass
+class ServingRequest:
+    pattern: bytes        # ASCII
+    text: bytes
+    span: str             # "end2end", "endsfree" (0.1 of each length free) or "extension"
+    heuristic: tuple      # one of SERVING_HEURISTICS
+    arrival_s: float      # Poisson arrival time from the start of the stream
+
+
+def serving_mix(n, rate, seed=0, lengths=SERVING_LENGTHS, div=(0.01, 0.10), indel_mean=2.0):
+    """`n` requests arriving at `rate` per second: half end-to-end, a quarter
+    ends-free (0.1 of each sequence's length free at every end), a quarter
+    extension; heuristic uniform over SERVING_HEURISTICS."""
+    rng = np.random.default_rng(seed)
+    shares = np.array([c[0] for c in lengths], dtype=np.float64)
+    cls = rng.choice(len(lengths), size=n, p=shares / shares.sum())
+    span = rng.choice(["end2end", "end2end", "endsfree", "extension"], size=n)
+    heur = rng.integers(0, len(SERVING_HEURISTICS), size=n)
+    arrival = np.cumsum(rng.exponential(1.0 / rate, size=n)) if n else np.zeros(0)
+    pats = [None] * n
+    txts = [None] * n
+    for c, (_, mode, lo, hi) in enumerate(lengths):
+        idx = np.nonzero(cls == c)[0]
+        if len(idx) == 0:
+            continue
+        w = Workload(f"serving{c}", len(idx), mode, lo, hi, div[0], div[1], indel_mean,
+                     seq_format=N.SEQ_ASCII, seed=seed * 7 + c + 1)
+        p = generate(w)
+        for j, i in enumerate(idx):
+            pats[i] = p.pattern(j)
+            txts[i] = p.text(j)
+    return [ServingRequest(pats[i], txts[i], str(span[i]), SERVING_HEURISTICS[heur[i]],
+                           float(arrival[i])) for i in range(n)]
