@@ -243,3 +243,33 @@ def test_heuristics_match_oracle(hname, h, sname, span, metric, pen, memory, pkg
             attr.heuristic_p[i] = h[i + 1]
         _compare(engine, oracle, attr, pairs, scope == "alignment",
                  label=f"{hname}/{sname}/{metric}/{memory}/{scope}")
+
+
+WIDE_SPANS = [("end2end", None), ("extension", "extension"), ("endsfree", [200, 200, 200, 200])]
+
+
+@pytest.mark.parametrize("metric,pen", CONFIGS)
+@pytest.mark.parametrize("span_name,span", WIDE_SPANS)
+@pytest.mark.parametrize("memory", ["high", "med"])
+def test_wide_kernel_several_waves_per_pair(metric, pen, span_name, span, memory, pkg, N, engine,
+                                            oracle):
+    """1-3 kbp pairs run on the wide kernel (rings in HBM, several waves per
+    pair sharing each row): every metric, span and memory mode, including
+    the ends-free first hit combined across waves, matches the oracle. An
+    ASCII batch with a non-ACGT byte also takes the byte-wise path there."""
+    from wfa2lib_rs_amd.synth import Workload, from_ascii, generate
+    if span is not None and pen[0] != 0:
+        pytest.skip("ends-free needs match 0 on the device path")
+    w = Workload("wide_waves", 24, 2, 1000, 3000, 0.01, 0.10, 5.0, seed=5)
+    pairs = generate(w)
+    attr = attr_for(N, metric, pen, "alignment", span=span)
+    attr.memory_mode = N.MEMORY_HIGH if memory == "high" else N.MEMORY_MED
+    _compare(engine, oracle, attr, pairs, True, label=f"wide/{metric}/{span_name}/{memory}")
+    attr_s = attr_for(N, metric, pen, "score", span=span)
+    _compare(engine, oracle, attr_s, pairs, False, label=f"wide/{metric}/{span_name}/score")
+    if memory == "high" and span_name == "end2end":
+        pats = [pairs.pattern(i) for i in range(6)]
+        txts = [pairs.text(i) for i in range(6)]
+        txts[2] = txts[2][:500] + b"N" + txts[2][501:]
+        asc = from_ascii(pats, txts, N.SEQ_ASCII)
+        _compare(engine, oracle, attr, asc, True, label=f"wide/{metric}/ascii")

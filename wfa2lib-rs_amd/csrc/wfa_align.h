@@ -169,9 +169,11 @@ struct PairCtx {
 
 // Stage pair `pair`'s sequences into LDS. Returns the symbol width (2 or 8)
 // or 0 if they do not fit (caller retries with a larger configuration).
+template <int NW>
 static __device__ int stage_sequences(const KParams& p, int pair, int plen, int tlen,
-                                      uint64_t* sP, uint64_t* sT) {
-  const int lane = threadIdx.x;
+                                      uint64_t* sP, uint64_t* sT, int* red) {
+  const int lane = threadIdx.x;  // all 64 * NW threads of the workgroup
+  constexpr int NT = 64 * NW;
   const uint8_t* P = p.seq + p.poff[pair];
   const uint8_t* T = p.seq + p.toff[pair];
   int bits = 2;
@@ -179,43 +181,49 @@ static __device__ int stage_sequences(const KParams& p, int pair, int plen, int 
     // 2-bit only if every byte is one of A,C,G,T (case-sensitive, as the
     // reference compares raw bytes)
     bool bad = false;
-    for (int i = lane; i < plen; i += 64) {
+    for (int i = lane; i < plen; i += NT) {
       const uint8_t c = P[i];
       bad |= !(c == 'A' || c == 'C' || c == 'G' || c == 'T');
     }
-    for (int i = lane; i < tlen; i += 64) {
+    for (int i = lane; i < tlen; i += NT) {
       const uint8_t c = T[i];
       bad |= !(c == 'A' || c == 'C' || c == 'G' || c == 'T');
     }
-    bits = ballot(bad) ? 8 : 2;
+    bool any_bad = ballot(bad) != 0;
+    if (NW > 1) {
+      if ((lane & 63) == 0) red[lane >> 6] = any_bad;
+      __syncthreads();
+      for (int w = 0; w < NW; ++w) any_bad |= red[w] != 0;
+    }
+    bits = any_bad ? 8 : 2;
   }
   const int per = 64 / bits;
   const int pw = (plen + per - 1) / per + 2;
   const int tw = (tlen + per - 1) / per + 2;
   if (pw > p.seq_words || tw > p.seq_words) return 0;
   // zero the words first (padding words are read but masked off)
-  for (int i = lane; i < pw; i += 64) sP[i] = 0;
-  for (int i = lane; i < tw; i += 64) sT[i] = 0;
+  for (int i = lane; i < pw; i += NT) sP[i] = 0;
+  for (int i = lane; i < tw; i += NT) sT[i] = 0;
   __syncthreads();
   uint8_t* bP = (uint8_t*)sP;
   uint8_t* bT = (uint8_t*)sT;
   if (p.fmt == WFA_SEQ_PACKED2BITS) {
     const int nb = (plen + 3) >> 2, mb = (tlen + 3) >> 2;
-    for (int i = lane; i < nb; i += 64) bP[i] = P[i];
-    for (int i = lane; i < mb; i += 64) bT[i] = T[i];
+    for (int i = lane; i < nb; i += NT) bP[i] = P[i];
+    for (int i = lane; i < mb; i += NT) bT[i] = T[i];
   } else if (bits == 8) {
-    for (int i = lane; i < plen; i += 64) bP[i] = P[i];
-    for (int i = lane; i < tlen; i += 64) bT[i] = T[i];
+    for (int i = lane; i < plen; i += NT) bP[i] = P[i];
+    for (int i = lane; i < tlen; i += NT) bT[i] = T[i];
   } else {
     // pack ASCII ACGT → 2 bits (A0 C1 G2 T3), 4 bases per byte
     auto code = [](uint8_t c) -> uint32_t { return c == 'A' ? 0u : c == 'C' ? 1u : c == 'G' ? 2u : 3u; };
     const int nb = (plen + 3) >> 2, mb = (tlen + 3) >> 2;
-    for (int i = lane; i < nb; i += 64) {
+    for (int i = lane; i < nb; i += NT) {
       uint32_t b = 0;
       for (int j = 0; j < 4 && 4 * i + j < plen; ++j) b |= code(P[4 * i + j]) << (2 * j);
       bP[i] = (uint8_t)b;
     }
-    for (int i = lane; i < mb; i += 64) {
+    for (int i = lane; i < mb; i += NT) {
       uint32_t b = 0;
       for (int j = 0; j < 4 && 4 * i + j < tlen; ++j) b |= code(T[4 * i + j]) << (2 * j);
       bT[i] = (uint8_t)b;
@@ -786,21 +794,29 @@ static __device__ bool heuristic_limits(const KParams& p, int heur, int band_lo,
 // ---------------------------------------------------------------------
 // The per-pair forward pass + backtrace.
 // ---------------------------------------------------------------------
-template <int KIND, int MODE, int BITS, typename RT, bool HEUR>
+// One pair per workgroup of NW waves. With NW > 1 the waves split every
+// row's 64-diagonal chunks round-robin and combine their per-chunk results
+// (trims, termination, the first ends-free hit) through `red` in LDS; one
+// thread writes metadata and results, and wave 0 alone runs the backtrace.
+template <int KIND, int MODE, int BITS, typename RT, bool HEUR, int U, int NW>
 static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen, uint64_t* sP,
-                                  uint64_t* sT, const Rows<RT>& R, const Hist& H) {
+                                  uint64_t* sT, const Rows<RT>& R, const Hist& H, int* red) {
   // MODE 0: score only; 1: high memory (M rows kept); 2: medium/low memory
   // (winner flags kept)
   constexpr bool CIG = MODE != 0;
   constexpr bool MED = MODE == 2;
-  const int lane = threadIdx.x;
+  static_assert(NW == 1 || (!HEUR && U == 1), "several waves per pair: no heuristics, U = 1");
+  constexpr int NT = 64 * NW;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = NW == 1 ? 0 : tid >> 6;
   const bool two = KIND == KIND_AFFINE2P;
   // ring rows: M: [0, ringM), I1: [ringM, ringM+ringE1), D1 next, then I2, D2
   const int ringM = p.ringM, ringE1 = p.ringE1, ringE2 = p.ringE2;
   const int rI1 = ringM, rD1 = ringM + ringE1, rI2 = ringM + 2 * ringE1,
             rD2 = ringM + 2 * ringE1 + ringE2;
   const int nrows = KIND == KIND_EDIT ? ringM : two ? rD2 + ringE2 : rD1 + ringE1;
-  for (int r = lane; r < nrows; r += 64) R.meta[r] = meta_absent();
+  for (int r = tid; r < nrows; r += NT) R.meta[r] = meta_absent();
   // ends-free limits of this pair (extension: free ends up to the full
   // lengths, no begin gaps; src/aligner.rs:713-718)
   const bool ef = p.endsfree;
@@ -814,10 +830,10 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   const int lo0 = -pb_free, hi0 = tb_free;
   bool overflow = hi0 - lo0 + 1 > R.wcap;
   if (!overflow) {
-    for (int k = lo0 + lane; k <= hi0; k += 64) R.set(k & R.mask, k > 0 ? k : 0);
+    for (int k = lo0 + tid; k <= hi0; k += NT) R.set(k & R.mask, k > 0 ? k : 0);
   }
   __syncthreads();
-  if (lane == 0) R.meta[0] = make_int4(lo0, hi0, lo0, hi0);
+  if (tid == 0) R.meta[0] = make_int4(lo0, hi0, lo0, hi0);
   __syncthreads();
 
   const int nwf = high_mem_nwf(p, plen, tlen);
@@ -839,12 +855,13 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     int hit_kb = 0;
     int4* hd = MODE == 1 ? &H.hdr[s] : nullptr;
     int64_t base = hpos;
-    for (int kb = lo; kb <= hi; kb += 64) {
+    // U chunks at a time: their offsets are read before the first one's
+    // extension is written back
+    auto ext_chunk = [&](int kb, int off) {
       const int k = kb + lane;
       bool term = false;
       if (k <= hi) {
         const int cell = row * R.wcap + (k & R.mask);
-        int off = R.at(cell);
         if (off >= 0) {
           off += lcp<BITS>(sP, sT, off - k, off, plen, tlen);
           R.set(cell, off);
@@ -866,16 +883,63 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         if (hit) {
           const int j = __builtin_ctzll(hit);
           end_k = hit_kb + j;
-          end_off = uniform(__shfl(R.at(row * R.wcap + ((hit_kb + lane) & R.mask)), j));
+          end_off = uniform(__shfl(off, j));
         }
+      }
+    };
+    if (NW == 1) {
+      int kb = lo;
+      for (; kb + 64 * (U - 1) <= hi; kb += 64 * U) {
+        int off[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = kb + 64 * u + lane;
+          off[u] = k <= hi ? R.at(row * R.wcap + (k & R.mask)) : WFA_NULL;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) ext_chunk(kb + 64 * u, off[u]);
+      }
+      for (; kb <= hi; kb += 64) {
+        const int k = kb + lane;
+        ext_chunk(kb, k <= hi ? R.at(row * R.wcap + (k & R.mask)) : WFA_NULL);
+      }
+    } else {
+      for (int kb = lo + 64 * wave; kb <= hi; kb += NT) {
+        const int k = kb + lane;
+        ext_chunk(kb, k <= hi ? R.at(row * R.wcap + (k & R.mask)) : WFA_NULL);
       }
     }
     if (ef) reached = hit != 0;
     if (MODE == 1) {
-      if (lane == 0) *hd = make_int4(lo, hi, (int)base, 0);
+      if (tid == 0) *hd = make_int4(lo, hi, (int)base, 0);
       if (lo <= hi) hpos += hi - lo + 1;
     }
-    return ballot(reached) != 0;
+    bool any_reached = ballot(reached) != 0;
+    if (NW > 1) {
+      // every wave's first hit is its lowest diagonal: the block's is the
+      // lowest of those
+      int* x = red + 10 * NW;
+      if (lane == 0) {
+        x[3 * wave] = any_reached;
+        x[3 * wave + 1] = hit ? end_k : INT32_MAX;
+        x[3 * wave + 2] = end_off;
+      }
+      __syncthreads();
+      any_reached = false;
+      int bk = INT32_MAX, boff = 0;
+      for (int w = 0; w < NW; ++w) {
+        any_reached |= x[3 * w] != 0;
+        if (x[3 * w + 1] < bk) {
+          bk = x[3 * w + 1];
+          boff = x[3 * w + 2];
+        }
+      }
+      if (ef && bk != INT32_MAX) {
+        end_k = bk;
+        end_off = boff;
+      }
+    }
+    return any_reached;
   };
 
   // ---- heuristic cut-off of M[s] after its extension (apply_heuristic_cutoff,
@@ -897,7 +961,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     if (nlo == lo && nhi == hi) return;
     const bool empty = nlo > nhi;
     __syncthreads();  // every lane has read M's cells and metadata
-    if (lane == 0) {
+    if (tid == 0) {
       // an emptied M is the null wavefront with an empty initialized range
       R.meta[rowM] = empty ? make_int4(nlo, nhi, 1, 0) : make_int4(nlo, nhi, mm.z, mm.w);
       if (KIND != KIND_EDIT) {
@@ -934,7 +998,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   // score 0
   {
     if (CIG && (H.hdr_cap < 1 || H.cap < hi0 - lo0 + 1)) overflow = true;
-    else if (MED && lane == 0) H.hdr[0] = make_int4(lo0, hi0, 0, 0);
+    else if (MED && tid == 0) H.hdr[0] = make_int4(lo0, hi0, 0, 0);
     if (MED) hpos = hi0 - lo0 + 1;
     if (!overflow && extend_row(0, lo0, hi0)) {
       status = WFA_STATUS_END_REACHED;
@@ -1043,7 +1107,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     if (!any) {
       // null step (src/aligner.rs:2031-2043, 1778-1782)
       nulls += 1;
-      if (lane == 0) {
+      if (tid == 0) {
         R.meta[rowM] = meta_absent();
         if (KIND != KIND_EDIT) {
           R.meta[rI1 + rowE] = meta_absent();
@@ -1066,7 +1130,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     if (lo > hi) {
       // a band (or the historic limits) left nothing to compute: no outputs
       // at this score (src/aligner.rs:2058-2081)
-      if (lane == 0) {
+      if (tid == 0) {
         R.meta[rowM] = meta_absent();
         if (KIND != KIND_EDIT) {
           R.meta[rI1 + rowE] = meta_absent();
@@ -1087,23 +1151,49 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     const int64_t fbase = hpos;
 
     // ---- compute all chunks; trims by ballot ----
+    // U chunks at a time: their input reads are all issued before the
+    // first chunk's outputs are written (outputs go to other ring rows), so
+    // a wave keeps U chunks of loads in flight on wide HBM-resident rows.
     int m_lo = INT32_MAX, m_hi = INT32_MIN, i_lo = INT32_MAX, i_hi = INT32_MIN;
     int d_lo = INT32_MAX, d_hi = INT32_MIN, i2_lo = INT32_MAX, i2_hi = INT32_MIN;
     int d2_lo = INT32_MAX, d2_hi = INT32_MIN;
-    for (int kb = lo; kb <= hi; kb += 64) {
+    struct In {
+      int a, b, c, d, e, f, g, h, i;
+    };
+    auto gather = [&](int k, In& in) {
+      if (k > hi) return;
+      if (KIND == KIND_EDIT) {
+        in.a = rd(R, row_x, gl_x, gh_x, k - 1);
+        in.b = rd(R, row_x, gl_x, gh_x, k + 1);
+        if (p.metric != WFA_METRIC_INDEL) in.c = rd(R, row_x, gl_x, gh_x, k);
+      } else {
+        in.a = rd(R, row_o, gl_o, gh_o, k - 1);
+        in.b = rd(R, rI1 + row_e, gil, gih, k - 1);
+        in.c = rd(R, row_o, gl_o, gh_o, k + 1);
+        in.d = rd(R, rD1 + row_e, gdl, gdh, k + 1);
+        in.e = rd(R, row_x, gl_x, gh_x, k);
+        if (two) {
+          in.f = rd(R, row_o2, gl_o2, gh_o2, k - 1);
+          in.g = rd(R, rI2 + row_e2, gil2, gih2, k - 1);
+          in.h = rd(R, row_o2, gl_o2, gh_o2, k + 1);
+          in.i = rd(R, rD2 + row_e2, gdl2, gdh2, k + 1);
+        }
+      }
+    };
+    auto chunk = [&](int kb, const In& in) {
       const int k = kb + lane;
       bool vm = false, vi = false, vd = false, vi2 = false, vd2 = false;
       if (k <= hi) {
         int m;
         int fl = 0;
         if (KIND == KIND_EDIT) {
-          const int ins = rd(R, row_x, gl_x, gh_x, k - 1);
-          const int del = rd(R, row_x, gl_x, gh_x, k + 1);
+          const int ins = in.a;
+          const int del = in.b;
           if (p.metric == WFA_METRIC_INDEL) {
             m = imax(del, ins + 1);
             if (MED) fl = m == del ? 1 : 0;
           } else {
-            const int mis = rd(R, row_x, gl_x, gh_x, k);
+            const int mis = in.c;
             m = imax(del, imax(ins, mis) + 1);
             // piggyback priority (src/compute/piggyback.rs:218-224):
             // mismatch, then deletion, then insertion
@@ -1111,21 +1201,19 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           }
           if (!in_bounds(m, k, plen, tlen)) m = WFA_NULL;
         } else {
-          const int io = rd(R, row_o, gl_o, gh_o, k - 1), ie = rd(R, rI1 + row_e, gil, gih, k - 1);
-          const int dop = rd(R, row_o, gl_o, gh_o, k + 1), de = rd(R, rD1 + row_e, gdl, gdh, k + 1);
+          const int io = in.a, ie = in.b;
+          const int dop = in.c, de = in.d;
           const int i1 = imax(io, ie) + 1;
           const int d1 = imax(dop, de);
-          const int mis = rd(R, row_x, gl_x, gh_x, k) + 1;
+          const int mis = in.e + 1;
           // a gap cell takes its extension input on ties
           // (src/compute/piggyback.rs:146-160)
           if (MED) fl |= (ie >= io ? FL_I1_EXT : 0) | (de >= dop ? FL_D1_EXT : 0);
           int ins = i1, del = d1;
           int i2 = WFA_NULL, d2 = WFA_NULL;
           if (two) {
-            const int io2 = rd(R, row_o2, gl_o2, gh_o2, k - 1);
-            const int ie2 = rd(R, rI2 + row_e2, gil2, gih2, k - 1);
-            const int do2 = rd(R, row_o2, gl_o2, gh_o2, k + 1);
-            const int de2 = rd(R, rD2 + row_e2, gdl2, gdh2, k + 1);
+            const int io2 = in.f, ie2 = in.g;
+            const int do2 = in.h, de2 = in.i;
             i2 = imax(io2, ie2) + 1;
             d2 = imax(do2, de2);
             if (MED) fl |= (ie2 >= io2 ? FL_I2_EXT : 0) | (de2 >= do2 ? FL_D2_EXT : 0);
@@ -1186,16 +1274,54 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           }
         }
       }
+    };
+    if (NW == 1) {
+      int kb = lo;
+      for (; kb + 64 * (U - 1) <= hi; kb += 64 * U) {
+        In in[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) gather(kb + 64 * u + lane, in[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) chunk(kb + 64 * u, in[u]);
+      }
+      for (; kb <= hi; kb += 64) {
+        In in;
+        gather(kb + lane, in);
+        chunk(kb, in);
+      }
+    } else {
+      for (int kb = lo + 64 * wave; kb <= hi; kb += NT) {
+        In in;
+        gather(kb + lane, in);
+        chunk(kb, in);
+      }
+      // every wave saw its own chunks: combine the trims
+      if (lane == 0) {
+        int* x = red + 10 * wave;
+        x[0] = m_lo; x[1] = m_hi; x[2] = i_lo; x[3] = i_hi; x[4] = d_lo;
+        x[5] = d_hi; x[6] = i2_lo; x[7] = i2_hi; x[8] = d2_lo; x[9] = d2_hi;
+      }
+      __syncthreads();
+      m_lo = i_lo = d_lo = i2_lo = d2_lo = INT32_MAX;
+      m_hi = i_hi = d_hi = i2_hi = d2_hi = INT32_MIN;
+      for (int w = 0; w < NW; ++w) {
+        const int* x = red + 10 * w;
+        m_lo = imin(m_lo, x[0]); m_hi = imax(m_hi, x[1]);
+        i_lo = imin(i_lo, x[2]); i_hi = imax(i_hi, x[3]);
+        d_lo = imin(d_lo, x[4]); d_hi = imax(d_hi, x[5]);
+        i2_lo = imin(i2_lo, x[6]); i2_hi = imax(i2_hi, x[7]);
+        d2_lo = imin(d2_lo, x[8]); d2_hi = imax(d2_hi, x[9]);
+      }
     }
     // trimmed-empty rows: lo > hi (but not META_ABSENT)
     auto fix = [](int& l, int& h) { if (l == INT32_MAX) { l = 1; h = 0; } };
     fix(m_lo, m_hi); fix(i_lo, i_hi); fix(d_lo, d_hi); fix(i2_lo, i2_hi); fix(d2_lo, d2_hi);
     __syncthreads();  // ring rows written before the extension reads them
     if (MED) {
-      if (lane == 0) H.hdr[s] = make_int4(lo, hi, (int)fbase, 0);
+      if (tid == 0) H.hdr[s] = make_int4(lo, hi, (int)fbase, 0);
       hpos += hi - lo + 1;
     }
-    if (lane == 0) {
+    if (tid == 0) {
       R.meta[rowM] = make_int4(m_lo, m_hi, m_lo, m_hi);
       if (KIND != KIND_EDIT) {
         R.meta[rI1 + rowE] = make_int4(i_lo, i_hi, i_lo, i_hi);
@@ -1220,7 +1346,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   }
 
   if (overflow) {
-    if (lane == 0) {
+    if (tid == 0) {
       const int slot = atomicAdd(&p.counters[1], 1);
       p.retry_list[slot] = pair;
       p.status[pair] = ST_RETRY;
@@ -1231,7 +1357,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   int ncig = 0;
   if (status == WFA_STATUS_END_REACHED) {
     out_status = WFA_STATUS_COMPLETED;
-    if (CIG) {
+    if (CIG && wave == 0) {
       __threadfence_block();
       uint8_t* out = p.cig + p.cig_off[pair];
       const int cap = plen + tlen;
@@ -1278,19 +1404,20 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     // (src/aligner.rs:819-825)
     out_status = WFA_STATUS_PARTIAL;
   }
-  if (lane == 0) {
+  if (tid == 0) {
     p.score[pair] = result;
     p.status[pair] = out_status;
     if (CIG) p.cig_len[pair] = ncig;
   }
 }
 
-template <int KIND, int MODE, bool LDS_RING, bool I16, bool PACKED, bool HEUR>
-__global__ void __launch_bounds__(64, 5) wfa_align_kernel(KParams p) {
+template <int KIND, int MODE, bool LDS_RING, bool I16, bool PACKED, bool HEUR, int NW>
+__global__ void __launch_bounds__(64 * NW, LDS_RING ? 5 : 4) wfa_align_kernel(KParams p) {
   using RT = typename std::conditional<I16, int16_t, int32_t>::type;
   constexpr bool CIG = MODE != 0;
+  static_assert(NW == 1 || !LDS_RING, "several waves per pair only with rings in HBM");
   extern __shared__ uint64_t smem[];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x;
   uint64_t* sP = smem;
   uint64_t* sT = smem + p.seq_words;
   int4* meta = (int4*)(sT + p.seq_words);
@@ -1300,6 +1427,8 @@ __global__ void __launch_bounds__(64, 5) wfa_align_kernel(KParams p) {
   R.mask = p.wcap - 1;
   if constexpr (LDS_RING) R.ring = (RT*)(meta + p.nrows_meta);
   else R.ring = (RT*)(p.gring + (int64_t)blockIdx.x * p.ring_rows * p.wcap);
+  // cross-wave reductions (NW > 1): 16 ints per wave after the metadata
+  int* red = LDS_RING ? nullptr : (int*)(meta + p.nrows_meta);
   Hist H;
   H.hdr = CIG ? p.hdr + (int64_t)blockIdx.x * p.hdr_cap : nullptr;
   H.cells = MODE == 1 ? p.hist + (int64_t)blockIdx.x * p.hist_cap : nullptr;
@@ -1310,15 +1439,22 @@ __global__ void __launch_bounds__(64, 5) wfa_align_kernel(KParams p) {
   H.cap = p.hist_cap;
   for (;;) {
     int item = 0;
-    if (lane == 0) item = atomicAdd(&p.counters[0], 1);
-    item = uniform(__shfl(item, 0));
+    if (NW == 1) {
+      if (tid == 0) item = atomicAdd(&p.counters[0], 1);
+      item = uniform(__shfl(item, 0));
+    } else {
+      if (tid == 0) red[15 * NW] = atomicAdd(&p.counters[0], 1);
+      __syncthreads();
+      item = uniform(red[15 * NW]);
+      __syncthreads();
+    }
     if (item >= p.num_pairs) break;
     // pair-level values are uniform: keep them in scalar registers
     const int pair = uniform(p.pair_list ? p.pair_list[item] : p.pair_base + item);
     const int plen = uniform(p.plen[pair]), tlen = uniform(p.tlen[pair]);
-    const int bits = stage_sequences(p, pair, plen, tlen, sP, sT);
+    const int bits = stage_sequences<NW>(p, pair, plen, tlen, sP, sT, red);
     if (bits == 0) {
-      if (lane == 0) {
+      if (tid == 0) {
         const int slot = atomicAdd(&p.counters[1], 1);
         p.retry_list[slot] = pair;
         p.status[pair] = ST_RETRY;
@@ -1326,10 +1462,12 @@ __global__ void __launch_bounds__(64, 5) wfa_align_kernel(KParams p) {
       continue;
     }
     // 2-bit packed input never needs the byte-wise path
+    // HBM-resident rings are wide: keep several chunks of loads in flight
+    constexpr int U = LDS_RING || NW > 1 ? 1 : MODE == 0 ? 4 : 2;
     if (PACKED || bits == 2)
-      align_pair<KIND, MODE, 2, RT, HEUR>(p, pair, plen, tlen, sP, sT, R, H);
+      align_pair<KIND, MODE, 2, RT, HEUR, U, NW>(p, pair, plen, tlen, sP, sT, R, H, red);
     else
-      align_pair<KIND, MODE, 8, RT, HEUR>(p, pair, plen, tlen, sP, sT, R, H);
+      align_pair<KIND, MODE, 8, RT, HEUR, U, NW>(p, pair, plen, tlen, sP, sT, R, H, red);
     __syncthreads();
   }
 }
@@ -1338,25 +1476,33 @@ __global__ void __launch_bounds__(64, 5) wfa_align_kernel(KParams p) {
 // Instantiation: every storage/format/heuristic variant of one kind
 // ---------------------------------------------------------------------
 template <int K, int M, bool HEUR>
-static KernelFn pick_kmh(bool lds, bool i16, bool packed) {
-  if (!lds) return packed ? wfa_align_kernel<K, M, false, false, true, HEUR>
-                          : wfa_align_kernel<K, M, false, false, false, HEUR>;
-  if (i16) return packed ? wfa_align_kernel<K, M, true, true, true, HEUR>
-                         : wfa_align_kernel<K, M, true, true, false, HEUR>;
-  return packed ? wfa_align_kernel<K, M, true, false, true, HEUR>
-                : wfa_align_kernel<K, M, true, false, false, HEUR>;
+static KernelFn pick_kmh(bool lds, bool i16, bool packed, int nw) {
+  if (!lds) {
+    if constexpr (!HEUR) {
+      if (nw > 1)
+        return packed ? wfa_align_kernel<K, M, false, false, true, false, kWideWaves>
+                      : wfa_align_kernel<K, M, false, false, false, false, kWideWaves>;
+    }
+    return packed ? wfa_align_kernel<K, M, false, false, true, HEUR, 1>
+                  : wfa_align_kernel<K, M, false, false, false, HEUR, 1>;
+  }
+  if (i16) return packed ? wfa_align_kernel<K, M, true, true, true, HEUR, 1>
+                         : wfa_align_kernel<K, M, true, true, false, HEUR, 1>;
+  return packed ? wfa_align_kernel<K, M, true, false, true, HEUR, 1>
+                : wfa_align_kernel<K, M, true, false, false, HEUR, 1>;
 }
 
 template <int K, int M>
-static KernelFn pick_km(bool lds, bool i16, bool packed, bool heur) {
-  return heur ? pick_kmh<K, M, true>(lds, i16, packed) : pick_kmh<K, M, false>(lds, i16, packed);
+static KernelFn pick_km(bool lds, bool i16, bool packed, bool heur, int nw) {
+  return heur ? pick_kmh<K, M, true>(lds, i16, packed, nw)
+              : pick_kmh<K, M, false>(lds, i16, packed, nw);
 }
 
 template <int K>
-static KernelFn pick_kind(int mode, bool lds, bool i16, bool packed, bool heur) {
-  if (mode == 1) return pick_km<K, 1>(lds, i16, packed, heur);
-  if (mode == 2) return pick_km<K, 2>(lds, i16, packed, heur);
-  return pick_km<K, 0>(lds, i16, packed, heur);
+static KernelFn pick_kind(int mode, bool lds, bool i16, bool packed, bool heur, int nw) {
+  if (mode == 1) return pick_km<K, 1>(lds, i16, packed, heur, nw);
+  if (mode == 2) return pick_km<K, 2>(lds, i16, packed, heur, nw);
+  return pick_km<K, 0>(lds, i16, packed, heur, nw);
 }
 
 }  // namespace wfa

@@ -32,6 +32,7 @@ struct LaunchCfg {
   bool lds_ring;
   bool i16;
   bool window;  // the short-read window kernel
+  int nw;       // waves per pair
   int wcap;
   int seq_words;
   int ring_rows;
@@ -75,7 +76,12 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   const size_t ring_bytes = (size_t)c->ring_rows * c->wcap * (c->i16 ? 2 : 4);
   c->lds_ring = seq_meta + ring_bytes <= 64 * 1024;
   if (!c->lds_ring) c->i16 = false;
-  c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : 0);
+  // rings in HBM are wide: several waves share each pair's rows (heuristic
+  // cut-offs need the single-wave kernel); their cross-wave reductions take
+  // 16 ints per wave of LDS
+  c->nw = !c->lds_ring && p.heuristic == WFA_HEURISTIC_NONE ? wfa::kWideWaves : 1;
+  const size_t red_bytes = c->lds_ring ? 0 : (size_t)64 * c->nw;
+  c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : red_bytes);
   if (c->lds_bytes > e->lds_max) return WFA_ERR_UNSUPPORTED;
   // First pass of gap-affine end-to-end score-only alignments without a
   // heuristic: the window kernel; pairs that leave its window come back for
@@ -92,9 +98,9 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
     per_cu = wfa::window_blocks_per_cu(mode, fmt == WFA_SEQ_PACKED2BITS, c->lds_bytes);
   } else {
     c->seq_words = ((max_len + per - 1) / per + 2 + 1) & ~1;
-    c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : 0);
+    c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : red_bytes);
     per_cu = wfa::max_blocks_per_cu(kind, mode, c->lds_ring, c->i16, fmt == WFA_SEQ_PACKED2BITS,
-                                    p.heuristic != WFA_HEURISTIC_NONE, c->lds_bytes);
+                                    p.heuristic != WFA_HEURISTIC_NONE, c->nw, c->lds_bytes);
   }
   int grid = e->num_cus * per_cu;
   grid = std::max(1, std::min(grid, npairs));
@@ -170,7 +176,8 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
   if (c.window)
     HIPCHK(wfa::launch_window(p, c.mode, c.grid, c.lds_bytes, st));
   else
-    HIPCHK(wfa::launch_align(p, c.kind, c.mode, c.lds_ring, c.i16, c.grid, c.lds_bytes, st));
+    HIPCHK(wfa::launch_align(p, c.kind, c.mode, c.lds_ring, c.i16, c.nw, c.grid, c.lds_bytes,
+                             st));
   return WFA_OK;
 }
 

@@ -130,9 +130,9 @@ hipError_t launch_biwfa_rescore(const KParams& p, const int32_t* pairs, int n,
                                 hipStream_t stream);
 // the alignment kernel for one configuration, from the kind's kernel set
 // (mode: 0 score only, 1 high memory CIGAR, 2 medium/low memory CIGAR)
-KernelFn pick_edit(int mode, bool lds, bool i16, bool packed, bool heur);
-KernelFn pick_affine(int mode, bool lds, bool i16, bool packed, bool heur);
-KernelFn pick_affine2p(int mode, bool lds, bool i16, bool packed, bool heur);
+KernelFn pick_edit(int mode, bool lds, bool i16, bool packed, bool heur, int nw);
+KernelFn pick_affine(int mode, bool lds, bool i16, bool packed, bool heur, int nw);
+KernelFn pick_affine2p(int mode, bool lds, bool i16, bool packed, bool heur, int nw);
 // the short-read window kernel (gap-affine, end-to-end, no heuristic;
 // mode 0 or 1)
 KernelFn pick_affine_window(int mode, bool packed);
@@ -143,9 +143,14 @@ hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, in
                              hipStream_t stream);
 // mode: 0 score only, 1 high memory CIGAR, 2 medium/low memory CIGAR
 // i16: ring offsets held in 16 bits (pairs of at most 8000 bases)
-hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, bool i16, int grid,
-                        size_t lds_bytes, hipStream_t stream);
-int max_blocks_per_cu(int kind, int mode, bool lds_ring, bool i16, bool packed, bool heur,
+// Waves per workgroup (one pair each) of the wide kernels, whose rings live
+// in HBM, without heuristics
+constexpr int kWideWaves = 4;
+
+// `nw` waves per pair (1, or kWideWaves for HBM rings without heuristics)
+hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, bool i16, int nw,
+                        int grid, size_t lds_bytes, hipStream_t stream);
+int max_blocks_per_cu(int kind, int mode, bool lds_ring, bool i16, bool packed, bool heur, int nw,
                       size_t lds_bytes);
 hipError_t launch_window(const KParams& p, int mode, int grid, size_t lds_bytes, hipStream_t stream);
 int window_blocks_per_cu(int mode, bool packed, size_t lds_bytes);

@@ -6,8 +6,8 @@
 
 namespace wfa {
 
-KernelFn pick_affine(int mode, bool lds, bool i16, bool packed, bool heur) {
-  return pick_kind<KIND_AFFINE>(mode, lds, i16, packed, heur);
+KernelFn pick_affine(int mode, bool lds, bool i16, bool packed, bool heur, int nw) {
+  return pick_kind<KIND_AFFINE>(mode, lds, i16, packed, heur, nw);
 }
 
 KernelFn pick_affine_window(int mode, bool packed) {

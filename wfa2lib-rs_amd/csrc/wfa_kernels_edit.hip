@@ -5,8 +5,8 @@
 
 namespace wfa {
 
-KernelFn pick_edit(int mode, bool lds, bool i16, bool packed, bool heur) {
-  return pick_kind<KIND_EDIT>(mode, lds, i16, packed, heur);
+KernelFn pick_edit(int mode, bool lds, bool i16, bool packed, bool heur, int nw) {
+  return pick_kind<KIND_EDIT>(mode, lds, i16, packed, heur, nw);
 }
 
 KernelFn2 pick_biwfa_edit(int which, bool heur) {

@@ -52,11 +52,11 @@ __global__ void __launch_bounds__(64) biwfa_rescore(KParams p, const int32_t* pa
 // ---------------------------------------------------------------------
 // Host-side launch helpers
 // ---------------------------------------------------------------------
-static KernelFn pick(int kind, int mode, bool lds, bool i16, bool packed, bool heur) {
+static KernelFn pick(int kind, int mode, bool lds, bool i16, bool packed, bool heur, int nw) {
   switch (kind) {
-    case KIND_EDIT: return pick_edit(mode, lds, i16, packed, heur);
-    case KIND_AFFINE: return pick_affine(mode, lds, i16, packed, heur);
-    case KIND_AFFINE2P: return pick_affine2p(mode, lds, i16, packed, heur);
+    case KIND_EDIT: return pick_edit(mode, lds, i16, packed, heur, nw);
+    case KIND_AFFINE: return pick_affine(mode, lds, i16, packed, heur, nw);
+    case KIND_AFFINE2P: return pick_affine2p(mode, lds, i16, packed, heur, nw);
     default: return nullptr;
   }
 }
@@ -76,12 +76,12 @@ hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, in
   return hipGetLastError();
 }
 
-hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, bool i16, int grid,
-                        size_t lds_bytes, hipStream_t stream) {
+hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, bool i16, int nw,
+                        int grid, size_t lds_bytes, hipStream_t stream) {
   KernelFn fn = pick(kind, mode, lds_ring, i16, p.fmt == WFA_SEQ_PACKED2BITS,
-                     p.heuristic != WFA_HEURISTIC_NONE);
+                     p.heuristic != WFA_HEURISTIC_NONE, nw);
   if (!fn) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(fn, dim3(grid), dim3(64), lds_bytes, stream, p);
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(64 * nw), lds_bytes, stream, p);
   return hipGetLastError();
 }
 
@@ -138,11 +138,12 @@ int window_blocks_per_cu(int mode, bool packed, size_t lds_bytes) {
   return n > 0 ? n : 1;
 }
 
-int max_blocks_per_cu(int kind, int mode, bool lds_ring, bool i16, bool packed, bool heur,
+int max_blocks_per_cu(int kind, int mode, bool lds_ring, bool i16, bool packed, bool heur, int nw,
                       size_t lds_bytes) {
-  KernelFn fn = pick(kind, mode, lds_ring, i16, packed, heur);
+  KernelFn fn = pick(kind, mode, lds_ring, i16, packed, heur, nw);
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, lds_bytes) != hipSuccess) return 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64 * nw, lds_bytes) != hipSuccess)
+    return 1;
   return n > 0 ? n : 1;
 }
 

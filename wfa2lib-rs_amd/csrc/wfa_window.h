@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(64, 5) wfa_window_kernel(KParams p) {
     // pair-level values are uniform: keep them in scalar registers
     const int pair = uniform(p.pair_list ? p.pair_list[item] : p.pair_base + item);
     const int plen = uniform(p.plen[pair]), tlen = uniform(p.tlen[pair]);
-    const int bits = stage_sequences(p, pair, plen, tlen, sP, sT);
+    const int bits = stage_sequences<1>(p, pair, plen, tlen, sP, sT, nullptr);
     if (bits != 2) {
       // does not fit, or not A/C/G/T: the general kernel takes it
       if (lane == 0) {
