@@ -234,18 +234,28 @@ static __device__ int stage_sequences(const KParams& p, int pair, int plen, int 
 }
 
 // ---------------------------------------------------------------------
-// CIGAR history (high memory): per slot, headers hdr[s] = {lo, hi, pos}
-// and the extended M row cells hist[pos + (k - lo)].
+// CIGAR history: per slot, headers hdr[s] = {lo, hi, offset in chunk,
+// chunk id}; row s holds the cells k = lo..hi at element
+// chunk id * chunk + offset + (k - lo) of the pool: extended M offsets
+// (high memory) or winner flags (medium/low memory).
 // ---------------------------------------------------------------------
 struct Hist {
   int4* hdr;
-  int32_t* cells;   // high memory: extended M offsets
-  uint8_t* bytes;   // medium/low memory: winner flags per (score, k)
+  int32_t* cells;   // high memory: the pool as extended M offsets
+  uint8_t* bytes;   // medium/low memory: the pool as winner flags per (score, k)
   uint8_t* chain;   // medium/low memory: op chain scratch
   int chain_cap;
   int hdr_cap;
-  int64_t cap;
+  int32_t* tab;     // this slot's chunk ids
+  int tab_cap;
+  int chunk;        // elements per chunk
+  int pool_chunks;
+  int32_t* next;    // the pool's next free chunk
 };
+
+static __device__ __forceinline__ int64_t hist_row(const Hist& H, const int4& h) {
+  return (int64_t)h.w * H.chunk + h.z;
+}
 
 // Winner flags of one (score, k) in medium/low memory. Bits 0-2: the
 // source of M (0 I1, 1 I2, 2 D1, 3 D2, 4 mismatch; for edit 0 ins, 1 del,
@@ -261,7 +271,7 @@ static __device__ __forceinline__ bool st_m(const Hist& H, int nscores, int nwf,
   if (s < 0 || s >= nwf || s >= nscores) return false;
   const int4 h = H.hdr[s];
   if (k < h.x || k > h.y) return false;
-  const int v = H.cells[h.z + (k - h.x)];
+  const int v = H.cells[hist_row(H, h) + (k - h.x)];
   if (v == WFA_NULL) return false;
   *out = v;
   return true;
@@ -383,7 +393,7 @@ static __device__ int backtrace_linear(const KParams& p, const Hist& H, int nsco
     if (s < 0 || s >= nscores) return false;
     const int4 hd = H.hdr[s];
     if (kk < hd.x || kk > hd.y) return false;
-    *val = H.cells[hd.z + (kk - hd.x)];
+    *val = H.cells[hist_row(H, hd) + (kk - hd.x)];
     return true;
   };
   while (v > 0 && h > 0 && score > 0) {
@@ -563,7 +573,7 @@ static __device__ int piggyback_backtrace(const KParams& p, const Hist& H, int n
     if (s < 0 || s >= nscores || pos < 0) { bad = true; break; }
     const int4 hd = H.hdr[s];
     if (kk < hd.x || kk > hd.y) { bad = true; break; }
-    const int f = H.bytes[hd.z + (kk - hd.x)];
+    const int f = H.bytes[hist_row(H, hd) + (kk - hd.x)];
     uint8_t op;
     if (KIND == KIND_EDIT) {
       const int w = f & 7;
@@ -800,7 +810,8 @@ static __device__ bool heuristic_limits(const KParams& p, int heur, int band_lo,
 // thread writes metadata and results, and wave 0 alone runs the backtrace.
 template <int KIND, int MODE, int BITS, typename RT, bool HEUR, int U, int NW>
 static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen, uint64_t* sP,
-                                  uint64_t* sT, const Rows<RT>& R, const Hist& H, int* red) {
+                                  uint64_t* sT, const Rows<RT>& R, const Hist& H, int* red,
+                                  int& owned) {
   // MODE 0: score only; 1: high memory (M rows kept); 2: medium/low memory
   // (winner flags kept)
   constexpr bool CIG = MODE != 0;
@@ -840,7 +851,28 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   const int k_end = tlen - plen;
   int end_k = k_end, end_off = tlen;
   const int hist_lo = -(plen + p.scope + 2), hist_hi = tlen + p.scope + 2;
-  int64_t hpos = 0;
+  // CIGAR history rows go to the slot's pool chunks in order: chunk `cur`
+  // (id `cid`) is filled up to `cpos`; a row that does not fit moves to the
+  // slot's next chunk, taking a new one from the pool when the slot owns no
+  // more (it keeps them for its next pairs). False: the pool is exhausted.
+  int cur = -1, cpos = H.chunk, cid = 0;
+  auto halloc = [&](int n, int4* hd_out, int lo_, int hi_) -> bool {
+    if (cpos + n > H.chunk) {
+      cur += 1;
+      if (cur >= H.tab_cap) return false;
+      if (cur >= owned) {
+        if (tid == 0) H.tab[cur] = atomicAdd(H.next, 1);
+        __syncthreads();
+        if (uniform(H.tab[cur]) >= H.pool_chunks) return false;
+        owned = cur + 1;
+      }
+      cid = uniform(H.tab[cur]);
+      cpos = 0;
+    }
+    *hd_out = make_int4(lo_, hi_, cpos, cid);
+    cpos += n;
+    return true;
+  };
   int nulls = 0;
   int score = 0;
   int status = WFA_STATUS_OK;
@@ -848,13 +880,12 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
 
   // extend + terminate + (CIGAR) record M[score] over its rows; `lo/hi`
   // is the computed (pre-trim) range; returns true if the end is reached.
-  auto extend_row = [&](int s, int lo, int hi) -> bool {
+  auto extend_row = [&](int s, int lo, int hi, int4 hrow) -> bool {
     const int row = s % ringM;
     bool reached = false;
     uint64_t hit = 0;  // ends-free: first chunk with a terminating diagonal
     int hit_kb = 0;
-    int4* hd = MODE == 1 ? &H.hdr[s] : nullptr;
-    int64_t base = hpos;
+    const int64_t base = MODE == 1 ? hist_row(H, hrow) : 0;
     // U chunks at a time: their offsets are read before the first one's
     // extension is written back
     auto ext_chunk = [&](int kb, int off) {
@@ -910,10 +941,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       }
     }
     if (ef) reached = hit != 0;
-    if (MODE == 1) {
-      if (tid == 0) *hd = make_int4(lo, hi, (int)base, 0);
-      if (lo <= hi) hpos += hi - lo + 1;
-    }
+    if (MODE == 1 && tid == 0) H.hdr[s] = hrow;
     bool any_reached = ballot(reached) != 0;
     if (NW > 1) {
       // every wave's first hit is its lowest diagonal: the block's is the
@@ -989,7 +1017,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       const int4 hd = H.hdr[s];
       if (lane < 4) {
         const int k = lane < 2 ? nlo - 2 + lane : nhi + lane - 1;
-        if (k >= hd.x && k <= hd.y) H.cells[hd.z + (k - hd.x)] = WFA_NULL;
+        if (k >= hd.x && k <= hd.y) H.cells[hist_row(H, hd) + (k - hd.x)] = WFA_NULL;
       }
     }
     __syncthreads();
@@ -997,10 +1025,11 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
 
   // score 0
   {
-    if (CIG && (H.hdr_cap < 1 || H.cap < hi0 - lo0 + 1)) overflow = true;
-    else if (MED && tid == 0) H.hdr[0] = make_int4(lo0, hi0, 0, 0);
-    if (MED) hpos = hi0 - lo0 + 1;
-    if (!overflow && extend_row(0, lo0, hi0)) {
+    int4 h0 = make_int4(lo0, hi0, 0, 0);
+    if (CIG && !overflow && (H.hdr_cap < 1 || !halloc(hi0 - lo0 + 1, &h0, lo0, hi0)))
+      overflow = true;
+    else if (MED && tid == 0) H.hdr[0] = h0;
+    if (!overflow && extend_row(0, lo0, hi0, h0)) {
       status = WFA_STATUS_END_REACHED;
       result = 0;
     }
@@ -1147,8 +1176,9 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       continue;
     }
     if (hi - lo + 1 > R.wcap) { overflow = true; break; }
-    if (MED && hpos + (hi - lo + 1) > H.cap) { overflow = true; break; }
-    const int64_t fbase = hpos;
+    int4 hrow = make_int4(lo, hi, 0, 0);
+    if (MED && !halloc(hi - lo + 1, &hrow, lo, hi)) { overflow = true; break; }
+    const int64_t fbase = MED ? hist_row(H, hrow) : 0;
 
     // ---- compute all chunks; trims by ballot ----
     // U chunks at a time: their input reads are all issued before the
@@ -1317,10 +1347,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     auto fix = [](int& l, int& h) { if (l == INT32_MAX) { l = 1; h = 0; } };
     fix(m_lo, m_hi); fix(i_lo, i_hi); fix(d_lo, d_hi); fix(i2_lo, i2_hi); fix(d2_lo, d2_hi);
     __syncthreads();  // ring rows written before the extension reads them
-    if (MED) {
-      if (tid == 0) H.hdr[s] = make_int4(lo, hi, (int)fbase, 0);
-      hpos += hi - lo + 1;
-    }
+    if (MED && tid == 0) H.hdr[s] = hrow;
     if (tid == 0) {
       R.meta[rowM] = make_int4(m_lo, m_hi, m_lo, m_hi);
       if (KIND != KIND_EDIT) {
@@ -1338,8 +1365,9 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     // In high memory the stored row keeps its computed range: M cells
     // outside the trimmed range are OFFSET_NULL there anyway.
     const int elo = MODE == 1 ? lo : m_lo, ehi = MODE == 1 ? hi : m_hi;
-    if (MODE == 1 && hpos + (hi - lo + 1) > H.cap) { overflow = true; break; }
-    const bool reached = extend_row(s, elo, ehi) && (ef || (m_lo <= k_end && k_end <= m_hi));
+    if (MODE == 1 && !halloc(hi - lo + 1, &hrow, elo, ehi)) { overflow = true; break; }
+    const bool reached =
+        extend_row(s, elo, ehi, hrow) && (ef || (m_lo <= k_end && k_end <= m_hi));
     __syncthreads();
     if (reached) { status = WFA_STATUS_END_REACHED; result = s; break; }
     if (heur != WFA_HEURISTIC_NONE) cutoff(s);
@@ -1431,12 +1459,17 @@ __global__ void __launch_bounds__(64 * NW, LDS_RING ? 5 : 4) wfa_align_kernel(KP
   int* red = LDS_RING ? nullptr : (int*)(meta + p.nrows_meta);
   Hist H;
   H.hdr = CIG ? p.hdr + (int64_t)blockIdx.x * p.hdr_cap : nullptr;
-  H.cells = MODE == 1 ? p.hist + (int64_t)blockIdx.x * p.hist_cap : nullptr;
-  H.bytes = MODE == 2 ? (uint8_t*)p.hist + (int64_t)blockIdx.x * p.hist_cap : nullptr;
+  H.cells = MODE == 1 ? p.hist : nullptr;
+  H.bytes = MODE == 2 ? (uint8_t*)p.hist : nullptr;
   H.chain = MODE == 2 ? p.chain + (int64_t)blockIdx.x * p.chain_cap : nullptr;
   H.chain_cap = p.chain_cap;
   H.hdr_cap = p.hdr_cap;
-  H.cap = p.hist_cap;
+  H.tab = CIG ? p.chunk_tab + (int64_t)blockIdx.x * p.tab_cap : nullptr;
+  H.tab_cap = p.tab_cap;
+  H.chunk = p.chunk;
+  H.pool_chunks = p.pool_chunks;
+  H.next = p.counters + 3;
+  int owned = 0;  // pool chunks this slot holds
   for (;;) {
     int item = 0;
     if (NW == 1) {
@@ -1465,9 +1498,9 @@ __global__ void __launch_bounds__(64 * NW, LDS_RING ? 5 : 4) wfa_align_kernel(KP
     // HBM-resident rings are wide: keep several chunks of loads in flight
     constexpr int U = LDS_RING || NW > 1 ? 1 : MODE == 0 ? 4 : 2;
     if (PACKED || bits == 2)
-      align_pair<KIND, MODE, 2, RT, HEUR, U, NW>(p, pair, plen, tlen, sP, sT, R, H, red);
+      align_pair<KIND, MODE, 2, RT, HEUR, U, NW>(p, pair, plen, tlen, sP, sT, R, H, red, owned);
     else
-      align_pair<KIND, MODE, 8, RT, HEUR, U, NW>(p, pair, plen, tlen, sP, sT, R, H, red);
+      align_pair<KIND, MODE, 8, RT, HEUR, U, NW>(p, pair, plen, tlen, sP, sT, R, H, red, owned);
     __syncthreads();
   }
 }

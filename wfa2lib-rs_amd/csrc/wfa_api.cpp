@@ -39,7 +39,10 @@ struct LaunchCfg {
   size_t lds_bytes;
   int grid;
   int hdr_cap;
-  int64_t hist_cap;
+  int64_t hist_cap;  // history elements per slot, on average
+  int chunk;         // history pool: elements per chunk
+  int pool_chunks;
+  int tab_cap;       // chunks one slot may hold
   int chain_cap;
 };
 
@@ -123,17 +126,28 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
     // medium/low memory keeps one flag byte per cell instead of an offset:
     // the same bytes hold four times the cells
     if (mode == 2) c->hist_cap *= 4;
+    // The history is one pool of chunks shared by the launch's slots
+    // (hist_cap elements per slot on average): a pair takes chunks as its
+    // rows need them, so one pair can hold far more than the average. A
+    // row (at most wcap cells) never spans chunks.
+    c->chunk = (int)std::max<int64_t>(16 * (int64_t)c->wcap, 1 << 16);
+    c->tab_cap = (int)std::min<int64_t>((int64_t)c->hdr_cap * c->wcap / c->chunk + 2, 1 << 20);
     per_slot += c->hist_cap * (mode == 2 ? 1 : 4) + (int64_t)c->hdr_cap * 16 +
-                (mode == 2 ? 4 * (int64_t)max_len + 64 : 0);
+                (int64_t)c->tab_cap * 4 + (mode == 2 ? 4 * (int64_t)max_len + 64 : 0);
   } else {
     c->hdr_cap = 0;
     c->hist_cap = 0;
+    c->chunk = c->pool_chunks = c->tab_cap = 0;
   }
   // keep the scratch of one launch within the engine's budget (48 GiB of
   // the 288 GB of HBM unless a serving loop splits it between engines)
   const int64_t budget = e->scratch_budget;
   if (per_slot > 0 && (int64_t)c->grid * per_slot > budget)
     c->grid = (int)std::max<int64_t>(1, budget / per_slot);
+  if (cig) {
+    const int64_t elems = (int64_t)c->grid * c->hist_cap;
+    c->pool_chunks = (int)std::max<int64_t>(c->grid, (elems + c->chunk - 1) / c->chunk);
+  }
   // op chain of one pair: at most every base consumed by a gap op plus one
   // gap-closing mismatch per gap
   c->chain_cap = 4 * max_len + 64;
@@ -157,12 +171,18 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
   }
   if (c.cig) {
     const size_t cell = c.mode == 2 ? 1 : 4;
-    int r = grow((void**)&e->d_hist, &e->hist_bytes, (size_t)c.grid * c.hist_cap * cell);
+    int r = grow((void**)&e->d_hist, &e->hist_bytes, (size_t)c.pool_chunks * c.chunk * cell);
     if (r) return r;
     r = grow((void**)&e->d_hdr, &e->hdr_bytes, (size_t)c.grid * c.hdr_cap * sizeof(int4));
     if (r) return r;
+    r = grow((void**)&e->d_tab, &e->tab_bytes, (size_t)c.grid * c.tab_cap * 4);
+    if (r) return r;
     p.hist = e->d_hist;
-    p.hist_cap = c.hist_cap;
+    p.hist_cap = 0;
+    p.chunk = c.chunk;
+    p.pool_chunks = c.pool_chunks;
+    p.chunk_tab = e->d_tab;
+    p.tab_cap = c.tab_cap;
     p.hdr = e->d_hdr;
     p.hdr_cap = c.hdr_cap;
     if (c.mode == 2) {
@@ -316,6 +336,7 @@ void wfa_engine_free(wfa_engine_t* e) {
   hipFree(e->d_hdr);
   hipFree(e->d_gring);
   hipFree(e->d_chain);
+  hipFree(e->d_tab);
   for (auto b : e->bi_buf) hipFree(b);
   hipHostFree(e->h_pinned);
   for (auto ev : e->ev) hipEventDestroy(ev);

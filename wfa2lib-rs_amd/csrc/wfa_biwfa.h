@@ -819,8 +819,9 @@ static __device__ void base_case(const BiParams& bp, int item, const BiJob& J, u
     }
     if (trace) {
       if (KIND == KIND_EDIT) {
+        // one contiguous history per sub-problem: chunk 0 of size 0
         Hist H;
-        H.hdr = hdr; H.cells = cells;
+        H.hdr = hdr; H.cells = cells; H.chunk = 0;
         n = backtrace_linear<BITS>(p, H, score + 1, plen, tlen, from_score, from_k, from_off,
                                    out, cap);
       } else {

@@ -57,6 +57,13 @@ struct KParams {
   int hdr_cap;
   uint8_t* chain;            // medium/low memory op-chain scratch, chain_cap per slot
   int chain_cap;
+  // CIGAR history pool (`hist`, pool_chunks chunks of `chunk` elements):
+  // every slot takes chunks as its pairs need them and keeps them for its
+  // next pairs; chunk_tab holds each slot's chunk ids (tab_cap per slot)
+  int32_t* chunk_tab;
+  int tab_cap;
+  int chunk;
+  int pool_chunks;
 };
 
 // ---------------------------------------------------------------------

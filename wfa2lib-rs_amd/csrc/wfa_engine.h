@@ -126,7 +126,7 @@ struct wfa_engine_t {
   int device = 0;
   int num_cus = 0;
   size_t lds_max = 0;
-  int32_t* d_counters = nullptr;  // [0] queue head, [1] retries, [2] scratch
+  int32_t* d_counters = nullptr;  // [0] queue head, [1] retries, [2] scratch, [3] next pool chunk
   int32_t* d_lists[2] = {nullptr, nullptr};
   size_t list_cap = 0;
   int32_t* h_pinned = nullptr;
@@ -139,6 +139,8 @@ struct wfa_engine_t {
   size_t gring_bytes = 0;
   uint8_t* d_chain = nullptr;
   size_t chain_bytes = 0;
+  int32_t* d_tab = nullptr;  // history pool: chunk ids per slot
+  size_t tab_bytes = 0;
   int last_retries = 0;
   // device scratch one launch may take (wavefront rings, CIGAR history)
   int64_t scratch_budget = (int64_t)48 << 30;

@@ -10,8 +10,10 @@ KernelFn pick_affine(int mode, bool lds, bool i16, bool packed, bool heur, int n
   return pick_kind<KIND_AFFINE>(mode, lds, i16, packed, heur, nw);
 }
 
+// The window kernel runs score-only launches (its high-memory CIGAR variant
+// keeps a per-slot history and measures slower than the general kernel)
 KernelFn pick_affine_window(int mode, bool packed) {
-  if (mode == 1) return packed ? wfa_window_kernel<1, true> : wfa_window_kernel<1, false>;
+  if (mode != 0) return nullptr;
   return packed ? wfa_window_kernel<0, true> : wfa_window_kernel<0, false>;
 }
 

@@ -36,7 +36,7 @@ static __device__ __forceinline__ int16_t sat16(int v) { return (int16_t)imax(v,
 template <int MODE>
 static __device__ void align_window(const KParams& p, int pair, int plen, int tlen,
                                     const uint64_t* sP, const uint64_t* sT, int16_t* rows,
-                                    int2* meta, int* vchunks, const Hist& H) {
+                                    int2* meta, int* vchunks, const Hist& H, int64_t hcap) {
   constexpr bool CIG = MODE == 1;
   const int lane = threadIdx.x;
   const int k_end = tlen - plen;
@@ -105,7 +105,7 @@ static __device__ void align_window(const KParams& p, int pair, int plen, int tl
   };
 
   // ---- score 0: M0 = {k = 0: offset 0} ----
-  if (CIG && (H.hdr_cap < 1 || H.cap < WIN)) overflow = true;
+  if (CIG && (H.hdr_cap < 1 || hcap < WIN)) overflow = true;
   if (!overflow) {
     const int c0 = (0 - base) >> 6;
 #pragma unroll
@@ -119,7 +119,7 @@ static __device__ void align_window(const KParams& p, int pair, int plen, int tl
     score += 1;
     const int s = score;
     if (!p.modular && s >= nwf) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
-    if (CIG && (s >= H.hdr_cap || hpos + WIN > H.cap)) { overflow = true; break; }
+    if (CIG && (s >= H.hdr_cap || hpos + WIN > hcap)) { overflow = true; break; }
     if (s > 16000) { overflow = true; break; }
     // ---- inputs: M[s-x], M[s-o-e], I1/D1[s-e] ----
     int2 mx = make_int2(1, 0), mo = mx, i1 = mx, d1 = mx;
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(64, 5) wfa_window_kernel(KParams p) {
   H.chain = nullptr;
   H.chain_cap = 0;
   H.hdr_cap = p.hdr_cap;
-  H.cap = p.hist_cap;
+  H.chunk = 0;  // one contiguous history per slot
   __syncthreads();
   for (;;) {
     int item = 0;
@@ -310,7 +310,7 @@ __global__ void __launch_bounds__(64, 5) wfa_window_kernel(KParams p) {
       }
       continue;
     }
-    align_window<MODE>(p, pair, plen, tlen, sP, sT, rows, meta, vchunks, H);
+    align_window<MODE>(p, pair, plen, tlen, sP, sT, rows, meta, vchunks, H, p.hist_cap);
     __syncthreads();
   }
 }
