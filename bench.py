@@ -56,12 +56,122 @@ METRIC_PEN = {"edit": (0, 0, 0, 0, 0, 0), "gap_affine": (0, 4, 6, 2, 0, 0),
               "gap_affine_2p": (0, 4, 6, 2, 24, 1)}
 
 
+# the serving mix without its 10-100 kbp share, for the default run
+SHORT_MIX = ((0.70, 0, 150, 150), (0.25, 2, 1000, 10_000))
+
+
+def serving_main(args, log):
+    """--workload serving: the full serving mix (150 bp, 1-10 kbp, 10-100 kbp;
+    spans and heuristics mixed), one JSON line with requests/s and p50/p99."""
+    from wfa2lib_rs_amd.synth import SERVING_LENGTHS
+    n = args.serving_requests or 2000
+    rate = args.serving_rate or 500.0
+    out = serving_leg(n, rate, SERVING_LENGTHS, 0, 300, log)
+    line = {
+        "metric": METRIC, "value": out["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(n / out["pairs_per_s"] * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"serving: {n} requests, Poisson at {rate}/s, 70% 150 bp, 25% "
+                               "1-10 kbp, 5% 10-100 kbp, half end-to-end, a quarter ends-free "
+                               "0.1, a quarter extension, heuristic none/wf-adaptive/X-drop/"
+                               "banded-adaptive, gap-affine (0,4,6,2) CIGAR high memory",
+                   **out},
+        "roofline": None,
+        "cpu_baseline": {"value": out["oracle_sample_pairs_per_s"], "unit": "pairs/s",
+                         "cores": 16, "kind": "port",
+                         "sample": "the parity sample of the same stream (requests of at most "
+                                   "10 kbp), grouped by configuration, 16 threads"},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def serving_leg(n, rate, lengths, seed, oracle_sample, log):
+    """Replay a Poisson stream of `n` mixed requests at `rate`/s through the
+    serving loop; latency = result received - scheduled arrival."""
+    import threading
+
+    import numpy as np
+
+    import __graft_entry__ as entry
+    from wfa2lib_rs_amd import _native as N
+    from wfa2lib_rs_amd.server import AlignmentServer, request_attr
+    from wfa2lib_rs_amd.synth import from_ascii, serving_mix
+
+    mix = serving_mix(n, rate, seed=seed, lengths=lengths)
+    attrs = [request_attr(r.span, r.heuristic, len(r.pattern), len(r.text)) for r in mix]
+    srv = AlignmentServer(device=0)
+    # warm the engines (device buffers, code objects) on a few requests
+    for i in range(min(64, n)):
+        srv.submit(attrs[i], mix[i].pattern, mix[i].text, tag=10 ** 9 + i)
+    srv.drain(min(64, n), timeout_s=300)
+    got, recv = {}, {}
+    done = threading.Event()
+
+    def poller():
+        while len(got) < n:
+            for r in srv.poll(timeout_us=2000):
+                got[r.tag] = r
+                recv[r.tag] = time.perf_counter()
+        done.set()
+
+    th = threading.Thread(target=poller, daemon=True)
+    t0 = time.perf_counter()
+    th.start()
+    for i, r in enumerate(mix):
+        wait = t0 + r.arrival_s - time.perf_counter()
+        if wait > 0:
+            time.sleep(wait)
+        srv.submit(attrs[i], r.pattern, r.text, tag=i)
+    done.wait(timeout=1200)
+    t_end = max(recv.values())
+    st = srv.stats()
+    srv.close()
+    assert len(got) == n, f"{len(got)}/{n} results"
+    lat = np.array([(recv[i] - (t0 + mix[i].arrival_s)) * 1e3 for i in range(n)])
+    short = np.array([len(r.pattern) <= 256 for r in mix])
+    errors = sum(1 for r in got.values() if r.error != 0)
+    oom = sum(1 for r in got.values() if r.status == N.STATUS_OOM)
+    # parity: the oracle on a sample (every configuration appears)
+    o = entry.load_oracle()
+    idx = [i for i in range(n) if len(mix[i].pattern) <= 10_000][:oracle_sample]
+    groups = {}
+    for i in idx:
+        groups.setdefault(bytes(attrs[i]), []).append(i)
+    same = 0
+    t = time.perf_counter()
+    for g in groups.values():
+        pairs = from_ascii([mix[i].pattern for i in g], [mix[i].text for i in g])
+        sc, sts, cg = o.align_batch(attrs[g[0]], pairs, cigar=True, threads=16)
+        for j, i in enumerate(g):
+            r = got[i]
+            same += int(r.score == sc[j] and r.status == sts[j] and r.cigar == cg[j])
+    cpu_sec = time.perf_counter() - t
+    out = {
+        "requests": n, "rate_per_s": rate,
+        "pairs_per_s": round(n / (t_end - t0), 3),
+        "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
+        "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),
+        "latency_ms_p50_150bp": round(float(np.percentile(lat[short], 50)), 3) if short.any() else None,
+        "latency_ms_p99_150bp": round(float(np.percentile(lat[short], 99)), 3) if short.any() else None,
+        "latency_ms_p99_long": round(float(np.percentile(lat[~short], 99)), 3) if (~short).any() else None,
+        "launches": int(st["launches"]), "errors": errors, "oom": oom,
+        "parity_sample": f"{same}/{len(idx)}",
+        "lengths": [list(x) for x in lengths],
+        "oracle_sample_pairs_per_s": round(len(idx) / cpu_sec, 3) if idx else None,
+    }
+    log(f"serving: {out['pairs_per_s']:,.1f} requests/s at {rate}/s offered, p50 "
+        f"{out['latency_ms_p50']:.2f} ms, p99 {out['latency_ms_p99']:.2f} ms, "
+        f"{out['launches']} launches, parity {out['parity_sample']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="reads100_300", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="reads100_300", choices=sorted(WORKLOADS) + ["serving"])
     ap.add_argument("--metric", default=None, choices=sorted(METRIC_PEN),
                     help="override the workload's distance metric")
     ap.add_argument("--pairs", type=int, default=None)
@@ -70,6 +180,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--parity-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=None)
+    ap.add_argument("--serving-rate", type=float, default=None, help="requests per second")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -103,6 +216,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
+    if args.workload == "serving":
+        serving_main(args, log)
+        return
     cfg = dict(WORKLOADS[args.workload])
     if args.metric:
         cfg["metric"] = args.metric
@@ -202,6 +318,13 @@ def main():
         same += int(g == ocig[i] and score[i] == os_[i] and status[i] == ost[i])
     log(f"parity: {same}/{ps} pairs identical to the oracle (score, status, CIGAR)")
 
+    # ---- serving: a short Poisson stream of mixed requests (reads up to
+    # 10 kbp; the full mix with 10-100 kbp reads is --workload serving) ----
+    serving = None
+    if not args.no_serving:
+        serving = serving_leg(args.serving_requests or 3000, args.serving_rate or 2000.0,
+                              SHORT_MIX, 0, 400, log)
+
     # ---- CPU baseline: the oracle on the host cores, bounded sample ----
     cpu = None
     if not args.no_cpu:
@@ -254,6 +377,7 @@ def main():
             "parity_sample": f"{same}/{ps}",
             "kernel_ms_first_pass": round(first_ms, 3),
             "kernel_ms_all_passes": round(all_ms, 3),
+            "serving": serving,
         },
         "roofline": {
             "bound": "hbm",

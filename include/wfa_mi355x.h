@@ -223,7 +223,31 @@ typedef struct wfa_batch_t {
     uint8_t* cigar_ops;              /* NULL in score scope */
     const int64_t* cigar_offset;     /* NULL in score scope */
     int32_t* cigar_length;           /* NULL in score scope */
+    const struct wfa_pair_config_t* pair_config;  /* optional, one per pair; NULL =
+                                        every pair as the attribute says */
 } wfa_batch_t;
+
+/* One pair's span and heuristic within a batch: what the reference's
+ * setters set on that pair's aligner (set_alignment_free_ends /
+ * set_alignment_extension / set_alignment_end_to_end / set_heuristic,
+ * src/aligner.rs:563-611). The launch's wfa_attr_t is the envelope: a pair
+ * runs ends-free or extension only when attr.span is WFA_SPAN_ENDSFREE, and
+ * its heuristic only when attr.heuristic is not WFA_HEURISTIC_NONE (the
+ * attribute's own free ends and heuristic parameters are then unused).
+ * The caller validates each pair's values as wfa_attr_t values are
+ * validated: free ends >= 0, heuristic not WFA_HEURISTIC_ZDROP. Not for
+ * WFA_MEMORY_ULTRALOW batches. */
+typedef struct wfa_pair_config_t {
+    int32_t span;                   /* WFA_SPAN_* */
+    int32_t extension;              /* 1 = extension (span WFA_SPAN_ENDSFREE) */
+    int32_t pattern_begin_free;
+    int32_t pattern_end_free;
+    int32_t text_begin_free;
+    int32_t text_end_free;
+    int32_t heuristic;              /* WFA_HEURISTIC_* */
+    int32_t heuristic_p[3];
+    int32_t reserved[2];            /* zero */
+} wfa_pair_config_t;
 
 /* Create an engine on HIP device `device` (owns the device work queue,
  * per-slot wavefront scratch and the launch configuration). */
@@ -236,7 +260,9 @@ void wfa_engine_free(wfa_engine_t* engine);
  * a retry counter after the launch, and pairs whose wavefronts outgrew the
  * first launch's per-pair storage are re-run by wider launches before it
  * returns. Scratch grows on demand (allocations happen only when a batch
- * needs more than any earlier one). */
+ * needs more than any earlier one), up to 48 GiB per launch; a pair whose
+ * CIGAR history still does not fit after six ever wider passes gets
+ * WFA_STATUS_OOM (score 0, no CIGAR) instead of failing the batch. */
 int32_t wfa_engine_align_batch(wfa_engine_t* engine, const wfa_attr_t* attr,
                                const wfa_batch_t* batch, void* hip_stream);
 
@@ -270,8 +296,8 @@ typedef struct wfa_server_config_t {
                                     stream with its own scratch (1..16, default 4) */
     int32_t max_batch_pairs;     /* most requests one launch takes (default 65536) */
     int64_t memory_budget;       /* predicted device bytes of the work in flight
-                                    (default 32 GiB); also split between the
-                                    engines as their scratch budget */
+                                    (default 64 GiB); each engine's launches may
+                                    use up to twice its even share of it */
 } wfa_server_config_t;
 
 /* One request: its own configuration and sequences (host memory, copied by
@@ -297,7 +323,7 @@ typedef struct wfa_server_result_t {
     double latency_us;           /* from submission to the result being ready */
 } wfa_server_result_t;
 
-/* Defaults: device 0, 4 engines, 65536 requests per launch, 32 GiB. */
+/* Defaults: device 0, 4 engines, 65536 requests per launch, 64 GiB. */
 void wfa_server_config_default(wfa_server_config_t* config);
 /* Start the engines and their worker threads; NULL and *err on failure. */
 wfa_server_t* wfa_server_new(const wfa_server_config_t* config, int32_t* err);

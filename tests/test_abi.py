@@ -96,7 +96,7 @@ def test_server_refuses_without_device(N):
         pass
     cfg = N.ServerConfig()
     N.lib().wfa_server_config_default(ctypes.byref(cfg))
-    assert (cfg.num_engines, cfg.max_batch_pairs, cfg.memory_budget) == (4, 65536, 32 << 30)
+    assert (cfg.num_engines, cfg.max_batch_pairs, cfg.memory_budget) == (4, 65536, 64 << 30)
     err = ctypes.c_int32(0)
     assert not N.lib().wfa_server_new(ctypes.byref(cfg), ctypes.byref(err))
     assert err.value in (-5, -4)

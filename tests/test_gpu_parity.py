@@ -273,3 +273,44 @@ def test_wide_kernel_several_waves_per_pair(metric, pen, span_name, span, memory
         txts[2] = txts[2][:500] + b"N" + txts[2][501:]
         asc = from_ascii(pats, txts, N.SEQ_ASCII)
         _compare(engine, oracle, attr, asc, True, label=f"wide/{metric}/ascii")
+
+
+def test_per_pair_configs_match_oracle(pkg, N, engine, oracle):
+    """A batch may carry each pair's span and heuristic (the serving loop
+    batches mixed requests that way): end-to-end, ends-free with per-pair
+    free ends and extension pairs, with and without each heuristic, in one
+    launch; every pair matches the oracle run with that pair's own aligner
+    configuration."""
+    from wfa2lib_rs_amd.engine import DeviceBatch, pair_config_row
+    from wfa2lib_rs_amd.synth import Workload, generate
+    import torch
+    heuristics = [None, ("wfadaptive", 10, 50, 1), ("xdrop", 200, 1),
+                  ("banded_adaptive", -50, 50, 1), ("banded_static", -20, 40)]
+    for w, use_heur in ((Workload("pc_short", 60, 0, 150, 150, 0.01, 0.10, 2.0), True),
+                        (Workload("pc_long", 12, 2, 1000, 3000, 0.01, 0.10, 2.0), True),
+                        (Workload("pc_long_nh", 12, 2, 1000, 3000, 0.01, 0.10, 2.0), False)):
+        pairs = generate(w)
+        n = len(pairs)
+        attrs = []
+        for i in range(n):
+            plen, tlen = int(pairs.plen[i]), int(pairs.tlen[i])
+            span = [None, "extension", [plen // 10, plen // 7, tlen // 10, tlen // 5]][i % 3]
+            a = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment", span=span)
+            h = heuristics[(i // 3) % len(heuristics)] if use_heur else None
+            if h is not None:
+                a.heuristic = {"wfadaptive": 1, "xdrop": 2, "banded_static": 4,
+                               "banded_adaptive": 5}[h[0]]
+                for j, v in enumerate(h[1:]):
+                    a.heuristic_p[j] = v
+            attrs.append(a)
+        env = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment", span=[0, 0, 0, 0])
+        if use_heur:
+            env.heuristic = N.HEURISTIC_WFADAPTIVE
+        cfg = np.array([pair_config_row(a) for a in attrs], dtype=np.int32)
+        b = DeviceBatch(pairs, device="cuda:0", cigar=True, pair_config=cfg)
+        engine.align(env, b)
+        torch.cuda.synchronize()
+        s, st, c = b.results()
+        for i in range(n):
+            os_, ost, oc = oracle.align_batch(attrs[i], pairs.subset([i]), cigar=True)
+            assert (s[i], st[i], c[i]) == (os_[0], ost[0], oc[0]), (w.name, i, list(cfg[i]))

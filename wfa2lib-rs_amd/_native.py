@@ -53,6 +53,7 @@ class Batch(ctypes.Structure):
         ("text_length", ctypes.c_void_p), ("score", ctypes.c_void_p),
         ("status", ctypes.c_void_p), ("cigar_ops", ctypes.c_void_p),
         ("cigar_offset", ctypes.c_void_p), ("cigar_length", ctypes.c_void_p),
+        ("pair_config", ctypes.c_void_p),
     ]
 
 

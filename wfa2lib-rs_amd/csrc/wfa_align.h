@@ -676,25 +676,37 @@ static __device__ int high_mem_nwf(const KParams& p, int plen, int tlen) {
 // cut at this step. `band_lo/hi` are the banded strategies' (min_k, max_k)
 // in this aligner's diagonals.
 // ---------------------------------------------------------------------
+// A pair's heuristic: its kind and parameters (HeuristicStrategy,
+// src/heuristic.rs), launch-wide or per pair.
+struct HeurCfg {
+  int kind, p0, p1, p2;
+};
+
+static __device__ __forceinline__ HeurCfg launch_heuristic(const KParams& p) {
+  return HeurCfg{p.heuristic, p.hp0, p.hp1, p.hp2};
+}
+
 struct HeurState {
   int steps_wait, max_sw;
   bool have_max;
   // HeuristicState::reset (src/heuristic.rs:122-143)
-  __device__ void reset(const KParams& p, int heur) {
+  __device__ void reset(const HeurCfg& hc) {
+    const int heur = hc.kind;
     steps_wait = heur == WFA_HEURISTIC_WFADAPTIVE || heur == WFA_HEURISTIC_BANDED_ADAPTIVE
-                     ? p.hp2
-                     : heur == WFA_HEURISTIC_XDROP ? p.hp1 : 0;
+                     ? hc.p2
+                     : heur == WFA_HEURISTIC_XDROP ? hc.p1 : 0;
     max_sw = 0;
     have_max = false;
   }
 };
 
 template <typename RT>
-static __device__ bool heuristic_limits(const KParams& p, int heur, int band_lo, int band_hi,
-                                        HeurState& hs, const Rows<RT>& R, int rowM, int lo,
-                                        int hi, int s, int plen, int tlen, int pb_free,
+static __device__ bool heuristic_limits(const KParams& p, const HeurCfg& hc, int band_lo,
+                                        int band_hi, HeurState& hs, const Rows<RT>& R, int rowM,
+                                        int lo, int hi, int s, int plen, int tlen, int pb_free,
                                         int tb_free, int* out_lo, int* out_hi) {
   const int lane = threadIdx.x;
+  const int heur = hc.kind;
   // banded-static inside its band changes nothing and skips the step
   // counter (src/aligner.rs:1645-1654)
   if (heur == WFA_HEURISTIC_BANDED_STATIC && lo >= band_lo && hi <= band_hi) return false;
@@ -708,7 +720,7 @@ static __device__ bool heuristic_limits(const KParams& p, int heur, int band_lo,
   };
   if (heur == WFA_HEURISTIC_WFADAPTIVE) {
     // src/heuristic.rs:258-346
-    if (hs.steps_wait > 0 || hi - lo + 1 < p.hp0) return false;
+    if (hs.steps_wait > 0 || hi - lo + 1 < hc.p0) return false;
     int md = imax(plen, tlen);
     for (int kb = lo; kb <= hi; kb += 64) {
       const int k = kb + lane;
@@ -718,7 +730,7 @@ static __device__ bool heuristic_limits(const KParams& p, int heur, int band_lo,
     int fg = INT32_MAX, lg = INT32_MIN;
     for (int kb = lo; kb <= hi; kb += 64) {
       const int k = kb + lane;
-      const bool good = k <= hi && (int64_t)dist(cell(k), k) - md <= (int64_t)p.hp1;
+      const bool good = k <= hi && (int64_t)dist(cell(k), k) - md <= (int64_t)hc.p1;
       const uint64_t b = ballot(good);
       if (b) {
         if (fg == INT32_MAX) fg = kb + __builtin_ctzll(b);
@@ -732,7 +744,7 @@ static __device__ bool heuristic_limits(const KParams& p, int heur, int band_lo,
     nlo = top <= lo ? lo : imin(fg, top);
     const int bottom = imax(ak, nlo);
     nhi = hi <= bottom ? hi : imax(lg, bottom);
-    hs.steps_wait = p.hp2;
+    hs.steps_wait = hc.p2;
   } else if (heur == WFA_HEURISTIC_BANDED_STATIC) {
     // src/heuristic.rs:400-407
     nlo = imax(lo, band_lo);
@@ -755,7 +767,7 @@ static __device__ bool heuristic_limits(const KParams& p, int heur, int band_lo,
       nlo = imax(nl, lo);
       nhi = imin(nl + maxlen - 1, hi);
     }
-    hs.steps_wait = p.hp2;
+    hs.steps_wait = hc.p2;
   } else if (heur == WFA_HEURISTIC_XDROP) {
     // src/heuristic.rs:410-448, 491-525: the Smith-Waterman-like score of
     // each cell (free begin gaps not counted), cells more than `xdrop`
@@ -775,7 +787,7 @@ static __device__ bool heuristic_limits(const KParams& p, int heur, int band_lo,
           const int sw = (swg * ((off - k - fv) + (off - fh)) - s) / 2;
           const int64_t key = ((int64_t)sw << 32) | (uint32_t)(0x7fffffffu - (uint32_t)k);
           best = best > key ? best : key;
-          good = hs.have_max && hs.max_sw - sw < p.hp0;
+          good = hs.have_max && hs.max_sw - sw < hc.p0;
         }
       }
       const uint64_t b = ballot(good);
@@ -794,7 +806,7 @@ static __device__ bool heuristic_limits(const KParams& p, int heur, int band_lo,
       hs.max_sw = cmax;
       hs.have_max = true;
     }
-    hs.steps_wait = p.hp1;
+    hs.steps_wait = hc.p1;
   }
   *out_lo = nlo;
   *out_hi = nhi;
@@ -830,11 +842,28 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   for (int r = tid; r < nrows; r += NT) R.meta[r] = meta_absent();
   // ends-free limits of this pair (extension: free ends up to the full
   // lengths, no begin gaps; src/aligner.rs:713-718)
-  const bool ef = p.endsfree;
-  const int pb_free = ef && !p.extension ? p.pb_free : 0;
-  const int tb_free = ef && !p.extension ? p.tb_free : 0;
-  const int pe_free = !ef ? 0 : p.extension ? plen : p.pe_free;
-  const int te_free = !ef ? 0 : p.extension ? tlen : p.te_free;
+  // span and heuristic: the launch's, or this pair's within the launch's
+  // envelope (ends-free only in an ends-free launch, a heuristic only in a
+  // heuristic launch)
+  bool ef = p.endsfree, ext = p.extension;
+  int4 fe = make_int4(p.pb_free, p.pe_free, p.tb_free, p.te_free);
+  HeurCfg hc = launch_heuristic(p);
+  if (p.pair_cfg) {
+    const int4 c0 = p.pair_cfg[3 * pair], c1 = p.pair_cfg[3 * pair + 1];
+    const int4 c2 = p.pair_cfg[3 * pair + 2];
+    ef = p.endsfree && uniform(c0.x) == WFA_SPAN_ENDSFREE;
+    ext = ef && uniform(c0.y) != 0;
+    fe = make_int4(imax(uniform(c0.z), 0), imax(uniform(c0.w), 0), imax(uniform(c1.x), 0),
+                   imax(uniform(c1.y), 0));
+    hc = HeurCfg{uniform(c1.z), uniform(c1.w), uniform(c2.x), uniform(c2.y)};
+    if (p.heuristic == WFA_HEURISTIC_NONE || hc.kind == WFA_HEURISTIC_ZDROP ||
+        hc.kind < WFA_HEURISTIC_NONE || hc.kind > WFA_HEURISTIC_BANDED_ADAPTIVE)
+      hc.kind = WFA_HEURISTIC_NONE;
+  }
+  const int pb_free = ef && !ext ? fe.x : 0;
+  const int tb_free = ef && !ext ? fe.z : 0;
+  const int pe_free = !ef ? 0 : ext ? plen : fe.y;
+  const int te_free = !ef ? 0 : ext ? tlen : fe.w;
   // M0: k = 0, offset 0, plus (ends-free, match == 0) the begin-free cells
   // (h, 0) at k = h and (0, v) at k = -v (init_alignment,
   // src/aligner.rs:938-998)
@@ -975,15 +1004,15 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   // The cut narrows M's limits; its cells stay readable within the
   // initialized range (an emptied M reads as the null wavefront) and the
   // gap components of the same score take M's new limits.
-  const int heur = HEUR ? p.heuristic : WFA_HEURISTIC_NONE;
+  const int heur = HEUR ? hc.kind : WFA_HEURISTIC_NONE;
   HeurState hs;
-  hs.reset(p, heur);
+  hs.reset(hc);
   auto cutoff = [&](int s) {
     const int rowM = s % ringM;
     const int4 mm = R.meta[rowM];
     const int lo = uniform(mm.x), hi = uniform(mm.y);
     int nlo = lo, nhi = hi;
-    if (!heuristic_limits(p, heur, p.hp0, p.hp1, hs, R, rowM, lo, hi, s, plen, tlen, pb_free,
+    if (!heuristic_limits(p, hc, hc.p0, hc.p1, hs, R, rowM, lo, hi, s, plen, tlen, pb_free,
                           tb_free, &nlo, &nhi))
       return;
     if (nlo == lo && nhi == hi) return;
@@ -1125,8 +1154,8 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       // clamp_to_band (src/aligner.rs:3308-3316); an emptied range takes
       // the no-output path below
       if (heur == WFA_HEURISTIC_BANDED_STATIC || heur == WFA_HEURISTIC_BANDED_ADAPTIVE) {
-        lo = imax(lo, p.hp0);
-        hi = imin(hi, p.hp1);
+        lo = imax(lo, hc.p0);
+        hi = imin(hi, hc.p1);
       }
     }
     const int rowM = s % ringM;
@@ -1416,7 +1445,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           __builtin_amdgcn_wave_barrier();
         }
       }
-      if (ef && p.extension && KIND != KIND_EDIT && ncig > 0) {
+      if (ef && ext && KIND != KIND_EDIT && ncig > 0) {
         __threadfence_block();
         bool trimmed = false;
         ncig = maxtrim_cigar(p, out, ncig, pb_free, tb_free, &trimmed);

@@ -106,6 +106,7 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
                                     p.heuristic != WFA_HEURISTIC_NONE, c->nw, c->lds_bytes);
   }
   int grid = e->num_cus * per_cu;
+  if (long_reads && e->long_grid_cap > 0) grid = std::min(grid, e->long_grid_cap);
   grid = std::max(1, std::min(grid, npairs));
   c->grid = grid;
   int64_t per_slot = c->lds_ring ? 0 : (int64_t)c->ring_rows * c->wcap * 4;
@@ -145,7 +146,15 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   if (per_slot > 0 && (int64_t)c->grid * per_slot > budget)
     c->grid = (int)std::max<int64_t>(1, budget / per_slot);
   if (cig) {
-    const int64_t elems = (int64_t)c->grid * c->hist_cap;
+    int64_t elems = (int64_t)c->grid * c->hist_cap;
+    if (long_reads || pass > 0) {
+      // few or large pairs: the pool takes what the budget leaves after
+      // the rings, headers and chunk tables, so one pair can grow large
+      const int64_t cell = mode == 2 ? 1 : 4;
+      const int64_t rest = budget - (int64_t)c->grid * (per_slot - c->hist_cap * cell);
+      elems = std::max(elems, rest / cell);
+    }
+    elems = std::min<int64_t>(elems, (int64_t)c->chunk * INT32_MAX);
     c->pool_chunks = (int)std::max<int64_t>(c->grid, (elems + c->chunk - 1) / c->chunk);
   }
   // op chain of one pair: at most every base consumed by a gap op plus one
@@ -240,7 +249,10 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
   if (r) return r;
   if (count == 0) return WFA_OK;
   HIPCHK(hipSetDevice(e->device));
-  if (attr->memory_mode == WFA_MEMORY_ULTRALOW) return biwfa_align_batch(e, attr, b, st, first, count);
+  if (attr->memory_mode == WFA_MEMORY_ULTRALOW) {
+    if (b->pair_config) return WFA_ERR_UNSUPPORTED;
+    return biwfa_align_batch(e, attr, b, st, first, count);
+  }
   KParams p;
   base_params(ip, *attr, *b, &p);
   const int kind = kind_of(ip.metric);
@@ -259,8 +271,24 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
   r = plan(e, p, kind, mode, b->seq_format, max_len, count, 0, &c);
   if (r) return r;
   p.pair_base = first;
+  // Long reads: hand the pairs out longest first, so the longest ones do
+  // not start last and stretch the launch (a device radix sort of the
+  // lengths; short reads are uniform enough to take in order)
+  const int32_t* order = nullptr;
+  if (max_len > 1024 && count > 1) {
+    const size_t tb = wfa::lpt_temp_bytes(count);
+    const size_t nb = ((size_t)count * 4 + 255) & ~(size_t)255;
+    r = grow(&e->d_order, &e->order_bytes, 4 * nb + tb);
+    if (r) return r;
+    uint8_t* ob = (uint8_t*)e->d_order;
+    int32_t* ord = (int32_t*)(ob + 3 * nb);
+    HIPCHK(wfa::launch_lpt_order(b->pattern_length, b->text_length, first, count,
+                                 (uint32_t*)ob, (uint32_t*)(ob + nb), (int32_t*)(ob + 2 * nb),
+                                 ord, ob + 4 * nb, tb, st));
+    order = ord;
+  }
   HIPCHK(hipEventRecord(e->ev[0], st));
-  r = run_launch(e, p, c, count, nullptr, e->d_lists[0], st);
+  r = run_launch(e, p, c, count, order, e->d_lists[0], st);
   if (r) return r;
   HIPCHK(hipEventRecord(e->ev[1], st));
   int32_t nretry = 0;
@@ -269,7 +297,13 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
   e->last_retries = nretry;
   int cur = 0;
   for (int pass = 1; nretry > 0; ++pass) {
-    if (pass > 6) return WFA_ERR_OUT_OF_MEMORY;
+    if (pass > 6) {
+      // what still does not fit the scratch budget gets the reference's
+      // out-of-memory status (AlignStatus::Oom, src/aligner.rs:120-135)
+      HIPCHK(wfa::launch_mark_oom(e->d_lists[cur], nretry, b->score, b->status, b->cigar_length,
+                                  st));
+      break;
+    }
     r = plan(e, p, kind, mode, b->seq_format, max_len, nretry, pass, &c);
     if (r) return r;
     r = run_launch(e, p, c, nretry, e->d_lists[cur], e->d_lists[cur ^ 1], st);
@@ -337,6 +371,7 @@ void wfa_engine_free(wfa_engine_t* e) {
   hipFree(e->d_gring);
   hipFree(e->d_chain);
   hipFree(e->d_tab);
+  hipFree(e->d_order);
   for (auto b : e->bi_buf) hipFree(b);
   hipHostFree(e->h_pinned);
   for (auto ev : e->ev) hipEventDestroy(ev);

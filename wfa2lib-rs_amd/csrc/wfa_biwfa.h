@@ -149,7 +149,7 @@ struct WfDir {
       R.meta[row(comp, 0)] = make_int4(0, 0, 0, 0);
     }
     __syncthreads();
-    hs.reset(*p, heur);
+    hs.reset(HeurCfg{heur, p->hp0, p->hp1, p->hp2});
     hist_lo = -(plen + scope + 2);
     hist_hi = tlen + scope + 2;
     nulls = 0;
@@ -386,8 +386,8 @@ struct WfDir {
   __device__ void cutoff(int s, int lo, int hi, int4 mm) {
     const int rM = row(C_M, s);
     int nlo = lo, nhi = hi;
-    if (!heuristic_limits(*p, heur, band_lo, band_hi, hs, R, rM, lo, hi, s, plen, tlen, 0, 0,
-                          &nlo, &nhi))
+    if (!heuristic_limits(*p, HeurCfg{heur, p->hp0, p->hp1, p->hp2}, band_lo, band_hi, hs, R,
+                          rM, lo, hi, s, plen, tlen, 0, 0, &nlo, &nhi))
       return;
     if (nlo == lo && nhi == hi) return;
     const bool empty = nlo > nhi;

@@ -22,6 +22,9 @@ struct KParams {
   // span (ends-free with match == 0; extension)
   int endsfree, extension;
   int pb_free, pe_free, tb_free, te_free;
+  // optional per-pair span and heuristic (wfa_pair_config_t, three int4
+  // per pair) within the envelope of the launch's span and heuristic
+  const int4* pair_cfg;
   // user penalties (max-trim of extension alignments and the heuristic
   // rescoring score with them)
   int u_match, u_x, u_o1, u_e1, u_o2, u_e2;
@@ -152,7 +155,18 @@ hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, in
 // i16: ring offsets held in 16 bits (pairs of at most 8000 bases)
 // Waves per workgroup (one pair each) of the wide kernels, whose rings live
 // in HBM, without heuristics
-constexpr int kWideWaves = 4;
+constexpr int kWideWaves = 8;
+
+// Status WFA_STATUS_OOM (score 0, no CIGAR) for the `n` pairs of `list`
+hipError_t launch_mark_oom(const int32_t* list, int n, int32_t* score, int32_t* status,
+                           int32_t* cig_len, hipStream_t stream);
+
+// Longest-first order of pairs first..first+n-1 into `order` (a device
+// radix sort; keys/keys_sorted/vals hold n entries, temp lpt_temp_bytes(n))
+size_t lpt_temp_bytes(int n);
+hipError_t launch_lpt_order(const int32_t* plen, const int32_t* tlen, int first, int n,
+                            uint32_t* keys, uint32_t* keys_sorted, int32_t* vals,
+                            int32_t* order, void* temp, size_t temp_bytes, hipStream_t stream);
 
 // `nw` waves per pair (1, or kWideWaves for HBM rings without heuristics)
 hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, bool i16, int nw,

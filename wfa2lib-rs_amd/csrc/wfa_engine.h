@@ -141,9 +141,14 @@ struct wfa_engine_t {
   size_t chain_bytes = 0;
   int32_t* d_tab = nullptr;  // history pool: chunk ids per slot
   size_t tab_bytes = 0;
+  void* d_order = nullptr;   // longest-first work order: keys, sorted keys, values, order, temp
+  size_t order_bytes = 0;
   int last_retries = 0;
   // device scratch one launch may take (wavefront rings, CIGAR history)
   int64_t scratch_budget = (int64_t)48 << 30;
+  // most workgroups of a long-read launch (0 = as many as fit): a serving
+  // loop leaves room on every CU for the short requests' launches
+  int long_grid_cap = 0;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   float first_ms = 0.f, all_ms = 0.f;
   // BiWFA scratch (wfa_biwfa.cpp), grown on demand
@@ -197,6 +202,7 @@ inline void base_params(const Internal& ip, const wfa_attr_t& attr, const wfa_ba
   p->pe_free = attr.pattern_end_free;
   p->tb_free = attr.text_begin_free;
   p->te_free = attr.text_end_free;
+  p->pair_cfg = (const int4*)b.pair_config;
   p->u_match = attr.penalties.match;
   p->u_x = attr.penalties.mismatch;
   p->u_o1 = attr.penalties.gap_opening1;

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include "wfa_align.h"
 #include "wfa_device.h"
 
@@ -20,6 +22,27 @@ __global__ void wfa_max_length(const int32_t* plen, const int32_t* tlen, int n, 
     m = max(m, max(plen[i], tlen[i]));
   for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
   if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// Longest-first work order: key = ~(plen + tlen) so an ascending sort puts
+// the longest pairs first; value = the pair index.
+__global__ void wfa_lpt_keys(const int32_t* plen, const int32_t* tlen, int first, int n,
+                             uint32_t* keys, int32_t* vals) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    keys[i] = ~(uint32_t)(plen[first + i] + tlen[first + i]);
+    vals[i] = first + i;
+  }
+}
+
+// Pairs that outgrew every pass: out of memory, no CIGAR
+__global__ void wfa_mark_oom(const int32_t* list, int n, int32_t* score, int32_t* status,
+                             int32_t* cig_len) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int pair = list[i];
+    score[pair] = 0;
+    status[pair] = WFA_STATUS_OOM;
+    if (cig_len) cig_len[pair] = 0;
+  }
 }
 
 // BiWFA: copy the CIGAR pieces (fragments or gap runs) into place
@@ -59,6 +82,35 @@ static KernelFn pick(int kind, int mode, bool lds, bool i16, bool packed, bool h
     case KIND_AFFINE2P: return pick_affine2p(mode, lds, i16, packed, heur, nw);
     default: return nullptr;
   }
+}
+
+size_t lpt_temp_bytes(int n) {
+  size_t bytes = 0;
+  hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                     (const int32_t*)nullptr, (int32_t*)nullptr, n);
+  return bytes;
+}
+
+hipError_t launch_lpt_order(const int32_t* plen, const int32_t* tlen, int first, int n,
+                            uint32_t* keys, uint32_t* keys_sorted, int32_t* vals,
+                            int32_t* order, void* temp, size_t temp_bytes, hipStream_t stream) {
+  int grid = (n + 255) / 256;
+  if (grid > 2048) grid = 2048;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(wfa_lpt_keys, dim3(grid), dim3(256), 0, stream, plen, tlen, first, n, keys,
+                     vals);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_sorted, vals, order, n,
+                                            0, 32, stream);
+}
+
+hipError_t launch_mark_oom(const int32_t* list, int n, int32_t* score, int32_t* status,
+                           int32_t* cig_len, hipStream_t stream) {
+  const int grid = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
+  hipLaunchKernelGGL(wfa_mark_oom, dim3(grid > 0 ? grid : 1), dim3(256), 0, stream, list, n,
+                     score, status, cig_len);
+  return hipGetLastError();
 }
 
 hipError_t launch_reset(int32_t* counters, hipStream_t stream) {

@@ -9,6 +9,13 @@
 // - Requests wait in per-class FIFO queues. A class is one configuration,
 //   one sequence format and one length bucket: a launch is planned for its
 //   longest pair, so short reads never share a launch with long ones.
+//   Requests that differ only in span (ends-free or extension, any free
+//   ends) or only in heuristic share a class: the launch takes each pair's
+//   span and heuristic from a per-pair configuration, within the class's
+//   envelope (end-to-end or ends-free; no heuristic or some heuristic).
+// - Long-read launches (longer than 1024 bases) may take at most half the
+//   engines and half a workgroup per CU each, so short requests always find
+//   an engine and room on the CUs.
 // - A few engines, each with its own HIP stream, scratch and worker thread,
 //   pull work. An idle worker takes the class whose oldest request has
 //   waited longest among those that fit, admits that class's requests in
@@ -46,15 +53,20 @@ struct Req {
   size_t pbytes, tbytes;
   std::vector<uint8_t> seq;  // pattern bytes, then text bytes
   int64_t footprint;         // predicted device bytes while it runs
+  wfa_pair_config_t cfg;     // this request's span and heuristic
   Clock::time_point t0;
 };
 
 struct Cls {
-  wfa_attr_t attr;
+  wfa_attr_t attr;  // the envelope: per-pair spans and heuristics within it
   int32_t fmt;
   int bucket;
+  bool per_pair;    // launches carry each request's span and heuristic
   std::deque<Req> q;
 };
+
+// buckets above this one hold reads longer than 1024 bases
+constexpr int kLongBucket = 1;
 
 struct Out {
   uint64_t tag;
@@ -119,6 +131,7 @@ struct wfa_server_t {
   std::deque<Out> done;
   std::vector<Out> polled;  // results of the last poll (their CIGARs stay valid)
   int64_t in_flight = 0;    // predicted device bytes of running launches
+  int long_running = 0;     // long-read launches on the device
   int64_t queued = 0, running = 0, launches = 0, launched_pairs = 0;
   bool stop = false;
 };
@@ -127,12 +140,14 @@ namespace {
 
 // Pick the next launch under s->mu: the fitting class whose head waited
 // longest, then its requests in order while they fit.
-bool pick(wfa_server_t* s, wfa_attr_t* attr_out, int* fmt_out, std::vector<Req>* batch,
-          int64_t* fp_out) {
+bool pick(wfa_server_t* s, wfa_attr_t* attr_out, int* fmt_out, bool* per_pair_out,
+          bool* long_out, std::vector<Req>* batch, int64_t* fp_out) {
   int best = -1;
   for (int i = 0; i < (int)s->classes.size(); ++i) {
     const Cls& c = s->classes[i];
     if (c.q.empty()) continue;
+    if (c.bucket > kLongBucket && 2 * (s->long_running + 1) > std::max(2, s->cfg.num_engines))
+      continue;
     const bool fits = s->in_flight == 0 || s->in_flight + c.q.front().footprint <= s->cfg.memory_budget;
     if (!fits) continue;
     if (best < 0 || c.q.front().t0 < s->classes[best].q.front().t0) best = i;
@@ -152,6 +167,8 @@ bool pick(wfa_server_t* s, wfa_attr_t* attr_out, int* fmt_out, std::vector<Req>*
   if (batch->empty()) return false;
   *attr_out = c.attr;
   *fmt_out = c.fmt;
+  *per_pair_out = c.per_pair;
+  *long_out = c.bucket > kLongBucket;
   *fp_out = fp;
   // classes come and go with the traffic (ends-free requests carry their
   // own free ends): forget emptied ones
@@ -186,7 +203,7 @@ int grow_device(uint8_t** p, size_t* cap, size_t need) {
 // offsets | CIGAR offsets (int64) | pattern lengths | text lengths (int32)
 // || scores | statuses | CIGAR lengths (int32) | CIGAR ops; the part after
 // || is output.
-int run_batch(Worker& w, const wfa_attr_t& attr, int fmt, std::vector<Req>& reqs,
+int run_batch(Worker& w, const wfa_attr_t& attr, int fmt, bool per_pair, std::vector<Req>& reqs,
               std::vector<Out>* outs) {
   const size_t n = reqs.size();
   const bool cig = attr.alignment_scope == WFA_SCOPE_ALIGNMENT;
@@ -202,7 +219,9 @@ int run_batch(Worker& w, const wfa_attr_t& attr, int fmt, std::vector<Req>& reqs
   const size_t o_coff = o_toff + align256(n * 8);
   const size_t o_plen = o_coff + align256(n * 8);
   const size_t o_tlen = o_plen + align256(n * 4);
-  const size_t o_score = o_tlen + align256(n * 4);  // outputs start here
+  const size_t o_cfg = o_tlen + align256(n * 4);
+  const size_t o_score =
+      o_cfg + (per_pair ? align256(n * sizeof(wfa_pair_config_t)) : 0);  // outputs start here
   const size_t o_status = o_score + align256(n * 4);
   const size_t o_clen = o_status + align256(n * 4);
   const size_t o_cig = o_clen + align256(n * 4);
@@ -216,6 +235,7 @@ int run_batch(Worker& w, const wfa_attr_t& attr, int fmt, std::vector<Req>& reqs
   int64_t* coff = (int64_t*)(w.h + o_coff);
   int32_t* plen = (int32_t*)(w.h + o_plen);
   int32_t* tlen = (int32_t*)(w.h + o_tlen);
+  wfa_pair_config_t* cfg = (wfa_pair_config_t*)(w.h + o_cfg);
   size_t pos = 0, cpos = 0;
   for (size_t i = 0; i < n; ++i) {
     const Req& q = reqs[i];
@@ -227,6 +247,7 @@ int run_batch(Worker& w, const wfa_attr_t& attr, int fmt, std::vector<Req>& reqs
     if (cig) cpos += (size_t)q.plen + q.tlen;
     plen[i] = q.plen;
     tlen[i] = q.tlen;
+    if (per_pair) cfg[i] = q.cfg;
   }
   HIPCHK(hipMemcpyAsync(w.d, w.h, o_score, hipMemcpyHostToDevice, w.st));
   wfa_batch_t b;
@@ -241,6 +262,7 @@ int run_batch(Worker& w, const wfa_attr_t& attr, int fmt, std::vector<Req>& reqs
   b.text_length = (const int32_t*)(w.d + o_tlen);
   b.score = (int32_t*)(w.d + o_score);
   b.status = (int32_t*)(w.d + o_status);
+  if (per_pair) b.pair_config = (const wfa_pair_config_t*)(w.d + o_cfg);
   if (cig) {
     b.cigar_ops = w.d + o_cig;
     b.cigar_offset = (const int64_t*)(w.d + o_coff);
@@ -282,9 +304,12 @@ void worker_main(wfa_server_t* s, Worker* w) {
     wfa_attr_t attr;
     int fmt = 0;
     int64_t fp = 0;
-    bool got = false;
-    s->cv_work.wait(lk, [&] { return s->stop || (got = pick(s, &attr, &fmt, &batch, &fp)); });
+    bool got = false, per_pair = false, is_long = false;
+    s->cv_work.wait(lk, [&] {
+      return s->stop || (got = pick(s, &attr, &fmt, &per_pair, &is_long, &batch, &fp));
+    });
     if (!got) break;
+    if (is_long) s->long_running += 1;
     const int64_t n = (int64_t)batch.size();
     s->queued -= n;
     s->running += n;
@@ -292,7 +317,7 @@ void worker_main(wfa_server_t* s, Worker* w) {
     s->launches += 1;
     s->launched_pairs += n;
     lk.unlock();
-    const int r = run_batch(*w, attr, fmt, batch, &outs);
+    const int r = run_batch(*w, attr, fmt, per_pair, batch, &outs);
     if (r) {
       const Clock::time_point t1 = Clock::now();
       outs.resize(batch.size());
@@ -311,6 +336,7 @@ void worker_main(wfa_server_t* s, Worker* w) {
     batch.clear();
     s->running -= n;
     s->in_flight -= fp;
+    if (is_long) s->long_running -= 1;
     s->cv_done.notify_all();
     s->cv_work.notify_all();
   }
@@ -344,7 +370,7 @@ void wfa_server_config_default(wfa_server_config_t* c) {
   c->device = 0;
   c->num_engines = 4;
   c->max_batch_pairs = 1 << 16;
-  c->memory_budget = (int64_t)32 << 30;
+  c->memory_budget = (int64_t)64 << 30;
 }
 
 wfa_server_t* wfa_server_new(const wfa_server_config_t* cfg, int32_t* err) {
@@ -367,8 +393,14 @@ wfa_server_t* wfa_server_new(const wfa_server_config_t* cfg, int32_t* err) {
       if (err) *err = e;
       return nullptr;
     }
-    // every engine may take its share of the device's scratch budget
-    w->e->scratch_budget = std::max<int64_t>(c.memory_budget / c.num_engines, (int64_t)1 << 30);
+    // Admission keeps the predicted total within the budget; an engine's
+    // own launches may use up to twice its even share (at most the whole
+    // budget), so one large request fits in one engine
+    w->e->scratch_budget = std::max<int64_t>(
+        std::min(c.memory_budget, 2 * c.memory_budget / c.num_engines), (int64_t)1 << 30);
+    // a long-read launch takes at most half a workgroup per CU on average,
+    // so short requests' launches always find room on the CUs
+    w->e->long_grid_cap = std::max(1, w->e->num_cus / 2);
     if (hipStreamCreateWithFlags(&w->st, hipStreamNonBlocking) != hipSuccess) {
       s->workers.push_back(std::move(w));
       destroy(s);
@@ -406,6 +438,25 @@ int32_t wfa_server_submit(wfa_server_t* s, const wfa_server_request_t* rq) {
   if (q.tbytes) std::memcpy(q.seq.data() + q.pbytes, rq->text, q.tbytes);
   q.footprint = predict_footprint(rq->attr, ip, q.plen, q.tlen);
   const int bucket = bucket_of(std::max(q.plen, q.tlen));
+  // the class key: the configuration with span and heuristic reduced to
+  // their envelope; the request's own values travel per pair
+  wfa_attr_t key = rq->attr;
+  std::memset(&q.cfg, 0, sizeof(q.cfg));
+  q.cfg.span = key.span;
+  q.cfg.extension = key.extension;
+  q.cfg.pattern_begin_free = key.pattern_begin_free;
+  q.cfg.pattern_end_free = key.pattern_end_free;
+  q.cfg.text_begin_free = key.text_begin_free;
+  q.cfg.text_end_free = key.text_end_free;
+  q.cfg.heuristic = key.heuristic;
+  for (int i = 0; i < 3; ++i) q.cfg.heuristic_p[i] = key.heuristic_p[i];
+  const bool per_pair = key.memory_mode != WFA_MEMORY_ULTRALOW;
+  if (per_pair) {
+    key.extension = 0;
+    key.pattern_begin_free = key.pattern_end_free = key.text_begin_free = key.text_end_free = 0;
+    if (key.heuristic != WFA_HEURISTIC_NONE) key.heuristic = WFA_HEURISTIC_WFADAPTIVE;
+    key.heuristic_p[0] = key.heuristic_p[1] = key.heuristic_p[2] = 0;
+  }
   {
     std::lock_guard<std::mutex> g(s->mu);
     if (s->stop) return WFA_ERR_INVALID_ARGUMENT;
@@ -413,12 +464,12 @@ int32_t wfa_server_submit(wfa_server_t* s, const wfa_server_request_t* rq) {
     Cls* c = nullptr;
     for (Cls& k : s->classes)
       if (k.fmt == rq->seq_format && k.bucket == bucket &&
-          std::memcmp(&k.attr, &rq->attr, sizeof(wfa_attr_t)) == 0) {
+          std::memcmp(&k.attr, &key, sizeof(wfa_attr_t)) == 0) {
         c = &k;
         break;
       }
     if (!c) {
-      s->classes.push_back(Cls{rq->attr, rq->seq_format, bucket, {}});
+      s->classes.push_back(Cls{key, rq->seq_format, bucket, per_pair, {}});
       c = &s->classes.back();
     }
     c->q.push_back(std::move(q));

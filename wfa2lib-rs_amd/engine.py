@@ -24,10 +24,20 @@ def make_attr(penalties: WavefrontPenalties, scope=N.SCOPE_SCORE, memory_mode=N.
     return attr
 
 
+def pair_config_row(attr):
+    """The wfa_pair_config_t row (12 int32) of an aligner configuration:
+    its span, extension, free ends and heuristic."""
+    return [attr.span, attr.extension, attr.pattern_begin_free, attr.pattern_end_free,
+            attr.text_begin_free, attr.text_end_free, attr.heuristic, *attr.heuristic_p, 0, 0]
+
+
 class DeviceBatch:
     """A batch resident in device memory (HBM)."""
 
-    def __init__(self, pairs, device="cuda", cigar=False):
+    def __init__(self, pairs, device="cuda", cigar=False, pair_config=None):
+        """`pair_config`: optional (n, 12) int32 rows of wfa_pair_config_t
+        (span, extension, four free ends, heuristic, three parameters, two
+        zeros), see ``pair_config_row``."""
         import torch
         self.n = len(pairs)
         self.seq_format = pairs.seq_format
@@ -41,6 +51,10 @@ class DeviceBatch:
         self.score = torch.zeros(max(self.n, 1), dtype=torch.int32, device=device)
         self.status = torch.zeros(max(self.n, 1), dtype=torch.int32, device=device)
         self.cigar = cigar
+        self.pair_config = None
+        if pair_config is not None:
+            pc = np.ascontiguousarray(np.asarray(pair_config, dtype=np.int32).reshape(self.n, 12))
+            self.pair_config = t(pc)
         if cigar:
             cap = pairs.plen.astype(np.int64) + pairs.tlen.astype(np.int64)
             off = np.concatenate([[0], np.cumsum(cap)[:-1]]).astype(np.int64) if self.n else \
@@ -65,6 +79,8 @@ class DeviceBatch:
             b.cigar_ops = self.cig.data_ptr()
             b.cigar_offset = self.cig_off.data_ptr()
             b.cigar_length = self.cig_len.data_ptr()
+        if self.pair_config is not None:
+            b.pair_config = self.pair_config.data_ptr()
         return b
 
     def results(self):

@@ -26,7 +26,7 @@ class ServerResult:
 
 class AlignmentServer:
     def __init__(self, device=0, num_engines=4, max_batch_pairs=1 << 16,
-                 memory_budget=32 << 30):
+                 memory_budget=64 << 30):
         cfg = N.ServerConfig()
         N.lib().wfa_server_config_default(ctypes.byref(cfg))
         cfg.device = device
