@@ -101,10 +101,13 @@ def serving_leg(n, rate, lengths, seed, oracle_sample, log):
     mix = serving_mix(n, rate, seed=seed, lengths=lengths)
     attrs = [request_attr(r.span, r.heuristic, len(r.pattern), len(r.text)) for r in mix]
     srv = AlignmentServer(device=0)
-    # warm the engines (device buffers, code objects) on a few requests
-    for i in range(min(64, n)):
-        srv.submit(attrs[i], mix[i].pattern, mix[i].text, tag=10 ** 9 + i)
-    srv.drain(min(64, n), timeout_s=300)
+    # warm the engines (device buffers, code objects) on requests of the
+    # same mix (another seed)
+    warm = serving_mix(min(400, n), 1e9, seed=seed + 1000, lengths=lengths)
+    for i, r in enumerate(warm):
+        srv.submit(request_attr(r.span, r.heuristic, len(r.pattern), len(r.text)), r.pattern,
+                   r.text, tag=10 ** 9 + i)
+    srv.drain(len(warm), timeout_s=600)
     got, recv = {}, {}
     done = threading.Event()
 

@@ -62,7 +62,14 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   c->mode = mode;
   c->cig = cig;
   c->ring_rows = ring_rows_of(p, kind);
-  const int per = fmt == WFA_SEQ_PACKED2BITS ? 32 : 8;
+  // ASCII pairs are staged 2-bit when every byte is A/C/G/T and byte-wise
+  // otherwise; when byte-wise staging of the longest pair would not fit in
+  // LDS, LDS is sized for 2-bit staging and a pair that needs bytes comes
+  // back for retry (and ends with WFA_STATUS_OOM)
+  const int per = fmt == WFA_SEQ_PACKED2BITS ||
+                          (size_t)2 * (((max_len + 7) / 8 + 3) & ~1) * 8 > e->lds_max / 2
+                      ? 32
+                      : 8;
   // even, so the per-row int4 metadata after the two sequences is 16-byte aligned
   c->seq_words = ((max_len + per - 1) / per + 2 + 1) & ~1;
   // the widest possible computed range: [-(plen+scope+1), tlen+scope+1]
@@ -147,9 +154,9 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
     c->grid = (int)std::max<int64_t>(1, budget / per_slot);
   if (cig) {
     int64_t elems = (int64_t)c->grid * c->hist_cap;
-    if (long_reads || pass > 0) {
-      // few or large pairs: the pool takes what the budget leaves after
-      // the rings, headers and chunk tables, so one pair can grow large
+    if (pass > 0) {
+      // the pairs that outgrew the first pass: the pool takes what the
+      // budget leaves after the rings, headers and chunk tables
       const int64_t cell = mode == 2 ? 1 : 4;
       const int64_t rest = budget - (int64_t)c->grid * (per_slot - c->hist_cap * cell);
       elems = std::max(elems, rest / cell);

@@ -120,6 +120,26 @@ int64_t predict_footprint(const wfa_attr_t& a, const Internal& ip, int plen, int
 
 inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+bool acgt_only(const uint8_t* s, int n) {
+  for (int i = 0; i < n; ++i)
+    if (s[i] != 'A' && s[i] != 'C' && s[i] != 'G' && s[i] != 'T') return false;
+  return true;
+}
+
+// 2-bit packing, 4 bases per byte, LSB first, A=0 C=1 G=2 T=3
+// (src/sequences.rs:55-74)
+void pack2(const uint8_t* s, int n, uint8_t* out) {
+  for (int i = 0; i < n; i += 4) {
+    uint8_t b = 0;
+    for (int j = 0; j < 4 && i + j < n; ++j) {
+      const uint8_t c = s[i + j];
+      const int v = c == 'A' ? 0 : c == 'C' ? 1 : c == 'G' ? 2 : 3;
+      b |= (uint8_t)(v << (2 * j));
+    }
+    out[i / 4] = b;
+  }
+}
+
 }  // namespace
 
 struct wfa_server_t {
@@ -138,6 +158,13 @@ struct wfa_server_t {
 
 namespace {
 
+// Long-read requests may fill the budget only up to 7/8: the rest stays
+// for short requests, whatever the long ones queue.
+int64_t limit_of(const wfa_server_t* s, const Cls& c) {
+  return c.bucket > kLongBucket ? s->cfg.memory_budget - s->cfg.memory_budget / 8
+                                : s->cfg.memory_budget;
+}
+
 // Pick the next launch under s->mu: the fitting class whose head waited
 // longest, then its requests in order while they fit.
 bool pick(wfa_server_t* s, wfa_attr_t* attr_out, int* fmt_out, bool* per_pair_out,
@@ -148,18 +175,19 @@ bool pick(wfa_server_t* s, wfa_attr_t* attr_out, int* fmt_out, bool* per_pair_ou
     if (c.q.empty()) continue;
     if (c.bucket > kLongBucket && 2 * (s->long_running + 1) > std::max(2, s->cfg.num_engines))
       continue;
-    const bool fits = s->in_flight == 0 || s->in_flight + c.q.front().footprint <= s->cfg.memory_budget;
+    const bool fits = s->in_flight == 0 || s->in_flight + c.q.front().footprint <= limit_of(s, c);
     if (!fits) continue;
     if (best < 0 || c.q.front().t0 < s->classes[best].q.front().t0) best = i;
   }
   if (best < 0) return false;
   Cls& c = s->classes[best];
+  const int64_t limit = limit_of(s, c);
   int64_t fp = 0;
   batch->clear();
   while (!c.q.empty() && (int)batch->size() < s->cfg.max_batch_pairs) {
     const int64_t f = c.q.front().footprint;
-    if (!batch->empty() && s->in_flight + fp + f > s->cfg.memory_budget) break;
-    if (batch->empty() && s->in_flight > 0 && s->in_flight + f > s->cfg.memory_budget) break;
+    if (!batch->empty() && s->in_flight + fp + f > limit) break;
+    if (batch->empty() && s->in_flight > 0 && s->in_flight + f > limit) break;
     fp += f;
     batch->push_back(std::move(c.q.front()));
     c.q.pop_front();
@@ -430,12 +458,22 @@ int32_t wfa_server_submit(wfa_server_t* s, const wfa_server_request_t* rq) {
   q.tag = rq->tag;
   q.plen = rq->pattern_length;
   q.tlen = rq->text_length;
-  const bool packed = rq->seq_format == WFA_SEQ_PACKED2BITS;
+  // ASCII sequences of A/C/G/T only travel 2-bit packed (a quarter of the
+  // bytes; the kernels stage them 2-bit either way)
+  int fmt = rq->seq_format;
+  if (fmt == WFA_SEQ_ASCII && acgt_only(rq->pattern, q.plen) && acgt_only(rq->text, q.tlen))
+    fmt = WFA_SEQ_PACKED2BITS;
+  const bool packed = fmt == WFA_SEQ_PACKED2BITS;
   q.pbytes = packed ? ((size_t)q.plen + 3) / 4 : (size_t)q.plen;
   q.tbytes = packed ? ((size_t)q.tlen + 3) / 4 : (size_t)q.tlen;
   q.seq.resize(q.pbytes + q.tbytes);
-  if (q.pbytes) std::memcpy(q.seq.data(), rq->pattern, q.pbytes);
-  if (q.tbytes) std::memcpy(q.seq.data() + q.pbytes, rq->text, q.tbytes);
+  if (rq->seq_format == WFA_SEQ_ASCII && packed) {
+    pack2(rq->pattern, q.plen, q.seq.data());
+    pack2(rq->text, q.tlen, q.seq.data() + q.pbytes);
+  } else {
+    if (q.pbytes) std::memcpy(q.seq.data(), rq->pattern, q.pbytes);
+    if (q.tbytes) std::memcpy(q.seq.data() + q.pbytes, rq->text, q.tbytes);
+  }
   q.footprint = predict_footprint(rq->attr, ip, q.plen, q.tlen);
   const int bucket = bucket_of(std::max(q.plen, q.tlen));
   // the class key: the configuration with span and heuristic reduced to
@@ -463,13 +501,13 @@ int32_t wfa_server_submit(wfa_server_t* s, const wfa_server_request_t* rq) {
     q.t0 = Clock::now();
     Cls* c = nullptr;
     for (Cls& k : s->classes)
-      if (k.fmt == rq->seq_format && k.bucket == bucket &&
+      if (k.fmt == fmt && k.bucket == bucket &&
           std::memcmp(&k.attr, &key, sizeof(wfa_attr_t)) == 0) {
         c = &k;
         break;
       }
     if (!c) {
-      s->classes.push_back(Cls{key, rq->seq_format, bucket, per_pair, {}});
+      s->classes.push_back(Cls{key, fmt, bucket, per_pair, {}});
       c = &s->classes.back();
     }
     c->q.push_back(std::move(q));
