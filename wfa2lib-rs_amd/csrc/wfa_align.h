@@ -1541,6 +1541,9 @@ template <int K, int M, bool HEUR>
 static KernelFn pick_kmh(bool lds, bool i16, bool packed, int nw) {
   if (!lds) {
     if constexpr (!HEUR) {
+      if (nw > 1 && i16)
+        return packed ? wfa_align_kernel<K, M, false, true, true, false, kWideWaves>
+                      : wfa_align_kernel<K, M, false, true, false, false, kWideWaves>;
       if (nw > 1)
         return packed ? wfa_align_kernel<K, M, false, false, true, false, kWideWaves>
                       : wfa_align_kernel<K, M, false, false, false, false, kWideWaves>;

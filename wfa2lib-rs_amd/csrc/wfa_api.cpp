@@ -85,11 +85,15 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   c->i16 = max_len <= 8000 && pass < 2;
   const size_t ring_bytes = (size_t)c->ring_rows * c->wcap * (c->i16 ? 2 : 4);
   c->lds_ring = seq_meta + ring_bytes <= 64 * 1024;
-  if (!c->lds_ring) c->i16 = false;
   // rings in HBM are wide: several waves share each pair's rows (heuristic
   // cut-offs need the single-wave kernel); their cross-wave reductions take
   // 16 ints per wave of LDS
   c->nw = !c->lds_ring && p.heuristic == WFA_HEURISTIC_NONE ? wfa::kWideWaves : 1;
+  // The wide kernels are bound by the rings' HBM traffic (every score reads
+  // and writes whole rows): 16-bit rows for pairs of at most 16000 bases
+  // halve it (offsets stay below 2^15; a NULL row value rises by at most
+  // one per score, and pairs whose score passes 16000 re-run with 32 bits)
+  if (!c->lds_ring) c->i16 = c->nw > 1 && max_len <= 16000 && pass < 2;
   const size_t red_bytes = c->lds_ring ? 0 : (size_t)64 * c->nw;
   c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : red_bytes);
   if (c->lds_bytes > e->lds_max) return WFA_ERR_UNSUPPORTED;
@@ -154,9 +158,10 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
     c->grid = (int)std::max<int64_t>(1, budget / per_slot);
   if (cig) {
     int64_t elems = (int64_t)c->grid * c->hist_cap;
-    if (pass > 0) {
-      // the pairs that outgrew the first pass: the pool takes what the
-      // budget leaves after the rings, headers and chunk tables
+    if (pass > 0 || (long_reads && e->long_pool_whole_budget)) {
+      // long reads, or the pairs that outgrew the first pass: the pool
+      // takes what the budget leaves after the rings, headers and chunk
+      // tables
       const int64_t cell = mode == 2 ? 1 : 4;
       const int64_t rest = budget - (int64_t)c->grid * (per_slot - c->hist_cap * cell);
       elems = std::max(elems, rest / cell);

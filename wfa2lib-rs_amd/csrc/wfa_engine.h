@@ -149,6 +149,10 @@ struct wfa_engine_t {
   // most workgroups of a long-read launch (0 = as many as fit): a serving
   // loop leaves room on every CU for the short requests' launches
   int long_grid_cap = 0;
+  // long reads: the first pass's history pool takes the whole budget (a
+  // batch engine, allocated once and reused); a serving loop's engines size
+  // it by the slots' average and leave the rest to retries
+  bool long_pool_whole_budget = true;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   float first_ms = 0.f, all_ms = 0.f;
   // BiWFA scratch (wfa_biwfa.cpp), grown on demand

@@ -429,6 +429,9 @@ wfa_server_t* wfa_server_new(const wfa_server_config_t* cfg, int32_t* err) {
     // a long-read launch takes at most half a workgroup per CU on average,
     // so short requests' launches always find room on the CUs
     w->e->long_grid_cap = std::max(1, w->e->num_cus / 2);
+    // and sizes its history pool by the slots' average: multi-GiB
+    // allocations while requests run would stall the other streams
+    w->e->long_pool_whole_budget = false;
     if (hipStreamCreateWithFlags(&w->st, hipStreamNonBlocking) != hipSuccess) {
       s->workers.push_back(std::move(w));
       destroy(s);
