@@ -114,7 +114,9 @@ int run_jobs(wfa_engine_t* e, const KParams& kp, int kind, bool heur, int which,
     bp.seq_words = words_for(32);
     if (pass > 0 && (size_t)2 * words_for(8) * 8 + meta_bytes <= e->lds_max)
       bp.seq_words = words_for(8);
-    const size_t lds = (size_t)2 * bp.seq_words * 8 + meta_bytes;
+    // breakpoint launches: 16 ints per wave for cross-wave reductions
+    const size_t lds = (size_t)2 * bp.seq_words * 8 + meta_bytes +
+                       (which == 0 ? (size_t)64 * wfa::kWideWaves : 0);
     if (lds > e->lds_max) return WFA_ERR_UNSUPPORTED;
     // rows: no row is wider than the sub-problem's diagonal range
     const int full = ceil_pow2(max_sum + 2 * kp.scope + 4);

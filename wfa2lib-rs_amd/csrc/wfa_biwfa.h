@@ -2,8 +2,9 @@
 //
 // The divide-and-conquer recursion of align_biwfa (src/aligner.rs:2563-2950)
 // runs level by level from the host: every level, one launch searches the
-// breakpoints of all pending sub-problems of all pairs (one 64-lane
-// wavefront per sub-problem, forward and reverse searches interleaved as in
+// breakpoints of all pending sub-problems of all pairs (one workgroup per
+// sub-problem: one wave with a heuristic, else kWideWaves waves sharing
+// every row's chunks; forward and reverse searches interleaved as in
 // bialign_find_breakpoint, src/aligner.rs:2958-3120), and one launch aligns
 // the base cases (bialign_base, src/aligner.rs:3127-3202) with full storage
 // and the component backtrace (backtrace_affine_c, src/backtrace.rs:492-635).
@@ -22,12 +23,14 @@ namespace wfa {
 // Stage bases [start, start + len) of one stored sequence as 64-bit words
 // (2-bit: base j at bits 2j; bytes: byte j at bits 8j), zero-padded to
 // `words` words.
+template <int NW>
 static __device__ void stage_range(const uint8_t* src, int fmt, int start, int len, int bits,
                                    uint64_t* dst, int words) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x;  // all 64 * NW threads
+  constexpr int NT = 64 * NW;
   if (fmt == WFA_SEQ_PACKED2BITS) {
     const int last_byte = len > 0 ? (start + len - 1) >> 2 : -1;
-    for (int w = lane; w < words; w += 64) {
+    for (int w = lane; w < words; w += NT) {
       uint64_t out = 0;
       const int b0 = start + 32 * w;  // first base of the word
       const int nb = imin(32, len - 32 * w);
@@ -43,7 +46,7 @@ static __device__ void stage_range(const uint8_t* src, int fmt, int start, int l
       dst[w] = out;
     }
   } else if (bits == 2) {
-    for (int w = lane; w < words; w += 64) {
+    for (int w = lane; w < words; w += NT) {
       uint64_t out = 0;
       const int nb = imin(32, len - 32 * w);
       for (int j = 0; j < nb; ++j) {
@@ -54,7 +57,7 @@ static __device__ void stage_range(const uint8_t* src, int fmt, int start, int l
       dst[w] = out;
     }
   } else {
-    for (int w = lane; w < words; w += 64) {
+    for (int w = lane; w < words; w += NT) {
       uint64_t out = 0;
       const int nb = imin(8, len - 8 * w);
       for (int j = 0; j < nb; ++j) out |= (uint64_t)src[start + 8 * w + j] << (8 * j);
@@ -64,29 +67,41 @@ static __device__ void stage_range(const uint8_t* src, int fmt, int start, int l
 }
 
 // Whether bases [start, start + len) of an ASCII sequence are all A/C/G/T.
-static __device__ bool all_acgt(const uint8_t* src, int start, int len) {
+// With NW > 1 waves the answer is combined through red[0..NW).
+template <int NW>
+static __device__ bool all_acgt(const uint8_t* src, int start, int len, int* red) {
   bool bad = false;
-  for (int i = threadIdx.x; i < len; i += 64) {
+  for (int i = threadIdx.x; i < len; i += 64 * NW) {
     const uint8_t c = src[start + i];
     bad |= !(c == 'A' || c == 'C' || c == 'G' || c == 'T');
   }
-  return ballot(bad) == 0;
+  bool any_bad = ballot(bad) != 0;
+  if (NW > 1) {
+    __syncthreads();  // earlier readers of red are done
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = any_bad;
+    __syncthreads();
+    for (int w = 0; w < NW; ++w) any_bad |= red[w] != 0;
+  }
+  return !any_bad;
 }
 
 // Stage one sub-problem's pattern and text. Returns the symbol width, or 0
 // when they do not fit in `seq_words` words each.
-static __device__ int stage_job(const BiParams& bp, const BiJob& J, uint64_t* sP, uint64_t* sT) {
+template <int NW>
+static __device__ int stage_job(const BiParams& bp, const BiJob& J, uint64_t* sP, uint64_t* sT,
+                                int* red) {
   const KParams& p = bp.k;
   const uint8_t* P = p.seq + p.poff[J.pair];
   const uint8_t* T = p.seq + p.toff[J.pair];
   const int plen = J.pe - J.pb, tlen = J.te - J.tb;
   int bits = 2;
-  if (p.fmt == WFA_SEQ_ASCII && !(all_acgt(P, J.pb, plen) && all_acgt(T, J.tb, tlen))) bits = 8;
+  if (p.fmt == WFA_SEQ_ASCII && !(all_acgt<NW>(P, J.pb, plen, red) && all_acgt<NW>(T, J.tb, tlen, red)))
+    bits = 8;
   const int per = 64 / bits;
   const int pw = (plen + per - 1) / per + 2, tw = (tlen + per - 1) / per + 2;
   if (pw > bp.seq_words || tw > bp.seq_words) return 0;
-  stage_range(P, p.fmt, J.pb, plen, bits, sP, pw);
-  stage_range(T, p.fmt, J.tb, tlen, bits, sT, tw);
+  stage_range<NW>(P, p.fmt, J.pb, plen, bits, sP, pw);
+  stage_range<NW>(T, p.fmt, J.tb, tlen, bits, sT, tw);
   __syncthreads();
   return bits;
 }
@@ -96,9 +111,17 @@ static __device__ int stage_job(const BiParams& bp, const BiJob& J, uint64_t* sP
 // forward or reverse search of a breakpoint, or (HISTORY) the base case,
 // which also keeps every row it computes.
 // ---------------------------------------------------------------------
-template <int KIND, int BITS, bool HEUR, bool HISTORY>
+// With NW > 1 waves (no heuristic, no history) the waves take every row's
+// 64-diagonal chunks round-robin and combine trims and maxima through `red`
+// (LDS): [0, 10 NW) trims, [10 NW, 11 NW) maxima, [11 NW, 14 NW) overlap
+// scans.
+template <int KIND, int BITS, bool HEUR, bool HISTORY, int NW = 1>
 struct WfDir {
   static constexpr int NC = KIND == KIND_EDIT ? 1 : KIND == KIND_AFFINE ? 3 : 5;
+  static constexpr int kNW = NW;
+  static constexpr int NT = 64 * NW;
+  static_assert(NW == 1 || (!HEUR && !HISTORY), "several waves: no heuristic, no history");
+  int* red;
   const KParams* p;
   Rows<int32_t> R;  // ring rows: component c, score s -> row c * scope + s % scope
   int scope;
@@ -142,7 +165,7 @@ struct WfDir {
   // (0, 0, 0).
   __device__ void init(int comp) {
     const int lane = threadIdx.x;
-    for (int r = lane; r < NC * scope; r += 64) R.meta[r] = meta_absent();
+    for (int r = lane; r < NC * scope; r += NT) R.meta[r] = meta_absent();
     __syncthreads();
     if (lane == 0) {
       R.set(row(comp, 0) * R.wcap, 0);
@@ -210,8 +233,8 @@ struct WfDir {
       if (hi - lo + 1 > R.wcap) { overflow = true; return; }
       const int rM = row(C_M, s);
       int m_lo = INT32_MAX, m_hi = INT32_MIN;
-      for (int kb = lo; kb <= hi; kb += 64) {
-        const int k = kb + lane;
+      for (int kb = lo + 64 * (lane >> 6); kb <= hi; kb += NT) {
+        const int k = kb + (lane & 63);
         bool vm = false;
         if (k <= hi) {
           const int ins = rd(R, x.row, x.gl, x.gh, k - 1);
@@ -231,6 +254,12 @@ struct WfDir {
           if (m_lo == INT32_MAX) m_lo = kb + __builtin_ctzll(b);
           m_hi = kb + 63 - __builtin_clzll(b);
         }
+      }
+      if (NW > 1) {
+        if ((lane & 63) == 0) { red[2 * (lane >> 6)] = m_lo; red[2 * (lane >> 6) + 1] = m_hi; }
+        __syncthreads();
+        m_lo = INT32_MAX; m_hi = INT32_MIN;
+        for (int w = 0; w < NW; ++w) { m_lo = imin(m_lo, red[2 * w]); m_hi = imax(m_hi, red[2 * w + 1]); }
       }
       if (m_lo == INT32_MAX) { m_lo = 1; m_hi = 0; }
       __syncthreads();
@@ -285,8 +314,8 @@ struct WfDir {
       const int rI2 = two ? row(C_I2, s) : 0, rD2 = two ? row(C_D2, s) : 0;
       int lim[5][2];
       for (int c = 0; c < 5; ++c) { lim[c][0] = INT32_MAX; lim[c][1] = INT32_MIN; }
-      for (int kb = lo; kb <= hi; kb += 64) {
-        const int k = kb + lane;
+      for (int kb = lo + 64 * (lane >> 6); kb <= hi; kb += NT) {
+        const int k = kb + (lane & 63);
         bool v[5] = {false, false, false, false, false};
         if (k <= hi) {
           const int iv = imax(rd(R, mo.row, mo.gl, mo.gh, k - 1), rd(R, i1.row, i1.gl, i1.gh, k - 1)) + 1;
@@ -322,6 +351,21 @@ struct WfDir {
           }
         }
       }
+      if (NW > 1) {
+        if ((lane & 63) == 0)
+          for (int c = 0; c < NC; ++c) {
+            red[10 * (lane >> 6) + 2 * c] = lim[c][0];
+            red[10 * (lane >> 6) + 2 * c + 1] = lim[c][1];
+          }
+        __syncthreads();
+        for (int c = 0; c < NC; ++c) {
+          lim[c][0] = INT32_MAX; lim[c][1] = INT32_MIN;
+          for (int w = 0; w < NW; ++w) {
+            lim[c][0] = imin(lim[c][0], red[10 * w + 2 * c]);
+            lim[c][1] = imax(lim[c][1], red[10 * w + 2 * c + 1]);
+          }
+        }
+      }
       for (int c = 0; c < NC; ++c)
         if (lim[c][0] == INT32_MAX) { lim[c][0] = 1; lim[c][1] = 0; }
       __syncthreads();
@@ -351,8 +395,8 @@ struct WfDir {
       return false;
     }
     int mx = 0;
-    for (int kb = lo; kb <= hi; kb += 64) {
-      const int k = kb + lane;
+    for (int kb = lo + 64 * (lane >> 6); kb <= hi; kb += NT) {
+      const int k = kb + (lane & 63);
       if (k <= hi) {
         const int idx = rM * R.wcap + (k & R.mask);
         int off = R.at(idx);
@@ -365,7 +409,13 @@ struct WfDir {
         }
       }
     }
-    *max_ak = uniform(wave_max(mx));
+    int amax = uniform(wave_max(mx));
+    if (NW > 1) {
+      if ((lane & 63) == 0) red[10 * NW + (lane >> 6)] = amax;
+      __syncthreads();
+      for (int w = 0; w < NW; ++w) amax = imax(amax, red[10 * NW + w]);
+    }
+    *max_ak = amax;
     __syncthreads();
     store(C_M, s, lo, hi);
     // check_component_termination (src/aligner.rs:3269-3302)
@@ -449,8 +499,12 @@ static __device__ void bi_scan(const D& a, const D& b, int c, int s0, int s1, in
   const int lo1 = k_inverse(bhi, plen, tlen), hi1 = k_inverse(blo, plen, tlen);
   if (hi1 < alo || ahi < lo1) return;
   const int lo = imax(alo, lo1), hi = imin(ahi, hi1);
-  for (int kb = lo; kb <= hi; kb += 64) {
-    const int k0 = kb + threadIdx.x;
+  constexpr int NW = D::kNW;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // every wave finds the first hit of its chunks; the lowest one wins
+  int hk = INT32_MAX, ho0 = 0, ho1 = 0;
+  for (int kb = lo + 64 * wave; kb <= hi; kb += 64 * NW) {
+    const int k0 = kb + lane;
     bool ok = false;
     int o0 = 0, o1 = 0;
     if (k0 <= hi) {
@@ -466,19 +520,30 @@ static __device__ void bi_scan(const D& a, const D& b, int c, int s0, int s1, in
     const uint64_t m = ballot(ok);
     if (m) {
       const int j = __builtin_ctzll(m);
-      const int k0j = kb + j;
-      const int o0j = uniform(__shfl(o0, j)), o1j = uniform(__shfl(o1, j));
-      if (fwd) {
-        bp.score_fwd = s0; bp.score_rev = s1; bp.k_fwd = k0j; bp.off_fwd = o0j;
-      } else {
-        bp.score_fwd = s1; bp.score_rev = s0;
-        bp.k_fwd = k_inverse(k0j, plen, tlen); bp.off_fwd = o1j;
-      }
-      bp.score = s0 + s1 - go;
-      bp.comp = c;
-      return;
+      hk = kb + j;
+      ho0 = uniform(__shfl(o0, j));
+      ho1 = uniform(__shfl(o1, j));
+      break;
     }
   }
+  if (NW > 1) {
+    int* x = a.red + 11 * NW;
+    if (lane == 0) { x[3 * wave] = hk; x[3 * wave + 1] = ho0; x[3 * wave + 2] = ho1; }
+    __syncthreads();
+    hk = INT32_MAX;
+    for (int w = 0; w < NW; ++w)
+      if (x[3 * w] < hk) { hk = x[3 * w]; ho0 = x[3 * w + 1]; ho1 = x[3 * w + 2]; }
+    __syncthreads();  // the next scan reuses x
+  }
+  if (hk == INT32_MAX) return;
+  if (fwd) {
+    bp.score_fwd = s0; bp.score_rev = s1; bp.k_fwd = hk; bp.off_fwd = ho0;
+  } else {
+    bp.score_fwd = s1; bp.score_rev = s0;
+    bp.k_fwd = k_inverse(hk, plen, tlen); bp.off_fwd = ho1;
+  }
+  bp.score = s0 + s1 - go;
+  bp.comp = c;
 }
 
 // bialign_overlap (src/bialign.rs:362-508): a at s0 against b's last
@@ -505,11 +570,11 @@ static __device__ void bi_overlap(const D& a, const D& b, int s0, int s1, bool f
 }
 
 // bialign_find_breakpoint (src/aligner.rs:2958-3120) for one sub-problem.
-template <int KIND, int BITS, bool HEUR>
+template <int KIND, int BITS, bool HEUR, int NW>
 static __device__ BiResult find_breakpoint(const BiParams& bp, const BiJob& J, uint64_t* sP,
                                            uint64_t* sT, int4* metaF, int4* metaR,
-                                           int32_t* ringF, int32_t* ringR) {
-  using Dir = WfDir<KIND, BITS, HEUR, false>;
+                                           int32_t* ringF, int32_t* ringR, int* red) {
+  using Dir = WfDir<KIND, BITS, HEUR, false, NW>;
   const KParams& p = bp.k;
   const int plen = J.pe - J.pb, tlen = J.te - J.tb;
   Dir f, r;
@@ -520,6 +585,7 @@ static __device__ BiResult find_breakpoint(const BiParams& bp, const BiJob& J, u
     d.sP = sP; d.sT = sT; d.plen = plen; d.tlen = tlen; d.rev = rev;
     d.heur = HEUR ? p.heuristic : WFA_HEURISTIC_NONE;
     d.hdr = nullptr; d.cells = nullptr; d.hdr_cap = 0; d.cap = 0;
+    d.red = red;
   };
   setup(f, metaF, ringF, false);
   setup(r, metaR, ringR, true);
@@ -614,8 +680,8 @@ static __device__ BiResult find_breakpoint(const BiParams& bp, const BiJob& J, u
   return res;
 }
 
-template <int KIND, bool HEUR>
-__global__ void __launch_bounds__(64) biwfa_breakpoint_kernel(BiParams bp) {
+template <int KIND, bool HEUR, int NW>
+__global__ void __launch_bounds__(64 * NW) biwfa_breakpoint_kernel(BiParams bp) {
   extern __shared__ uint64_t smem[];
   const int lane = threadIdx.x;
   constexpr int NC = KIND == KIND_EDIT ? 1 : KIND == KIND_AFFINE ? 3 : 5;
@@ -626,22 +692,31 @@ __global__ void __launch_bounds__(64) biwfa_breakpoint_kernel(BiParams bp) {
   const int64_t dir_ints = (int64_t)NC * bp.k.scope * bp.wcap;
   int32_t* ringF = bp.ring + (int64_t)blockIdx.x * 2 * dir_ints;
   int32_t* ringR = ringF + dir_ints;
+  // cross-wave reductions: 16 ints per wave after the metadata
+  int* red = (int*)(metaR + NC * bp.k.scope);
   for (;;) {
     int item = 0;
-    if (lane == 0) item = atomicAdd(&bp.counters[0], 1);
-    item = uniform(__shfl(item, 0));
+    if (NW == 1) {
+      if (lane == 0) item = atomicAdd(&bp.counters[0], 1);
+      item = uniform(__shfl(item, 0));
+    } else {
+      if (lane == 0) red[15 * NW] = atomicAdd(&bp.counters[0], 1);
+      __syncthreads();
+      item = uniform(red[15 * NW]);
+      __syncthreads();
+    }
     if (item >= bp.njobs) break;
     const BiJob J = bp.jobs[item];
-    const int bits = stage_job(bp, J, sP, sT);
+    const int bits = stage_job<NW>(bp, J, sP, sT, red);
     BiResult res;
     if (bits == 0) {
       res.status = BI_RETRY;
       res.score = res.score_fwd = res.score_rev = res.k_fwd = res.off_fwd = res.end_score = 0;
       res.comp = 0;
     } else if (bits == 2) {
-      res = find_breakpoint<KIND, 2, HEUR>(bp, J, sP, sT, metaF, metaR, ringF, ringR);
+      res = find_breakpoint<KIND, 2, HEUR, NW>(bp, J, sP, sT, metaF, metaR, ringF, ringR, red);
     } else {
-      res = find_breakpoint<KIND, 8, HEUR>(bp, J, sP, sT, metaF, metaR, ringF, ringR);
+      res = find_breakpoint<KIND, 8, HEUR, NW>(bp, J, sP, sT, metaF, metaR, ringF, ringR, red);
     }
     if (lane == 0) bp.results[item] = res;
     __syncthreads();
@@ -873,7 +948,7 @@ __global__ void __launch_bounds__(64) biwfa_base_kernel(BiParams bp) {
     item = uniform(__shfl(item, 0));
     if (item >= bp.njobs) break;
     const BiJob J = bp.jobs[item];
-    const int bits = stage_job(bp, J, sP, sT);
+    const int bits = stage_job<1>(bp, J, sP, sT, nullptr);
     if (bits == 0) {
       if (lane == 0) {
         bp.frag_len[item] = 0;
@@ -891,7 +966,8 @@ __global__ void __launch_bounds__(64) biwfa_base_kernel(BiParams bp) {
 
 template <int K>
 static KernelFn2 pick_biwfa_kind(int which, bool heur) {
-  if (which == 0) return heur ? biwfa_breakpoint_kernel<K, true> : biwfa_breakpoint_kernel<K, false>;
+  if (which == 0)
+    return heur ? biwfa_breakpoint_kernel<K, true, 1> : biwfa_breakpoint_kernel<K, false, kWideWaves>;
   return heur ? biwfa_base_kernel<K, true> : biwfa_base_kernel<K, false>;
 }
 

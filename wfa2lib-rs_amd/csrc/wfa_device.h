@@ -157,6 +157,11 @@ hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, in
 // in HBM, without heuristics
 constexpr int kWideWaves = 8;
 
+// Waves per workgroup of a BiWFA launch (`which` 0 = breakpoint search,
+// 1 = base cases): breakpoint searches without a heuristic take kWideWaves
+// (heuristic cut-offs are single-wave); base cases one.
+inline int biwfa_waves(int which, bool heur) { return which == 0 && !heur ? kWideWaves : 1; }
+
 // Status WFA_STATUS_OOM (score 0, no CIGAR) for the `n` pairs of `list`
 hipError_t launch_mark_oom(const int32_t* list, int n, int32_t* score, int32_t* status,
                            int32_t* cig_len, hipStream_t stream);

@@ -150,14 +150,15 @@ hipError_t launch_biwfa(int kind, int which, bool heur, const BiParams& bp, int 
                         size_t lds_bytes, hipStream_t stream) {
   KernelFn2 fn = pick_biwfa(kind, which, heur);
   if (!fn) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(fn, dim3(grid), dim3(64), lds_bytes, stream, bp);
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(64 * biwfa_waves(which, heur)), lds_bytes, stream, bp);
   return hipGetLastError();
 }
 
 int biwfa_blocks_per_cu(int kind, int which, bool heur, size_t lds_bytes) {
   KernelFn2 fn = pick_biwfa(kind, which, heur);
   int n = 0;
-  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, lds_bytes) != hipSuccess)
+  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64 * biwfa_waves(which, heur),
+                                                          lds_bytes) != hipSuccess)
     return 1;
   return n > 0 ? n : 1;
 }
