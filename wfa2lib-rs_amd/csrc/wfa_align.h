@@ -156,6 +156,26 @@ static __device__ __forceinline__ int rd(const Rows<RT>& R, int row, int lo, int
   return (k >= lo && k <= hi) ? R.at(row * R.wcap + (k & R.mask)) : WFA_NULL;
 }
 
+// A row's readable diagonals as {first, count - 1}: k is readable iff
+// (unsigned)(k - first) <= span. An empty range becomes a first diagonal
+// no pair reaches, with span 0.
+struct Guard {
+  int first, span;
+};
+
+static __device__ __forceinline__ Guard make_guard(int lo, int hi) {
+  return lo <= hi ? Guard{lo, hi - lo} : Guard{INT32_MIN / 2, 0};
+}
+
+// Branch-free guarded read: the ring index is always inside the ring, so
+// the read happens for every lane and a select applies the range (the
+// reference's NULL halo, src/wavefront.rs:422-457).
+template <typename RT>
+static __device__ __forceinline__ int rdg(const Rows<RT>& R, int rowbase, Guard g, int k) {
+  const int v = R.at(rowbase + (k & R.mask));
+  return (uint32_t)(k - g.first) <= (uint32_t)g.span ? v : WFA_NULL;
+}
+
 // ---------------------------------------------------------------------
 // Per-pair result codes internal to the kernel
 // ---------------------------------------------------------------------
@@ -820,7 +840,7 @@ static __device__ bool heuristic_limits(const KParams& p, const HeurCfg& hc, int
 // row's 64-diagonal chunks round-robin and combine their per-chunk results
 // (trims, termination, the first ends-free hit) through `red` in LDS; one
 // thread writes metadata and results, and wave 0 alone runs the backtrace.
-template <int KIND, int MODE, int BITS, typename RT, bool HEUR, int U, int NW>
+template <int KIND, int MODE, int BITS, typename RT, bool HEUR, int U, int NW, bool BFREE>
 static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen, uint64_t* sP,
                                   uint64_t* sT, const Rows<RT>& R, const Hist& H, int* red,
                                   int& owned) {
@@ -909,8 +929,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
 
   // extend + terminate + (CIGAR) record M[score] over its rows; `lo/hi`
   // is the computed (pre-trim) range; returns true if the end is reached.
-  auto extend_row = [&](int s, int lo, int hi, int4 hrow) -> bool {
-    const int row = s % ringM;
+  auto extend_row = [&](int s, int row, int lo, int hi, int4 hrow) -> bool {
     bool reached = false;
     uint64_t hit = 0;  // ends-free: first chunk with a terminating diagonal
     int hit_kb = 0;
@@ -1007,8 +1026,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   const int heur = HEUR ? hc.kind : WFA_HEURISTIC_NONE;
   HeurState hs;
   hs.reset(hc);
-  auto cutoff = [&](int s) {
-    const int rowM = s % ringM;
+  auto cutoff = [&](int s, int rowM, int rowE, int rowE2) {
     const int4 mm = R.meta[rowM];
     const int lo = uniform(mm.x), hi = uniform(mm.y);
     int nlo = lo, nhi = hi;
@@ -1022,7 +1040,6 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       // an emptied M is the null wavefront with an empty initialized range
       R.meta[rowM] = empty ? make_int4(nlo, nhi, 1, 0) : make_int4(nlo, nhi, mm.z, mm.w);
       if (KIND != KIND_EDIT) {
-        const int rowE = s % ringE1;
         auto equate = [&](int row) {
           const int4 g = R.meta[row];
           if (g.x > g.y) return;
@@ -1032,7 +1049,6 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         equate(rI1 + rowE);
         equate(rD1 + rowE);
         if (two) {
-          const int rowE2 = s % ringE2;
           equate(rI2 + rowE2);
           equate(rD2 + rowE2);
         }
@@ -1058,17 +1074,24 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     if (CIG && !overflow && (H.hdr_cap < 1 || !halloc(hi0 - lo0 + 1, &h0, lo0, hi0)))
       overflow = true;
     else if (MED && tid == 0) H.hdr[0] = h0;
-    if (!overflow && extend_row(0, lo0, hi0, h0)) {
+    if (!overflow && extend_row(0, 0, lo0, hi0, h0)) {
       status = WFA_STATUS_END_REACHED;
       result = 0;
     }
     __syncthreads();
-    if (!overflow && status == WFA_STATUS_OK && heur != WFA_HEURISTIC_NONE) cutoff(0);
+    if (!overflow && status == WFA_STATUS_OK && heur != WFA_HEURISTIC_NONE) cutoff(0, 0, 0, 0);
   }
   __syncthreads();
 
+  // s mod ringM / ringE1 / ringE2, kept as the score advances
+  int sM = 0, sE1 = 0, sE2 = 0;
+  // (s - d) mod ring for 0 <= d < ring
+  auto back = [](int cur, int d, int ring) { return cur >= d ? cur - d : cur - d + ring; };
   while (status == WFA_STATUS_OK && !overflow) {
     score += 1;
+    sM = sM + 1 == ringM ? 0 : sM + 1;
+    if (KIND != KIND_EDIT) sE1 = sE1 + 1 == ringE1 ? 0 : sE1 + 1;
+    if (two) sE2 = sE2 + 1 == ringE2 ? 0 : sE2 + 1;
     if (!p.modular && score >= nwf) { status = WFA_STATUS_MAX_STEPS_REACHED; result = score; break; }
     if (CIG && score >= H.hdr_cap) { overflow = true; break; }
     // int16 rows hold offsets up to tlen + score: leave long runs to int32
@@ -1089,7 +1112,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       gl = uniform(m.z); gh = uniform(m.w);
     };
     if (KIND == KIND_EDIT) {
-      row_x = (s - 1) % ringM;
+      row_x = back(sM, 1, ringM);
       load(row_x, ml_x, mh_x, gl_x, gh_x);
       // the edit step nulls prev's cells next to [lo, hi] before reading
       // them (src/aligner.rs:1795-1808): reads outside [lo, hi] are NULL
@@ -1101,25 +1124,25 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       }
     } else {
       if (s - p.x >= 0) {
-        row_x = (s - p.x) % ringM;
+        row_x = back(sM, p.x, ringM);
         load(row_x, ml_x, mh_x, gl_x, gh_x);
       }
       if (s - p.oe1 >= 0) {
-        row_o = (s - p.oe1) % ringM;
+        row_o = back(sM, p.oe1, ringM);
         load(row_o, ml_o, mh_o, gl_o, gh_o);
       }
       if (s - p.e1 >= 0) {
-        row_e = (s - p.e1) % ringE1;
+        row_e = back(sE1, p.e1, ringE1);
         load(rI1 + row_e, il, ih, gil, gih);
         load(rD1 + row_e, dl, dh, gdl, gdh);
       }
       if (two) {
         if (s - p.oe2 >= 0) {
-          row_o2 = (s - p.oe2) % ringM;
+          row_o2 = back(sM, p.oe2, ringM);
           load(row_o2, ml_o2, mh_o2, gl_o2, gh_o2);
         }
         if (s - p.e2 >= 0) {
-          row_e2 = (s - p.e2) % ringE2;
+          row_e2 = back(sE2, p.e2, ringE2);
           load(rI2 + row_e2, il2, ih2, gil2, gih2);
           load(rD2 + row_e2, dl2, dh2, gdl2, gdh2);
         }
@@ -1158,9 +1181,9 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         hi = imin(hi, hc.p1);
       }
     }
-    const int rowM = s % ringM;
-    const int rowE = s % ringE1;
-    const int rowE2 = two ? s % ringE2 : 0;
+    const int rowM = sM;
+    const int rowE = sE1;
+    const int rowE2 = two ? sE2 : 0;
     __syncthreads();  // every lane has read the metadata before it is rewritten
     if (!any) {
       // null step (src/aligner.rs:2031-2043, 1778-1782)
@@ -1219,23 +1242,55 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     struct In {
       int a, b, c, d, e, f, g, h, i;
     };
+    // input rows: base index and readable range
+    const int bx = row_x * R.wcap, bo = row_o * R.wcap, bi1 = (rI1 + row_e) * R.wcap,
+              bd1 = (rD1 + row_e) * R.wcap, bo2 = row_o2 * R.wcap,
+              bi2 = (rI2 + row_e2) * R.wcap, bd2 = (rD2 + row_e2) * R.wcap;
+    const Guard g_x = make_guard(gl_x, gh_x), g_o = make_guard(gl_o, gh_o),
+                g_i1 = make_guard(gil, gih), g_d1 = make_guard(gdl, gdh),
+                g_o2 = make_guard(gl_o2, gh_o2), g_i2 = make_guard(gil2, gih2),
+                g_d2 = make_guard(gdl2, gdh2);
+    // BFREE: branch-free reads (every lane reads, a select applies the
+    // range); otherwise reads outside a row's range are skipped, which the
+    // wide CIGAR kernels measure faster (their register pressure would
+    // serialize nine unconditional HBM loads)
     auto gather = [&](int k, In& in) {
-      if (k > hi) return;
+      if constexpr (!BFREE) {
+        if (k > hi) return;
+        if (KIND == KIND_EDIT) {
+          in.a = rd(R, row_x, gl_x, gh_x, k - 1);
+          in.b = rd(R, row_x, gl_x, gh_x, k + 1);
+          if (p.metric != WFA_METRIC_INDEL) in.c = rd(R, row_x, gl_x, gh_x, k);
+        } else {
+          in.a = rd(R, row_o, gl_o, gh_o, k - 1);
+          in.b = rd(R, rI1 + row_e, gil, gih, k - 1);
+          in.c = rd(R, row_o, gl_o, gh_o, k + 1);
+          in.d = rd(R, rD1 + row_e, gdl, gdh, k + 1);
+          in.e = rd(R, row_x, gl_x, gh_x, k);
+          if (two) {
+            in.f = rd(R, row_o2, gl_o2, gh_o2, k - 1);
+            in.g = rd(R, rI2 + row_e2, gil2, gih2, k - 1);
+            in.h = rd(R, row_o2, gl_o2, gh_o2, k + 1);
+            in.i = rd(R, rD2 + row_e2, gdl2, gdh2, k + 1);
+          }
+        }
+        return;
+      }
       if (KIND == KIND_EDIT) {
-        in.a = rd(R, row_x, gl_x, gh_x, k - 1);
-        in.b = rd(R, row_x, gl_x, gh_x, k + 1);
-        if (p.metric != WFA_METRIC_INDEL) in.c = rd(R, row_x, gl_x, gh_x, k);
+        in.a = rdg(R, bx, g_x, k - 1);
+        in.b = rdg(R, bx, g_x, k + 1);
+        if (p.metric != WFA_METRIC_INDEL) in.c = rdg(R, bx, g_x, k);
       } else {
-        in.a = rd(R, row_o, gl_o, gh_o, k - 1);
-        in.b = rd(R, rI1 + row_e, gil, gih, k - 1);
-        in.c = rd(R, row_o, gl_o, gh_o, k + 1);
-        in.d = rd(R, rD1 + row_e, gdl, gdh, k + 1);
-        in.e = rd(R, row_x, gl_x, gh_x, k);
+        in.a = rdg(R, bo, g_o, k - 1);
+        in.b = rdg(R, bi1, g_i1, k - 1);
+        in.c = rdg(R, bo, g_o, k + 1);
+        in.d = rdg(R, bd1, g_d1, k + 1);
+        in.e = rdg(R, bx, g_x, k);
         if (two) {
-          in.f = rd(R, row_o2, gl_o2, gh_o2, k - 1);
-          in.g = rd(R, rI2 + row_e2, gil2, gih2, k - 1);
-          in.h = rd(R, row_o2, gl_o2, gh_o2, k + 1);
-          in.i = rd(R, rD2 + row_e2, gdl2, gdh2, k + 1);
+          in.f = rdg(R, bo2, g_o2, k - 1);
+          in.g = rdg(R, bi2, g_i2, k - 1);
+          in.h = rdg(R, bo2, g_o2, k + 1);
+          in.i = rdg(R, bd2, g_d2, k + 1);
         }
       }
     };
@@ -1396,10 +1451,10 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     const int elo = MODE == 1 ? lo : m_lo, ehi = MODE == 1 ? hi : m_hi;
     if (MODE == 1 && !halloc(hi - lo + 1, &hrow, elo, ehi)) { overflow = true; break; }
     const bool reached =
-        extend_row(s, elo, ehi, hrow) && (ef || (m_lo <= k_end && k_end <= m_hi));
+        extend_row(s, rowM, elo, ehi, hrow) && (ef || (m_lo <= k_end && k_end <= m_hi));
     __syncthreads();
     if (reached) { status = WFA_STATUS_END_REACHED; result = s; break; }
-    if (heur != WFA_HEURISTIC_NONE) cutoff(s);
+    if (heur != WFA_HEURISTIC_NONE) cutoff(s, rowM, rowE, rowE2);
   }
 
   if (overflow) {
@@ -1526,10 +1581,13 @@ __global__ void __launch_bounds__(64 * NW, LDS_RING ? 5 : 4) wfa_align_kernel(KP
     // 2-bit packed input never needs the byte-wise path
     // HBM-resident rings are wide: keep several chunks of loads in flight
     constexpr int U = LDS_RING || NW > 1 ? 1 : MODE == 0 ? 4 : 2;
+    constexpr bool BFREE = LDS_RING || MODE == 0;
     if (PACKED || bits == 2)
-      align_pair<KIND, MODE, 2, RT, HEUR, U, NW>(p, pair, plen, tlen, sP, sT, R, H, red, owned);
+      align_pair<KIND, MODE, 2, RT, HEUR, U, NW, BFREE>(p, pair, plen, tlen, sP, sT, R, H, red,
+                                                        owned);
     else
-      align_pair<KIND, MODE, 8, RT, HEUR, U, NW>(p, pair, plen, tlen, sP, sT, R, H, red, owned);
+      align_pair<KIND, MODE, 8, RT, HEUR, U, NW, BFREE>(p, pair, plen, tlen, sP, sT, R, H, red,
+                                                        owned);
     __syncthreads();
   }
 }
