@@ -272,6 +272,13 @@ int32_t wfa_engine_align_batch(wfa_engine_t* engine, const wfa_attr_t* attr,
 int32_t wfa_engine_align_batch_per_pair(wfa_engine_t* engine, const wfa_attr_t* attr,
                                         const wfa_batch_t* batch, void* hip_stream);
 
+/* Cap the device scratch (wavefront rings, CIGAR history) one launch of
+ * this engine may use, in bytes (default 48 GiB; at least 64 MiB). Pairs
+ * whose CIGAR history does not fit the cap get WFA_STATUS_OOM. The
+ * reference has no counterpart (its memory is the host's); this is the
+ * engine's own limit. */
+int32_t wfa_engine_set_scratch_budget(wfa_engine_t* engine, int64_t bytes);
+
 /* Launch statistics of the last batch call: number of pairs that needed a
  * wider wavefront or longer history than the first pass provides and were
  * re-run by a second, wider pass. */

@@ -314,3 +314,38 @@ def test_per_pair_configs_match_oracle(pkg, N, engine, oracle):
         for i in range(n):
             os_, ost, oc = oracle.align_batch(attrs[i], pairs.subset([i]), cigar=True)
             assert (s[i], st[i], c[i]) == (os_[0], ost[0], oc[0]), (w.name, i, list(cfg[i]))
+
+
+def test_scratch_budget_gives_oom_status(pkg, N, oracle):
+    """Under a small scratch cap, pairs whose CIGAR history does not fit get
+    WFA_STATUS_OOM (score 0, no CIGAR) instead of failing the batch; the
+    pairs that fit still match the oracle, and with the default cap every
+    pair completes."""
+    from wfa2lib_rs_amd.engine import BatchEngine, DeviceBatch
+    from wfa2lib_rs_amd.synth import Workload, generate
+    import torch
+    pairs = generate(Workload("oom", 24, 1, 300, 6000, 0.08, 0.12, 2.0, seed=2))
+    attr = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment")
+    os_, ost, oc = oracle.align_batch(attr, pairs, cigar=True, threads=8)
+    eng = BatchEngine(0)
+    eng.set_scratch_budget(64 << 20)
+    b = DeviceBatch(pairs, device="cuda:0", cigar=True)
+    eng.align(attr, b)
+    torch.cuda.synchronize()
+    s, st, c = b.results()
+    oom = st == N.STATUS_OOM
+    assert oom.any() and not oom.all(), st
+    for i in range(len(pairs)):
+        if oom[i]:
+            assert s[i] == 0 and c[i] == b""
+        else:
+            assert (s[i], st[i], c[i]) == (os_[i], ost[i], oc[i]), i
+    with pytest.raises(RuntimeError):
+        eng.set_scratch_budget(1 << 20)
+    eng.set_scratch_budget(48 << 30)
+    b2 = DeviceBatch(pairs, device="cuda:0", cigar=True)
+    eng.align(attr, b2)
+    torch.cuda.synchronize()
+    s2, st2, c2 = b2.results()
+    assert (st2 == N.STATUS_COMPLETED).all()
+    assert all((s2[i], st2[i], c2[i]) == (os_[i], ost[i], oc[i]) for i in range(len(pairs)))

@@ -110,6 +110,7 @@ SIGNATURES = {
     "wfa_engine_free": (None, [_VP]),
     "wfa_engine_align_batch": (_I32, [_VP, _P(Attr), _P(Batch), _VP]),
     "wfa_engine_align_batch_per_pair": (_I32, [_VP, _P(Attr), _P(Batch), _VP]),
+    "wfa_engine_set_scratch_budget": (_I32, [_VP, ctypes.c_int64]),
     "wfa_engine_last_retries": (_I32, [_VP]),
     "wfa_engine_last_kernel_ms": (_I32, [_VP, _P(ctypes.c_float), _P(ctypes.c_float)]),
     "wfa_server_config_default": (None, [_P(ServerConfig)]),

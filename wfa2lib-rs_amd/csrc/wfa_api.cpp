@@ -158,14 +158,13 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
     c->grid = (int)std::max<int64_t>(1, budget / per_slot);
   if (cig) {
     int64_t elems = (int64_t)c->grid * c->hist_cap;
-    if (pass > 0 || (long_reads && e->long_pool_whole_budget)) {
-      // long reads, or the pairs that outgrew the first pass: the pool
-      // takes what the budget leaves after the rings, headers and chunk
-      // tables
-      const int64_t cell = mode == 2 ? 1 : 4;
-      const int64_t rest = budget - (int64_t)c->grid * (per_slot - c->hist_cap * cell);
-      elems = std::max(elems, rest / cell);
-    }
+    // what the budget leaves after the rings, headers and chunk tables
+    const int64_t cell = mode == 2 ? 1 : 4;
+    const int64_t rest = budget - (int64_t)c->grid * (per_slot - c->hist_cap * cell);
+    // long reads, or the pairs that outgrew the first pass: the pool takes
+    // all of it; never more than it (at least one chunk)
+    if (pass > 0 || (long_reads && e->long_pool_whole_budget)) elems = std::max(elems, rest / cell);
+    elems = std::min(elems, std::max<int64_t>(rest / cell, c->chunk));
     elems = std::min<int64_t>(elems, (int64_t)c->chunk * INT32_MAX);
     c->pool_chunks = (int)std::max<int64_t>(c->grid, (elems + c->chunk - 1) / c->chunk);
   }
@@ -410,6 +409,13 @@ int32_t wfa_engine_align_batch_per_pair(wfa_engine_t* e, const wfa_attr_t* attr,
     r = align_batch_impl(e, attr, &one, (hipStream_t)stream, i, 1);
     if (r) return r;
   }
+  return WFA_OK;
+}
+
+int32_t wfa_engine_set_scratch_budget(wfa_engine_t* e, int64_t bytes) {
+  if (!e || bytes < ((int64_t)64 << 20)) return WFA_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> g(e->mu);
+  e->scratch_budget = bytes;
   return WFA_OK;
 }
 

@@ -125,12 +125,14 @@ int run_jobs(wfa_engine_t* e, const KParams& kp, int kind, bool heur, int which,
     const size_t ring_slot = (size_t)dirs * NC * kp.scope * bp.wcap * 4;
     int grid = e->num_cus * std::max(1, wfa::biwfa_blocks_per_cu(kind, which, heur, lds));
     grid = std::min(grid, m);
-    grid = (int)std::max<size_t>(1, std::min<size_t>(grid, kRingBudget / ring_slot));
+    const size_t ring_budget = std::min<size_t>(kRingBudget, (size_t)e->scratch_budget / 2);
+    const size_t hist_budget = std::min<size_t>(kHistBudget, (size_t)e->scratch_budget / 4);
+    grid = (int)std::max<size_t>(1, std::min<size_t>(grid, ring_budget / ring_slot));
     if (which == 1) {
       bp.hdr_cap = pass == 0 ? 1024 : 1024 << (2 * pass);
       bp.hist_cap = pass == 0 ? (int64_t(1) << 20) : (int64_t(1) << (20 + 2 * pass));
       const size_t per_slot = (size_t)bp.hdr_cap * NC * 16 + (size_t)bp.hist_cap * 4;
-      grid = (int)std::max<size_t>(1, std::min<size_t>(grid, kHistBudget / per_slot));
+      grid = (int)std::max<size_t>(1, std::min<size_t>(grid, hist_budget / per_slot));
     }
     BiJob* d_jobs;
     int r = dev(e, B_JOBS, m, &d_jobs);

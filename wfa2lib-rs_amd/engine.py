@@ -119,6 +119,11 @@ class BatchEngine:
         N.check(fn(self._h, ctypes.byref(attr), ctypes.byref(b), ctypes.c_void_p(stream)),
                 "wfa_engine_align_batch")
 
+    def set_scratch_budget(self, nbytes):
+        """Cap one launch's device scratch (pairs that do not fit: OOM)."""
+        N.check(N.lib().wfa_engine_set_scratch_budget(self._h, int(nbytes)),
+                "wfa_engine_set_scratch_budget")
+
     def last_retries(self):
         return N.lib().wfa_engine_last_retries(self._h)
 
