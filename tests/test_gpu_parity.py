@@ -324,7 +324,7 @@ def test_scratch_budget_gives_oom_status(pkg, N, oracle):
     from wfa2lib_rs_amd.engine import BatchEngine, DeviceBatch
     from wfa2lib_rs_amd.synth import Workload, generate
     import torch
-    pairs = generate(Workload("oom", 24, 1, 300, 6000, 0.08, 0.12, 2.0, seed=2))
+    pairs = generate(Workload("oom", 24, 1, 300, 10000, 0.15, 0.20, 2.0, seed=2))
     attr = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment")
     os_, ost, oc = oracle.align_batch(attr, pairs, cigar=True, threads=8)
     eng = BatchEngine(0)
