@@ -349,8 +349,7 @@ wfo_aligner* wfo_aligner_new(const wfa_attr_t* attr, int32_t* err) {
     if ((attr->span == WFA_SPAN_ENDSFREE && a->pen.match != 0) ||
         attr->heuristic == WFA_HEURISTIC_ZDROP ||
         (attr->memory_mode == WFA_MEMORY_ULTRALOW &&
-         (attr->span != WFA_SPAN_END2END || attr->heuristic == WFA_HEURISTIC_XDROP)) ||
-        attr->penalties.distance_metric == WFA_METRIC_GAP_LINEAR) {
+         (attr->span != WFA_SPAN_END2END || attr->heuristic == WFA_HEURISTIC_XDROP))) {
         if (err) *err = WFA_ERR_UNSUPPORTED;
         free(a);
         return NULL;
@@ -1035,10 +1034,82 @@ static void compute_edit_step(wfo_aligner* a, int score) {
     trim_ends(plen, tlen, cur);
 }
 
+/* compute_linear_step (src/aligner.rs:1866-2002) with compute_linear_idm
+ * (src/compute/linear.rs:20-131) and, for med/low memory, the piggyback
+ * kernel (src/compute/piggyback.rs:279-302, the edit cell update over the
+ * two inputs) and offload_linear (src/compute/piggyback.rs:556-613).
+ * M[s,k] = max(M[s-x,k] + 1, M[s-o,k-1] + 1, M[s-o,k+1]). Unlike the edit
+ * step, the inputs' ends are NULL-filled only beyond their initialized
+ * ranges (init_ends), so cells a heuristic cut keep their offsets. Ends-free
+ * with a match reward (begin-free cells added during compute) is refused at
+ * construction, as for every metric. */
+static void compute_linear_step(wfo_aligner* a, int score) {
+    const int plen = a->plen, tlen = a->tlen;
+    const pen_t* p = &a->pen;
+    wf_t* mx = score - p->mismatch >= 0 ? get(a, C_M, score - p->mismatch) : NULL;
+    wf_t* mg = score - p->o1 >= 0 ? get(a, C_M, score - p->o1) : NULL;
+    if (is_null_wf(mx) && is_null_wf(mg)) {
+        a->num_null_steps++;
+        put(a, C_M, score, NULL);
+        return;
+    }
+    a->num_null_steps = 0;
+    /* trimmed-empty inputs read as the null wavefront; its bounds (1, -1)
+     * still enter the union */
+    mx = live_or_none(mx);
+    mg = live_or_none(mg);
+    int lo = nlo(mx), hi = nhi(mx);
+    if (nlo(mg) - 1 < lo) lo = nlo(mg) - 1;
+    if (nhi(mg) + 1 > hi) hi = nhi(mg) + 1;
+    /* clamp_to_band (src/aligner.rs:3308-3316) */
+    if (a->attr.heuristic == WFA_HEURISTIC_BANDED_STATIC ||
+        a->attr.heuristic == WFA_HEURISTIC_BANDED_ADAPTIVE) {
+        if (lo < a->attr.heuristic_p[0]) lo = a->attr.heuristic_p[0];
+        if (hi > a->attr.heuristic_p[1]) hi = a->attr.heuristic_p[1];
+    }
+    if (lo < a->hist_lo + 1) lo = a->hist_lo + 1;
+    if (hi > a->hist_hi - 1) hi = a->hist_hi - 1;
+    if (lo > hi) { put(a, C_M, score, NULL); return; }
+    int alo, ahi;
+    output_alloc_range(a, lo, hi, &alo, &ahi);
+    if (mx) { wf_init_ends_higher(mx, hi); wf_init_ends_lower(mx, lo); }
+    if (mg) { wf_init_ends_higher(mg, hi + 1); wf_init_ends_lower(mg, lo - 1); }
+    wf_t* cur = alloc_wf(a, alo, ahi);
+    wf_set_limits(cur, lo, hi);
+    for (int k = lo; k <= hi; ++k) {
+        const int32_t mis = rd(mx, k) + 1, ins = rd(mg, k - 1) + 1, del = rd(mg, k + 1);
+        int32_t m = ins > mis ? ins : mis;
+        if (del > m) m = del;
+        if (a->piggyback) {
+            /* later checks win: ins, then del, then misms */
+            uint32_t pc = (rd_pc(mg, k - 1) << 2) | PC_INS, pv = rd_pv(mg, k - 1);
+            if (m == del) { pc = (rd_pc(mg, k + 1) << 2) | PC_DEL; pv = rd_pv(mg, k + 1); }
+            if (m == mis) { pc = (rd_pc(mx, k) << 2) | PC_MISMS; pv = rd_pv(mx, k); }
+            cur->pc[k - cur->base] = pc;
+            cur->pv[k - cur->base] = pv;
+        }
+        if ((uint32_t)m > (uint32_t)tlen || (uint32_t)(m - k) > (uint32_t)plen) m = OFFSET_NULL;
+        wf_set(cur, k, m);
+    }
+    if (a->piggyback) {
+        /* offload_linear: every cell holds at most one op more than its
+         * inputs' fullest; full chains move to the block store */
+        int o = occ(mg) > occ(mx) ? occ(mg) : occ(mx);
+        cur->occ = o + 1;
+        if (cur->occ >= 16) {
+            offload_selective(a, cur, lo, hi, a->low ? PC_FULL_MASK : PC_HALF_FULL_MASK);
+            cur->occ = a->low ? 16 - 1 : 16 / 2;
+        }
+    }
+    put(a, C_M, score, cur);
+    trim_ends(plen, tlen, cur);
+}
+
 static void compute_step(wfo_aligner* a, int score) {
     switch (a->pen.metric) {
     case WFA_METRIC_EDIT:
     case WFA_METRIC_INDEL: compute_edit_step(a, score); break;
+    case WFA_METRIC_GAP_LINEAR: compute_linear_step(a, score); break;
     default: compute_affine_step(a, score); break;
     }
 }
@@ -1945,6 +2016,9 @@ static int cigar_wfa_score_all(const wfo_aligner* a, int plen, int tlen) {
     switch (a->pen.metric) {
     case WFA_METRIC_INDEL:
     case WFA_METRIC_EDIT: return wfo_cigar_score_edit(ops, n);
+    case WFA_METRIC_GAP_LINEAR:
+        classic = wfo_cigar_score_gap_linear(ops, n, u->match, u->mismatch, u->gap_opening1);
+        break;
     case WFA_METRIC_GAP_AFFINE:
         classic = wfo_cigar_score_gap_affine(ops, n, u->match, u->mismatch, u->gap_opening1,
                                              u->gap_extension1);

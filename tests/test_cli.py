@@ -19,6 +19,8 @@ def _args(vec):
         a = ["-a", "gap-affine-wfa", "-g", f"{m},{x},{o1},{e1}"]
     elif vec["metric"] == "gap_affine_2p":
         a = ["-a", "gap-affine2p-wfa", "--affine2p-penalties", f"{m},{x},{o1},{e1},{o2},{e2}"]
+    elif vec["metric"] == "gap_linear":
+        a = ["-a", "gap-linear-wfa", "-p", f"{m},{x},{o1}"]
     else:
         a = ["-a", f"{vec['metric']}-wfa"]
     if vec["scope"] == "score":
@@ -71,8 +73,9 @@ def test_cli_output_lines(args, vecs, pkg, tmp_path):
      "Invalid --wfa-heuristic-parameters"),
     (["--wfa-span", "ends-free,1,2"], "Invalid --wfa-span"),
     (["--wfa-memory", "huge"], "--wfa-memory"),
-    (["-a", "gap-linear-wfa"], "not available on the device path"),
+    (["--no-c-compat"], "not available on the device path"),
     (["-g", "1,4,6,2"], "invalid penalties"),
+    (["-a", "gap-linear-wfa", "-p", "0,4,0"], "invalid penalties"),
 ])
 def test_cli_rejections(args, msg, pkg, tmp_path, capsys):
     """The reference driver's rejections (src/bin/align_benchmark.rs:

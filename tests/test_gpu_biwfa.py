@@ -58,7 +58,7 @@ def test_biwfa_packed_input(pkg, N, engine, oracle):
                          ids=["wfadaptive_10_50_1", "wfadaptive_4_6_2", "banded_m40_5",
                               "banded_m3_3", "badaptive_m50_50_1", "badaptive_m5_5_1"])
 @pytest.mark.parametrize("metric,pen", [BIWFA_PENALTIES[0], BIWFA_PENALTIES[2],
-                                        BIWFA_PENALTIES[4]])
+                                        BIWFA_PENALTIES[4], BIWFA_PENALTIES[6]])
 def test_biwfa_heuristics_match_oracle(h, metric, pen, pkg, N, engine, oracle):
     pairs = _pairs(12, seed=13)
     for scope in ("score", "alignment"):

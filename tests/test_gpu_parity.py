@@ -67,6 +67,8 @@ CONFIGS = [
     ("gap_affine_2p", (0, 4, 6, 2, 24, 1)),
     ("edit", (0, 0, 0, 0, 0, 0)),
     ("indel", (0, 0, 0, 0, 0, 0)),
+    ("gap_linear", (0, 4, 2, 0, 0, 0)),
+    ("gap_linear", (-1, 3, 2, 0, 0, 0)),
 ]
 
 
@@ -135,6 +137,11 @@ def test_aligner_api_mirrors_reference(pkg, N, engine):
     p, t = b"ACGTACGTAC", b"ACGTACGTACGGAAA"
     assert A.align_packed2bits(pkg.encode_packed2bits(p), len(p), pkg.encode_packed2bits(t),
                                len(t)) == A.align_end2end(p, t)
+    # gap-linear, src/aligner.rs:4076-4090
+    L = pkg.EditAligner(pkg.WavefrontPenalties.new_linear(pkg.LinearPenalties(0, 4, 2)))
+    L.set_alignment_scope(pkg.AlignmentScope.COMPUTE_ALIGNMENT)
+    assert L.align_end2end(b"ACT", b"ACGT") == 2
+    assert L.cigar().check_alignment(b"ACT", b"ACGT") is None
     E = pkg.EditAligner(pkg.WavefrontPenalties.new_edit())
     E.set_max_alignment_steps(2)
     E.align_end2end(b"AAAA", b"TTTT")

@@ -15,9 +15,43 @@ def _penalty_obj(pkg, metric, pen):
         return pkg.WavefrontPenalties.new_affine(pkg.AffinePenalties(m, x, o1, e1))
     if metric == "gap_affine_2p":
         return pkg.WavefrontPenalties.new_affine2p(pkg.Affine2pPenalties(m, x, o1, e1, o2, e2))
+    if metric == "gap_linear":
+        return pkg.WavefrontPenalties.new_linear(pkg.LinearPenalties(m, x, o1))
     if metric == "edit":
         return pkg.WavefrontPenalties.new_edit()
     return pkg.WavefrontPenalties.new_indel()
+
+
+def _dp_score(oracle, metric, pen, p, t):
+    """The optimal score from the exact DP, in the aligner's internal scale:
+    with a match reward m < 0 the penalties are Eizenga-transformed
+    (src/penalties.rs:156-296) and the DP charges matches nothing."""
+    m, x, o1, e1, o2, e2 = pen
+    if metric == "edit":
+        return oracle.dp_edit(p, t)
+    if metric == "indel":
+        return oracle.dp_edit(p, t, indel=True)
+    if metric == "gap_linear":
+        # a gap of length l costs l * indel: affine with o = 0, e = indel
+        if m < 0:
+            return oracle.dp_affine(p, t, 2 * x - 2 * m, 0, 2 * o1 - m)
+        return oracle.dp_affine(p, t, x, 0, o1)
+    return oracle.dp_affine(p, t, x, o1, e1, o2, e2, metric == "gap_affine_2p")
+
+
+def _cigar_wf_score(pkg, metric, pen, cig, plen, tlen):
+    """A CIGAR's score on the user penalties, in the aligner's internal scale
+    (cigar_wfa_score, src/aligner.rs:2700-2719)."""
+    m, x, o1, e1, o2, e2 = pen
+    if metric in ("edit", "indel"):
+        return cig.score_edit()
+    if metric == "gap_affine":
+        classic = cig.score_gap_affine(pkg.AffinePenalties(m, x, o1, e1))
+    elif metric == "gap_affine_2p":
+        classic = cig.score_gap_affine2p(pkg.Affine2pPenalties(m, x, o1, e1, o2, e2))
+    else:
+        classic = cig.score_gap_linear(pkg.LinearPenalties(m, x, o1))
+    return -m * (plen + tlen) - 2 * classic if m < 0 else -classic
 
 
 @pytest.mark.parametrize("vec", load_vectors(), ids=lambda v: v["id"])
@@ -92,6 +126,9 @@ PENALTY_SETS = [
     ("gap_affine_2p", (0, 5, 4, 3, 12, 1)),
     ("edit", (0, 0, 0, 0, 0, 0)),
     ("indel", (0, 0, 0, 0, 0, 0)),
+    ("gap_linear", (0, 4, 2, 0, 0, 0)),
+    ("gap_linear", (0, 3, 5, 0, 0, 0)),
+    ("gap_linear", (-1, 3, 2, 0, 0, 0)),
 ]
 
 
@@ -100,27 +137,15 @@ def test_oracle_scores_match_exact_dp(metric, pen, pkg, N, oracle):
     """Optimal scores from an independent Gotoh / Levenshtein DP."""
     attr = attr_for(N, metric, pen, "alignment")
     A = oracle.OracleAligner(attr)
-    _, x, o1, e1, o2, e2 = pen
     for p, t in _lcg_pairs(60, 120) + [(b"", b""), (b"A", b""), (b"", b"ACG"), (b"AAAA", b"TTTT")]:
         score, status, ops = A.align_end2end(p, t)
         assert status == 5
-        if metric == "edit":
-            want = oracle.dp_edit(p, t)
-        elif metric == "indel":
-            want = oracle.dp_edit(p, t, indel=True)
-        else:
-            want = oracle.dp_affine(p, t, x, o1, e1, o2, e2, metric == "gap_affine_2p")
-        assert score == want, (p, t)
+        assert score == _dp_score(oracle, metric, pen, p, t), (p, t)
         cig = pkg.Cigar(ops)
         if p or t:
             assert cig.check_alignment(p, t) is None
         # rescoring the CIGAR gives the score back (tests/singletrack.rs:116-121)
-        if metric == "gap_affine":
-            assert -cig.score_gap_affine(pkg.AffinePenalties(0, x, o1, e1)) == score
-        elif metric == "gap_affine_2p":
-            assert -cig.score_gap_affine2p(pkg.Affine2pPenalties(0, x, o1, e1, o2, e2)) == score
-        elif metric == "edit":
-            assert cig.score_edit() == score
+        assert _cigar_wf_score(pkg, metric, pen, cig, len(p), len(t)) == score
 
 
 def test_oracle_score_only_equals_cigar_mode(N, oracle):
@@ -141,6 +166,14 @@ def test_plausible_mistakes_are_caught(pkg, N, oracle):
     score = A.align_end2end(p, t)[0]
     assert score == oracle.dp_affine(p, t, 4, 6, 2)
     assert score != oracle.dp_affine(p, t, 4, 4, 2)
+    # gap-linear: a gap costs indel per base, never an extra opening, and the
+    # mismatch and indel inputs are not interchangeable
+    A = oracle.OracleAligner(attr_for(N, "gap_linear", (0, 4, 2, 0, 0, 0), "score"))
+    score = A.align_end2end(p, t)[0]
+    assert score == oracle.dp_affine(p, t, 4, 0, 2)
+    assert score != oracle.dp_affine(p, t, 4, 2, 2)
+    assert A.align_end2end(b"ACGTACGT", b"ACCTACCT")[0] == 8
+    assert oracle.dp_affine(b"ACGTACGT", b"ACCTACCT", 2, 0, 4) != 8
 
 
 def test_generator_deterministic_and_shaped(pkg):
@@ -190,7 +223,6 @@ def test_oracle_med_low_memory(metric, pen, pkg, N, oracle):
     al = attr_for(N, metric, pen, "alignment")
     al.memory_mode = N.MEMORY_LOW
     a_m, a_l = oracle.OracleAligner(am), oracle.OracleAligner(al)
-    _, x, o1, e1, o2, e2 = pen
     for p, t in _lcg_pairs(80, 250, seed=5) + [(b"", b""), (b"ACGT", b""), (b"", b"GG")]:
         sh = a_h.align_end2end(p, t)[0]
         sm, stm, cm = a_m.align_end2end(p, t)
@@ -200,12 +232,7 @@ def test_oracle_med_low_memory(metric, pen, pkg, N, oracle):
         cig = pkg.Cigar(cm)
         if p or t:
             assert cig.check_alignment(p, t) is None
-        if metric == "gap_affine":
-            assert -cig.score_gap_affine(pkg.AffinePenalties(0, x, o1, e1)) == sm
-        elif metric == "gap_affine_2p":
-            assert -cig.score_gap_affine2p(pkg.Affine2pPenalties(0, x, o1, e1, o2, e2)) == sm
-        elif metric == "edit":
-            assert cig.score_edit() == sm
+        assert _cigar_wf_score(pkg, metric, pen, cig, len(p), len(t)) == sm
 
 
 def test_oracle_indel_literal_med_low(pkg, N, oracle):

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from conftest import attr_for
+from test_oracle import _cigar_wf_score, _dp_score
 
 BIWFA_PENALTIES = [
     ("edit", (0, 0, 0, 0, 0, 0)),
@@ -15,6 +16,8 @@ BIWFA_PENALTIES = [
     ("gap_affine", (-3, 5, 3, 2, 0, 0)),
     ("gap_affine_2p", (0, 4, 6, 2, 24, 1)),
     ("gap_affine_2p", (-1, 4, 6, 2, 24, 1)),
+    ("gap_linear", (0, 4, 2, 0, 0, 0)),
+    ("gap_linear", (-1, 4, 2, 0, 0, 0)),
 ]
 
 
@@ -102,17 +105,13 @@ def test_biwfa_recursion_optimal_and_valid(metric, pen, pkg, N, oracle):
     bi = _biwfa(N, oracle, metric, pen)
     bis = _biwfa(N, oracle, metric, pen, "score")
     uni = oracle.OracleAligner(attr_for(N, metric, pen, "score"))
-    m, x, o1, e1, o2, e2 = pen
+    m = pen[0]
     split = 0
     for p, t in pairs:
         score, status, ops = bi.align(p, t)
         assert status == N.STATUS_COMPLETED
-        if metric == "edit":
-            want = oracle.dp_edit(p, t)
-        elif metric == "indel":
-            want = oracle.dp_edit(p, t, indel=True)
-        elif m == 0:
-            want = oracle.dp_affine(p, t, x, o1, e1, o2, e2, metric == "gap_affine_2p")
+        if m == 0 or metric == "gap_linear":
+            want = _dp_score(oracle, metric, pen, p, t)
         else:
             want = uni.align(p, t)[0]
         assert score == want
@@ -120,14 +119,7 @@ def test_biwfa_recursion_optimal_and_valid(metric, pen, pkg, N, oracle):
         cig = pkg.Cigar(ops)
         assert cig.check_alignment(p, t) is None
         assert bi.cigar_score() == score
-        if metric in ("edit", "indel"):
-            assert cig.score_edit() == score
-        elif metric == "gap_affine":
-            classic = cig.score_gap_affine(pkg.AffinePenalties(m, x, o1, e1))
-            assert (-m * (len(p) + len(t)) - 2 * classic if m < 0 else -classic) == score
-        else:
-            classic = cig.score_gap_affine2p(pkg.Affine2pPenalties(m, x, o1, e1, o2, e2))
-            assert (-m * (len(p) + len(t)) - 2 * classic if m < 0 else -classic) == score
+        assert _cigar_wf_score(pkg, metric, pen, cig, len(p), len(t)) == score
         split += score > 250
     assert split >= 4, "the pairs must exercise the breakpoint search"
 

@@ -4,8 +4,8 @@ output lines, aligning the input's pairs in batches on the device instead of
 one aligner call per pair.
 
     python wfa2lib-rs_amd/align_benchmark.py -i pairs.seq [-o out.alg]
-        [-a gap-affine-wfa|gap-affine2p-wfa|edit-wfa|indel-wfa]
-        [-g M,X,O,E] [--affine2p-penalties M,X,O1,E1,O2,E2]
+        [-a gap-affine-wfa|gap-affine2p-wfa|gap-linear-wfa|edit-wfa|indel-wfa]
+        [-p M,X,I] [-g M,X,O,E] [--affine2p-penalties M,X,O1,E1,O2,E2]
         [--wfa-score-only] [--wfa-memory high|med|low|ultralow]
         [--wfa-span global|extension|ends-free,P0,Pf,T0,Tf]
         [--wfa-heuristic NAME --wfa-heuristic-parameters P,...]
@@ -15,8 +15,8 @@ Input: lines alternating `>pattern` and `<text` (prefixes optional; an
 unpaired last line is ignored). Output: one `score<TAB>CIGAR` line per pair
 (`-` for no CIGAR), exactly as the reference prints them; `Time.Alignment`
 (device time of the batches) and, with -c, `Alignments.Correct` on stderr.
-Configurations the device path does not run (gap-linear, Z-drop,
---no-c-compat) exit with an error instead of falling back to a CPU.
+Configurations the device path does not run (Z-drop, --no-c-compat) exit
+with an error instead of falling back to a CPU.
 """
 import argparse
 import ctypes
@@ -30,12 +30,12 @@ if __package__ in (None, ""):
     _entry.load_package()
     from wfa2lib_rs_amd import _native as N  # noqa: E402
     from wfa2lib_rs_amd.aligner import (Affine2pPenalties, AffinePenalties, Cigar,  # noqa: E402
-                                        DistanceMetric, WavefrontPenalties,
-                                        compute_output_score)
+                                        DistanceMetric, LinearPenalties,
+                                        WavefrontPenalties, compute_output_score)
 else:
     from . import _native as N
     from .aligner import (Affine2pPenalties, AffinePenalties, Cigar, DistanceMetric,
-                          WavefrontPenalties, compute_output_score)
+                          LinearPenalties, WavefrontPenalties, compute_output_score)
 
 INT32_MIN = -2 ** 31
 OFFSET_NULL = INT32_MIN // 2
@@ -117,7 +117,8 @@ def build_penalties(a):
         if a.algorithm == "edit-wfa":
             return WavefrontPenalties.new_edit()
         if a.algorithm == "gap-linear-wfa":
-            fail("[WFA] gap-linear is not available on the device path")
+            v = vals(a.linear_penalties) if a.linear_penalties else [0, 4, 2]
+            return WavefrontPenalties.new_linear(LinearPenalties(*v[:3]))
         if a.algorithm == "gap-affine-wfa":
             v = vals(a.affine_penalties) if a.affine_penalties else [0, 4, 6, 2]
             return WavefrontPenalties.new_affine(AffinePenalties(*v[:4]))

@@ -453,9 +453,9 @@ static __device__ int backtrace_linear(const KParams& p, const Hist& H, int nsco
   return cap - 1 - pos;
 }
 
-// maxtrim_cigar (src/aligner.rs:1545-1583) with Cigar::maxtrim_gap_affine
-// / _affine2p (src/cigar.rs:272-419) on the user penalties: keep the
-// maximal-scoring prefix, a free leading gap skipped and put back; an
+// maxtrim_cigar (src/aligner.rs:1545-1583) with Cigar::maxtrim_gap_linear
+// / _affine / _affine2p (src/cigar.rs:207-419) on the user penalties: keep
+// the maximal-scoring prefix, a free leading gap skipped and put back; an
 // alignment whose best prefix scores 0 is cleared. Returns the new length.
 static __device__ int maxtrim_cigar(const KParams& p, const uint8_t* ops, int n, int pb_free,
                                     int tb_free, bool* trimmed) {
@@ -470,6 +470,7 @@ static __device__ int maxtrim_cigar(const KParams& p, const uint8_t* ops, int n,
   if (b >= n) { *trimmed = true; return 0; }
   const int mscore = p.u_match != 0 ? p.u_match : -1;
   const bool two = p.metric == WFA_METRIC_GAP_AFFINE_2P;
+  const bool lin = p.metric == WFA_METRIC_GAP_LINEAR;
   int max_score = 0, max_off = b, score = 0, last = 0, len = 0;
   auto run_cost = [&](int op, int l) -> int {
     if (op == 'M') return mscore * l;
@@ -494,6 +495,7 @@ static __device__ int maxtrim_cigar(const KParams& p, const uint8_t* ops, int n,
       } else {
         if (op == 'M') score -= mscore;
         else if (op == 'X') score -= p.u_x;
+        else if (lin) score -= p.u_o1;  // gap_opening1 is the indel penalty
         else score -= p.u_e1 + (last == op ? 0 : p.u_o1);
         last = op;
         if (max_score < score) { max_score = score; max_off = i; }
@@ -518,7 +520,8 @@ static __device__ __forceinline__ int wave_sum(int v) {
 // (src/aligner.rs:2700-2719): with a heuristic the returned score of a
 // gap-affine/2p alignment is its CIGAR's score on the user penalties
 // (ends-free: a leading free gap and the trailing free ends not counted),
-// in the internal scale. `ops` holds the n ops at the head of the region.
+// in the internal scale; a BiWFA alignment of any gap metric returns its
+// CIGAR's score too. `ops` holds the n ops at the head of the region.
 static __device__ int rescore_cigar(const KParams& p, const uint8_t* ops, int n, bool ef,
                                     int pb_free, int tb_free, int trailing) {
   const int lane = threadIdx.x;
@@ -531,7 +534,17 @@ static __device__ int rescore_cigar(const KParams& p, const uint8_t* ops, int n,
   }
   if (e <= b) return 0;
   int classic = 0, vh = 0;
-  if (p.metric == WFA_METRIC_GAP_AFFINE) {
+  if (p.metric == WFA_METRIC_GAP_LINEAR) {
+    // Cigar::score_gap_linear (src/cigar.rs:126-136): every gap op costs
+    // the indel penalty
+    for (int i = b + lane; i < e; i += 64) {
+      const int op = ops[i];
+      classic -= op == 'M' ? p.u_match : op == 'X' ? p.u_x : p.u_o1;
+      vh += (op != 'I') + (op != 'D');
+    }
+    classic = wave_sum(classic);
+    vh = wave_sum(vh);
+  } else if (p.metric == WFA_METRIC_GAP_AFFINE) {
     // Cigar::score_gap_affine (src/cigar.rs:139-169): each op on its own,
     // a gap op opening unless it follows the same op
     for (int i = b + lane; i < e; i += 64) {
@@ -595,11 +608,13 @@ static __device__ int piggyback_backtrace(const KParams& p, const Hist& H, int n
     if (kk < hd.x || kk > hd.y) { bad = true; break; }
     const int f = H.bytes[hist_row(H, hd) + (kk - hd.x)];
     uint8_t op;
-    if (KIND == KIND_EDIT) {
+    if (KIND == KIND_EDIT || KIND == KIND_LINEAR) {
+      // one M component: a mismatch came from M[s - x], a gap from
+      // M[s - indel] (edit and indel: both 1)
       const int w = f & 7;
       op = w == 0 ? 'I' : w == 1 ? 'D' : 'X';
       kk += w == 0 ? -1 : w == 1 ? 1 : 0;
-      s -= 1;
+      s -= KIND == KIND_EDIT ? 1 : w == 2 ? p.x : p.o1;
     } else if (comp == CM) {
       op = 'X';
       const int w = f & 7;
@@ -628,7 +643,7 @@ static __device__ int piggyback_backtrace(const KParams& p, const Hist& H, int n
   const int bv = kk < 0 ? -kk : 0, bh = kk > 0 ? kk : 0;
   const int nchain = H.chain_cap - 1 - pos;
   const uint8_t* ch = H.chain + pos + 1;
-  const bool affine = KIND != KIND_EDIT;
+  const bool affine = KIND == KIND_AFFINE || KIND == KIND_AFFINE2P;
   // unpack: begin at (0, 0); re-extend matches before each op (affine: only
   // while in M); drop the X that closes a gap; pad to the end cell and the
   // sequence ends
@@ -679,6 +694,7 @@ static __device__ int high_mem_nwf(const KParams& p, int plen, int tlen) {
     switch (p.metric) {
       case WFA_METRIC_INDEL: return pl + tl + 1;
       case WFA_METRIC_EDIT: return imax(pl, tl) + 1;
+      case WFA_METRIC_GAP_LINEAR: return mn * p.x + p.o1 * ad + 1;
       case WFA_METRIC_GAP_AFFINE: return mn * p.x + p.o1 + ad * p.e1 + 1;
       default: {
         const int i1 = p.o1 + ad * p.e1, i2 = p.o2 + ad * p.e2;
@@ -850,6 +866,8 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   constexpr bool MED = MODE == 2;
   static_assert(NW == 1 || (!HEUR && U == 1), "several waves per pair: no heuristics, U = 1");
   constexpr int NT = 64 * NW;
+  // gap-affine kinds keep I/D components; edit and gap-linear only M
+  constexpr bool GAPS = KIND == KIND_AFFINE || KIND == KIND_AFFINE2P;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = NW == 1 ? 0 : tid >> 6;
@@ -858,7 +876,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   const int ringM = p.ringM, ringE1 = p.ringE1, ringE2 = p.ringE2;
   const int rI1 = ringM, rD1 = ringM + ringE1, rI2 = ringM + 2 * ringE1,
             rD2 = ringM + 2 * ringE1 + ringE2;
-  const int nrows = KIND == KIND_EDIT ? ringM : two ? rD2 + ringE2 : rD1 + ringE1;
+  const int nrows = !GAPS ? ringM : two ? rD2 + ringE2 : rD1 + ringE1;
   for (int r = tid; r < nrows; r += NT) R.meta[r] = meta_absent();
   // ends-free limits of this pair (extension: free ends up to the full
   // lengths, no begin gaps; src/aligner.rs:713-718)
@@ -1039,7 +1057,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     if (tid == 0) {
       // an emptied M is the null wavefront with an empty initialized range
       R.meta[rowM] = empty ? make_int4(nlo, nhi, 1, 0) : make_int4(nlo, nhi, mm.z, mm.w);
-      if (KIND != KIND_EDIT) {
+      if (GAPS) {
         auto equate = [&](int row) {
           const int4 g = R.meta[row];
           if (g.x > g.y) return;
@@ -1090,7 +1108,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   while (status == WFA_STATUS_OK && !overflow) {
     score += 1;
     sM = sM + 1 == ringM ? 0 : sM + 1;
-    if (KIND != KIND_EDIT) sE1 = sE1 + 1 == ringE1 ? 0 : sE1 + 1;
+    if (GAPS) sE1 = sE1 + 1 == ringE1 ? 0 : sE1 + 1;
     if (two) sE2 = sE2 + 1 == ringE2 ? 0 : sE2 + 1;
     if (!p.modular && score >= nwf) { status = WFA_STATUS_MAX_STEPS_REACHED; result = score; break; }
     if (CIG && score >= H.hdr_cap) { overflow = true; break; }
@@ -1123,15 +1141,19 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         hi = mh_x + 1;
       }
     } else {
+      // gap-linear (compute_linear_step, src/aligner.rs:1866-2002): M at
+      // s - x and s - indel; its inputs' ends are NULL-filled only beyond
+      // their initialized ranges, as the affine inputs'
+      const int go = GAPS ? p.oe1 : p.o1;
       if (s - p.x >= 0) {
         row_x = back(sM, p.x, ringM);
         load(row_x, ml_x, mh_x, gl_x, gh_x);
       }
-      if (s - p.oe1 >= 0) {
-        row_o = back(sM, p.oe1, ringM);
+      if (s - go >= 0) {
+        row_o = back(sM, go, ringM);
         load(row_o, ml_o, mh_o, gl_o, gh_o);
       }
-      if (s - p.e1 >= 0) {
+      if (GAPS && s - p.e1 >= 0) {
         row_e = back(sE1, p.e1, ringE1);
         load(rI1 + row_e, il, ih, gil, gih);
         load(rD1 + row_e, dl, dh, gdl, gdh);
@@ -1167,8 +1189,10 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       };
       bound(ml_x, mh_x, 0, 0);
       bound(ml_o, mh_o, -1, 1);
-      bound(il, ih, 1, 1);
-      bound(dl, dh, -1, -1);
+      if (GAPS) {
+        bound(il, ih, 1, 1);
+        bound(dl, dh, -1, -1);
+      }
       if (two) {
         bound(ml_o2, mh_o2, -1, 1);
         bound(il2, ih2, 1, 1);
@@ -1190,7 +1214,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       nulls += 1;
       if (tid == 0) {
         R.meta[rowM] = meta_absent();
-        if (KIND != KIND_EDIT) {
+        if (GAPS) {
           R.meta[rI1 + rowE] = meta_absent();
           R.meta[rD1 + rowE] = meta_absent();
         }
@@ -1213,7 +1237,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       // at this score (src/aligner.rs:2058-2081)
       if (tid == 0) {
         R.meta[rowM] = meta_absent();
-        if (KIND != KIND_EDIT) {
+        if (GAPS) {
           R.meta[rI1 + rowE] = meta_absent();
           R.meta[rD1 + rowE] = meta_absent();
         }
@@ -1263,9 +1287,9 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           if (p.metric != WFA_METRIC_INDEL) in.c = rd(R, row_x, gl_x, gh_x, k);
         } else {
           in.a = rd(R, row_o, gl_o, gh_o, k - 1);
-          in.b = rd(R, rI1 + row_e, gil, gih, k - 1);
+          if (GAPS) in.b = rd(R, rI1 + row_e, gil, gih, k - 1);
           in.c = rd(R, row_o, gl_o, gh_o, k + 1);
-          in.d = rd(R, rD1 + row_e, gdl, gdh, k + 1);
+          if (GAPS) in.d = rd(R, rD1 + row_e, gdl, gdh, k + 1);
           in.e = rd(R, row_x, gl_x, gh_x, k);
           if (two) {
             in.f = rd(R, row_o2, gl_o2, gh_o2, k - 1);
@@ -1282,9 +1306,9 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         if (p.metric != WFA_METRIC_INDEL) in.c = rdg(R, bx, g_x, k);
       } else {
         in.a = rdg(R, bo, g_o, k - 1);
-        in.b = rdg(R, bi1, g_i1, k - 1);
+        if (GAPS) in.b = rdg(R, bi1, g_i1, k - 1);
         in.c = rdg(R, bo, g_o, k + 1);
-        in.d = rdg(R, bd1, g_d1, k + 1);
+        if (GAPS) in.d = rdg(R, bd1, g_d1, k + 1);
         in.e = rdg(R, bx, g_x, k);
         if (two) {
           in.f = rdg(R, bo2, g_o2, k - 1);
@@ -1313,6 +1337,13 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
             // mismatch, then deletion, then insertion
             if (MED) fl = m == mis + 1 ? 2 : m == del ? 1 : 0;
           }
+          if (!in_bounds(m, k, plen, tlen)) m = WFA_NULL;
+        } else if (KIND == KIND_LINEAR) {
+          // compute_linear_idm (src/compute/linear.rs:20-131); the same
+          // piggyback priority as edit (src/compute/piggyback.rs:279-302)
+          const int ins = in.a + 1, del = in.c, mis = in.e + 1;
+          m = imax(del, imax(ins, mis));
+          if (MED) fl = m == mis ? 2 : m == del ? 1 : 0;
           if (!in_bounds(m, k, plen, tlen)) m = WFA_NULL;
         } else {
           const int io = in.a, ie = in.b;
@@ -1364,7 +1395,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         if (m_lo == INT32_MAX) m_lo = kb + __builtin_ctzll(b);
         m_hi = kb + 63 - __builtin_clzll(b);
       }
-      if (KIND != KIND_EDIT) {
+      if (GAPS) {
         b = ballot(vi);
         if (b) {
           if (i_lo == INT32_MAX) i_lo = kb + __builtin_ctzll(b);
@@ -1434,7 +1465,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     if (MED && tid == 0) H.hdr[s] = hrow;
     if (tid == 0) {
       R.meta[rowM] = make_int4(m_lo, m_hi, m_lo, m_hi);
-      if (KIND != KIND_EDIT) {
+      if (GAPS) {
         R.meta[rI1 + rowE] = make_int4(i_lo, i_hi, i_lo, i_hi);
         R.meta[rD1 + rowE] = make_int4(d_lo, d_hi, d_lo, d_hi);
       }
@@ -1476,7 +1507,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       if (MED)
         ncig = piggyback_backtrace<KIND, BITS>(p, H, score + 1, sP, sT, plen, tlen, score, end_k,
                                                end_off, out, cap);
-      else if (KIND == KIND_EDIT)
+      else if (!GAPS)
         ncig = backtrace_linear<BITS>(p, H, score + 1, plen, tlen, score, end_k, end_off, out,
                                       cap);
       else
@@ -1505,7 +1536,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         bool trimmed = false;
         ncig = maxtrim_cigar(p, out, ncig, pb_free, tb_free, &trimmed);
         if (trimmed) out_status = WFA_STATUS_PARTIAL;
-      } else if (heur != WFA_HEURISTIC_NONE && KIND != KIND_EDIT) {
+      } else if (heur != WFA_HEURISTIC_NONE && GAPS) {
         __threadfence_block();
         const int trailing = ef ? (plen - (end_off - end_k)) + (tlen - end_off) : 0;
         result = rescore_cigar(p, out, ncig, ef, pb_free, tb_free, trailing);

@@ -48,7 +48,7 @@ struct LaunchCfg {
 
 
 int ring_rows_of(const KParams& p, int kind) {
-  if (kind == wfa::KIND_EDIT) return p.ringM;
+  if (kind == wfa::KIND_EDIT || kind == wfa::KIND_LINEAR) return p.ringM;
   if (kind == wfa::KIND_AFFINE) return p.ringM + 2 * p.ringE1;
   return p.ringM + 2 * p.ringE1 + 2 * p.ringE2;
 }

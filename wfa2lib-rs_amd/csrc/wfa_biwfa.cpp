@@ -73,7 +73,7 @@ int ceil_pow2(int v) {
   return p;
 }
 
-int nc_of(int kind) { return kind == wfa::KIND_EDIT ? 1 : kind == wfa::KIND_AFFINE ? 3 : 5; }
+int nc_of(int kind) { return wfa::kind_components(kind); }
 
 // Run one kind of BiWFA launch over `jobs`, re-running jobs that asked for
 // more room (longer LDS sequences, a wider ring, more history) with larger

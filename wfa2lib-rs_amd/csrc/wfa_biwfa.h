@@ -117,7 +117,7 @@ static __device__ int stage_job(const BiParams& bp, const BiJob& J, uint64_t* sP
 // scans.
 template <int KIND, int BITS, bool HEUR, bool HISTORY, int NW = 1>
 struct WfDir {
-  static constexpr int NC = KIND == KIND_EDIT ? 1 : KIND == KIND_AFFINE ? 3 : 5;
+  static constexpr int NC = kind_components(KIND);
   static constexpr int kNW = NW;
   static constexpr int NT = 64 * NW;
   static_assert(NW == 1 || (!HEUR && !HISTORY), "several waves: no heuristic, no history");
@@ -266,9 +266,17 @@ struct WfDir {
       if (lane == 0) R.meta[rM] = make_int4(m_lo, m_hi, m_lo, m_hi);
       __syncthreads();
     } else {
+      // gap-affine kinds, and gap-linear (compute_linear_step,
+      // src/aligner.rs:1866-2002): M from M[s-x] and M[s-indel], whose ends
+      // are NULL-filled only beyond their initialized ranges
+      constexpr bool GAPS = KIND != KIND_LINEAR;
       const bool two = KIND == KIND_AFFINE2P;
-      In mx = load(C_M, s - p->x), mo = load(C_M, s - p->oe1);
-      In i1 = load(C_I1, s - p->e1), d1 = load(C_D1, s - p->e1);
+      In mx = load(C_M, s - p->x), mo = load(C_M, s - (GAPS ? p->oe1 : p->o1));
+      In i1{0, 1, 0, 1, 0}, d1{0, 1, 0, 1, 0};
+      if (GAPS) {
+        i1 = load(C_I1, s - p->e1);
+        d1 = load(C_D1, s - p->e1);
+      }
       In mo2{0, 1, 0, 1, 0}, i2{0, 1, 0, 1, 0}, d2{0, 1, 0, 1, 0};
       if (two) {
         mo2 = load(C_M, s - p->oe2);
@@ -289,8 +297,10 @@ struct WfDir {
       };
       bound(mx, 0, 0);
       bound(mo, -1, 1);
-      bound(i1, 1, 1);
-      bound(d1, -1, -1);
+      if (GAPS) {
+        bound(i1, 1, 1);
+        bound(d1, -1, -1);
+      }
       if (two) {
         bound(mo2, -1, 1);
         bound(i2, 1, 1);
@@ -310,7 +320,8 @@ struct WfDir {
       hi = imin(hi, hist_hi - 1);
       if (lo > hi) { set_absent(s); return; }
       if (hi - lo + 1 > R.wcap) { overflow = true; return; }
-      const int rM = row(C_M, s), rI1 = row(C_I1, s), rD1 = row(C_D1, s);
+      const int rM = row(C_M, s);
+      const int rI1 = GAPS ? row(C_I1, s) : 0, rD1 = GAPS ? row(C_D1, s) : 0;
       const int rI2 = two ? row(C_I2, s) : 0, rD2 = two ? row(C_D2, s) : 0;
       int lim[5][2];
       for (int c = 0; c < 5; ++c) { lim[c][0] = INT32_MAX; lim[c][1] = INT32_MIN; }
@@ -318,8 +329,13 @@ struct WfDir {
         const int k = kb + (lane & 63);
         bool v[5] = {false, false, false, false, false};
         if (k <= hi) {
-          const int iv = imax(rd(R, mo.row, mo.gl, mo.gh, k - 1), rd(R, i1.row, i1.gl, i1.gh, k - 1)) + 1;
-          const int dv = imax(rd(R, mo.row, mo.gl, mo.gh, k + 1), rd(R, d1.row, d1.gl, d1.gh, k + 1));
+          int iv = rd(R, mo.row, mo.gl, mo.gh, k - 1);
+          int dv = rd(R, mo.row, mo.gl, mo.gh, k + 1);
+          if (GAPS) {
+            iv = imax(iv, rd(R, i1.row, i1.gl, i1.gh, k - 1));
+            dv = imax(dv, rd(R, d1.row, d1.gl, d1.gh, k + 1));
+          }
+          iv += 1;
           const int mis = rd(R, mx.row, mx.gl, mx.gh, k) + 1;
           int ins = iv, del = dv;
           if (two) {
@@ -338,10 +354,12 @@ struct WfDir {
           v[C_M] = in_bounds(m, k, plen, tlen);
           if (!v[C_M]) m = WFA_NULL;
           R.set(rM * R.wcap + (k & R.mask), m);
-          R.set(rI1 * R.wcap + (k & R.mask), iv);
-          R.set(rD1 * R.wcap + (k & R.mask), dv);
-          v[C_I1] = in_bounds(iv, k, plen, tlen);
-          v[C_D1] = in_bounds(dv, k, plen, tlen);
+          if (GAPS) {
+            R.set(rI1 * R.wcap + (k & R.mask), iv);
+            R.set(rD1 * R.wcap + (k & R.mask), dv);
+            v[C_I1] = in_bounds(iv, k, plen, tlen);
+            v[C_D1] = in_bounds(dv, k, plen, tlen);
+          }
         }
         for (int c = 0; c < NC; ++c) {
           const uint64_t b = ballot(v[c]);
@@ -560,7 +578,7 @@ static __device__ void bi_overlap(const D& a, const D& b, int s0, int s1, bool f
       bi_scan(a, b, C_D2, s0, si, p.o2, fwd, bp);
       bi_scan(a, b, C_I2, s0, si, p.o2, fwd, bp);
     }
-    if (KIND != KIND_EDIT && s0 + si - p.o1 < bp.score) {
+    if ((KIND == KIND_AFFINE || KIND == KIND_AFFINE2P) && s0 + si - p.o1 < bp.score) {
       bi_scan(a, b, C_D1, s0, si, p.o1, fwd, bp);
       bi_scan(a, b, C_I1, s0, si, p.o1, fwd, bp);
     }
@@ -619,7 +637,8 @@ static __device__ BiResult find_breakpoint(const BiParams& bp, const BiJob& J, u
   // Both searches together never need more than twice the score of the
   // all-gap alignment (plus the null steps that end a search): a bound that
   // keeps every wavefront finite whatever the input.
-  const int64_t pen_max = imax(imax(p.x, p.oe1), KIND == KIND_AFFINE2P ? p.oe2 : 0);
+  const int64_t pen_max = imax(imax(p.x, KIND == KIND_LINEAR ? p.o1 : p.oe1),
+                               KIND == KIND_AFFINE2P ? p.oe2 : 0);
   const int64_t step_bound = 4 * ((int64_t)(plen + tlen + 2) * imax((int)pen_max, 1)) + 4 * p.scope;
   int s_fwd = 0, s_rev = 0, ak = 0, fmax = 0, rmax = 0;
   auto runaway = [&]() { return (int64_t)s_fwd + s_rev > step_bound; };
@@ -684,7 +703,7 @@ template <int KIND, bool HEUR, int NW>
 __global__ void __launch_bounds__(64 * NW) biwfa_breakpoint_kernel(BiParams bp) {
   extern __shared__ uint64_t smem[];
   const int lane = threadIdx.x;
-  constexpr int NC = KIND == KIND_EDIT ? 1 : KIND == KIND_AFFINE ? 3 : 5;
+  constexpr int NC = kind_components(KIND);
   uint64_t* sP = smem;
   uint64_t* sT = smem + bp.seq_words;
   int4* metaF = (int4*)(sT + bp.seq_words);
@@ -893,7 +912,7 @@ static __device__ void base_case(const BiParams& bp, int item, const BiJob& J, u
       trace = false;  // max steps: the CIGAR is cleared
     }
     if (trace) {
-      if (KIND == KIND_EDIT) {
+      if constexpr (KIND == KIND_EDIT || KIND == KIND_LINEAR) {
         // one contiguous history per sub-problem: chunk 0 of size 0
         Hist H;
         H.hdr = hdr; H.cells = cells; H.chunk = 0;
@@ -935,10 +954,10 @@ template <int KIND, bool HEUR>
 __global__ void __launch_bounds__(64) biwfa_base_kernel(BiParams bp) {
   extern __shared__ uint64_t smem[];
   const int lane = threadIdx.x;
-  constexpr int NC = KIND == KIND_EDIT ? 1 : KIND == KIND_AFFINE ? 3 : 5;
+  constexpr int NC = kind_components(KIND);
   uint64_t* sP = smem;
   uint64_t* sT = smem + bp.seq_words;
-  int4* meta = (int4*)(sT + bp.seq_words);
+  int4* meta =(int4*)(sT + bp.seq_words);
   int32_t* ring = bp.ring + (int64_t)blockIdx.x * NC * bp.k.scope * bp.wcap;
   int4* hdr = bp.hdr + (int64_t)blockIdx.x * bp.hdr_cap * NC;
   int32_t* cells = bp.hist + (int64_t)blockIdx.x * bp.hist_cap;

@@ -8,7 +8,14 @@
 
 namespace wfa {
 
-enum { KIND_EDIT = 0, KIND_AFFINE = 1, KIND_AFFINE2P = 2 };
+// Kernel kinds: edit / indel (M from M[s-1]), gap-affine, 2-piece
+// gap-affine, gap-linear (M from M[s-x] and M[s-indel], no gap components)
+enum { KIND_EDIT = 0, KIND_AFFINE = 1, KIND_AFFINE2P = 2, KIND_LINEAR = 3 };
+
+// wavefront components per score of a kind: M, then I1/D1, then I2/D2
+constexpr int kind_components(int kind) {
+  return kind == KIND_AFFINE ? 3 : kind == KIND_AFFINE2P ? 5 : 1;
+}
 
 // Everything one persistent launch needs, passed by value.
 struct KParams {
@@ -129,6 +136,7 @@ using KernelFn2 = void (*)(BiParams);
 KernelFn2 pick_biwfa_edit(int which, bool heur);
 KernelFn2 pick_biwfa_affine(int which, bool heur);
 KernelFn2 pick_biwfa_affine2p(int which, bool heur);
+KernelFn2 pick_biwfa_linear(int which, bool heur);
 hipError_t launch_biwfa(int kind, int which, bool heur, const BiParams& bp, int grid,
                         size_t lds_bytes, hipStream_t stream);
 int biwfa_blocks_per_cu(int kind, int which, bool heur, size_t lds_bytes);
@@ -143,6 +151,7 @@ hipError_t launch_biwfa_rescore(const KParams& p, const int32_t* pairs, int n,
 KernelFn pick_edit(int mode, bool lds, bool i16, bool packed, bool heur, int nw);
 KernelFn pick_affine(int mode, bool lds, bool i16, bool packed, bool heur, int nw);
 KernelFn pick_affine2p(int mode, bool lds, bool i16, bool packed, bool heur, int nw);
+KernelFn pick_linear(int mode, bool lds, bool i16, bool packed, bool heur, int nw);
 // the short-read window kernel (gap-affine, end-to-end, no heuristic;
 // mode 0 or 1)
 KernelFn pick_affine_window(int mode, bool packed);

@@ -78,8 +78,6 @@ inline int score_scope(const Internal& p) {
 // Which configurations the device path runs. The rest return
 // WFA_ERR_UNSUPPORTED rather than silently falling back to anything.
 inline int check_supported(const wfa_attr_t& a) {
-  const int m = a.penalties.distance_metric;
-  if (m == WFA_METRIC_GAP_LINEAR) return WFA_ERR_UNSUPPORTED;
   if (a.span != WFA_SPAN_END2END && a.span != WFA_SPAN_ENDSFREE) return WFA_ERR_INVALID_ARGUMENT;
   // ends-free with a match reward adds begin-free cells during compute
   // (src/aligner.rs:3369-3389): not on the device path yet
@@ -177,6 +175,7 @@ inline int grow(void** ptr, size_t* have, size_t need) {
 inline int kind_of(int metric) {
   if (metric == WFA_METRIC_GAP_AFFINE) return wfa::KIND_AFFINE;
   if (metric == WFA_METRIC_GAP_AFFINE_2P) return wfa::KIND_AFFINE2P;
+  if (metric == WFA_METRIC_GAP_LINEAR) return wfa::KIND_LINEAR;
   return wfa::KIND_EDIT;
 }
 
@@ -193,8 +192,9 @@ inline void base_params(const Internal& ip, const wfa_attr_t& attr, const wfa_ba
   p->oe2 = ip.o2 + ip.e2;
   p->scope = score_scope(ip);
   const int kind = kind_of(ip.metric);
+  const bool gaps = kind == wfa::KIND_AFFINE || kind == wfa::KIND_AFFINE2P;
   p->ringM = kind == wfa::KIND_EDIT ? 2 : p->scope;
-  p->ringE1 = kind == wfa::KIND_EDIT ? 0 : ip.e1 + 1;
+  p->ringE1 = gaps ? ip.e1 + 1 : 0;
   p->ringE2 = kind == wfa::KIND_AFFINE2P ? ip.e2 + 1 : 0;
   // score-only and medium/low memory keep only the wavefronts in scope
   // (src/aligner.rs:855-866): no score-indexed slot limit

@@ -80,6 +80,7 @@ static KernelFn pick(int kind, int mode, bool lds, bool i16, bool packed, bool h
     case KIND_EDIT: return pick_edit(mode, lds, i16, packed, heur, nw);
     case KIND_AFFINE: return pick_affine(mode, lds, i16, packed, heur, nw);
     case KIND_AFFINE2P: return pick_affine2p(mode, lds, i16, packed, heur, nw);
+    case KIND_LINEAR: return pick_linear(mode, lds, i16, packed, heur, nw);
     default: return nullptr;
   }
 }
@@ -142,6 +143,7 @@ static KernelFn2 pick_biwfa(int kind, int which, bool heur) {
     case KIND_EDIT: return pick_biwfa_edit(which, heur);
     case KIND_AFFINE: return pick_biwfa_affine(which, heur);
     case KIND_AFFINE2P: return pick_biwfa_affine2p(which, heur);
+    case KIND_LINEAR: return pick_biwfa_linear(which, heur);
     default: return nullptr;
   }
 }

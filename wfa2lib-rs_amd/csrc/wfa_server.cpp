@@ -105,6 +105,9 @@ int64_t predict_footprint(const wfa_attr_t& a, const Internal& ip, int plen, int
   if (ip.metric == WFA_METRIC_GAP_AFFINE || ip.metric == WFA_METRIC_GAP_AFFINE_2P) {
     cost = std::max(ip.x, ip.o1 + ip.e1);
     e = ip.metric == WFA_METRIC_GAP_AFFINE_2P ? std::min(ip.e1, ip.e2) : ip.e1;
+  } else if (ip.metric == WFA_METRIC_GAP_LINEAR) {
+    cost = std::max(ip.x, ip.o1);
+    e = ip.o1;
   }
   const int64_t s = std::max<int64_t>(1, (len * cost + 9) / 10);
   const int64_t w = std::min<int64_t>(2 * s / std::max(e, 1) + 1, (int64_t)plen + tlen + 1);
