@@ -54,7 +54,10 @@ extern "C" {
 #define WFA_HEURISTIC_NONE             0
 #define WFA_HEURISTIC_WFADAPTIVE       1  /* p0 = min_wavefront_length, p1 = max_distance_threshold, p2 = steps_between_cutoffs */
 #define WFA_HEURISTIC_XDROP            2  /* p0 = xdrop, p1 = steps_between_cutoffs */
-#define WFA_HEURISTIC_ZDROP            3  /* p0 = zdrop, p1 = steps_between_cutoffs */
+#define WFA_HEURISTIC_ZDROP            3  /* p0 = zdrop, p1 = steps_between_cutoffs;
+                                              score-only or high-memory CIGARs
+                                              (a medium/low-memory CIGAR or BiWFA:
+                                              WFA_ERR_UNSUPPORTED) */
 #define WFA_HEURISTIC_BANDED_STATIC    4  /* p0 = min_k, p1 = max_k */
 #define WFA_HEURISTIC_BANDED_ADAPTIVE  5  /* p0 = min_k, p1 = max_k, p2 = steps_between_cutoffs */
 
@@ -192,6 +195,11 @@ int32_t wfa_aligner_cigar(const wfa_aligner_t* aligner, const uint8_t** ops);
 /* Cigar.score (src/cigar.rs:26): the WFA score the CIGAR was traced at,
  * INT32_MIN when there is no CIGAR. */
 int32_t wfa_aligner_cigar_score(const wfa_aligner_t* aligner);
+/* alignment_end_pos (src/aligner.rs:146-204, a public field there) after
+ * the last unidirectional alignment: out[0..3) = {score, k, offset} of the
+ * end cell or of a Z-drop's best cell; unset {-1, INT32_MAX, INT32_MIN/2}
+ * when neither exists or after a BiWFA alignment. */
+void wfa_aligner_end_position(const wfa_aligner_t* aligner, int32_t* out);
 /* Error code of the last call on this aligner (WFA_OK if it succeeded). */
 int32_t wfa_aligner_error(const wfa_aligner_t* aligner);
 
@@ -225,6 +233,13 @@ typedef struct wfa_batch_t {
     int32_t* cigar_length;           /* NULL in score scope */
     const struct wfa_pair_config_t* pair_config;  /* optional, one per pair; NULL =
                                         every pair as the attribute says */
+    int32_t* end_position;           /* optional, 3 per pair: the aligner's
+                                        alignment_end_pos {score, k, offset}
+                                        (src/aligner.rs:146-204): the end
+                                        cell, a Z-drop's best cell (found at
+                                        that earlier score), else unset
+                                        {-1, INT32_MAX, INT32_MIN/2}. Not
+                                        written for WFA_MEMORY_ULTRALOW. */
 } wfa_batch_t;
 
 /* One pair's span and heuristic within a batch: what the reference's
@@ -235,7 +250,8 @@ typedef struct wfa_batch_t {
  * its heuristic only when attr.heuristic is not WFA_HEURISTIC_NONE (the
  * attribute's own free ends and heuristic parameters are then unused).
  * The caller validates each pair's values as wfa_attr_t values are
- * validated: free ends >= 0, heuristic not WFA_HEURISTIC_ZDROP. Not for
+ * validated: free ends >= 0; WFA_HEURISTIC_ZDROP only for score-only and
+ * high-memory batches (elsewhere the pair runs without a heuristic). Not for
  * WFA_MEMORY_ULTRALOW batches. */
 typedef struct wfa_pair_config_t {
     int32_t span;                   /* WFA_SPAN_* */

@@ -40,6 +40,8 @@ def lib(attr_type):
             f.argtypes = [VP, ctypes.c_char_p, I32, ctypes.c_char_p, I32]
         l.wfo_status.restype = I32
         l.wfo_status.argtypes = [VP]
+        l.wfo_end_position.restype = None
+        l.wfo_end_position.argtypes = [VP, P(I32)]
         l.wfo_cigar.restype = I32
         l.wfo_cigar.argtypes = [VP, P(P(ctypes.c_uint8))]
         l.wfo_cigar_score.restype = I32
@@ -95,6 +97,12 @@ class OracleAligner:
     def cigar_score(self):
         """The CIGAR's score field (Cigar.score) after the last alignment."""
         return self.L.wfo_cigar_score(self.h)
+
+    def end_position(self):
+        """alignment_end_pos (score, k, offset) after the last alignment."""
+        out = (ctypes.c_int32 * 3)()
+        self.L.wfo_end_position(self.h, out)
+        return tuple(out)
 
     def _finish(self, s):
         st = self.L.wfo_status(self.h)

@@ -237,6 +237,9 @@ struct wfo_aligner {
     int piggyback, low;
     /* heuristic state (src/heuristic.rs:60-143) */
     int h_steps_wait, h_max_sw_score, h_max_sw_k;
+    /* Z-drop: the best cell's offset and WFA score, the last score with no
+     * drop (HeuristicState, src/heuristic.rs:60-69) */
+    int h_max_sw_off, h_max_wf_score, h_zlast;
     int32_t* dist; int cap_dist;
     /* BiWFA sub-aligners (forward, reverse: score-only; base: CIGAR),
      * created on first use (src/aligner.rs:2721-2740) */
@@ -346,10 +349,16 @@ wfo_aligner* wfo_aligner_new(const wfa_attr_t* attr, int32_t* err) {
         free(a);
         return NULL;
     }
+    /* Z-drop with a medium/low-memory CIGAR: Compat::C backtraces the best
+     * cell at the current score, reading whatever backtrace chain that
+     * wavefront holds there (HPCBio/wfa2lib-rs#22, src/aligner.rs:1443-1460):
+     * not restated. BiWFA rejects X-drop and Z-drop as the driver does. */
+    const int zdrop = attr->heuristic == WFA_HEURISTIC_ZDROP;
     if ((attr->span == WFA_SPAN_ENDSFREE && a->pen.match != 0) ||
-        attr->heuristic == WFA_HEURISTIC_ZDROP ||
+        (zdrop && attr->alignment_scope == WFA_SCOPE_ALIGNMENT &&
+         (attr->memory_mode == WFA_MEMORY_MED || attr->memory_mode == WFA_MEMORY_LOW)) ||
         (attr->memory_mode == WFA_MEMORY_ULTRALOW &&
-         (attr->span != WFA_SPAN_END2END || attr->heuristic == WFA_HEURISTIC_XDROP))) {
+         (attr->span != WFA_SPAN_END2END || attr->heuristic == WFA_HEURISTIC_XDROP || zdrop))) {
         if (err) *err = WFA_ERR_UNSUPPORTED;
         free(a);
         return NULL;
@@ -564,6 +573,9 @@ static void heuristic_reset(wfo_aligner* a) {
         a->h_steps_wait = a->attr.heuristic_p[1];
     a->h_max_sw_score = 0;
     a->h_max_sw_k = DIAG_NULL;
+    a->h_max_sw_off = OFFSET_NULL;
+    a->h_max_wf_score = 0;
+    a->h_zlast = 0;
 }
 /* wf_distance_end2end (src/heuristic.rs:150-160) */
 static inline int32_t wf_distance(int32_t off, int k, int plen, int tlen) {
@@ -579,18 +591,19 @@ static int32_t* dist_at(wfo_aligner* a, const wf_t* w) {
     }
     return a->dist;
 }
-/* heuristic_cutoff (src/heuristic.rs:167-255) without Z-drop; narrows
- * m->lo/hi only (cut cells keep their offsets and stay readable). */
-static void heuristic_cutoff(wfo_aligner* a, wf_t* wf, int score) {
-    if (wf->lo > wf->hi) return;
+/* heuristic_cutoff (src/heuristic.rs:167-255); narrows m->lo/hi only (cut
+ * cells keep their offsets and stay readable). Returns 1 when Z-drop ends
+ * the alignment. */
+static int heuristic_cutoff(wfo_aligner* a, wf_t* wf, int score) {
+    if (wf->lo > wf->hi) return 0;
     a->h_steps_wait -= 1;
     const int plen = a->plen, tlen = a->tlen;
     const int* hp = a->attr.heuristic_p;
     switch (a->attr.heuristic) {
     case WFA_HEURISTIC_WFADAPTIVE: { /* src/heuristic.rs:258-346 */
-        if (a->h_steps_wait > 0) return;
+        if (a->h_steps_wait > 0) return 0;
         int lo = wf->lo, hi = wf->hi;
-        if (hi - lo + 1 < hp[0]) return;
+        if (hi - lo + 1 < hp[0]) return 0;
         int32_t* d = dist_at(a, wf);
         int min_d = plen > tlen ? plen : tlen;
         for (int k = lo; k <= hi; ++k) {
@@ -614,16 +627,16 @@ static void heuristic_cutoff(wfo_aligner* a, wf_t* wf, int score) {
         }
         wf->hi = hi_r;
         a->h_steps_wait = hp[2];
-        return;
+        return 0;
     }
     case WFA_HEURISTIC_BANDED_STATIC: /* src/heuristic.rs:400-407 */
         if (wf->lo < hp[0]) wf->lo = hp[0];
         if (wf->hi > hp[1]) wf->hi = hp[1];
-        return;
+        return 0;
     case WFA_HEURISTIC_BANDED_ADAPTIVE: { /* src/heuristic.rs:355-394 */
-        if (a->h_steps_wait > 0) return;
+        if (a->h_steps_wait > 0) return 0;
         int lo = wf->lo, hi = wf->hi, len = hi - lo + 1;
-        if (len < 4) return;
+        if (len < 4) return 0;
         int maxlen = hp[1] - hp[0] + 1;
         if (len > maxlen) {
             int leeway = (len - maxlen) / 2, q = len / 4;
@@ -639,10 +652,10 @@ static void heuristic_cutoff(wfo_aligner* a, wf_t* wf, int score) {
             wf->hi = nhi < hi ? nhi : hi;
         }
         a->h_steps_wait = hp[2];
-        return;
+        return 0;
     }
     case WFA_HEURISTIC_XDROP: { /* src/heuristic.rs:410-448, 491-525 */
-        if (a->h_steps_wait > 0) return;
+        if (a->h_steps_wait > 0) return 0;
         const int swg_match = a->pen.match != 0 ? -a->pen.match : 1;
         const int bfree = a->pen.match == 0 && a->endsfree;
         const int pbf = bfree ? a->pb_free : 0, tbf = bfree ? a->tb_free : 0;
@@ -675,21 +688,58 @@ static void heuristic_cutoff(wfo_aligner* a, wf_t* wf, int score) {
             a->h_max_sw_k = cmax_k;
         }
         a->h_steps_wait = hp[1];
-        return;
+        return 0;
+    }
+    case WFA_HEURISTIC_ZDROP: { /* zdrop_cutoff + compute_sw_scores, src/heuristic.rs:449-525 */
+        if (a->h_steps_wait > 0) return 0;
+        const int swg_match = a->pen.match != 0 ? -a->pen.match : 1;
+        const int bfree = a->pen.match == 0 && a->endsfree;
+        const int pbf = bfree ? a->pb_free : 0, tbf = bfree ? a->tb_free : 0;
+        /* the wavefront's best cell: the first k with the largest score */
+        int cmax = INT32_MIN, cmax_k = 0, cmax_off = 0;
+        for (int k = wf->lo; k <= wf->hi; ++k) {
+            int32_t off = wf_get(wf, k);
+            if (off < 0) continue;
+            int fv = k < 0 ? ((-k) < pbf ? -k : pbf) : 0;
+            int fh = k > 0 ? (k < tbf ? k : tbf) : 0;
+            int sws = (swg_match * ((off - k - fv) + (off - fh)) - score) / 2;
+            if (cmax < sws) { cmax = sws; cmax_k = k; cmax_off = off; }
+        }
+        if (a->h_max_sw_k == DIAG_NULL || cmax > a->h_max_sw_score) {
+            /* a new best: its cell and the score it was reached at */
+            a->h_max_sw_score = cmax;
+            a->h_max_wf_score = score;
+            a->h_max_sw_k = cmax_k;
+            a->h_max_sw_off = cmax_off;
+            a->h_zlast = score;
+        } else if ((int64_t)a->h_max_sw_score - cmax <= hp[0]) {
+            a->h_zlast = score;
+        } else if (score - a->h_zlast >= a->scope - 1) {
+            /* dropped for at least the largest single penalty */
+            return 1;
+        }
+        a->h_steps_wait = hp[1];
+        return 0;
     }
     default:
-        return;
+        return 0;
     }
 }
 
-/* apply_heuristic_cutoff (src/aligner.rs:1644-1766), without Z-drop. */
-static void apply_heuristic_cutoff(wfo_aligner* a, int score, wf_t* wf) {
+/* apply_heuristic_cutoff (src/aligner.rs:1644-1766). Returns 1 when Z-drop
+ * ends the alignment: the end position is then the best cell seen. */
+static int apply_heuristic_cutoff(wfo_aligner* a, int score, wf_t* wf) {
     if (a->attr.heuristic == WFA_HEURISTIC_BANDED_STATIC &&
         wf->lo >= a->attr.heuristic_p[0] && wf->hi <= a->attr.heuristic_p[1])
-        return;
+        return 0;
     int lo_base = wf->lo, hi_base = wf->hi;
-    heuristic_cutoff(a, wf, score);
-    if (lo_base > hi_base || (wf->lo == lo_base && wf->hi == hi_base)) return;
+    if (heuristic_cutoff(a, wf, score)) {
+        a->end_score = a->h_max_wf_score;
+        a->end_k = a->h_max_sw_k;
+        a->end_offset = a->h_max_sw_off;
+        return 1;
+    }
+    if (lo_base > hi_base || (wf->lo == lo_base && wf->hi == hi_base)) return 0;
     if (wf->lo > wf->hi) {
         wf->null = 1;
         wf->init_min = wf->lo; wf->init_max = wf->hi; /* clear_init_range */
@@ -702,6 +752,7 @@ static void apply_heuristic_cutoff(wfo_aligner* a, int score, wf_t* wf) {
         if (wf->hi < d->hi) d->hi = wf->hi;
         if (d->lo > d->hi) { d->null = 1; d->init_min = d->lo; d->init_max = d->hi; }
     }
+    return 0;
 }
 
 /* extend_end2end (src/aligner.rs:1004-1049) */
@@ -725,7 +776,10 @@ static int extend_end2end(wfo_aligner* a, int score) {
         a->status = WFA_STATUS_END_REACHED;
         return 1;
     }
-    if (a->attr.heuristic != WFA_HEURISTIC_NONE) apply_heuristic_cutoff(a, score, m);
+    if (a->attr.heuristic != WFA_HEURISTIC_NONE && apply_heuristic_cutoff(a, score, m)) {
+        a->status = WFA_STATUS_END_UNREACHABLE; /* Z-dropped */
+        return 1;
+    }
     return 0;
 }
 
@@ -761,7 +815,10 @@ static int extend_endsfree(wfo_aligner* a, int score) {
             }
         }
     }
-    if (a->attr.heuristic != WFA_HEURISTIC_NONE) apply_heuristic_cutoff(a, score, m);
+    if (a->attr.heuristic != WFA_HEURISTIC_NONE && apply_heuristic_cutoff(a, score, m)) {
+        a->status = WFA_STATUS_END_UNREACHABLE; /* Z-dropped */
+        return 1;
+    }
     return 0;
 }
 
@@ -1360,6 +1417,14 @@ static void backtrace_from(wfo_aligner* a, int score, int k, int offset) {
         backtrace_linear(a, score, k, offset);
 }
 
+/* backtrace_zdrop_end (src/aligner.rs:1393-1406), Compat::C: the best cell,
+ * backtraced from the score it was found at (high memory; Z-drop with a
+ * medium/low-memory CIGAR is refused at construction). */
+static void backtrace_zdrop_end(wfo_aligner* a, int score) {
+    const int from = !a->piggyback && a->end_score >= 0 ? a->end_score : score;
+    backtrace_from(a, from, a->end_k, a->end_offset);
+}
+
 /* ------------------------------------------------------------------ */
 /* align_end2end (src/aligner.rs:620-698)                              */
 /* ------------------------------------------------------------------ */
@@ -1433,7 +1498,33 @@ static int maxtrim(const wfo_aligner* a, const uint8_t* ops, int b, int e, int* 
     return max_off + 1;
 }
 
-/* maxtrim_cigar (src/aligner.rs:1545-1583) */
+/* cigar_wfa_score (src/aligner.rs:2700-2719) of ops[0, n): their classic
+ * score on the user penalties, in the WFA scale over the bases they span. */
+static int wf_score_ops(const wfo_aligner* a, const uint8_t* ops, int n) {
+    const wfa_penalties_t* u = &a->pen.user;
+    int v = 0, h = 0, classic;
+    for (int i = 0; i < n; ++i) { v += ops[i] != 'I'; h += ops[i] != 'D'; }
+    switch (a->pen.metric) {
+    case WFA_METRIC_INDEL:
+    case WFA_METRIC_EDIT: return wfo_cigar_score_edit(ops, n);
+    case WFA_METRIC_GAP_LINEAR:
+        classic = wfo_cigar_score_gap_linear(ops, n, u->match, u->mismatch, u->gap_opening1);
+        break;
+    case WFA_METRIC_GAP_AFFINE:
+        classic = wfo_cigar_score_gap_affine(ops, n, u->match, u->mismatch, u->gap_opening1,
+                                             u->gap_extension1);
+        break;
+    default:
+        classic = wfo_cigar_score_gap_affine2p(ops, n, u->match, u->mismatch, u->gap_opening1,
+                                               u->gap_extension1, u->gap_opening2,
+                                               u->gap_extension2);
+        break;
+    }
+    return a->pen.match < 0 ? -a->pen.match * (v + h) - 2 * classic : -classic;
+}
+
+/* maxtrim_cigar (src/aligner.rs:1545-1583): the kept CIGAR's score is its
+ * WFA score, a free leading gap excluded. */
 static int maxtrim_cigar(wfo_aligner* a) {
     if (a->pen.metric == WFA_METRIC_EDIT || a->pen.metric == WFA_METRIC_INDEL) return 0;
     int begin_v = 0, begin_h = 0;
@@ -1461,6 +1552,7 @@ static int maxtrim_cigar(wfo_aligner* a) {
         return trimmed;
     }
     a->cig_end = ne;
+    a->cig_score = wf_score_ops(a, a->cig + a->cig_begin, ne - a->cig_begin);
     a->cig_begin -= begin_v + begin_h;
     return trimmed;
 }
@@ -1524,7 +1616,8 @@ int32_t wfo_align_endsfree(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
         int unreachable = a->status == WFA_STATUS_END_UNREACHABLE;
         if (a->attr.alignment_scope == WFA_SCOPE_ALIGNMENT &&
             (a->status == WFA_STATUS_END_REACHED || a->end_offset != OFFSET_NULL)) {
-            backtrace_from(a, score, a->end_k, a->end_offset);
+            if (unreachable) backtrace_zdrop_end(a, score);
+            else backtrace_from(a, score, a->end_k, a->end_offset);
             if (a->attr.extension || unreachable) {
                 int trimmed = maxtrim_cigar(a);
                 a->status = (trimmed || unreachable) ? WFA_STATUS_PARTIAL : WFA_STATUS_COMPLETED;
@@ -1564,8 +1657,22 @@ int32_t wfo_align_end2end(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
             result = rescore_heuristic_cigar(a, score, a->plen, a->tlen);
         }
         a->status = WFA_STATUS_COMPLETED;
+    } else if (a->status == WFA_STATUS_END_UNREACHABLE && a->end_offset != OFFSET_NULL) {
+        /* Z-dropped: the partial alignment up to the best cell, max-trimmed
+         * (src/aligner.rs:683-694) */
+        if (a->attr.alignment_scope == WFA_SCOPE_ALIGNMENT) {
+            backtrace_zdrop_end(a, score);
+            maxtrim_cigar(a);
+        }
+        a->status = WFA_STATUS_PARTIAL;
     }
     return result;
+}
+
+void wfo_end_position(const wfo_aligner* a, int32_t* out) {
+    out[0] = a->end_score;
+    out[1] = a->end_k;
+    out[2] = a->end_offset;
 }
 
 /* ------------------------------------------------------------------ */

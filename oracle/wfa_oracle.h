@@ -48,6 +48,9 @@ int32_t wfo_align_biwfa(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
 int32_t wfo_align_biwfa_score(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
                               const uint8_t* text, int32_t tlen);
 int32_t wfo_status(const wfo_aligner* a);
+/* alignment_end_pos (src/aligner.rs:146-204) after the last alignment:
+ * {score, k, offset}; a Z-drop end is the best cell seen */
+void wfo_end_position(const wfo_aligner* a, int32_t* out);
 int32_t wfo_cigar(const wfo_aligner* a, const uint8_t** ops);
 int32_t wfo_cigar_score(const wfo_aligner* a);
 

@@ -9,8 +9,8 @@ import pytest
 
 from conftest import expand, have_gpu, load_vectors
 
-HEUR_NAMES = {"wfadaptive": "wfa-adaptive", "xdrop": "xdrop", "banded_static": "banded-static",
-              "banded_adaptive": "banded-adaptive"}
+HEUR_NAMES = {"wfadaptive": "wfa-adaptive", "xdrop": "xdrop", "zdrop": "zdrop",
+              "banded_static": "banded-static", "banded_adaptive": "banded-adaptive"}
 
 
 def _args(vec):
@@ -74,6 +74,8 @@ def test_cli_output_lines(args, vecs, pkg, tmp_path):
     (["--wfa-span", "ends-free,1,2"], "Invalid --wfa-span"),
     (["--wfa-memory", "huge"], "--wfa-memory"),
     (["--no-c-compat"], "not available on the device path"),
+    (["--wfa-heuristic", "zdrop", "--wfa-heuristic-parameters", "20,1", "--wfa-memory", "med"],
+     "medium/low-memory CIGAR"),
     (["-g", "1,4,6,2"], "invalid penalties"),
     (["-a", "gap-linear-wfa", "-p", "0,4,0"], "invalid penalties"),
 ])

@@ -219,6 +219,8 @@ HEURISTIC_CASES = [
     ("banded_static_m12_9", (4, -12, 9, 0)),
     ("banded_adaptive_m50_50_1", (5, -50, 50, 1)),
     ("banded_adaptive_m6_7_1", (5, -6, 7, 1)),
+    ("zdrop_5_1", (3, 5, 1, 0)),
+    ("zdrop_20_2", (3, 20, 2, 0)),
 ]
 HEUR_SPANS = [("end2end", None), ("extension", "extension"), ("endsfree_5_5", [5, 5, 5, 5])]
 
@@ -242,7 +244,10 @@ def test_heuristics_match_oracle(hname, h, sname, span, metric, pen, memory, pkg
     if span is not None and pen[0] != 0:
         pytest.skip("ends-free on the device needs match == 0")
     pairs = _heuristic_pairs()
-    for scope in ("score", "alignment"):
+    # Z-drop with a medium-memory CIGAR is refused (score only runs)
+    scopes = ("score",) if h[0] == N.HEURISTIC_ZDROP and memory == "med" else \
+        ("score", "alignment")
+    for scope in scopes:
         attr = attr_for(N, metric, pen, scope, span=span)
         attr.memory_mode = N.MEMORY_HIGH if memory == "high" else N.MEMORY_MED
         attr.heuristic = h[0]
@@ -282,6 +287,69 @@ def test_wide_kernel_several_waves_per_pair(metric, pen, span_name, span, memory
         _compare(engine, oracle, attr, asc, True, label=f"wide/{metric}/ascii")
 
 
+ZDROP_CONFIGS = [c for c in CONFIGS if c[0] != "indel"]
+ZDROP_SPANS = [("end2end", None), ("extension", "extension"), ("endsfree", [20, 20, 20, 20])]
+
+
+@pytest.mark.parametrize("metric,pen", ZDROP_CONFIGS)
+@pytest.mark.parametrize("span_name,span", ZDROP_SPANS)
+def test_zdrop_end_positions_match_oracle(metric, pen, span_name, span, pkg, N, engine,
+                                          oracle):
+    """Z-drop (score only and high-memory CIGARs): scores, statuses, CIGARs
+    and each pair's end position (the best cell and the score it was found
+    at) equal the oracle's, on pairs whose unrelated tails make it fire."""
+    import torch
+    from wfa2lib_rs_amd.engine import DeviceBatch
+    from wfa2lib_rs_amd.synth import Workload, generate
+    if span is not None and pen[0] != 0:
+        pytest.skip("ends-free on the device needs match == 0")
+    w = Workload("zdrop_mix", 80, 1, 60, 600, 0.0, 0.35, 3.0, run_frac=0.2, run_mean=30.0,
+                 seq_format=N.SEQ_ASCII, seed=23)
+    pairs = generate(w)
+    fired = 0
+    for scope in ("score", "alignment"):
+        attr = attr_for(N, metric, pen, scope, span=span)
+        attr.heuristic = N.HEURISTIC_ZDROP
+        attr.heuristic_p[0], attr.heuristic_p[1] = 5, 1
+        b = DeviceBatch(pairs, device="cuda:0", cigar=scope == "alignment", end_position=True)
+        engine.align(attr, b)
+        torch.cuda.synchronize()
+        s, st, c = b.results()
+        ends = b.end_positions()
+        A = oracle.OracleAligner(attr)
+        for i in range(len(pairs)):
+            os_, ost, oc = A.align(pairs.pattern(i), pairs.text(i))
+            got = (int(s[i]), int(st[i]), c[i] if c is not None else b"", tuple(ends[i]))
+            want = (os_, ost, oc if c is not None else b"", A.end_position())
+            assert got == want, f"{metric}/{span_name}/{scope} pair {i}: {got[:2]} {want[:2]}"
+            fired += ost == STATUS["PARTIAL"]
+    if metric != "edit" and pen[1] >= 4 and span is None:
+        assert fired >= 10, "the pairs must make Z-drop fire"
+
+
+def test_zdrop_issue22_on_gpu(pkg, N, engine):
+    """tests/corrected_zdrop.rs:77-89 through the aligner API: Z-drop 50,10
+    with (-3,5,3,2), high memory: a valid partial alignment scoring 4498;
+    the CIGAR's score is its WFA score and the end position is the best
+    cell, found before the score the alignment stopped at."""
+    from test_oracle import _issue22_pair, valid_prefix
+    p, t = _issue22_pair()
+    pen = pkg.AffinePenalties(-3, 5, 3, 2)
+    A = pkg.AffineAligner(pkg.WavefrontPenalties.new_affine(pen))
+    A.set_alignment_scope(pkg.AlignmentScope.COMPUTE_ALIGNMENT)
+    A.set_heuristic(pkg.HeuristicStrategy.zdrop(50, 10))
+    s = A.align_end2end(p, t)
+    assert A.status() == pkg.AlignStatus.PARTIAL
+    cig = A.cigar()
+    assert valid_prefix(cig.operations, p, t)
+    assert cig.score_gap_affine(pen) == 4498
+    v = sum(op != ord("I") for op in cig.operations)
+    h = sum(op != ord("D") for op in cig.operations)
+    assert cig.score == 3 * (v + h) - 2 * 4498
+    pos = A.alignment_end_pos
+    assert 0 <= pos.score < s and pos.offset - pos.k >= v and pos.offset >= h
+
+
 def test_per_pair_configs_match_oracle(pkg, N, engine, oracle):
     """A batch may carry each pair's span and heuristic (the serving loop
     batches mixed requests that way): end-to-end, ends-free with per-pair
@@ -292,7 +360,7 @@ def test_per_pair_configs_match_oracle(pkg, N, engine, oracle):
     from wfa2lib_rs_amd.synth import Workload, generate
     import torch
     heuristics = [None, ("wfadaptive", 10, 50, 1), ("xdrop", 200, 1),
-                  ("banded_adaptive", -50, 50, 1), ("banded_static", -20, 40)]
+                  ("banded_adaptive", -50, 50, 1), ("banded_static", -20, 40), ("zdrop", 15, 1)]
     for w, use_heur in ((Workload("pc_short", 60, 0, 150, 150, 0.01, 0.10, 2.0), True),
                         (Workload("pc_long", 12, 2, 1000, 3000, 0.01, 0.10, 2.0), True),
                         (Workload("pc_long_nh", 12, 2, 1000, 3000, 0.01, 0.10, 2.0), False)):
@@ -305,7 +373,7 @@ def test_per_pair_configs_match_oracle(pkg, N, engine, oracle):
             a = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment", span=span)
             h = heuristics[(i // 3) % len(heuristics)] if use_heur else None
             if h is not None:
-                a.heuristic = {"wfadaptive": 1, "xdrop": 2, "banded_static": 4,
+                a.heuristic = {"wfadaptive": 1, "xdrop": 2, "zdrop": 3, "banded_static": 4,
                                "banded_adaptive": 5}[h[0]]
                 for j, v in enumerate(h[1:]):
                     a.heuristic_p[j] = v

@@ -75,8 +75,13 @@ def test_serving_mix_matches_oracle_and_alone(pkg, N, server, oracle):
         attrs[i].alignment_scope = N.SCOPE_SCORE
     for i in range(3, len(attrs), 11):
         attrs[i].memory_mode = N.MEMORY_MED
+    # Z-drop requests (high memory or score only) share the heuristic launches
+    for i in range(2, len(attrs), 17):
+        if attrs[i].memory_mode == N.MEMORY_HIGH:
+            attrs[i].heuristic = N.HEURISTIC_ZDROP
+            attrs[i].heuristic_p[0], attrs[i].heuristic_p[1] = 10, 1
     for i in range(5, len(attrs), 13):
-        if attrs[i].heuristic != N.HEURISTIC_XDROP:
+        if attrs[i].heuristic not in (N.HEURISTIC_XDROP, N.HEURISTIC_ZDROP):
             attrs[i].memory_mode = N.MEMORY_ULTRALOW
             attrs[i].span, attrs[i].extension = N.SPAN_END2END, 0
             attrs[i].pattern_begin_free = attrs[i].pattern_end_free = 0
@@ -129,8 +134,10 @@ def test_serving_rejects_invalid_requests(pkg, N, server):
     bad = attr_for(N, "gap_affine", (1, 4, 6, 2, 0, 0), "alignment")  # match > 0
     with pytest.raises(RuntimeError, match="invalid penalties"):
         server.submit(bad, b"ACGT", b"ACGT", tag=1)
+    # Z-drop with a medium-memory CIGAR is not on the device path
     z = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment")
     z.heuristic = N.HEURISTIC_ZDROP
+    z.memory_mode = N.MEMORY_MED
     with pytest.raises(RuntimeError, match="unsupported"):
         server.submit(z, b"ACGT", b"ACGT", tag=2)
     # an empty pair is served

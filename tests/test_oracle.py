@@ -1,9 +1,11 @@
 """CPU tests: the oracle pinned against the reference's literal vectors, an
 exact Gotoh DP, CIGAR validity/rescoring invariants, and the generator."""
+import os
+
 import numpy as np
 import pytest
 
-from conftest import attr_for, attr_for_vec, expand, load_vectors
+from conftest import GOLDEN, attr_for, attr_for_vec, expand, load_vectors
 
 STATUS = {"OK": 0, "END_REACHED": 1, "END_UNREACHABLE": 2, "MAX_STEPS_REACHED": 3, "OOM": 4,
           "COMPLETED": 5, "PARTIAL": 6}
@@ -242,3 +244,108 @@ def test_oracle_indel_literal_med_low(pkg, N, oracle):
         a.memory_mode = mem
         s, st, ops = oracle.OracleAligner(a).align_end2end(b"A" * 1500, b"C" * 1500)
         assert (s, st, pkg.Cigar(ops).to_string_rle(True)) == (3000, 5, "1500I1500D")
+
+
+def _issue22_pair():
+    """tests/data/issue22_zdrop.seq (transcribed to tests/golden/)."""
+    with open(os.path.join(GOLDEN, "issue22_zdrop.seq")) as f:
+        lines = f.read().split("\n")
+    return lines[0].lstrip(">").encode(), lines[1].lstrip("<").encode()
+
+
+def valid_prefix(ops, p, t):
+    """tests/corrected_zdrop.rs:56-75: every M a match and every X a
+    mismatch, within both sequences (trailing bases may be left)."""
+    i = j = 0
+    for op in ops:
+        if op in b"MX":
+            if i >= len(p) or j >= len(t) or (p[i] == t[j]) != (op == ord("M")):
+                return False
+            i, j = i + 1, j + 1
+        elif op == ord("D"):
+            i += 1
+        elif op == ord("I"):
+            j += 1
+        else:
+            return False
+    return i <= len(p) and j <= len(t)
+
+
+def _zdrop_attr(N, metric, pen, scope, zdrop, steps, span=None):
+    a = attr_for(N, metric, pen, scope, span=span)
+    a.heuristic = N.HEURISTIC_ZDROP
+    a.heuristic_p[0], a.heuristic_p[1] = zdrop, steps
+    return a
+
+
+def test_oracle_zdrop_issue22_literal(pkg, N, oracle):
+    """tests/corrected_zdrop.rs:77-89 (Compat::C, high memory): Z-drop 50,10
+    with (-3,5,3,2) ends early in a valid partial alignment whose gap-affine
+    score is 4498; score-only ends at the same score with the same status."""
+    p, t = _issue22_pair()
+    pen = (-3, 5, 3, 2, 0, 0)
+    A = oracle.OracleAligner(_zdrop_attr(N, "gap_affine", pen, "alignment", 50, 10))
+    s, st, ops = A.align_end2end(p, t)
+    assert st == STATUS["PARTIAL"] and ops
+    assert valid_prefix(ops, p, t)
+    assert pkg.Cigar(ops).score_gap_affine(pkg.AffinePenalties(-3, 5, 3, 2)) == 4498
+    best = A.end_position()
+    assert 0 <= best[0] < s and 0 <= best[2] - best[1] <= len(p) and 0 <= best[2] <= len(t)
+    S = oracle.OracleAligner(_zdrop_attr(N, "gap_affine", pen, "score", 50, 10))
+    assert S.align_end2end(p, t)[:2] == (s, st)
+    assert S.end_position() == best
+
+
+@pytest.mark.parametrize("metric,pen", [m for m in PENALTY_SETS if m[0] != "indel"])
+def test_oracle_zdrop_invariants(metric, pen, pkg, N, oracle):
+    """A Z-drop that never fires changes nothing; one that fires leaves a
+    valid partial alignment whose end is the recorded best cell (or, max
+    trimmed, a prefix of the path to it), and score-only agrees on the score,
+    status and end position. (With a mismatch of 1, or a gap extension of 1
+    and no opening, the Smith-Waterman score of unrelated sequence rises, so
+    Z-drop cannot fire: those sets only check the first property.)"""
+    from wfa2lib_rs_amd.synth import Workload, generate
+    import wfa2lib_rs_amd._native as Nn
+    w = Workload("zdrop_mix", 60, 1, 60, 600, 0.0, 0.35, 3.0, run_frac=0.2, run_mean=30.0,
+                 seq_format=Nn.SEQ_ASCII, seed=23)
+    hp = generate(w)
+    plain = oracle.OracleAligner(attr_for(N, metric, pen, "alignment"))
+    never = oracle.OracleAligner(_zdrop_attr(N, metric, pen, "alignment", 10 ** 8, 1))
+    zd = oracle.OracleAligner(_zdrop_attr(N, metric, pen, "alignment", 5, 1))
+    zs = oracle.OracleAligner(_zdrop_attr(N, metric, pen, "score", 5, 1))
+    fired = 0
+    for i in range(len(hp)):
+        p, t = hp.pattern(i), hp.text(i)
+        assert never.align_end2end(p, t) == plain.align_end2end(p, t)
+        s, st, ops = zd.align_end2end(p, t)
+        assert zs.align_end2end(p, t)[:2] == (s, st)
+        assert zs.end_position() == zd.end_position()
+        if st == STATUS["PARTIAL"]:
+            fired += 1
+            assert valid_prefix(ops, p, t)
+            bs, bk, boff = zd.end_position()
+            assert bs < s
+            v = sum(op != ord("I") for op in ops)
+            h = sum(op != ord("D") for op in ops)
+            assert v <= boff - bk and h <= boff
+            if ops and metric != "edit":
+                # max-trimmed: the kept part's WFA score (tests/heuristic_scores.rs:124-127)
+                assert zd.cigar_score() == _cigar_wf_score(pkg, metric, pen, pkg.Cigar(ops), v, h)
+        else:
+            assert st == STATUS["COMPLETED"]
+    if metric != "edit" and pen[1] >= 4:
+        assert fired >= 5, "the pairs must make Z-drop fire"
+
+
+def test_oracle_zdrop_refusals(N, oracle):
+    """Z-drop with a medium/low-memory CIGAR (the reference's known issue
+    HPCBio/wfa2lib-rs#22) and with BiWFA is refused; score-only medium
+    memory runs."""
+    for mem in (N.MEMORY_MED, N.MEMORY_LOW, N.MEMORY_ULTRALOW):
+        a = _zdrop_attr(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment", 20, 1)
+        a.memory_mode = mem
+        with pytest.raises(ValueError):
+            oracle.OracleAligner(a)
+    a = _zdrop_attr(N, "gap_affine", (0, 4, 6, 2, 0, 0), "score", 20, 1)
+    a.memory_mode = N.MEMORY_MED
+    oracle.OracleAligner(a)

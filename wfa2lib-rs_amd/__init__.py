@@ -9,12 +9,12 @@ from . import _native
 from .aligner import (Affine2pAligner, Affine2pPenalties, AffineAligner, AffinePenalties,
                       AlignmentScope, AlignmentSpan, AlignStatus, Cigar, DistanceMetric,
                       EditAligner, HeuristicStrategy, LinearPenalties, MemoryMode,
-                      WavefrontAligner, WavefrontPenalties, compute_output_score,
-                      encode_packed2bits)
+                      WavefrontAligner, WavefrontPenalties, WavefrontPos,
+                      compute_output_score, encode_packed2bits)
 
 __all__ = [
     "Affine2pAligner", "Affine2pPenalties", "AffineAligner", "AffinePenalties", "AlignmentScope",
     "AlignmentSpan", "AlignStatus", "Cigar", "DistanceMetric", "EditAligner", "HeuristicStrategy",
-    "LinearPenalties", "MemoryMode", "WavefrontAligner", "WavefrontPenalties",
+    "LinearPenalties", "MemoryMode", "WavefrontAligner", "WavefrontPenalties", "WavefrontPos",
     "compute_output_score", "encode_packed2bits",
 ]

@@ -53,7 +53,7 @@ class Batch(ctypes.Structure):
         ("text_length", ctypes.c_void_p), ("score", ctypes.c_void_p),
         ("status", ctypes.c_void_p), ("cigar_ops", ctypes.c_void_p),
         ("cigar_offset", ctypes.c_void_p), ("cigar_length", ctypes.c_void_p),
-        ("pair_config", ctypes.c_void_p),
+        ("pair_config", ctypes.c_void_p), ("end_position", ctypes.c_void_p),
     ]
 
 
@@ -105,6 +105,7 @@ SIGNATURES = {
     "wfa_aligner_status": (_I32, [_VP]),
     "wfa_aligner_cigar": (_I32, [_VP, _P(_P(ctypes.c_uint8))]),
     "wfa_aligner_cigar_score": (_I32, [_VP]),
+    "wfa_aligner_end_position": (None, [_VP, _VP]),
     "wfa_aligner_error": (_I32, [_VP]),
     "wfa_engine_new": (_VP, [_I32, _P(_I32)]),
     "wfa_engine_free": (None, [_VP]),

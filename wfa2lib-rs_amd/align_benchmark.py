@@ -15,8 +15,8 @@ Input: lines alternating `>pattern` and `<text` (prefixes optional; an
 unpaired last line is ignored). Output: one `score<TAB>CIGAR` line per pair
 (`-` for no CIGAR), exactly as the reference prints them; `Time.Alignment`
 (device time of the batches) and, with -c, `Alignments.Correct` on stderr.
-Configurations the device path does not run (Z-drop, --no-c-compat) exit
-with an error instead of falling back to a CPU.
+Configurations the device path does not run (Z-drop with a medium/low-memory
+CIGAR, --no-c-compat) exit with an error instead of falling back to a CPU.
 """
 import argparse
 import ctypes
@@ -38,7 +38,6 @@ else:
                           LinearPenalties, WavefrontPenalties, compute_output_score)
 
 INT32_MIN = -2 ** 31
-OFFSET_NULL = INT32_MIN // 2
 
 
 def fail(msg):
@@ -203,8 +202,9 @@ def main(argv=None, out=None):
         fail("[WFA] BiWFA is not compatible with X-drop/Z-drop heuristics")
     if is_drop and penalties.distance_metric in (DistanceMetric.EDIT, DistanceMetric.INDEL):
         fail("[WFA] Heuristics drops are not compatible with 'edit'/'indel' distance metrics")
-    if heur is not None and heur[0] == N.HEURISTIC_ZDROP:
-        fail("[WFA] Z-drop is not available on the device path")
+    if (heur is not None and heur[0] == N.HEURISTIC_ZDROP and not score_only and
+            memory in (N.MEMORY_MED, N.MEMORY_LOW)):
+        fail("[WFA] Z-drop with a medium/low-memory CIGAR is not available on the device path")
     if a.no_c_compat:
         fail("[WFA] --no-c-compat (Compat::Corrected) is not available on the device path")
 
@@ -257,13 +257,15 @@ def main(argv=None, out=None):
                                nominal_prop(len(t), span[1][2]), nominal_prop(len(t), span[1][3])]
             cfg[:, 6] = attr.heuristic
             cfg[:, 7:10] = list(attr.heuristic_p)
-        b = DeviceBatch(hp, device=dev, cigar=not score_only, pair_config=cfg)
+        b = DeviceBatch(hp, device=dev, cigar=not score_only, pair_config=cfg,
+                        end_position=not use_biwfa)
         torch.cuda.synchronize()
         t0 = time.perf_counter_ns()
         eng.align(attr, b)
         torch.cuda.synchronize()
         align_ns += time.perf_counter_ns() - t0
         scores, status, cigars = b.results()
+        ends = b.end_positions() if not use_biwfa else None
         lines = []
         for i in range(n):
             p, t = batch_pairs[i]
@@ -272,11 +274,16 @@ def main(argv=None, out=None):
             cig = Cigar(cigars[i] if cigars is not None else b"")
             biwfa_failed = use_biwfa and score_only and st != N.STATUS_COMPLETED
             # the end position the driver scores a score-only result at
-            # (src/bin/align_benchmark.rs:376-391): (|P|, |T|) when the end
-            # was reached; else the unset position, whose v + h wraps to 1
+            # (src/bin/align_benchmark.rs:376-391, 630-641): (|P|, |T|) when
+            # the end was reached; else the aligner's end position (a
+            # Z-drop's best cell, at the score it was found at, or the unset
+            # position, whose v = offset - k wraps)
             if not use_biwfa and st in (N.STATUS_PARTIAL, N.STATUS_END_UNREACHABLE):
-                v = (OFFSET_NULL - (2 ** 31 - 1) + 2 ** 31) % 2 ** 32 - 2 ** 31
-                end_v, end_h = v, OFFSET_NULL
+                ps, pk, poff = (int(x) for x in ends[i])
+                end_v = (poff - pk + 2 ** 31) % 2 ** 32 - 2 ** 31
+                end_h = poff
+                if ps >= 0:
+                    wfa_score = ps
             else:
                 end_v, end_h = len(p), len(t)
             if biwfa_failed:

@@ -373,6 +373,23 @@ class WavefrontAligner:
     def cigar(self) -> Cigar:
         return self._cigar
 
+    @property
+    def alignment_end_pos(self) -> "WavefrontPos":
+        """alignment_end_pos (src/aligner.rs:146-204): the end cell, or a
+        Z-drop's best cell and the score it was found at; unset
+        (-1, INT32_MAX, OFFSET_NULL) otherwise."""
+        out = (ctypes.c_int32 * 3)()
+        N.lib().wfa_aligner_end_position(self._h, out)
+        return WavefrontPos(out[0], out[1], out[2])
+
+
+@dataclass(frozen=True)
+class WavefrontPos:
+    """WavefrontPos (src/wavefront.rs:24-29)."""
+    score: int
+    k: int
+    offset: int
+
 
 EditAligner = WavefrontAligner
 AffineAligner = WavefrontAligner

@@ -725,16 +725,81 @@ static __device__ __forceinline__ HeurCfg launch_heuristic(const KParams& p) {
 struct HeurState {
   int steps_wait, max_sw;
   bool have_max;
+  // Z-drop: the best cell (k, offset), the score it was reached at, and the
+  // last score with no drop
+  int best_k, best_off, max_wf, zlast;
   // HeuristicState::reset (src/heuristic.rs:122-143)
   __device__ void reset(const HeurCfg& hc) {
     const int heur = hc.kind;
     steps_wait = heur == WFA_HEURISTIC_WFADAPTIVE || heur == WFA_HEURISTIC_BANDED_ADAPTIVE
                      ? hc.p2
-                     : heur == WFA_HEURISTIC_XDROP ? hc.p1 : 0;
+                     : heur == WFA_HEURISTIC_XDROP || heur == WFA_HEURISTIC_ZDROP ? hc.p1 : 0;
     max_sw = 0;
     have_max = false;
+    best_k = 0;
+    best_off = WFA_NULL;
+    max_wf = 0;
+    zlast = 0;
   }
 };
+
+// Z-drop (zdrop_cutoff, src/heuristic.rs:449-478, with compute_sw_scores,
+// :491-525) on the extended M row [lo, hi] at score s: every p1 steps the
+// row's best Smith-Waterman cell (the first k with the largest score) is
+// compared with the best seen; true when the best has dropped by more than
+// p0 for at least the largest single penalty (scope - 1) scores. The row's
+// limits are not changed.
+template <typename RT>
+static __device__ bool zdrop_check(const KParams& p, const HeurCfg& hc, HeurState& hs,
+                                   const Rows<RT>& R, int rowM, int lo, int hi, int s, int pb_free,
+                                   int tb_free) {
+  const int lane = threadIdx.x & 63;
+  if (lo > hi) return false;
+  hs.steps_wait -= 1;
+  if (hs.steps_wait > 0) return false;
+  const int swg = p.u_match < 0 ? -p.u_match : 1;
+  int64_t best = INT64_MIN;  // (sw << 32) | (~biased k): max sw, first k
+  int boff = 0;
+  for (int kb = lo; kb <= hi; kb += 64) {
+    const int k = kb + lane;
+    if (k <= hi) {
+      const int off = R.at(rowM * R.wcap + (k & R.mask));
+      if (off >= 0) {
+        const int fv = k < 0 ? imin(-k, pb_free) : 0;
+        const int fh = k > 0 ? imin(k, tb_free) : 0;
+        const int sw = (swg * ((off - k - fv) + (off - fh)) - s) / 2;
+        const int64_t key = ((int64_t)sw << 32) | (uint32_t)(0x7fffffffu - (uint32_t)k);
+        if (key > best) {
+          best = key;
+          boff = off;
+        }
+      }
+    }
+  }
+  const int64_t wb = wave_max64(best);
+  // no valid cell: the reference's (INT32_MIN, k 0, offset 0)
+  int cmax = INT32_MIN, ck = 0, coff = 0;
+  if (wb != INT64_MIN) {
+    const uint64_t holder = ballot(best == wb);
+    coff = uniform(__shfl(boff, (int)__builtin_ctzll(holder)));
+    cmax = (int)(wb >> 32);
+    ck = (int)(0x7fffffffu - (uint32_t)(wb & 0xffffffffu));
+  }
+  if (!hs.have_max || cmax > hs.max_sw) {
+    hs.have_max = true;
+    hs.max_sw = cmax;
+    hs.max_wf = s;
+    hs.best_k = ck;
+    hs.best_off = coff;
+    hs.zlast = s;
+  } else if ((int64_t)hs.max_sw - cmax <= hc.p0) {
+    hs.zlast = s;
+  } else if (s - hs.zlast >= p.scope - 1) {
+    return true;
+  }
+  hs.steps_wait = hc.p1;
+  return false;
+}
 
 template <typename RT>
 static __device__ bool heuristic_limits(const KParams& p, const HeurCfg& hc, int band_lo,
@@ -894,7 +959,9 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     fe = make_int4(imax(uniform(c0.z), 0), imax(uniform(c0.w), 0), imax(uniform(c1.x), 0),
                    imax(uniform(c1.y), 0));
     hc = HeurCfg{uniform(c1.z), uniform(c1.w), uniform(c2.x), uniform(c2.y)};
-    if (p.heuristic == WFA_HEURISTIC_NONE || hc.kind == WFA_HEURISTIC_ZDROP ||
+    // (Z-drop with a medium/low-memory CIGAR is refused for launches; a
+    // pair asking for it there runs without a heuristic)
+    if (p.heuristic == WFA_HEURISTIC_NONE || (MED && hc.kind == WFA_HEURISTIC_ZDROP) ||
         hc.kind < WFA_HEURISTIC_NONE || hc.kind > WFA_HEURISTIC_BANDED_ADAPTIVE)
       hc.kind = WFA_HEURISTIC_NONE;
   }
@@ -1044,14 +1111,17 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   const int heur = HEUR ? hc.kind : WFA_HEURISTIC_NONE;
   HeurState hs;
   hs.reset(hc);
-  auto cutoff = [&](int s, int rowM, int rowE, int rowE2) {
+  // Returns true when Z-drop ends the alignment.
+  auto cutoff = [&](int s, int rowM, int rowE, int rowE2) -> bool {
     const int4 mm = R.meta[rowM];
     const int lo = uniform(mm.x), hi = uniform(mm.y);
+    if (hc.kind == WFA_HEURISTIC_ZDROP)
+      return zdrop_check(p, hc, hs, R, rowM, lo, hi, s, pb_free, tb_free);
     int nlo = lo, nhi = hi;
     if (!heuristic_limits(p, hc, hc.p0, hc.p1, hs, R, rowM, lo, hi, s, plen, tlen, pb_free,
                           tb_free, &nlo, &nhi))
-      return;
-    if (nlo == lo && nhi == hi) return;
+      return false;
+    if (nlo == lo && nhi == hi) return false;
     const bool empty = nlo > nhi;
     __syncthreads();  // every lane has read M's cells and metadata
     if (tid == 0) {
@@ -1084,7 +1154,11 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       }
     }
     __syncthreads();
+    return false;
   };
+  // Z-drop ended the alignment: the end position is the best cell seen
+  // (src/aligner.rs:1690-1700); the alignment is partial
+  bool zdropped = false;
 
   // score 0
   {
@@ -1097,7 +1171,11 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       result = 0;
     }
     __syncthreads();
-    if (!overflow && status == WFA_STATUS_OK && heur != WFA_HEURISTIC_NONE) cutoff(0, 0, 0, 0);
+    if (!overflow && status == WFA_STATUS_OK && heur != WFA_HEURISTIC_NONE && cutoff(0, 0, 0, 0)) {
+      status = WFA_STATUS_END_UNREACHABLE;
+      result = 0;
+      zdropped = true;
+    }
   }
   __syncthreads();
 
@@ -1485,7 +1563,12 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         extend_row(s, rowM, elo, ehi, hrow) && (ef || (m_lo <= k_end && k_end <= m_hi));
     __syncthreads();
     if (reached) { status = WFA_STATUS_END_REACHED; result = s; break; }
-    if (heur != WFA_HEURISTIC_NONE) cutoff(s, rowM, rowE, rowE2);
+    if (heur != WFA_HEURISTIC_NONE && cutoff(s, rowM, rowE, rowE2)) {
+      status = WFA_STATUS_END_UNREACHABLE;
+      result = s;
+      zdropped = true;
+      break;
+    }
   }
 
   if (overflow) {
@@ -1498,20 +1581,32 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   }
   int out_status = status;
   int ncig = 0;
+  // the end position (alignment_end_pos): the end reached, the Z-drop best
+  // cell (found at an earlier score), or unset
+  int pos_s = -1, pos_k = INT32_MAX, pos_off = WFA_NULL;
   if (status == WFA_STATUS_END_REACHED) {
-    out_status = WFA_STATUS_COMPLETED;
+    pos_s = score; pos_k = end_k; pos_off = end_off;
+  } else if (zdropped) {
+    pos_s = hs.max_wf; pos_k = hs.best_k; pos_off = hs.best_off;
+  }
+  if (status == WFA_STATUS_END_REACHED || zdropped) {
+    // a Z-dropped alignment is partial: backtraced from its best cell at
+    // the score that cell was found at (high memory, Compat::C,
+    // src/aligner.rs:1393-1406) and max-trimmed (src/aligner.rs:683-694,
+    // 784-806); its score is the score it stopped at
+    out_status = zdropped ? WFA_STATUS_PARTIAL : WFA_STATUS_COMPLETED;
     if (CIG && wave == 0) {
       __threadfence_block();
       uint8_t* out = p.cig + p.cig_off[pair];
       const int cap = plen + tlen;
       if (MED)
-        ncig = piggyback_backtrace<KIND, BITS>(p, H, score + 1, sP, sT, plen, tlen, score, end_k,
-                                               end_off, out, cap);
+        ncig = piggyback_backtrace<KIND, BITS>(p, H, score + 1, sP, sT, plen, tlen, pos_s, pos_k,
+                                               pos_off, out, cap);
       else if (!GAPS)
-        ncig = backtrace_linear<BITS>(p, H, score + 1, plen, tlen, score, end_k, end_off, out,
+        ncig = backtrace_linear<BITS>(p, H, score + 1, plen, tlen, pos_s, pos_k, pos_off, out,
                                       cap);
       else
-        ncig = singletrack<BITS>(p, H, score + 1, nwf, sP, sT, plen, tlen, score, end_k, end_off,
+        ncig = singletrack<BITS>(p, H, score + 1, nwf, sP, sT, plen, tlen, pos_s, pos_k, pos_off,
                                  out, cap);
       if (ncig < 0) {
         // the stored flags do not lead back to the start: never expected;
@@ -1531,12 +1626,12 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           __builtin_amdgcn_wave_barrier();
         }
       }
-      if (ef && ext && KIND != KIND_EDIT && ncig > 0) {
+      if ((zdropped || (ef && ext)) && KIND != KIND_EDIT && ncig > 0) {
         __threadfence_block();
         bool trimmed = false;
         ncig = maxtrim_cigar(p, out, ncig, pb_free, tb_free, &trimmed);
         if (trimmed) out_status = WFA_STATUS_PARTIAL;
-      } else if (heur != WFA_HEURISTIC_NONE && GAPS) {
+      } else if (!zdropped && heur != WFA_HEURISTIC_NONE && GAPS) {
         __threadfence_block();
         const int trailing = ef ? (plen - (end_off - end_k)) + (tlen - end_off) : 0;
         result = rescore_cigar(p, out, ncig, ef, pb_free, tb_free, trailing);
@@ -1551,6 +1646,11 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     p.score[pair] = result;
     p.status[pair] = out_status;
     if (CIG) p.cig_len[pair] = ncig;
+    if (p.end_pos) {
+      p.end_pos[3 * pair] = pos_s;
+      p.end_pos[3 * pair + 1] = pos_k;
+      p.end_pos[3 * pair + 2] = pos_off;
+    }
   }
 }
 

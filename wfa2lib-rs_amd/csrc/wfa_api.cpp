@@ -312,7 +312,7 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
       // what still does not fit the scratch budget gets the reference's
       // out-of-memory status (AlignStatus::Oom, src/aligner.rs:120-135)
       HIPCHK(wfa::launch_mark_oom(e->d_lists[cur], nretry, b->score, b->status, b->cigar_length,
-                                  st));
+                                  b->end_position, st));
       break;
     }
     r = plan(e, p, kind, mode, b->seq_format, max_len, nretry, pass, &c);
@@ -475,22 +475,63 @@ struct wfa_aligner_t {
   int32_t score = 0;
   std::vector<uint8_t> cigar;
   int32_t cigar_score = INT32_MIN;
+  int32_t end_pos[3] = {-1, INT32_MAX, INT32_MIN / 2};
   // device staging for one pair: [pattern | text | cigar], plus arrays
   uint8_t* d_buf = nullptr;
   size_t d_cap = 0;
   int64_t* d_i64 = nullptr;  // poff, toff, coff
-  int32_t* d_i32 = nullptr;  // plen, tlen, score, status, clen
+  int32_t* d_i32 = nullptr;  // plen, tlen, score, status, clen, end position (3)
   uint8_t* h_buf = nullptr;  // pinned
   size_t h_cap = 0;
 };
 
 namespace {
 
+// cigar_wfa_score (src/aligner.rs:2700-2719) of ops[0, n): the classic
+// score (Cigar::score_*, src/cigar.rs:118-202) on the user penalties, in
+// the WFA scale over the bases the ops span.
+int32_t wfa_score_of_ops(const wfa_penalties_t& u, const uint8_t* ops, int n) {
+  int v = 0, h = 0, classic = 0, edits = 0;
+  uint8_t last = 0;
+  for (int i = 0; i < n; ++i) {
+    const uint8_t op = ops[i];
+    v += op != 'I';
+    h += op != 'D';
+    edits += op != 'M';
+    if (op == 'M') classic -= u.match;
+    else if (op == 'X') classic -= u.mismatch;
+    else if (u.distance_metric == WFA_METRIC_GAP_LINEAR) classic -= u.gap_opening1;
+    else if (u.distance_metric == WFA_METRIC_GAP_AFFINE)
+      classic -= u.gap_extension1 + (last == op ? 0 : u.gap_opening1);
+    last = op;
+  }
+  if (u.distance_metric == WFA_METRIC_EDIT || u.distance_metric == WFA_METRIC_INDEL) return edits;
+  if (u.distance_metric == WFA_METRIC_GAP_AFFINE_2P) {
+    // runs, each gap run priced by the cheaper piece
+    classic = 0;
+    for (int i = 0; i < n;) {
+      int j = i;
+      while (j < n && ops[j] == ops[i]) ++j;
+      const int len = j - i;
+      if (ops[i] == 'M') classic -= u.match * len;
+      else if (ops[i] == 'X') classic -= u.mismatch * len;
+      else
+        classic -= std::min(u.gap_opening1 + u.gap_extension1 * len,
+                            u.gap_opening2 + u.gap_extension2 * len);
+      i = j;
+    }
+  }
+  return u.match < 0 ? -u.match * (v + h) - 2 * classic : -classic;
+}
+
 int aligner_run(wfa_aligner_t* a, const uint8_t* P, int plen, const uint8_t* T, int tlen,
                 int fmt) {
   a->cigar.clear();
   a->cigar_score = INT32_MIN;
   a->status = WFA_STATUS_OK;
+  a->end_pos[0] = -1;
+  a->end_pos[1] = INT32_MAX;
+  a->end_pos[2] = INT32_MIN / 2;
   if (plen < 0 || tlen < 0 || (plen && !P) || (tlen && !T)) return WFA_ERR_INVALID_ARGUMENT;
   int r = check_supported(a->attr);
   if (r) return r;
@@ -545,22 +586,44 @@ int aligner_run(wfa_aligner_t* a, const uint8_t* P, int plen, const uint8_t* T, 
     b.cigar_offset = a->d_i64 + 2;
     b.cigar_length = a->d_i32 + 4;
   }
+  const bool biwfa = a->attr.memory_mode == WFA_MEMORY_ULTRALOW;
+  if (!biwfa) b.end_position = a->d_i32 + 5;
   {
     std::lock_guard<std::mutex> g(a->engine->mu);
     r = align_batch_impl(a->engine, &a->attr, &b, st, 0, 1);
   }
   if (r) return r;
-  int32_t res[3];
+  int32_t res[6];
   if (hipMemcpy(res, a->d_i32 + 2, sizeof(res), hipMemcpyDeviceToHost) != hipSuccess)
     return WFA_ERR_DEVICE;
   a->score = res[0];
   a->status = res[1];
+  if (!biwfa) std::memcpy(a->end_pos, res + 3, sizeof(a->end_pos));
   if (cig && res[2] > 0) {
     a->cigar.resize(res[2]);
     if (hipMemcpy(a->cigar.data(), a->d_buf + pb + tb, res[2], hipMemcpyDeviceToHost) !=
         hipSuccess)
       return WFA_ERR_DEVICE;
     a->cigar_score = a->score;
+    // a max-trimmed CIGAR (extension, or a Z-drop end) scores its kept
+    // part, a free leading gap excluded (maxtrim_cigar, src/aligner.rs:1545-1583)
+    const int m = a->attr.penalties.distance_metric;
+    const bool ef = a->attr.span == WFA_SPAN_ENDSFREE;
+    const bool ext = ef && a->attr.extension;
+    const bool zdrop_end =
+        a->attr.heuristic == WFA_HEURISTIC_ZDROP && a->status == WFA_STATUS_PARTIAL;
+    if ((ext || zdrop_end) && m != WFA_METRIC_EDIT && m != WFA_METRIC_INDEL) {
+      int b = 0, bh = 0;
+      const int n = (int)a->cigar.size();
+      if (ef && !ext) {
+        while (bh < a->attr.text_begin_free && b < n && a->cigar[b] == 'I') { ++b; ++bh; }
+        for (int bv = 0; bh == 0 && bv < a->attr.pattern_begin_free && b < n &&
+                         a->cigar[b] == 'D';
+             ++bv)
+          ++b;
+      }
+      a->cigar_score = wfa_score_of_ops(a->attr.penalties, a->cigar.data() + b, n - b);
+    }
   }
   return WFA_OK;
 }
@@ -728,6 +791,10 @@ int32_t wfa_aligner_cigar(const wfa_aligner_t* a, const uint8_t** ops) {
   return (int32_t)a->cigar.size();
 }
 int32_t wfa_aligner_cigar_score(const wfa_aligner_t* a) { return a ? a->cigar_score : INT32_MIN; }
+void wfa_aligner_end_position(const wfa_aligner_t* a, int32_t* out) {
+  if (!a || !out) return;
+  std::memcpy(out, a->end_pos, sizeof(a->end_pos));
+}
 int32_t wfa_aligner_error(const wfa_aligner_t* a) { return a ? a->err : WFA_ERR_INVALID_ARGUMENT; }
 
 }  // extern "C"

@@ -52,6 +52,7 @@ struct KParams {
   uint8_t* cig;
   const int64_t* cig_off;
   int32_t* cig_len;
+  int32_t* end_pos;          // optional {score, k, offset} per pair
   // scheduling
   int32_t* counters;         // [0] work-queue head, [1] retry count
   int32_t* retry_list;
@@ -173,7 +174,7 @@ inline int biwfa_waves(int which, bool heur) { return which == 0 && !heur ? kWid
 
 // Status WFA_STATUS_OOM (score 0, no CIGAR) for the `n` pairs of `list`
 hipError_t launch_mark_oom(const int32_t* list, int n, int32_t* score, int32_t* status,
-                           int32_t* cig_len, hipStream_t stream);
+                           int32_t* cig_len, int32_t* end_pos, hipStream_t stream);
 
 // Longest-first order of pairs first..first+n-1 into `order` (a device
 // radix sort; keys/keys_sorted/vals hold n entries, temp lpt_temp_bytes(n))

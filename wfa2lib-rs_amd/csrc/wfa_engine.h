@@ -86,19 +86,25 @@ inline int check_supported(const wfa_attr_t& a) {
       (a.pattern_begin_free < 0 || a.pattern_end_free < 0 || a.text_begin_free < 0 ||
        a.text_end_free < 0))
     return WFA_ERR_INVALID_ARGUMENT;
-  // Z-drop ends alignments early through its own backtrace path: not on
-  // the device path
   if (a.heuristic < WFA_HEURISTIC_NONE || a.heuristic > WFA_HEURISTIC_BANDED_ADAPTIVE)
     return WFA_ERR_INVALID_ARGUMENT;
-  if (a.heuristic == WFA_HEURISTIC_ZDROP) return WFA_ERR_UNSUPPORTED;
   if (a.alignment_scope != WFA_SCOPE_SCORE && a.alignment_scope != WFA_SCOPE_ALIGNMENT)
     return WFA_ERR_INVALID_ARGUMENT;
   if (a.memory_mode < WFA_MEMORY_HIGH || a.memory_mode > WFA_MEMORY_ULTRALOW)
     return WFA_ERR_INVALID_ARGUMENT;
+  // Z-drop with a medium/low-memory CIGAR: the reference (Compat::C)
+  // backtraces the best cell at the score the alignment stopped at, from
+  // whatever backtrace chain that wavefront holds there
+  // (src/aligner.rs:1393-1406, 1443-1460; HPCBio/wfa2lib-rs#22): not on
+  // the device path
+  if (a.heuristic == WFA_HEURISTIC_ZDROP && a.alignment_scope == WFA_SCOPE_ALIGNMENT &&
+      (a.memory_mode == WFA_MEMORY_MED || a.memory_mode == WFA_MEMORY_LOW))
+    return WFA_ERR_UNSUPPORTED;
   // BiWFA: end-to-end only, and no X-drop / Z-drop, as the reference's
   // driver checks (src/bin/align_benchmark.rs:529-560)
   if (a.memory_mode == WFA_MEMORY_ULTRALOW &&
-      (a.span != WFA_SPAN_END2END || a.heuristic == WFA_HEURISTIC_XDROP))
+      (a.span != WFA_SPAN_END2END || a.heuristic == WFA_HEURISTIC_XDROP ||
+       a.heuristic == WFA_HEURISTIC_ZDROP))
     return WFA_ERR_UNSUPPORTED;
   return WFA_OK;
 }
@@ -228,6 +234,7 @@ inline void base_params(const Internal& ip, const wfa_attr_t& attr, const wfa_ba
   p->cig = b.cigar_ops;
   p->cig_off = b.cigar_offset;
   p->cig_len = b.cigar_length;
+  p->end_pos = b.end_position;
 }
 
 // BiWFA batches (memory mode ULTRALOW): wfa_biwfa.cpp

@@ -34,14 +34,19 @@ __global__ void wfa_lpt_keys(const int32_t* plen, const int32_t* tlen, int first
   }
 }
 
-// Pairs that outgrew every pass: out of memory, no CIGAR
+// Pairs that outgrew every pass: out of memory, no CIGAR, no end position
 __global__ void wfa_mark_oom(const int32_t* list, int n, int32_t* score, int32_t* status,
-                             int32_t* cig_len) {
+                             int32_t* cig_len, int32_t* end_pos) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int pair = list[i];
     score[pair] = 0;
     status[pair] = WFA_STATUS_OOM;
     if (cig_len) cig_len[pair] = 0;
+    if (end_pos) {
+      end_pos[3 * pair] = -1;
+      end_pos[3 * pair + 1] = INT32_MAX;
+      end_pos[3 * pair + 2] = INT32_MIN / 2;
+    }
   }
 }
 
@@ -107,10 +112,10 @@ hipError_t launch_lpt_order(const int32_t* plen, const int32_t* tlen, int first,
 }
 
 hipError_t launch_mark_oom(const int32_t* list, int n, int32_t* score, int32_t* status,
-                           int32_t* cig_len, hipStream_t stream) {
+                           int32_t* cig_len, int32_t* end_pos, hipStream_t stream) {
   const int grid = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
   hipLaunchKernelGGL(wfa_mark_oom, dim3(grid > 0 ? grid : 1), dim3(256), 0, stream, list, n,
-                     score, status, cig_len);
+                     score, status, cig_len, end_pos);
   return hipGetLastError();
 }
 

@@ -264,6 +264,13 @@ static __device__ void align_window(const KParams& p, int pair, int plen, int tl
     p.score[pair] = result;
     p.status[pair] = out_status;
     if (CIG) p.cig_len[pair] = ncig;
+    if (p.end_pos) {
+      // the end cell, or unset (src/aligner.rs:903-909)
+      const bool end = status == WFA_STATUS_END_REACHED;
+      p.end_pos[3 * pair] = end ? score : -1;
+      p.end_pos[3 * pair + 1] = end ? k_end : INT32_MAX;
+      p.end_pos[3 * pair + 2] = end ? tlen : WFA_NULL;
+    }
   }
 }
 

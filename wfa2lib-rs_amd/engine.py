@@ -34,10 +34,11 @@ def pair_config_row(attr):
 class DeviceBatch:
     """A batch resident in device memory (HBM)."""
 
-    def __init__(self, pairs, device="cuda", cigar=False, pair_config=None):
+    def __init__(self, pairs, device="cuda", cigar=False, pair_config=None, end_position=False):
         """`pair_config`: optional (n, 12) int32 rows of wfa_pair_config_t
         (span, extension, four free ends, heuristic, three parameters, two
-        zeros), see ``pair_config_row``."""
+        zeros), see ``pair_config_row``. `end_position`: also return each
+        pair's alignment_end_pos (score, k, offset), ``end_positions()``."""
         import torch
         self.n = len(pairs)
         self.seq_format = pairs.seq_format
@@ -51,6 +52,8 @@ class DeviceBatch:
         self.score = torch.zeros(max(self.n, 1), dtype=torch.int32, device=device)
         self.status = torch.zeros(max(self.n, 1), dtype=torch.int32, device=device)
         self.cigar = cigar
+        self.end_pos = (torch.zeros(3 * max(self.n, 1), dtype=torch.int32, device=device)
+                        if end_position else None)
         self.pair_config = None
         if pair_config is not None:
             pc = np.ascontiguousarray(np.asarray(pair_config, dtype=np.int32).reshape(self.n, 12))
@@ -81,7 +84,13 @@ class DeviceBatch:
             b.cigar_length = self.cig_len.data_ptr()
         if self.pair_config is not None:
             b.pair_config = self.pair_config.data_ptr()
+        if self.end_pos is not None:
+            b.end_position = self.end_pos.data_ptr()
         return b
+
+    def end_positions(self):
+        """(n, 3) int32 host array of alignment_end_pos {score, k, offset}."""
+        return self.end_pos.cpu().numpy()[:3 * self.n].reshape(self.n, 3)
 
     def results(self):
         """(scores, status, list of CIGAR bytes or None) on the host."""
