@@ -7,7 +7,9 @@
  * distance and indel (LCS) distance. With match = 0 the WFA score is the
  * optimum of exactly these costs, so these pin the oracle's and the
  * engine's scores on random pairs (the reference's own DP oracle,
- * tests/corrected_endsfree.rs:56-107, is the same Gotoh recurrence).
+ * tests/corrected_endsfree.rs:56-107, is the same Gotoh recurrence). The
+ * ends-free variant restates that model (free starts and ends, a match
+ * reward allowed).
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -61,6 +63,61 @@ int32_t dp_affine_score(const uint8_t* P, int32_t plen, const uint8_t* T, int32_
     int32_t r = M[(plen & 1) * W + tlen];
     free(M); free(I1); free(I2); free(D1); free(D2);
     return r;
+}
+
+/* Optimal ends-free penalty (lower is better; matches cost `match`, <= 0)
+ * under the wavefront model: an alignment starts at (v, 0) with
+ * v <= pb or at (0, h) with h <= tb, and ends at (v, |T|) with |P| - v <= pe
+ * or at (|P|, h) with |T| - h <= te; a gap of length l costs
+ * min(o1 + l*e1, o2 + l*e2) (one piece when two_piece is 0). Full matrices:
+ * for short sequences only. */
+int64_t dp_endsfree_penalty(const uint8_t* P, int32_t n, const uint8_t* T, int32_t m,
+                            int32_t match, int32_t x, int32_t o1, int32_t e1, int32_t o2,
+                            int32_t e2, int32_t two_piece, int32_t pb, int32_t pe, int32_t tb,
+                            int32_t te) {
+    const int64_t INF = INT64_MAX / 4;
+    const int W = m + 1, np = two_piece ? 2 : 1;
+    const int64_t go[2] = {o1, o2}, ge[2] = {e1, e2};
+    int64_t* H = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1) * W);
+    int64_t* I = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1) * W * 2);
+    int64_t* D = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1) * W * 2);
+    for (int v = 0; v <= n; ++v) {
+        for (int j = 0; j <= m; ++j) {
+            int64_t best = INF;
+            if ((j == 0 && v <= pb) || (v == 0 && j <= tb)) best = 0;
+            if (v > 0 && j > 0 && H[(v - 1) * W + j - 1] < INF) {
+                int64_t c = H[(v - 1) * W + j - 1] + (P[v - 1] == T[j - 1] ? match : x);
+                if (c < best) best = c;
+            }
+            for (int q = 0; q < 2; ++q) {
+                int64_t* iq = &I[((size_t)v * W + j) * 2 + q];
+                int64_t* dq = &D[((size_t)v * W + j) * 2 + q];
+                *iq = INF;
+                *dq = INF;
+                if (q >= np) continue;
+                if (j > 0) {
+                    int64_t a = H[v * W + j - 1] + go[q] + ge[q];
+                    int64_t b = I[((size_t)v * W + j - 1) * 2 + q] + ge[q];
+                    *iq = a < b ? a : b;
+                    if (*iq < best) best = *iq;
+                }
+                if (v > 0) {
+                    int64_t a = H[(v - 1) * W + j] + go[q] + ge[q];
+                    int64_t b = D[((size_t)(v - 1) * W + j) * 2 + q] + ge[q];
+                    *dq = a < b ? a : b;
+                    if (*dq < best) best = *dq;
+                }
+            }
+            H[v * W + j] = best;
+        }
+    }
+    int64_t best = INF;
+    for (int v = 0; v <= n; ++v)
+        if (n - v <= pe && H[v * W + m] < best) best = H[v * W + m];
+    for (int j = 0; j <= m; ++j)
+        if (m - j <= te && H[n * W + j] < best) best = H[n * W + j];
+    free(H); free(I); free(D);
+    return best;
 }
 
 int32_t dp_edit_score(const uint8_t* P, int32_t plen, const uint8_t* T, int32_t tlen,

@@ -57,6 +57,8 @@ def lib(attr_type):
                                       I32, I32, I32]
         l.dp_edit_score.restype = I32
         l.dp_edit_score.argtypes = [ctypes.c_char_p, I32, ctypes.c_char_p, I32, I32]
+        l.dp_endsfree_penalty.restype = ctypes.c_int64
+        l.dp_endsfree_penalty.argtypes = [ctypes.c_char_p, I32, ctypes.c_char_p, I32] + [I32] * 11
         _lib = l
     return _lib
 
@@ -144,3 +146,11 @@ def dp_affine(pattern, text, x, o1, e1, o2=0, e2=0, two_piece=False, attr_type=N
 
 def dp_edit(pattern, text, indel=False):
     return _lib.dp_edit_score(pattern, len(pattern), text, len(text), 1 if indel else 0)
+
+
+def dp_endsfree(pattern, text, match, x, o1, e1, o2, e2, two_piece, ends):
+    """Optimal ends-free penalty (matches cost `match` <= 0); ends =
+    (pattern begin, pattern end, text begin, text end) free lengths."""
+    pb, pe, tb, te = ends
+    return _lib.dp_endsfree_penalty(pattern, len(pattern), text, len(text), match, x, o1, e1,
+                                    o2, e2, 1 if two_piece else 0, pb, pe, tb, te)

@@ -338,6 +338,84 @@ static wf_t* alloc_wf(wfo_aligner* a, int alo, int ahi) {
     return w;
 }
 
+static void set_bt_init(wfo_aligner* a, wf_t* w, int k, int v, int h);
+
+/* endsfree_begin_k (src/aligner.rs:3369-3386): with a match reward the
+ * begin-free cells (h, 0) and (0, v), h = v = d, are reached at score
+ * d * -match; d when `score` is such a score within a free range, else -1.
+ * (With match == 0 they are all seeded at score 0.) */
+static int endsfree_begin_k(const wfo_aligner* a, int score) {
+    const int m = a->pen.match;
+    if (m == 0 || !a->endsfree || (a->tb_free == 0 && a->pb_free == 0)) return -1;
+    if (score % (-m) != 0) return -1;
+    const int k = score / (-m);
+    return a->tb_free >= k || a->pb_free >= k ? k : -1;
+}
+
+/* output_alloc_range with the ends-free widening of a begin-free score
+ * (src/aligner.rs:3318-3342) */
+static void output_alloc_range_at(wfo_aligner* a, int score, int lo, int hi, int* alo, int* ahi) {
+    const int k = score >= 0 ? endsfree_begin_k(a, score) : -1;
+    if (k >= 0) {
+        if (-k < lo) lo = -k;
+        if (k > hi) hi = k;
+    }
+    output_alloc_range(a, lo, hi, alo, ahi);
+}
+
+/* endsfree_init_offset (src/aligner.rs:3421-3437): a begin-free cell (v, h)
+ * on diagonal k, starting a backtrace chain there */
+static void endsfree_init_offset(wfo_aligner* a, wf_t* w, int k, int v, int h) {
+    wf_set(w, k, h);
+    if (a->piggyback) set_bt_init(a, w, k, v, h);
+}
+static void set_null_cell(wf_t* w, int k) {
+    wf_set(w, k, OFFSET_NULL);
+    wf_zero_bt(w, k);
+}
+
+/* endsfree_init (src/aligner.rs:3388-3419): the begin-free cells of score
+ * d * -match added to a computed M before its ends are trimmed: (h = d, 0)
+ * on diagonal d and (0, v = d) on -d, replacing a cell that reaches no
+ * further, the range widened over NULL cells to reach them. */
+static void endsfree_init(wfo_aligner* a, wf_t* wf, int k) {
+    const int lo = wf->lo, hi = wf->hi;
+    if (a->tb_free >= k) {
+        if (hi >= k) {
+            if (wf_get(wf, k) <= k) endsfree_init_offset(a, wf, k, 0, k);
+        } else {
+            for (int kk = hi + 1; kk < k; ++kk) set_null_cell(wf, kk);
+            endsfree_init_offset(a, wf, k, 0, k);
+            wf->hi = k;
+        }
+    }
+    if (a->pb_free >= k) {
+        if (lo <= -k) {
+            if (wf_get(wf, -k) <= 0) endsfree_init_offset(a, wf, -k, k, 0);
+        } else {
+            endsfree_init_offset(a, wf, -k, k, 0);
+            for (int kk = -k + 1; kk < lo; ++kk) set_null_cell(wf, kk);
+            wf->lo = -k;
+        }
+    }
+}
+
+/* endsfree_null_m (src/aligner.rs:3439-3465): at a begin-free score whose
+ * inputs are all null, M holds just the begin-free cells. */
+static wf_t* endsfree_null_m(wfo_aligner* a, int k) {
+    const int text = a->tb_free >= k, pattern = a->pb_free >= k;
+    const int lo = pattern ? -k : text ? k : 0;
+    const int hi = text ? k : pattern ? -k : 0;
+    int alo, ahi;
+    output_alloc_range(a, lo, hi, &alo, &ahi);
+    wf_t* w = alloc_wf(a, alo, ahi);
+    for (int kk = lo; kk <= hi; ++kk) set_null_cell(w, kk);
+    if (text) endsfree_init_offset(a, w, k, 0, k);
+    if (pattern) endsfree_init_offset(a, w, -k, k, 0);
+    wf_set_limits(w, lo, hi);
+    return w;
+}
+
 /* ------------------------------------------------------------------ */
 /* Aligner construction                                                */
 /* ------------------------------------------------------------------ */
@@ -354,8 +432,7 @@ wfo_aligner* wfo_aligner_new(const wfa_attr_t* attr, int32_t* err) {
      * wavefront holds there (HPCBio/wfa2lib-rs#22, src/aligner.rs:1443-1460):
      * not restated. BiWFA rejects X-drop and Z-drop as the driver does. */
     const int zdrop = attr->heuristic == WFA_HEURISTIC_ZDROP;
-    if ((attr->span == WFA_SPAN_ENDSFREE && a->pen.match != 0) ||
-        (zdrop && attr->alignment_scope == WFA_SCOPE_ALIGNMENT &&
+    if ((zdrop && attr->alignment_scope == WFA_SCOPE_ALIGNMENT &&
          (attr->memory_mode == WFA_MEMORY_MED || attr->memory_mode == WFA_MEMORY_LOW)) ||
         (attr->memory_mode == WFA_MEMORY_ULTRALOW &&
          (attr->span != WFA_SPAN_END2END || attr->heuristic == WFA_HEURISTIC_XDROP || zdrop))) {
@@ -850,10 +927,12 @@ static void compute_affine_step(wfo_aligner* a, int score) {
         d2e = score - e2 >= 0 ? get(a, C_D2, score - e2) : NULL;
     }
     const int ncomp = two ? 5 : 3;
+    const int bk = endsfree_begin_k(a, score);
     if (is_null_wf(mx) && is_null_wf(mo1) && is_null_wf(i1e) && is_null_wf(d1e) &&
         is_null_wf(mo2) && is_null_wf(i2e) && is_null_wf(d2e)) {
         a->num_null_steps++;
         free_outputs(a, score, ncomp);
+        if (bk >= 0) put(a, C_M, score, endsfree_null_m(a, bk));
         return;
     }
     a->num_null_steps = 0;
@@ -892,7 +971,7 @@ static void compute_affine_step(wfo_aligner* a, int score) {
         return;
     }
     int alo, ahi;
-    output_alloc_range(a, lo, hi, &alo, &ahi);
+    output_alloc_range_at(a, score, lo, hi, &alo, &ahi);
     if (mx) { wf_init_ends_higher(mx, hi); wf_init_ends_lower(mx, lo); }
     if (mo1) { wf_init_ends_higher(mo1, hi + 1); wf_init_ends_lower(mo1, lo - 1); }
     if (mo2) { wf_init_ends_higher(mo2, hi + 1); wf_init_ends_lower(mo2, lo - 1); }
@@ -1008,6 +1087,7 @@ static void compute_affine_step(wfo_aligner* a, int score) {
     }
     put(a, C_M, score, m); put(a, C_I1, score, i1); put(a, C_D1, score, d1);
     if (two) { put(a, C_I2, score, i2); put(a, C_D2, score, d2); }
+    if (bk >= 0) endsfree_init(a, m, bk);
     trim_ends(plen, tlen, m);
     trim_ends(plen, tlen, i1);
     trim_ends(plen, tlen, d1);
@@ -1098,16 +1178,17 @@ static void compute_edit_step(wfo_aligner* a, int score) {
  * M[s,k] = max(M[s-x,k] + 1, M[s-o,k-1] + 1, M[s-o,k+1]). Unlike the edit
  * step, the inputs' ends are NULL-filled only beyond their initialized
  * ranges (init_ends), so cells a heuristic cut keep their offsets. Ends-free
- * with a match reward (begin-free cells added during compute) is refused at
- * construction, as for every metric. */
+ * with a match reward adds the begin-free cells of this score
+ * (endsfree_init / endsfree_null_m). */
 static void compute_linear_step(wfo_aligner* a, int score) {
     const int plen = a->plen, tlen = a->tlen;
     const pen_t* p = &a->pen;
     wf_t* mx = score - p->mismatch >= 0 ? get(a, C_M, score - p->mismatch) : NULL;
     wf_t* mg = score - p->o1 >= 0 ? get(a, C_M, score - p->o1) : NULL;
+    const int bk = endsfree_begin_k(a, score);
     if (is_null_wf(mx) && is_null_wf(mg)) {
         a->num_null_steps++;
-        put(a, C_M, score, NULL);
+        put(a, C_M, score, bk >= 0 ? endsfree_null_m(a, bk) : NULL);
         return;
     }
     a->num_null_steps = 0;
@@ -1128,7 +1209,7 @@ static void compute_linear_step(wfo_aligner* a, int score) {
     if (hi > a->hist_hi - 1) hi = a->hist_hi - 1;
     if (lo > hi) { put(a, C_M, score, NULL); return; }
     int alo, ahi;
-    output_alloc_range(a, lo, hi, &alo, &ahi);
+    output_alloc_range_at(a, score, lo, hi, &alo, &ahi);
     if (mx) { wf_init_ends_higher(mx, hi); wf_init_ends_lower(mx, lo); }
     if (mg) { wf_init_ends_higher(mg, hi + 1); wf_init_ends_lower(mg, lo - 1); }
     wf_t* cur = alloc_wf(a, alo, ahi);
@@ -1159,6 +1240,7 @@ static void compute_linear_step(wfo_aligner* a, int score) {
         }
     }
     put(a, C_M, score, cur);
+    if (bk >= 0) endsfree_init(a, cur, bk);
     trim_ends(plen, tlen, cur);
 }
 

@@ -192,11 +192,11 @@ def test_long_reads_affine2p_med_match_oracle(pkg, N, engine, oracle):
 
 
 SPANS = [("extension", "extension"), ("endsfree_5_5", [5, 5, 5, 5]), ("endsfree_txt", [0, 0, 0, 20]),
-         ("begin_free", [15, 0, 15, 0])]
+         ("begin_free", [15, 0, 15, 0]), ("begin_free_wide", [80, 10, 60, 200])]
 
 
 @pytest.mark.parametrize("name,span", SPANS)
-@pytest.mark.parametrize("metric,pen", [c for c in CONFIGS if c[1][0] == 0])
+@pytest.mark.parametrize("metric,pen", CONFIGS)
 @pytest.mark.parametrize("memory", ["high", "med"])
 def test_endsfree_spans_match_oracle(name, span, metric, pen, memory, pkg, N, engine, oracle):
     """Ends-free and extension spans, score and CIGAR, high and medium memory."""
@@ -241,8 +241,6 @@ def test_heuristics_match_oracle(hname, h, sname, span, metric, pen, memory, pkg
                                  oracle):
     """WF-adaptive, X-drop, banded static/adaptive: cut-offs, band clamp,
     statuses and (gap-affine/2p) CIGAR rescoring, against the oracle."""
-    if span is not None and pen[0] != 0:
-        pytest.skip("ends-free on the device needs match == 0")
     pairs = _heuristic_pairs()
     # Z-drop with a medium-memory CIGAR is refused (score only runs)
     scopes = ("score",) if h[0] == N.HEURISTIC_ZDROP and memory == "med" else \
@@ -270,8 +268,6 @@ def test_wide_kernel_several_waves_per_pair(metric, pen, span_name, span, memory
     the ends-free first hit combined across waves, matches the oracle. An
     ASCII batch with a non-ACGT byte also takes the byte-wise path there."""
     from wfa2lib_rs_amd.synth import Workload, from_ascii, generate
-    if span is not None and pen[0] != 0:
-        pytest.skip("ends-free needs match 0 on the device path")
     w = Workload("wide_waves", 24, 2, 1000, 3000, 0.01, 0.10, 5.0, seed=5)
     pairs = generate(w)
     attr = attr_for(N, metric, pen, "alignment", span=span)
@@ -301,8 +297,6 @@ def test_zdrop_end_positions_match_oracle(metric, pen, span_name, span, pkg, N, 
     import torch
     from wfa2lib_rs_amd.engine import DeviceBatch
     from wfa2lib_rs_amd.synth import Workload, generate
-    if span is not None and pen[0] != 0:
-        pytest.skip("ends-free on the device needs match == 0")
     w = Workload("zdrop_mix", 80, 1, 60, 600, 0.0, 0.35, 3.0, run_frac=0.2, run_mean=30.0,
                  seq_format=N.SEQ_ASCII, seed=23)
     pairs = generate(w)

@@ -38,6 +38,9 @@ def _dp_score(oracle, metric, pen, p, t):
         if m < 0:
             return oracle.dp_affine(p, t, 2 * x - 2 * m, 0, 2 * o1 - m)
         return oracle.dp_affine(p, t, x, 0, o1)
+    if m < 0:
+        return oracle.dp_affine(p, t, 2 * x - 2 * m, 2 * o1, 2 * e1 - m, 2 * o2, 2 * e2 - m,
+                                metric == "gap_affine_2p")
     return oracle.dp_affine(p, t, x, o1, e1, o2, e2, metric == "gap_affine_2p")
 
 
@@ -131,6 +134,8 @@ PENALTY_SETS = [
     ("gap_linear", (0, 4, 2, 0, 0, 0)),
     ("gap_linear", (0, 3, 5, 0, 0, 0)),
     ("gap_linear", (-1, 3, 2, 0, 0, 0)),
+    ("gap_affine", (-3, 5, 3, 2, 0, 0)),
+    ("gap_affine_2p", (-1, 4, 6, 2, 24, 1)),
 ]
 
 
@@ -335,6 +340,121 @@ def test_oracle_zdrop_invariants(metric, pen, pkg, N, oracle):
             assert st == STATUS["COMPLETED"]
     if metric != "edit" and pen[1] >= 4:
         assert fired >= 5, "the pairs must make Z-drop fire"
+
+
+def _endsfree_dp(oracle, metric, pen, p, t, ends):
+    m, x, o1, e1, o2, e2 = pen
+    if metric == "gap_linear":
+        return oracle.dp_endsfree(p, t, m, x, 0, o1, 0, o1, False, ends)
+    return oracle.dp_endsfree(p, t, m, x, o1, e1, o2, e2, metric == "gap_affine_2p", ends)
+
+
+def test_oracle_endsfree_match0_equals_dp(N, oracle):
+    """Ends-free with match == 0 is exact (tests/corrected_endsfree.rs:
+    333-354 checks C == corrected there, and corrected against this DP): the
+    WFA score equals the optimal ends-free penalty on random short pairs and
+    free ends, for every gap metric."""
+    import random
+    rng = random.Random(7)
+    seq = lambda n: bytes(rng.choice(b"ACGT") for _ in range(n))
+    for _ in range(400):
+        metric = rng.choice(["gap_affine", "gap_affine_2p", "gap_linear"])
+        pen = (0, 1 + rng.randrange(6), rng.randrange(6), 1 + rng.randrange(4),
+               rng.randrange(12), 1 + rng.randrange(2))
+        if metric == "gap_linear":
+            pen = (0, pen[1], 1 + rng.randrange(5), 0, 0, 0)
+        p, t = seq(rng.randrange(16)), seq(rng.randrange(16))
+        ends = [rng.randrange(len(p) + 1), rng.randrange(len(p) + 1),
+                rng.randrange(len(t) + 1), rng.randrange(len(t) + 1)]
+        s, st, _ = oracle.OracleAligner(attr_for(N, metric, pen, "score", span=ends)).align(p, t)
+        assert st == STATUS["COMPLETED"]
+        assert s == _endsfree_dp(oracle, metric, pen, p, t, ends), (metric, pen, p, t, ends)
+
+
+def test_oracle_endsfree_match_reward_literals(pkg, N, oracle):
+    """Ends-free with a match reward in the reference's default (C) mode:
+    the begin-free cells are reached at score d * -match during compute.
+    tests/corrected_endsfree.rs:293-301 (issue 37: IIIMMMMMMMMDDDD) and
+    :317-324 (issue 38: the all-gap alignment), src/aligner.rs:3945-3967
+    (2-piece, score only: completes)."""
+    a = attr_for(N, "gap_affine", (-5, 4, 0, 8, 0, 0), "alignment", span=[12, 12, 11, 11])
+    for mem in (N.MEMORY_HIGH, N.MEMORY_MED, N.MEMORY_LOW):
+        a.memory_mode = mem
+        s, st, ops = oracle.OracleAligner(a).align(b"ACGTACGTACGT", b"ACGACGTACGT")
+        assert (st, ops) == (STATUS["COMPLETED"], b"IIIMMMMMMMMDDDD")
+    pat = b"AAGAAGCACCGGCTAACTCCGTGCCAGCAGCCGCGGT"
+    txt = b"AGTCGTGACTCTTTTTGCATGTACCTTATGAATAAGGACCGGCTAATTCCG"
+    a = attr_for(N, "gap_affine", (-5, 4, 0, 8, 0, 0), "alignment",
+                 span=[len(pat), len(pat), len(txt), len(txt)])
+    s, st, ops = oracle.OracleAligner(a).align(pat, txt)
+    assert ops and set(ops) <= set(b"ID") and pkg.Cigar(ops).check_alignment(pat, txt) is None
+    a = attr_for(N, "gap_affine_2p", (-3, 6, 1, 5, 9, 1), "score", span=[0, 33, 0, 35])
+    s, st, _ = oracle.OracleAligner(a).align(b"TGGATTGAGATTCCCCGGTATGCTGACAGAAGC",
+                                             b"ACATTCCGACATAGAGCCATATGATAAAAGTTAGT")
+    assert st == STATUS["COMPLETED"]
+
+
+@pytest.mark.parametrize("metric,pen", [("gap_affine", (-3, 5, 3, 2, 0, 0)),
+                                        ("gap_affine", (-5, 4, 0, 8, 0, 0)),
+                                        ("gap_affine_2p", (-1, 4, 6, 2, 24, 1)),
+                                        ("gap_linear", (-1, 3, 2, 0, 0, 0))])
+def test_oracle_endsfree_match_reward_invariants(metric, pen, pkg, N, oracle):
+    """Ends-free with a match reward (C mode): a valid CIGAR whose ends-free
+    penalty is no better than the optimum (the mode is not always optimal:
+    issues 37/38); medium and low memory give the high-memory score and
+    each other's CIGAR; score-only gives the same score."""
+    import random
+    rng = random.Random(11)
+    seq = lambda n: bytes(rng.choice(b"ACGT") for _ in range(n))
+    m = pen[0]
+    for _ in range(150):
+        p = seq(1 + rng.randrange(40))
+        t = bytes(c if rng.random() > 0.2 else rng.choice(b"ACGT") for c in p)
+        t = seq(rng.randrange(6)) + t[rng.randrange(4):] + seq(rng.randrange(6))
+        ends = [rng.randrange(len(p) + 1), rng.randrange(len(p) + 1),
+                rng.randrange(len(t) + 1), rng.randrange(len(t) + 1)]
+        res = {}
+        for mem, scope in ((N.MEMORY_HIGH, "alignment"), (N.MEMORY_MED, "alignment"),
+                           (N.MEMORY_LOW, "alignment"), (N.MEMORY_HIGH, "score")):
+            a = attr_for(N, metric, pen, scope, span=ends)
+            a.memory_mode = mem
+            res[(mem, scope)] = oracle.OracleAligner(a).align(p, t)
+        s, st, ops = res[(N.MEMORY_HIGH, "alignment")]
+        assert st == STATUS["COMPLETED"]
+        assert pkg.Cigar(ops).check_alignment(p, t) is None
+        assert res[(N.MEMORY_HIGH, "score")][:2] == (s, st)
+        assert res[(N.MEMORY_MED, "alignment")][0] == s
+        assert res[(N.MEMORY_MED, "alignment")] == res[(N.MEMORY_LOW, "alignment")]
+        for _, _, c in res.values():
+            if c:
+                pb, pe, tb, te = ends
+                # the CIGAR's ends-free penalty: leading/trailing gap runs
+                # free up to the limits (tests/corrected_endsfree.rs:109-146)
+                runs = []
+                for op in c:
+                    if runs and runs[-1][0] == op:
+                        runs[-1][1] += 1
+                    else:
+                        runs.append([op, 1])
+                pen_total = 0
+                for i, (op, ln) in enumerate(runs):
+                    ch = ln
+                    if i == 0:
+                        ch -= min(ch, pb if op == ord("D") else tb if op == ord("I") else 0)
+                    if i == len(runs) - 1:
+                        ch -= min(ch, pe if op == ord("D") else te if op == ord("I") else 0)
+                    if op == ord("M"):
+                        pen_total += ln * m
+                    elif op == ord("X"):
+                        pen_total += ln * pen[1]
+                    elif ch > 0:
+                        if metric == "gap_linear":
+                            pen_total += ch * pen[2]
+                        elif metric == "gap_affine":
+                            pen_total += pen[2] + ch * pen[3]
+                        else:
+                            pen_total += min(pen[2] + ch * pen[3], pen[4] + ch * pen[5])
+                assert pen_total >= _endsfree_dp(oracle, metric, pen, p, t, ends)
 
 
 def test_oracle_zdrop_refusals(N, oracle):

@@ -278,8 +278,10 @@ static __device__ __forceinline__ int64_t hist_row(const Hist& H, const int4& h)
 }
 
 // Winner flags of one (score, k) in medium/low memory. Bits 0-2: the
-// source of M (0 I1, 1 I2, 2 D1, 3 D2, 4 mismatch; for edit 0 ins, 1 del,
-// 2 mismatch); bits 3-6: I1, D1, I2, D2 took their extension input.
+// source of M (0 I1, 1 I2, 2 D1, 3 D2, 4 mismatch; for edit and gap-linear
+// 0 ins, 1 del, 2 mismatch; 7 a begin-free cell, where a chain starts);
+// bits 3-6: I1, D1, I2, D2 took their extension input.
+#define FL_M_SEED 7
 #define FL_I1_EXT 8
 #define FL_D1_EXT 16
 #define FL_I2_EXT 32
@@ -607,6 +609,8 @@ static __device__ int piggyback_backtrace(const KParams& p, const Hist& H, int n
     const int4 hd = H.hdr[s];
     if (kk < hd.x || kk > hd.y) { bad = true; break; }
     const int f = H.bytes[hist_row(H, hd) + (kk - hd.x)];
+    // a begin-free cell added during compute starts its chain
+    if (comp == CM && (f & 7) == FL_M_SEED) break;
     uint8_t op;
     if (KIND == KIND_EDIT || KIND == KIND_LINEAR) {
       // one M component: a mismatch came from M[s - x], a gap from
@@ -969,10 +973,15 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   const int tb_free = ef && !ext ? fe.z : 0;
   const int pe_free = !ef ? 0 : ext ? plen : fe.y;
   const int te_free = !ef ? 0 : ext ? tlen : fe.w;
+  // with a match reward the begin-free cells come during compute
+  const bool seedable = p.u_match < 0 && ef && (pb_free > 0 || tb_free > 0);
+  // X-drop and Z-drop leave free leading gaps out of the Smith-Waterman
+  // score only with match == 0 (DropParams::new, src/heuristic.rs:92-107)
+  const int sw_pb = p.u_match == 0 ? pb_free : 0, sw_tb = p.u_match == 0 ? tb_free : 0;
   // M0: k = 0, offset 0, plus (ends-free, match == 0) the begin-free cells
   // (h, 0) at k = h and (0, v) at k = -v (init_alignment,
   // src/aligner.rs:938-998)
-  const int lo0 = -pb_free, hi0 = tb_free;
+  const int lo0 = p.u_match == 0 ? -pb_free : 0, hi0 = p.u_match == 0 ? tb_free : 0;
   bool overflow = hi0 - lo0 + 1 > R.wcap;
   if (!overflow) {
     for (int k = lo0 + tid; k <= hi0; k += NT) R.set(k & R.mask, k > 0 ? k : 0);
@@ -1116,10 +1125,10 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     const int4 mm = R.meta[rowM];
     const int lo = uniform(mm.x), hi = uniform(mm.y);
     if (hc.kind == WFA_HEURISTIC_ZDROP)
-      return zdrop_check(p, hc, hs, R, rowM, lo, hi, s, pb_free, tb_free);
+      return zdrop_check(p, hc, hs, R, rowM, lo, hi, s, sw_pb, sw_tb);
     int nlo = lo, nhi = hi;
-    if (!heuristic_limits(p, hc, hc.p0, hc.p1, hs, R, rowM, lo, hi, s, plen, tlen, pb_free,
-                          tb_free, &nlo, &nhi))
+    if (!heuristic_limits(p, hc, hc.p0, hc.p1, hs, R, rowM, lo, hi, s, plen, tlen, sw_pb,
+                          sw_tb, &nlo, &nhi))
       return false;
     if (nlo == lo && nhi == hi) return false;
     const bool empty = nlo > nhi;
@@ -1257,10 +1266,11 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       // wavefront, whose bounds (1, -1) still enter the range union
       // (src/aligner.rs:2045-2062, 2265-2289). Without a heuristic the extra
       // cells only see NULL and are trimmed away, so those kernels leave the
-      // null bounds out; after a cut they can read the cut cells.
+      // null bounds out; after a cut they can read the cut cells, and with
+      // begin-free cells to add the computed range decides which stay.
       auto bound = [&](int l, int h, int dlo, int dhi) {
         if (l <= h) any = true;
-        else if (!HEUR) return;
+        else if (!HEUR && !seedable) return;
         else { l = 1; h = -1; }
         lo = imin(lo, l + dlo);
         hi = imax(hi, h + dhi);
@@ -1286,8 +1296,20 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     const int rowM = sM;
     const int rowE = sE1;
     const int rowE2 = two ? sE2 : 0;
+    // ends-free with a match reward: the begin-free cells (h = d, 0) and
+    // (0, v = d) are reached at score d * -match (endsfree_begin_k,
+    // src/aligner.rs:3369-3386); -1 when this score has none
+    int seed_k = -1;
+    if (KIND != KIND_EDIT && seedable && s % (-p.u_match) == 0) {
+      const int d = s / (-p.u_match);
+      if (tb_free >= d || pb_free >= d) seed_k = d;
+    }
+    // a begin-free score whose inputs are all null: M holds just the
+    // begin-free cells, and the step still counts as null
+    // (endsfree_null_m, src/aligner.rs:3439-3465)
+    const bool seed_only = !any && seed_k >= 0;
     __syncthreads();  // every lane has read the metadata before it is rewritten
-    if (!any) {
+    if (!any && !seed_only) {
       // null step (src/aligner.rs:2031-2043, 1778-1782)
       nulls += 1;
       if (tid == 0) {
@@ -1307,10 +1329,14 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       if (nulls > p.scope) { status = WFA_STATUS_END_UNREACHABLE; result = s; break; }
       continue;
     }
-    nulls = 0;
-    lo = imax(lo, hist_lo + 1);
-    hi = imin(hi, hist_hi - 1);
-    if (lo > hi) {
+    if (seed_only) {
+      nulls += 1;
+    } else {
+      nulls = 0;
+      lo = imax(lo, hist_lo + 1);
+      hi = imin(hi, hist_hi - 1);
+    }
+    if (!seed_only && lo > hi) {
       // a band (or the historic limits) left nothing to compute: no outputs
       // at this score (src/aligner.rs:2058-2081)
       if (tid == 0) {
@@ -1328,6 +1354,27 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       __syncthreads();
       if (s >= p.max_steps) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
       continue;
+    }
+    // [clo, chi]: the cells the recurrences compute. The begin-free cells
+    // widen [lo, hi] over NULL cells to reach them (endsfree_init,
+    // src/aligner.rs:3388-3419): the text cell replaces a computed cell
+    // that reaches no further, and is lost below the computed range; the
+    // pattern cell likewise above it
+    const int clo = seed_only ? 1 : lo, chi = seed_only ? 0 : hi;
+    bool tseed = false, pseed = false;
+    if (seed_k >= 0) {
+      const int d = seed_k;
+      if (seed_only) {
+        tseed = tb_free >= d;
+        pseed = pb_free >= d;
+        lo = pseed ? -d : d;
+        hi = tseed ? d : -d;
+      } else {
+        tseed = tb_free >= d && d >= lo;
+        if (tseed) hi = imax(hi, d);
+        pseed = pb_free >= d && -d <= hi;
+        if (pseed) lo = imin(lo, -d);
+      }
     }
     if (hi - lo + 1 > R.wcap) { overflow = true; break; }
     int4 hrow = make_int4(lo, hi, 0, 0);
@@ -1463,6 +1510,27 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           R.set((rD1 + rowE) * R.wcap + (k & R.mask), d1);
           vi = in_bounds(i1, k, plen, tlen);
           vd = in_bounds(d1, k, plen, tlen);
+        }
+        if (KIND != KIND_EDIT && seed_k >= 0) {
+          if (k < clo || k > chi) {
+            // beyond the computed range: NULL, the gap components too
+            m = WFA_NULL;
+            if (GAPS) {
+              R.set((rI1 + rowE) * R.wcap + (k & R.mask), WFA_NULL);
+              R.set((rD1 + rowE) * R.wcap + (k & R.mask), WFA_NULL);
+              vi = vd = false;
+              if (two) {
+                R.set((rI2 + rowE2) * R.wcap + (k & R.mask), WFA_NULL);
+                R.set((rD2 + rowE2) * R.wcap + (k & R.mask), WFA_NULL);
+                vi2 = vd2 = false;
+              }
+            }
+          }
+          // endsfree_init_offset (src/aligner.rs:3421-3437)
+          if ((tseed && k == seed_k && m <= seed_k) || (pseed && k == -seed_k && m <= 0)) {
+            m = k > 0 ? k : 0;
+            if (MED) fl = (fl & ~7) | FL_M_SEED;
+          }
         }
         vm = m != WFA_NULL;
         R.set(rowM * R.wcap + (k & R.mask), m);

@@ -79,9 +79,6 @@ inline int score_scope(const Internal& p) {
 // WFA_ERR_UNSUPPORTED rather than silently falling back to anything.
 inline int check_supported(const wfa_attr_t& a) {
   if (a.span != WFA_SPAN_END2END && a.span != WFA_SPAN_ENDSFREE) return WFA_ERR_INVALID_ARGUMENT;
-  // ends-free with a match reward adds begin-free cells during compute
-  // (src/aligner.rs:3369-3389): not on the device path yet
-  if (a.span == WFA_SPAN_ENDSFREE && a.penalties.match != 0) return WFA_ERR_UNSUPPORTED;
   if (a.span == WFA_SPAN_ENDSFREE && !a.extension &&
       (a.pattern_begin_free < 0 || a.pattern_end_free < 0 || a.text_begin_free < 0 ||
        a.text_end_free < 0))
