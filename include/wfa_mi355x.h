@@ -50,6 +50,15 @@ extern "C" {
 #define WFA_SPAN_END2END     0
 #define WFA_SPAN_ENDSFREE    1
 
+/* ---- Compat (src/aligner.rs:53-94) ---- */
+/* C: results identical to C WFA2-lib, its known bugs included (the
+ * default). CORRECTED: ends-free with a match reward returns the optimal
+ * alignment (candidate ends charged for the bases their free end gaps skip;
+ * the score is the whole alignment's), and a Z-drop end is backtraced from
+ * the best cell at the score it was found at in every memory mode. */
+#define WFA_COMPAT_C          0
+#define WFA_COMPAT_CORRECTED  1
+
 /* ---- Heuristics (src/heuristic.rs, HeuristicStrategy) ---- */
 #define WFA_HEURISTIC_NONE             0
 #define WFA_HEURISTIC_WFADAPTIVE       1  /* p0 = min_wavefront_length, p1 = max_distance_threshold, p2 = steps_between_cutoffs */
@@ -117,6 +126,7 @@ typedef struct wfa_attr_t {
     int32_t heuristic;              /* WFA_HEURISTIC_* */
     int32_t heuristic_p[3];
     int32_t max_alignment_steps;
+    int32_t compat;                 /* WFA_COMPAT_* (set_compat, src/aligner.rs:53-94) */
 } wfa_attr_t;
 
 /* Defaults of WavefrontAligner::new (src/aligner.rs:409-458) for `metric`,

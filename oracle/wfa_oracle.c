@@ -241,6 +241,15 @@ struct wfo_aligner {
      * drop (HeuristicState, src/heuristic.rs:60-69) */
     int h_max_sw_off, h_max_wf_score, h_zlast;
     int32_t* dist; int cap_dist;
+    /* Compat::Corrected (src/aligner.rs:53-94, 1111-1360): the best ends-free
+     * end candidate so far (EndsfreeCandidate) with its piggyback chain; the
+     * begin-free dominance record (furthest offset per component and
+     * diagonal); the Z-drop best cell's chain */
+    int cand_valid, cand_total, cand_score, cand_k, cand_off;
+    uint32_t cand_pc, cand_pv;
+    int dom_on, dom_base, dom_len, cap_dom;
+    int32_t* dom;
+    uint32_t zbest_pc, zbest_pv;
     /* BiWFA sub-aligners (forward, reverse: score-only; base: CIGAR),
      * created on first use (src/aligner.rs:2721-2740) */
     struct wfo_aligner *bi_fwd, *bi_rev, *bi_base;
@@ -432,7 +441,8 @@ wfo_aligner* wfo_aligner_new(const wfa_attr_t* attr, int32_t* err) {
      * wavefront holds there (HPCBio/wfa2lib-rs#22, src/aligner.rs:1443-1460):
      * not restated. BiWFA rejects X-drop and Z-drop as the driver does. */
     const int zdrop = attr->heuristic == WFA_HEURISTIC_ZDROP;
-    if ((zdrop && attr->alignment_scope == WFA_SCOPE_ALIGNMENT &&
+    if ((zdrop && attr->compat != WFA_COMPAT_CORRECTED &&
+         attr->alignment_scope == WFA_SCOPE_ALIGNMENT &&
          (attr->memory_mode == WFA_MEMORY_MED || attr->memory_mode == WFA_MEMORY_LOW)) ||
         (attr->memory_mode == WFA_MEMORY_ULTRALOW &&
          (attr->span != WFA_SPAN_END2END || attr->heuristic == WFA_HEURISTIC_XDROP || zdrop))) {
@@ -460,6 +470,7 @@ void wfo_aligner_free(wfo_aligner* a) {
     free(a->init_h);
     free(a->packed);
     free(a->dist);
+    free(a->dom);
     wfo_aligner_free(a->bi_fwd);
     wfo_aligner_free(a->bi_rev);
     wfo_aligner_free(a->bi_base);
@@ -603,6 +614,28 @@ static void init_alignment_component(wfo_aligner* a, int comp) {
     a->end_score = 0; a->end_k = 0; a->end_offset = 0;
     a->hist_lo = -(plen + scope + 2);
     a->hist_hi = tlen + scope + 2;
+    a->cand_valid = 0;
+    a->zbest_pc = 0; a->zbest_pv = BT_NULL;
+    /* begin-free dominance pruning (src/aligner.rs:925-945): corrected
+     * ends-free with a match reward and begin-free gaps (at most 4 for the
+     * affine kinds), no heuristic reading the wavefronts */
+    a->dom_on = 0;
+    if (a->attr.compat == WFA_COMPAT_CORRECTED && a->pen.match < 0 && a->endsfree) {
+        const int bf = a->pb_free > a->tb_free ? a->pb_free : a->tb_free;
+        const int heur = a->attr.heuristic;
+        if (bf > 0 && (a->pen.metric == WFA_METRIC_GAP_LINEAR || bf <= 4) &&
+            (heur == WFA_HEURISTIC_NONE || heur == WFA_HEURISTIC_BANDED_STATIC)) {
+            a->dom_on = 1;
+            a->dom_base = a->hist_lo;
+            a->dom_len = a->hist_hi - a->hist_lo + 1;
+            const int n = ncomp_of(a->pen.metric) * a->dom_len;
+            if (n > a->cap_dom) {
+                a->dom = (int32_t*)realloc(a->dom, sizeof(int32_t) * (size_t)n);
+                a->cap_dom = n;
+            }
+            for (int i = 0; i < n; ++i) a->dom[i] = -1;
+        }
+    }
     int alo, ahi;
     int lo0 = 0, hi0 = 0;
     if (a->endsfree && a->pen.match == 0) { lo0 = -a->pb_free; hi0 = a->tb_free; }
@@ -810,7 +843,18 @@ static int apply_heuristic_cutoff(wfo_aligner* a, int score, wf_t* wf) {
         wf->lo >= a->attr.heuristic_p[0] && wf->hi <= a->attr.heuristic_p[1])
         return 0;
     int lo_base = wf->lo, hi_base = wf->hi;
-    if (heuristic_cutoff(a, wf, score)) {
+    const int max_wf_base = a->h_max_wf_score;
+    const int zdropped = heuristic_cutoff(a, wf, score);
+    /* Corrected: keep the backtrace of a new Z-drop best cell, which the
+     * end backtrace needs after its wavefront is gone (med/low,
+     * src/aligner.rs:1676-1688) */
+    if (a->attr.compat == WFA_COMPAT_CORRECTED && a->piggyback &&
+        a->h_max_wf_score == score && max_wf_base != score) {
+        const int k = a->h_max_sw_k;
+        a->zbest_pc = wf->pc[k - wf->base];
+        a->zbest_pv = wf->pv[k - wf->base];
+    }
+    if (zdropped) {
         a->end_score = a->h_max_wf_score;
         a->end_k = a->h_max_sw_k;
         a->end_offset = a->h_max_sw_off;
@@ -828,6 +872,160 @@ static int apply_heuristic_cutoff(wfo_aligner* a, int score, wf_t* wf) {
         if (wf->lo > d->lo) d->lo = wf->lo;
         if (wf->hi < d->hi) d->hi = wf->hi;
         if (d->lo > d->hi) { d->null = 1; d->init_min = d->lo; d->init_max = d->hi; }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Compat::Corrected ends-free with a match reward                     */
+/* ------------------------------------------------------------------ */
+static inline int term_endsfree(const wfo_aligner* a, int k, int offset);
+
+/* Wavefront::trim_nulled (src/wavefront.rs:459-490): narrow [lo, hi] to
+ * [lo, hi] ∩ [blo, bhi], NULLing the cells dropped (the initialized range
+ * is kept). */
+static void trim_nulled(wf_t* w, int blo, int bhi) {
+    if (!w || w->null || w->lo > w->hi || (blo <= w->lo && bhi >= w->hi)) return;
+    const int olo = w->lo, ohi = w->hi;
+    const int nlo = blo > olo ? blo : olo, nhi = bhi < ohi ? bhi : ohi;
+    if (nlo > nhi) {
+        for (int k = olo; k <= ohi; ++k) wf_set(w, k, OFFSET_NULL);
+        w->null = 1;
+    } else {
+        for (int k = olo; k < nlo; ++k) wf_set(w, k, OFFSET_NULL);
+        for (int k = nhi + 1; k <= ohi; ++k) wf_set(w, k, OFFSET_NULL);
+    }
+    w->lo = nlo;
+    w->hi = nhi;
+}
+
+/* endsfree_prune (src/aligner.rs:1203-1245): every completion of a cell on
+ * diagonal k at `score` totals at least score + -match * |k - k_end|; the
+ * diagonals that cannot beat (tie) `total` are dropped from every
+ * component at this score. */
+static void endsfree_prune(wfo_aligner* a, int score, int total) {
+    const int radius = (total - score) / -a->pen.match;
+    const int k_end = a->tlen - a->plen;
+    const int nc = ncomp_of(a->pen.metric);
+    for (int c = 0; c < nc; ++c) trim_nulled(get(a, c, score), k_end - radius, k_end + radius);
+}
+
+/* endsfree_dominance_prune (src/aligner.rs:1247-1312): drop from the ends
+ * of a component just computed the cells that reach no further than a cell
+ * of that component did at an earlier score; `record` adds the kept ones
+ * to the record (I/D; M is recorded after its extension). */
+static void dominance_prune(wfo_aligner* a, wf_t* w, int c, int record) {
+    if (!w || w->lo > w->hi) return;
+    const int lo = w->lo, hi = w->hi;
+    int32_t* best = a->dom + c * a->dom_len - a->dom_base;
+    int first = lo, last = hi;
+    while (first <= hi && wf_get(w, first) <= best[first]) ++first;
+    while (last >= first && wf_get(w, last) <= best[last]) --last;
+    if (record)
+        for (int k = first; k <= last; ++k) {
+            const int32_t o = wf_get(w, k);
+            if (o > best[k]) best[k] = o;
+        }
+    for (int k = lo; k < first; ++k) wf_set(w, k, OFFSET_NULL);
+    for (int k = last + 1; k <= hi; ++k) wf_set(w, k, OFFSET_NULL);
+    if (first > last) {
+        w->lo = 1; w->hi = 0; w->null = 1;
+    } else {
+        w->lo = first; w->hi = last;
+    }
+}
+
+/* endsfree_dominance_update (src/aligner.rs:1314-1330) */
+static void dominance_update(wfo_aligner* a, const wf_t* w, int c) {
+    if (!w || w->null || w->lo > w->hi) return;
+    int32_t* best = a->dom + c * a->dom_len - a->dom_base;
+    for (int k = w->lo; k <= w->hi; ++k) {
+        const int32_t o = wf_get(w, k);
+        if (o > best[k]) best[k] = o;
+    }
+}
+
+/* after a compute step: dominance pruning of every component (order and
+ * recording as src/aligner.rs:2183-2190, 2540-2551, 1998-2001) */
+static void dominance_prune_step(wfo_aligner* a, int score) {
+    if (!a->dom_on) return;
+    dominance_prune(a, get(a, C_M, score), 0, 0);
+    if (a->pen.metric == WFA_METRIC_GAP_AFFINE || a->pen.metric == WFA_METRIC_GAP_AFFINE_2P) {
+        dominance_prune(a, get(a, C_I1, score), 1, 1);
+        dominance_prune(a, get(a, C_D1, score), 2, 1);
+    }
+    if (a->pen.metric == WFA_METRIC_GAP_AFFINE_2P) {
+        dominance_prune(a, get(a, C_I2, score), 3, 1);
+        dominance_prune(a, get(a, C_D2, score), 4, 1);
+    }
+}
+
+/* endsfree_left (src/termination.rs:65-86): the bases a free end gap skips
+ * from a cell at a free end (-1 if none) */
+static int endsfree_left(const wfo_aligner* a, int k, int offset) {
+    const int h = offset, v = offset - k;
+    const int pl = a->plen - v, tl = a->tlen - h;
+    const int by_text = h >= a->tlen && pl <= a->pe_free ? pl : -1;
+    const int by_pattern = v >= a->plen && tl <= a->te_free ? tl : -1;
+    if (by_text >= 0 && by_pattern >= 0) return by_text < by_pattern ? by_text : by_pattern;
+    return by_text >= 0 ? by_text : by_pattern;
+}
+
+/* extend_endsfree_corrected (src/aligner.rs:1111-1200): a cell reaching a
+ * free end at `score` with `left` bases to skip completes an alignment of
+ * total score + -match * left; the best candidate ends the search once the
+ * score reaches its total. */
+static int extend_endsfree_corrected(wfo_aligner* a, int score) {
+    wf_t* m = get(a, C_M, score);
+    const int heur = a->attr.heuristic;
+    const int prune = heur == WFA_HEURISTIC_NONE || heur == WFA_HEURISTIC_BANDED_STATIC;
+    const int mr = -a->pen.match;
+    if (prune && a->cand_valid && a->cand_total > score) endsfree_prune(a, score, a->cand_total);
+    if (m) {
+        /* extend_matches_packed_endsfree_best (src/extend.rs:179-232): the
+         * first cell in ascending k with the fewest bases left */
+        int bk = 0, boff = 0, bleft = -1;
+        for (int k = m->lo; k <= m->hi; ++k) {
+            int32_t off = wf_get(m, k);
+            if (off == OFFSET_NULL) continue;
+            off += lcp(a, off - k, off);
+            wf_set(m, k, off);
+            if (!term_endsfree(a, k, off)) continue;
+            const int left = endsfree_left(a, k, off);
+            if (left >= 0 && (bleft < 0 || left < bleft)) { bk = k; boff = off; bleft = left; }
+        }
+        if (bleft >= 0) {
+            const int total = score + mr * bleft;
+            if (!a->cand_valid || total < a->cand_total) {
+                a->cand_valid = 1;
+                a->cand_total = total;
+                a->cand_score = score; a->cand_k = bk; a->cand_off = boff;
+                /* bt_or_zero (src/wavefront.rs:405-416) */
+                a->cand_pc = a->piggyback ? m->pc[bk - m->base] : 0;
+                a->cand_pv = a->piggyback ? m->pv[bk - m->base] : BT_NULL;
+                if (prune && total > score) endsfree_prune(a, score, total);
+            }
+        }
+    }
+    if (a->dom_on && !is_null_wf(m)) dominance_update(a, m, 0);
+    if (a->cand_valid && a->cand_total <= score) {
+        a->end_score = a->cand_score; a->end_k = a->cand_k; a->end_offset = a->cand_off;
+        a->status = WFA_STATUS_END_REACHED;
+        return 1;
+    }
+    int done;
+    if (!m) /* with a candidate null steps do not end the search */
+        done = !a->cand_valid && a->num_null_steps > a->scope;
+    else
+        done = heur != WFA_HEURISTIC_NONE && apply_heuristic_cutoff(a, score, m);
+    if (done) {
+        if (a->cand_valid) {
+            a->end_score = a->cand_score; a->end_k = a->cand_k; a->end_offset = a->cand_off;
+            a->status = WFA_STATUS_END_REACHED;
+        } else {
+            a->status = WFA_STATUS_END_UNREACHABLE;
+        }
+        return 1;
     }
     return 0;
 }
@@ -872,6 +1070,8 @@ static inline int term_endsfree(const wfo_aligner* a, int k, int offset) {
  * extend_matches_packed_endsfree (src/extend.rs:112-170): the first
  * diagonal in ascending k that reaches a free end wins. */
 static int extend_endsfree(wfo_aligner* a, int score) {
+    if (a->attr.compat == WFA_COMPAT_CORRECTED && a->pen.match < 0)
+        return extend_endsfree_corrected(a, score);
     wf_t* m = get(a, C_M, score);
     if (!m) {
         if (a->num_null_steps > a->scope) {
@@ -1093,6 +1293,7 @@ static void compute_affine_step(wfo_aligner* a, int score) {
     trim_ends(plen, tlen, d1);
     if (i2) trim_ends(plen, tlen, i2);
     if (d2) trim_ends(plen, tlen, d2);
+    dominance_prune_step(a, score);
 }
 
 /* compute_edit_step (src/aligner.rs:1769-1865) with compute_edit_idm /
@@ -1242,6 +1443,7 @@ static void compute_linear_step(wfo_aligner* a, int score) {
     put(a, C_M, score, cur);
     if (bk >= 0) endsfree_init(a, cur, bk);
     trim_ends(plen, tlen, cur);
+    dominance_prune_step(a, score);
 }
 
 static void compute_step(wfo_aligner* a, int score) {
@@ -1431,10 +1633,18 @@ static void backtrace_linear(wfo_aligner* a, int score, int k, int offset) {
 
 /* backtrace_piggyback (src/aligner.rs:1443-1481) -> traceback_from
  * (src/bt_buffer.rs:198-209) -> unpack_piggyback (src/aligner.rs:232-311). */
+static void backtrace_piggyback_bt(wfo_aligner* a, uint32_t pcigar, uint32_t prev, int score,
+                                   int k, int offset);
 static void backtrace_piggyback(wfo_aligner* a, int score, int k, int offset) {
     const wf_t* m = get(a, C_M, score);
     if (!m || !wf_allocated(m, k)) { a->cig_begin = a->cig_end = 0; return; }
-    uint32_t pcigar = m->pc[k - m->base], prev = m->pv[k - m->base];
+    backtrace_piggyback_bt(a, m->pc[k - m->base], m->pv[k - m->base], score, k, offset);
+}
+
+/* backtrace_piggyback_bt (src/aligner.rs:1462-1487): unpack the chain
+ * (pcigar, prev) of M cell (k, offset) at `score`. */
+static void backtrace_piggyback_bt(wfo_aligner* a, uint32_t pcigar, uint32_t prev, int score,
+                                   int k, int offset) {
     /* chain: collect pcigars end-first until the initial block */
     int np = 0;
     while (prev != BT_NULL) {
@@ -1499,10 +1709,16 @@ static void backtrace_from(wfo_aligner* a, int score, int k, int offset) {
         backtrace_linear(a, score, k, offset);
 }
 
-/* backtrace_zdrop_end (src/aligner.rs:1393-1406), Compat::C: the best cell,
- * backtraced from the score it was found at (high memory; Z-drop with a
- * medium/low-memory CIGAR is refused at construction). */
+/* backtrace_zdrop_end (src/aligner.rs:1393-1406): the best cell, backtraced
+ * from the score it was found at (Compat::C: high memory; its
+ * medium/low-memory CIGAR is refused at construction. Corrected: in every
+ * memory mode, med/low from the chain kept when the cell was found). */
 static void backtrace_zdrop_end(wfo_aligner* a, int score) {
+    if (a->attr.compat == WFA_COMPAT_CORRECTED && a->piggyback) {
+        backtrace_piggyback_bt(a, a->zbest_pc, a->zbest_pv, a->end_score, a->end_k,
+                               a->end_offset);
+        return;
+    }
     const int from = !a->piggyback && a->end_score >= 0 ? a->end_score : score;
     backtrace_from(a, from, a->end_k, a->end_offset);
 }
@@ -1696,16 +1912,31 @@ int32_t wfo_align_endsfree(wfo_aligner* a, const uint8_t* pattern, int32_t plen,
     int result = score;
     if (a->status == WFA_STATUS_END_REACHED || a->status == WFA_STATUS_END_UNREACHABLE) {
         int unreachable = a->status == WFA_STATUS_END_UNREACHABLE;
+        /* Corrected: the end candidate's cell can be from an earlier score,
+         * and its total is the alignment's score (src/aligner.rs:769-776) */
+        const int candidate = a->cand_valid && !unreachable;
+        if (candidate) result = a->cand_total;
         if (a->attr.alignment_scope == WFA_SCOPE_ALIGNMENT &&
             (a->status == WFA_STATUS_END_REACHED || a->end_offset != OFFSET_NULL)) {
-            if (unreachable) backtrace_zdrop_end(a, score);
-            else backtrace_from(a, score, a->end_k, a->end_offset);
+            if (candidate) {
+                if (a->piggyback)
+                    backtrace_piggyback_bt(a, a->cand_pc, a->cand_pv, a->cand_score, a->cand_k,
+                                           a->cand_off);
+                else
+                    backtrace_from(a, a->cand_score, a->cand_k, a->cand_off);
+                a->cig_score = a->cand_total;
+            } else if (unreachable) {
+                backtrace_zdrop_end(a, score);
+            } else {
+                backtrace_from(a, score, a->end_k, a->end_offset);
+            }
             if (a->attr.extension || unreachable) {
                 int trimmed = maxtrim_cigar(a);
                 a->status = (trimmed || unreachable) ? WFA_STATUS_PARTIAL : WFA_STATUS_COMPLETED;
             } else {
-                result = rescore_heuristic_cigar(a, score, a->end_offset - a->end_k,
-                                                 a->end_offset);
+                if (!candidate)
+                    result = rescore_heuristic_cigar(a, score, a->end_offset - a->end_k,
+                                                     a->end_offset);
                 a->status = WFA_STATUS_COMPLETED;
             }
         } else {

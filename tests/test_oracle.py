@@ -457,6 +457,103 @@ def test_oracle_endsfree_match_reward_invariants(metric, pen, pkg, N, oracle):
                 assert pen_total >= _endsfree_dp(oracle, metric, pen, p, t, ends)
 
 
+def _penalty_of(w, m, plen, tlen):
+    """The real penalty of a WFA score (wf_score_to_sw_score,
+    src/penalties.rs; tests/corrected_endsfree.rs:178-190)."""
+    return -((-m * (plen + tlen) - w) // 2) if m < 0 else w
+
+
+def _corrected(N, metric, pen, scope, span, mem=None):
+    a = attr_for(N, metric, pen, scope, span=span)
+    a.compat = N.COMPAT_CORRECTED
+    if mem is not None:
+        a.memory_mode = mem
+    return a
+
+
+def test_oracle_corrected_literals(pkg, N, oracle):
+    """Compat::Corrected: tests/corrected_endsfree.rs:293-312 (issue 37:
+    penalty -47, MMMDMMMMMMMM) and :314-330 (issue 38: -78, the DP
+    optimum); tests/corrected_zdrop.rs:91-102 (Z-drop 50,10: high, med and
+    low memory give valid partial alignments of the same score)."""
+    pen = (-5, 4, 0, 8, 0, 0)
+    s, st, ops = oracle.OracleAligner(_corrected(N, "gap_affine", pen, "alignment",
+                                                 [12, 12, 11, 11])).align(b"ACGTACGTACGT",
+                                                                          b"ACGACGTACGT")
+    assert (_penalty_of(s, -5, 12, 11), ops) == (-47, b"MMMDMMMMMMMM")
+    pat = b"AAGAAGCACCGGCTAACTCCGTGCCAGCAGCCGCGGT"
+    txt = b"AGTCGTGACTCTTTTTGCATGTACCTTATGAATAAGGACCGGCTAATTCCG"
+    ends = [len(pat), len(pat), len(txt), len(txt)]
+    assert _endsfree_dp(oracle, "gap_affine", pen, pat, txt, ends) == -78
+    for scope in ("score", "alignment"):
+        s, st, _ = oracle.OracleAligner(_corrected(N, "gap_affine", pen, scope, ends)).align(pat, txt)
+        assert _penalty_of(s, -5, len(pat), len(txt)) == -78
+    p, t = _issue22_pair()
+    scores = set()
+    for mem in (N.MEMORY_HIGH, N.MEMORY_MED, N.MEMORY_LOW):
+        a = _zdrop_attr(N, "gap_affine", (-3, 5, 3, 2, 0, 0), "alignment", 50, 10)
+        a.compat, a.memory_mode = N.COMPAT_CORRECTED, mem
+        s, st, ops = oracle.OracleAligner(a).align_end2end(p, t)
+        assert st == STATUS["PARTIAL"] and ops and valid_prefix(ops, p, t)
+        scores.add(pkg.Cigar(ops).score_gap_affine(pkg.AffinePenalties(-3, 5, 3, 2)))
+    assert len(scores) == 1
+
+
+def test_oracle_corrected_endsfree_is_optimal(pkg, N, oracle):
+    """Corrected ends-free (and extension) with a match reward returns the
+    optimal alignment (tests/corrected_endsfree.rs:356-420 checks the same
+    against this DP): on random short pairs, free ends and penalties
+    (dominance pruning included), every memory mode's penalty equals the
+    DP optimum and its CIGAR is valid (a valid prefix for extension)."""
+    import random
+    rng = random.Random(5)
+    seq = lambda n: bytes(rng.choice(b"ACGT") for _ in range(n))
+    for _ in range(300):
+        metric = rng.choice(["gap_affine", "gap_affine_2p", "gap_linear"])
+        m = -(1 + rng.randrange(5))
+        pen = (m, 1 + rng.randrange(6), rng.randrange(6), 1 + rng.randrange(4),
+               rng.randrange(12), 1 + rng.randrange(2))
+        if metric == "gap_linear":
+            pen = (m, pen[1], 1 + rng.randrange(5), 0, 0, 0)
+        p, t = seq(rng.randrange(18)), seq(rng.randrange(18))
+        ends = [rng.randrange(len(p) + 1), rng.randrange(len(p) + 1),
+                rng.randrange(len(t) + 1), rng.randrange(len(t) + 1)]
+        if rng.random() < 0.2:
+            ends = "extension"
+        e = [0, len(p), 0, len(t)] if ends == "extension" else ends
+        want = _endsfree_dp(oracle, metric, pen, p, t, e)
+        for mem in (N.MEMORY_HIGH, N.MEMORY_MED, N.MEMORY_LOW):
+            s, st, ops = oracle.OracleAligner(_corrected(N, metric, pen, "alignment", ends,
+                                                         mem)).align(p, t)
+            assert _penalty_of(s, m, len(p), len(t)) == want, (metric, pen, p, t, ends, mem)
+            if ends == "extension":
+                assert valid_prefix(ops, p, t)
+            elif ops:
+                assert pkg.Cigar(ops).check_alignment(p, t) is None
+
+
+def test_oracle_corrected_equals_c_with_match0(N, oracle):
+    """tests/corrected_endsfree.rs:333-354: with match == 0 both modes give
+    identical results."""
+    import random
+    rng = random.Random(7)
+    seq = lambda n: bytes(rng.choice(b"ACGT") for _ in range(n))
+    for _ in range(150):
+        metric = rng.choice(["gap_affine", "gap_affine_2p", "gap_linear"])
+        pen = (0, 1 + rng.randrange(6), rng.randrange(6), 1 + rng.randrange(4),
+               rng.randrange(12), 1 + rng.randrange(2))
+        if metric == "gap_linear":
+            pen = (0, pen[1], 1 + rng.randrange(5), 0, 0, 0)
+        p, t = seq(rng.randrange(16)), seq(rng.randrange(16))
+        ends = [rng.randrange(len(p) + 1), rng.randrange(len(p) + 1),
+                rng.randrange(len(t) + 1), rng.randrange(len(t) + 1)]
+        for mem in (N.MEMORY_HIGH, N.MEMORY_MED):
+            c = attr_for(N, metric, pen, "alignment", span=ends)
+            c.memory_mode = mem
+            assert oracle.OracleAligner(c).align(p, t) == \
+                oracle.OracleAligner(_corrected(N, metric, pen, "alignment", ends, mem)).align(p, t)
+
+
 def test_oracle_zdrop_refusals(N, oracle):
     """Z-drop with a medium/low-memory CIGAR (the reference's known issue
     HPCBio/wfa2lib-rs#22) and with BiWFA is refused; score-only medium
@@ -468,4 +565,8 @@ def test_oracle_zdrop_refusals(N, oracle):
             oracle.OracleAligner(a)
     a = _zdrop_attr(N, "gap_affine", (0, 4, 6, 2, 0, 0), "score", 20, 1)
     a.memory_mode = N.MEMORY_MED
+    oracle.OracleAligner(a)
+    # Compat::Corrected backtraces the best cell's own chain: runs
+    a = _zdrop_attr(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment", 20, 1)
+    a.memory_mode, a.compat = N.MEMORY_MED, N.COMPAT_CORRECTED
     oracle.OracleAligner(a)

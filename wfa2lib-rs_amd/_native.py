@@ -17,6 +17,7 @@ METRIC_INDEL, METRIC_EDIT, METRIC_GAP_LINEAR, METRIC_GAP_AFFINE, METRIC_GAP_AFFI
 SCOPE_SCORE, SCOPE_ALIGNMENT = 0, 1
 MEMORY_HIGH, MEMORY_MED, MEMORY_LOW, MEMORY_ULTRALOW = range(4)
 SPAN_END2END, SPAN_ENDSFREE = 0, 1
+COMPAT_C, COMPAT_CORRECTED = 0, 1
 (HEURISTIC_NONE, HEURISTIC_WFADAPTIVE, HEURISTIC_XDROP, HEURISTIC_ZDROP,
  HEURISTIC_BANDED_STATIC, HEURISTIC_BANDED_ADAPTIVE) = range(6)
 (STATUS_OK, STATUS_END_REACHED, STATUS_END_UNREACHABLE, STATUS_MAX_STEPS_REACHED,
@@ -41,7 +42,8 @@ class Attr(ctypes.Structure):
     _fields_ = [("penalties", Penalties)] + [(n, ctypes.c_int32) for n in (
         "alignment_scope", "memory_mode", "span", "extension", "pattern_begin_free",
         "pattern_end_free", "text_begin_free", "text_end_free", "heuristic")] + [
-        ("heuristic_p", ctypes.c_int32 * 3), ("max_alignment_steps", ctypes.c_int32)]
+        ("heuristic_p", ctypes.c_int32 * 3), ("max_alignment_steps", ctypes.c_int32),
+        ("compat", ctypes.c_int32)]
 
 
 class Batch(ctypes.Structure):
