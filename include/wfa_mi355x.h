@@ -156,6 +156,8 @@ int32_t wfa_aligner_set_alignment_scope(wfa_aligner_t* aligner, int32_t scope);
 /* set_memory_mode (src/aligner.rs:462); WFA_MEMORY_ULTRALOW selects BiWFA as
  * the CLI does (src/bin/align_benchmark.rs:510-519). */
 int32_t wfa_aligner_set_memory_mode(wfa_aligner_t* aligner, int32_t mode);
+/* set_compat (src/aligner.rs:606-608): WFA_COMPAT_C or WFA_COMPAT_CORRECTED */
+int32_t wfa_aligner_set_compat(wfa_aligner_t* aligner, int32_t compat);
 /* set_heuristic (src/aligner.rs:611) */
 int32_t wfa_aligner_set_heuristic(wfa_aligner_t* aligner, int32_t heuristic,
                                   int32_t p0, int32_t p1, int32_t p2);

@@ -618,9 +618,13 @@ static void init_alignment_component(wfo_aligner* a, int comp) {
     a->zbest_pc = 0; a->zbest_pv = BT_NULL;
     /* begin-free dominance pruning (src/aligner.rs:925-945): corrected
      * ends-free with a match reward and begin-free gaps (at most 4 for the
-     * affine kinds), no heuristic reading the wavefronts */
+     * affine kinds), no heuristic reading the wavefronts. Begin-free gaps
+     * longer than their sequence (outside the reference's contract: its
+     * driver refuses them) put seeds beyond the record's diagonals, which
+     * then stays off. */
     a->dom_on = 0;
-    if (a->attr.compat == WFA_COMPAT_CORRECTED && a->pen.match < 0 && a->endsfree) {
+    if (a->attr.compat == WFA_COMPAT_CORRECTED && a->pen.match < 0 && a->endsfree &&
+        a->pb_free <= a->plen && a->tb_free <= a->tlen) {
         const int bf = a->pb_free > a->tb_free ? a->pb_free : a->tb_free;
         const int heur = a->attr.heuristic;
         if (bf > 0 && (a->pen.metric == WFA_METRIC_GAP_LINEAR || bf <= 4) &&

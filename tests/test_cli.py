@@ -73,7 +73,6 @@ def test_cli_output_lines(args, vecs, pkg, tmp_path):
      "Invalid --wfa-heuristic-parameters"),
     (["--wfa-span", "ends-free,1,2"], "Invalid --wfa-span"),
     (["--wfa-memory", "huge"], "--wfa-memory"),
-    (["--no-c-compat"], "not available on the device path"),
     (["--wfa-heuristic", "zdrop", "--wfa-heuristic-parameters", "20,1", "--wfa-memory", "med"],
      "medium/low-memory CIGAR"),
     (["-g", "1,4,6,2"], "invalid penalties"),
@@ -90,6 +89,39 @@ def test_cli_rejections(args, msg, pkg, tmp_path, capsys):
         align_benchmark.main(["-i", str(seq)] + args, out=io.StringIO())
     assert e.value.code == 1
     assert msg in capsys.readouterr().err
+
+
+@pytest.mark.gpu
+def test_cli_corrected_unreachable_prints_minus_one(pkg, tmp_path):
+    """tests/cli_output.rs:391-437 (HPCBio/wfa2lib-rs#23): a score-only
+    result with no alignment prints C's converted give-up score (-113 with
+    a match reward, the negated score with match == 0, INT32_MIN for a
+    failed BiWFA score); --no-c-compat prints -1 for each."""
+    if not have_gpu():
+        pytest.skip("no GPU")
+    from wfa2lib_rs_amd import align_benchmark
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+    def run(path, args):
+        out = io.StringIO()
+        assert align_benchmark.main(["-i", path] + args, out=out) == 0
+        return out.getvalue().splitlines()
+
+    issue23 = os.path.join(golden, "issue23_unreachable.seq")
+    xdrop = ["-a", "gap-affine-wfa", "-g", "-3,5,3,2", "--wfa-heuristic", "xdrop",
+             "--wfa-heuristic-parameters", "20,1", "--wfa-score-only"]
+    assert run(issue23, xdrop) == ["-113\t-"]
+    assert run(issue23, xdrop + ["--no-c-compat"]) == ["-1\t-"]
+    pattern = b"ACGTTGCAAGCT" * 10
+    seq = tmp_path / "u.seq"
+    seq.write_bytes(b">" + pattern + b"\n<" + pattern[:90] + b"\n")
+    band = ["-a", "gap-affine-wfa", "--wfa-heuristic", "banded-static",
+            "--wfa-heuristic-parameters", "-5,5", "--wfa-score-only"]
+    assert run(str(seq), band) != ["-1\t-"]
+    assert run(str(seq), band + ["--no-c-compat"]) == ["-1\t-"]
+    bi = band + ["-g", "-3,5,3,2", "--wfa-memory", "ultralow"]
+    assert run(str(seq), bi) == ["-2147483648\t-"]
+    assert run(str(seq), bi + ["--no-c-compat"]) == ["-1\t-"]
 
 
 def test_cli_reader_and_nominal_prop(pkg, tmp_path):

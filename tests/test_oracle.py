@@ -532,6 +532,45 @@ def test_oracle_corrected_endsfree_is_optimal(pkg, N, oracle):
                 assert pkg.Cigar(ops).check_alignment(p, t) is None
 
 
+def seq_pair(name):
+    """The first pair of a tests/data/*.seq fixture (copied to tests/golden/)."""
+    with open(os.path.join(GOLDEN, name)) as f:
+        lines = f.read().split("\n")
+    return lines[0].lstrip(">").encode(), lines[1].lstrip("<").encode()
+
+
+def test_oracle_corrected_fixtures(pkg, N, oracle):
+    """tests/corrected_endsfree.rs:523-554: gap-linear (-2,4,3), banded-static
+    -10..200 off the end diagonal, whole free ends: the optimum is a
+    begin-free seed reached after every wavefront emptied, so null steps
+    must not end the search while a candidate is pending; every memory mode
+    gives a valid alignment of the DP optimum. tests/corrected_endsfree.rs:
+    611-624 (issue 38 pair B, 16S reads, (-5,4,0,8), whole free ends):
+    Compat::C returns an all-gap alignment, Corrected the DP optimum."""
+    pat, txt = seq_pair("corrected_banded_seeds.seq")
+    pen = (-2, 4, 3, 0, 0, 0)
+    ends = [len(pat), len(pat), len(txt), len(txt)]
+    want = _endsfree_dp(oracle, "gap_linear", pen, pat, txt, ends)
+    for mem in (N.MEMORY_HIGH, N.MEMORY_MED, N.MEMORY_LOW):
+        a = _corrected(N, "gap_linear", pen, "alignment", ends, mem)
+        a.heuristic = N.HEURISTIC_BANDED_STATIC
+        a.heuristic_p[0], a.heuristic_p[1] = -10, 200
+        s, st, ops = oracle.OracleAligner(a).align(pat, txt)
+        assert _penalty_of(s, -2, len(pat), len(txt)) == want, mem
+        assert pkg.Cigar(ops).check_alignment(pat, txt) is None
+    pat, txt = seq_pair("issue38_pair_b.seq")
+    pen = (-5, 4, 0, 8, 0, 0)
+    ends = [len(pat), len(pat), len(txt), len(txt)]
+    _, _, ops = oracle.OracleAligner(attr_for(N, "gap_affine", pen, "alignment",
+                                              span=ends)).align(pat, txt)
+    assert ops and set(ops) <= set(b"ID")
+    want = _endsfree_dp(oracle, "gap_affine", pen, pat, txt, ends)
+    s, st, ops = oracle.OracleAligner(_corrected(N, "gap_affine", pen, "alignment",
+                                                 ends)).align(pat, txt)
+    assert _penalty_of(s, -5, len(pat), len(txt)) == want < 0
+    assert pkg.Cigar(ops).check_alignment(pat, txt) is None
+
+
 def test_oracle_corrected_equals_c_with_match0(N, oracle):
     """tests/corrected_endsfree.rs:333-354: with match == 0 both modes give
     identical results."""

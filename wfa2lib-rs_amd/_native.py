@@ -18,6 +18,8 @@ SCOPE_SCORE, SCOPE_ALIGNMENT = 0, 1
 MEMORY_HIGH, MEMORY_MED, MEMORY_LOW, MEMORY_ULTRALOW = range(4)
 SPAN_END2END, SPAN_ENDSFREE = 0, 1
 COMPAT_C, COMPAT_CORRECTED = 0, 1
+# the offset of an empty cell (WFA_NULL: INT32_MIN / 2)
+OFFSET_NULL = -(2 ** 30)
 (HEURISTIC_NONE, HEURISTIC_WFADAPTIVE, HEURISTIC_XDROP, HEURISTIC_ZDROP,
  HEURISTIC_BANDED_STATIC, HEURISTIC_BANDED_ADAPTIVE) = range(6)
 (STATUS_OK, STATUS_END_REACHED, STATUS_END_UNREACHABLE, STATUS_MAX_STEPS_REACHED,
@@ -94,6 +96,7 @@ SIGNATURES = {
     "wfa_aligner_free": (None, [_VP]),
     "wfa_aligner_set_alignment_scope": (_I32, [_VP, _I32]),
     "wfa_aligner_set_memory_mode": (_I32, [_VP, _I32]),
+    "wfa_aligner_set_compat": (_I32, [_VP, _I32]),
     "wfa_aligner_set_heuristic": (_I32, [_VP, _I32, _I32, _I32, _I32]),
     "wfa_aligner_set_alignment_free_ends": (_I32, [_VP, _I32, _I32, _I32, _I32]),
     "wfa_aligner_set_alignment_extension": (_I32, [_VP]),

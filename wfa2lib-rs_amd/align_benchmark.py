@@ -16,7 +16,8 @@ unpaired last line is ignored). Output: one `score<TAB>CIGAR` line per pair
 (`-` for no CIGAR), exactly as the reference prints them; `Time.Alignment`
 (device time of the batches) and, with -c, `Alignments.Correct` on stderr.
 Configurations the device path does not run (Z-drop with a medium/low-memory
-CIGAR, --no-c-compat) exit with an error instead of falling back to a CPU.
+CIGAR in C-compatible mode) exit with an error instead of falling back to a
+CPU; --no-c-compat selects Compat::Corrected.
 """
 import argparse
 import ctypes
@@ -203,16 +204,16 @@ def main(argv=None, out=None):
     if is_drop and penalties.distance_metric in (DistanceMetric.EDIT, DistanceMetric.INDEL):
         fail("[WFA] Heuristics drops are not compatible with 'edit'/'indel' distance metrics")
     if (heur is not None and heur[0] == N.HEURISTIC_ZDROP and not score_only and
-            memory in (N.MEMORY_MED, N.MEMORY_LOW)):
+            memory in (N.MEMORY_MED, N.MEMORY_LOW) and not a.no_c_compat):
         fail("[WFA] Z-drop with a medium/low-memory CIGAR is not available on the device path")
-    if a.no_c_compat:
-        fail("[WFA] --no-c-compat (Compat::Corrected) is not available on the device path")
 
     attr = N.Attr()
     N.lib().wfa_attr_default(ctypes.byref(attr), int(penalties.distance_metric))
     attr.penalties = penalties._c
     attr.alignment_scope = N.SCOPE_SCORE if score_only else N.SCOPE_ALIGNMENT
     attr.memory_mode = memory
+    if a.no_c_compat:
+        attr.compat = N.COMPAT_CORRECTED
     if heur is not None:
         attr.heuristic = heur[0]
         for i, v in enumerate(heur[1]):
@@ -278,15 +279,21 @@ def main(argv=None, out=None):
             # the end was reached; else the aligner's end position (a
             # Z-drop's best cell, at the score it was found at, or the unset
             # position, whose v = offset - k wraps)
+            no_alignment = False
             if not use_biwfa and st in (N.STATUS_PARTIAL, N.STATUS_END_UNREACHABLE):
                 ps, pk, poff = (int(x) for x in ends[i])
+                no_alignment = poff == N.OFFSET_NULL
                 end_v = (poff - pk + 2 ** 31) % 2 ** 32 - 2 ** 31
                 end_h = poff
                 if ps >= 0:
                     wfa_score = ps
             else:
                 end_v, end_h = len(p), len(t)
-            if biwfa_failed:
+            # Corrected mode prints -1 for a score-only result with no
+            # alignment, as CIGAR mode does (src/bin/align_benchmark.rs:642-651)
+            if score_only and a.no_c_compat and (biwfa_failed or no_alignment):
+                score = -1
+            elif biwfa_failed:
                 score = INT32_MIN
             else:
                 score = compute_output_score(cig, wfa_score, penalties, score_only, end_v, end_h)

@@ -310,6 +310,11 @@ class WavefrontAligner:
     def set_memory_mode(self, mode: MemoryMode):
         N.check(N.lib().wfa_aligner_set_memory_mode(self._h, int(mode)))
 
+    def set_compat(self, compat: int):
+        """Compat::C (COMPAT_C, the default) or Compat::Corrected
+        (COMPAT_CORRECTED; src/aligner.rs:606-608)."""
+        N.check(N.lib().wfa_aligner_set_compat(self._h, int(compat)))
+
     def set_heuristic(self, strategy: HeuristicStrategy):
         N.check(N.lib().wfa_aligner_set_heuristic(self._h, strategy.kind, *strategy.params))
 

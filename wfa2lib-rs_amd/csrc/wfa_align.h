@@ -963,9 +963,10 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     fe = make_int4(imax(uniform(c0.z), 0), imax(uniform(c0.w), 0), imax(uniform(c1.x), 0),
                    imax(uniform(c1.y), 0));
     hc = HeurCfg{uniform(c1.z), uniform(c1.w), uniform(c2.x), uniform(c2.y)};
-    // (Z-drop with a medium/low-memory CIGAR is refused for launches; a
-    // pair asking for it there runs without a heuristic)
-    if (p.heuristic == WFA_HEURISTIC_NONE || (MED && hc.kind == WFA_HEURISTIC_ZDROP) ||
+    // (Z-drop with a medium/low-memory CIGAR is refused for Compat::C
+    // launches; a pair asking for it there runs without a heuristic)
+    if (p.heuristic == WFA_HEURISTIC_NONE ||
+        (MED && hc.kind == WFA_HEURISTIC_ZDROP && p.compat != WFA_COMPAT_CORRECTED) ||
         hc.kind < WFA_HEURISTIC_NONE || hc.kind > WFA_HEURISTIC_BANDED_ADAPTIVE)
       hc.kind = WFA_HEURISTIC_NONE;
   }
@@ -1021,12 +1022,47 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   int status = WFA_STATUS_OK;
   int result = 0;
 
+  // ---- Compat::Corrected ends-free with a match reward
+  // (src/aligner.rs:1111-1360; the heuristic kernels, one wave): a cell
+  // reaching a free end at score s with `left` bases to skip completes an
+  // alignment of total s + -match * left; the best candidate ends the
+  // search once the score reaches its total. Band pruning against the
+  // candidate's total (no heuristic or banded-static only) and begin-free
+  // dominance pruning drop cells that cannot do better.
+  const bool corrected = HEUR && NW == 1 && ef && corrected_endsfree(p);
+  const bool cprune =
+      corrected && (hc.kind == WFA_HEURISTIC_NONE || hc.kind == WFA_HEURISTIC_BANDED_STATIC);
+  const int mr = -p.u_match;
+  bool ext_found = false;  // the last extension's best free-end cell
+  int ext_left = 0, ext_k = 0, ext_off = 0;
+  bool cand_valid = false;
+  int cand_total = 0, cand_s = 0, cand_k = 0, cand_off = 0;
+  int end_s = -1;  // the end cell's score when it is a candidate's
+  // dominance record: per component and diagonal the furthest offset seen
+  // (src/aligner.rs:925-945: begin-free gaps, at most 4 for gap-affine)
+  constexpr int NCOMP = kind_components(KIND);
+  const int bfree = imax(pb_free, tb_free);
+  // (begin-free gaps longer than their sequence put seeds beyond the
+  // record's diagonals [hist_lo, hist_hi]: the record stays off)
+  const bool dom_on = cprune && bfree > 0 && (KIND == KIND_LINEAR || bfree <= 4) &&
+                      pb_free <= plen && tb_free <= tlen;
+  const int dlen = hist_hi - hist_lo + 1;
+  int32_t* const dom = dom_on ? p.dom + (int64_t)blockIdx.x * p.dom_stride : nullptr;
+  if (dom_on) {
+    for (int i = tid; i < NCOMP * dlen; i += NT) dom[i] = -1;
+    __syncthreads();
+  }
+
   // extend + terminate + (CIGAR) record M[score] over its rows; `lo/hi`
   // is the computed (pre-trim) range; returns true if the end is reached.
   auto extend_row = [&](int s, int row, int lo, int hi, int4 hrow) -> bool {
     bool reached = false;
     uint64_t hit = 0;  // ends-free: first chunk with a terminating diagonal
     int hit_kb = 0;
+    // corrected: the fewest bases left, then the lowest k
+    // (extend_matches_packed_endsfree_best, src/extend.rs:179-232)
+    int64_t cbest = INT64_MAX;
+    int cboff = 0;
     const int64_t base = MODE == 1 ? hist_row(H, hrow) : 0;
     // U chunks at a time: their offsets are read before the first one's
     // extension is written back
@@ -1046,10 +1082,21 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           const int v = off - k;
           term = off >= 0 &&
                  ((off >= tlen && plen - v <= pe_free) || (v >= plen && tlen - off <= te_free));
+          if (corrected && term) {
+            // endsfree_left (src/termination.rs:65-86)
+            int left = INT32_MAX;
+            if (off >= tlen && plen - v <= pe_free) left = plen - v;
+            if (v >= plen && tlen - off <= te_free) left = imin(left, tlen - off);
+            const int64_t key = ((int64_t)left << 32) | (uint32_t)(k + (1 << 30));
+            if (key < cbest) {
+              cbest = key;
+              cboff = off;
+            }
+          }
         }
         if (MODE == 1) H.cells[base + (k - lo)] = off < 0 ? WFA_NULL : off;
       }
-      if (ef && !hit) {
+      if (ef && !corrected && !hit) {
         // the lowest terminating diagonal wins (src/extend.rs:133-166)
         hit = ballot(term);
         hit_kb = kb;
@@ -1083,6 +1130,16 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       }
     }
     if (ef) reached = hit != 0;
+    if (corrected) {
+      const int64_t w = ~wave_max64(~cbest);  // the wave's minimum key
+      ext_found = w != INT64_MAX;
+      if (ext_found) {
+        const uint64_t holder = ballot(cbest == w);
+        ext_off = uniform(__shfl(cboff, (int)__builtin_ctzll(holder)));
+        ext_left = (int)(w >> 32);
+        ext_k = (int)(uint32_t)(w & 0xffffffffu) - (1 << 30);
+      }
+    }
     if (MODE == 1 && tid == 0) H.hdr[s] = hrow;
     bool any_reached = ballot(reached) != 0;
     if (NW > 1) {
@@ -1169,18 +1226,111 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   // (src/aligner.rs:1690-1700); the alignment is partial
   bool zdropped = false;
 
+  // Corrected: every component at score s keeps only the diagonals within
+  // (total - s) / -match of k_end, the others NULLed (endsfree_prune,
+  // src/aligner.rs:1203-1245, with trim_nulled, src/wavefront.rs:459-490);
+  // `hist`: M's stored CIGAR row is NULLed too (high memory, after the
+  // extension)
+  auto band_prune = [&](int s, int total, int rowM, int rowE, int rowE2, bool hist) {
+    const int r = (total - s) / mr;
+    const int blo = k_end - r, bhi = k_end + r;
+    auto trim = [&](int row, bool is_m) {
+      const int4 mm = R.meta[row];
+      const int l = uniform(mm.x), h = uniform(mm.y);
+      if (l > h || (blo <= l && bhi >= h)) return;
+      const int nl = imax(blo, l), nh = imin(bhi, h);
+      int4 hd = make_int4(1, 0, 0, 0);
+      if (MODE == 1 && is_m && hist) hd = H.hdr[s];
+      for (int k = l + tid; k <= h; k += NT) {
+        if (k >= nl && k <= nh) continue;
+        R.set(row * R.wcap + (k & R.mask), WFA_NULL);
+        if (MODE == 1 && k >= hd.x && k <= hd.y) H.cells[hist_row(H, hd) + (k - hd.x)] = WFA_NULL;
+      }
+      __syncthreads();
+      if (tid == 0) R.meta[row] = make_int4(nl, nh, mm.z, mm.w);
+      __syncthreads();
+    };
+    trim(rowM, true);
+    if (GAPS) {
+      trim(rI1 + rowE, false);
+      trim(rD1 + rowE, false);
+    }
+    if (two) {
+      trim(rI2 + rowE2, false);
+      trim(rD2 + rowE2, false);
+    }
+  };
+  // Corrected: M's extended offsets enter the dominance record
+  // (endsfree_dominance_update, src/aligner.rs:1314-1330)
+  auto dom_update = [&](int rowM) {
+    const int4 mm = R.meta[rowM];
+    const int l = uniform(mm.x), h = uniform(mm.y);
+    int32_t* best = dom - hist_lo;
+    for (int k = l + tid; k <= h; k += NT) {
+      const int o = R.at(rowM * R.wcap + (k & R.mask));
+      if (o > best[k]) best[k] = o;
+    }
+    __syncthreads();
+  };
+  auto cand_end = [&]() {
+    status = WFA_STATUS_END_REACHED;
+    result = cand_total;
+    end_s = cand_s;
+    end_k = cand_k;
+    end_off = cand_off;
+  };
+  // Corrected: the extension of M[s] and its end test
+  // (extend_endsfree_corrected, src/aligner.rs:1111-1200); true when the
+  // alignment ends here
+  auto corrected_step = [&](int s, int rowM, int rowE, int rowE2, int elo, int ehi,
+                            int4 hrow) -> bool {
+    if (cprune && cand_valid && cand_total > s) band_prune(s, cand_total, rowM, rowE, rowE2, false);
+    extend_row(s, rowM, elo, ehi, hrow);
+    __syncthreads();
+    if (ext_found) {
+      const int total = s + mr * ext_left;
+      if (!cand_valid || total < cand_total) {
+        cand_valid = true;
+        cand_total = total;
+        cand_s = s;
+        cand_k = ext_k;
+        cand_off = ext_off;
+        if (cprune && total > s) band_prune(s, total, rowM, rowE, rowE2, true);
+      }
+    }
+    if (dom_on) dom_update(rowM);
+    if (cand_valid && cand_total <= s) {
+      cand_end();
+      return true;
+    }
+    if (heur != WFA_HEURISTIC_NONE && cutoff(s, rowM, rowE, rowE2)) {
+      if (cand_valid) {
+        cand_end();
+      } else {
+        status = WFA_STATUS_END_UNREACHABLE;
+        result = s;
+        zdropped = true;
+      }
+      return true;
+    }
+    return false;
+  };
+
   // score 0
   {
     int4 h0 = make_int4(lo0, hi0, 0, 0);
     if (CIG && !overflow && (H.hdr_cap < 1 || !halloc(hi0 - lo0 + 1, &h0, lo0, hi0)))
       overflow = true;
     else if (MED && tid == 0) H.hdr[0] = h0;
-    if (!overflow && extend_row(0, 0, lo0, hi0, h0)) {
+    if (corrected) {
+      if (!overflow) corrected_step(0, 0, 0, 0, lo0, hi0, h0);
+    } else if (!overflow && extend_row(0, 0, lo0, hi0, h0)) {
       status = WFA_STATUS_END_REACHED;
       result = 0;
     }
     __syncthreads();
-    if (!overflow && status == WFA_STATUS_OK && heur != WFA_HEURISTIC_NONE && cutoff(0, 0, 0, 0)) {
+    if (!corrected && !overflow && status == WFA_STATUS_OK && heur != WFA_HEURISTIC_NONE &&
+        cutoff(0, 0, 0, 0)) {
       status = WFA_STATUS_END_UNREACHABLE;
       result = 0;
       zdropped = true;
@@ -1326,6 +1476,11 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       }
       __syncthreads();
       if (s >= p.max_steps) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
+      // Corrected: with a candidate, null steps do not end the search
+      if (corrected && cand_valid) {
+        if (cand_total <= s) { cand_end(); break; }
+        continue;
+      }
       if (nulls > p.scope) { status = WFA_STATUS_END_UNREACHABLE; result = s; break; }
       continue;
     }
@@ -1353,6 +1508,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       }
       __syncthreads();
       if (s >= p.max_steps) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
+      if (corrected && cand_valid && cand_total <= s) { cand_end(); break; }
       continue;
     }
     // [clo, chi]: the cells the recurrences compute. The begin-free cells
@@ -1530,6 +1686,10 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           if ((tseed && k == seed_k && m <= seed_k) || (pseed && k == -seed_k && m <= 0)) {
             m = k > 0 ? k : 0;
             if (MED) fl = (fl & ~7) | FL_M_SEED;
+            // a seed beyond its sequence (a free begin gap longer than it)
+            // is out of bounds, the end cell trim_ends drops
+            // (src/wavefront.rs); a seed-only M keeps it (endsfree_null_m)
+            if (!seed_only && !in_bounds(m, k, plen, tlen)) m = WFA_NULL;
           }
         }
         vm = m != WFA_NULL;
@@ -1608,6 +1768,52 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     auto fix = [](int& l, int& h) { if (l == INT32_MAX) { l = 1; h = 0; } };
     fix(m_lo, m_hi); fix(i_lo, i_hi); fix(d_lo, d_hi); fix(i2_lo, i2_hi); fix(d2_lo, d2_hi);
     __syncthreads();  // ring rows written before the extension reads them
+    if (dom_on && !seed_only) {
+      // Corrected: drop from each component's ends the cells that reach no
+      // further than that component did on their diagonal at an earlier
+      // score; I/D record what they keep (endsfree_dominance_prune,
+      // src/aligner.rs:1247-1312, order as src/aligner.rs:2183-2190)
+      auto dom_prune = [&](int row, int c, bool record, int& l, int& h) {
+        if (l > h) return;
+        int32_t* best = dom + c * dlen - hist_lo;
+        const int rb = row * R.wcap;
+        int first = INT32_MAX, last = INT32_MIN;
+        for (int kb = l; kb <= h; kb += 64) {
+          const int k = kb + lane;
+          const uint64_t b = ballot(k <= h && R.at(rb + (k & R.mask)) > best[k]);
+          if (b) {
+            if (first == INT32_MAX) first = kb + __builtin_ctzll(b);
+            last = kb + 63 - __builtin_clzll(b);
+          }
+        }
+        for (int k = l + lane; k <= h; k += 64) {
+          const int cell = rb + (k & R.mask);
+          if (k < first || k > last) {
+            R.set(cell, WFA_NULL);
+          } else if (record) {
+            const int o = R.at(cell);
+            if (o > best[k]) best[k] = o;
+          }
+        }
+        if (first == INT32_MAX) {
+          l = 1;
+          h = 0;
+        } else {
+          l = first;
+          h = last;
+        }
+      };
+      dom_prune(rowM, 0, false, m_lo, m_hi);
+      if (GAPS) {
+        dom_prune(rI1 + rowE, 1, true, i_lo, i_hi);
+        dom_prune(rD1 + rowE, 2, true, d_lo, d_hi);
+      }
+      if (two) {
+        dom_prune(rI2 + rowE2, 3, true, i2_lo, i2_hi);
+        dom_prune(rD2 + rowE2, 4, true, d2_lo, d2_hi);
+      }
+      __syncthreads();
+    }
     if (MED && tid == 0) H.hdr[s] = hrow;
     if (tid == 0) {
       R.meta[rowM] = make_int4(m_lo, m_hi, m_lo, m_hi);
@@ -1627,6 +1833,10 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     // outside the trimmed range are OFFSET_NULL there anyway.
     const int elo = MODE == 1 ? lo : m_lo, ehi = MODE == 1 ? hi : m_hi;
     if (MODE == 1 && !halloc(hi - lo + 1, &hrow, elo, ehi)) { overflow = true; break; }
+    if (corrected) {
+      if (corrected_step(s, rowM, rowE, rowE2, elo, ehi, hrow)) break;
+      continue;
+    }
     const bool reached =
         extend_row(s, rowM, elo, ehi, hrow) && (ef || (m_lo <= k_end && k_end <= m_hi));
     __syncthreads();
@@ -1653,7 +1863,8 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   // cell (found at an earlier score), or unset
   int pos_s = -1, pos_k = INT32_MAX, pos_off = WFA_NULL;
   if (status == WFA_STATUS_END_REACHED) {
-    pos_s = score; pos_k = end_k; pos_off = end_off;
+    // (a Corrected candidate can be from an earlier score)
+    pos_s = end_s >= 0 ? end_s : score; pos_k = end_k; pos_off = end_off;
   } else if (zdropped) {
     pos_s = hs.max_wf; pos_k = hs.best_k; pos_off = hs.best_off;
   }
@@ -1699,7 +1910,8 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         bool trimmed = false;
         ncig = maxtrim_cigar(p, out, ncig, pb_free, tb_free, &trimmed);
         if (trimmed) out_status = WFA_STATUS_PARTIAL;
-      } else if (!zdropped && heur != WFA_HEURISTIC_NONE && GAPS) {
+      } else if (!zdropped && !corrected && heur != WFA_HEURISTIC_NONE && GAPS) {
+        // (a Corrected candidate's total is its score)
         __threadfence_block();
         const int trailing = ef ? (plen - (end_off - end_k)) + (tlen - end_off) : 0;
         result = rescore_cigar(p, out, ncig, ef, pb_free, tb_free, trailing);

@@ -44,6 +44,7 @@ struct LaunchCfg {
   int pool_chunks;
   int tab_cap;       // chunks one slot may hold
   int chain_cap;
+  int dom_stride;    // corrected ends-free dominance record per slot (ints)
 };
 
 
@@ -88,7 +89,7 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   // rings in HBM are wide: several waves share each pair's rows (heuristic
   // cut-offs need the single-wave kernel); their cross-wave reductions take
   // 16 ints per wave of LDS
-  c->nw = !c->lds_ring && p.heuristic == WFA_HEURISTIC_NONE ? wfa::kWideWaves : 1;
+  c->nw = !c->lds_ring && !wfa::heur_kernel(p) ? wfa::kWideWaves : 1;
   // The wide kernels are bound by the rings' HBM traffic (every score reads
   // and writes whole rows): 16-bit rows for pairs of at most 16000 bases
   // halve it (offsets stay below 2^15; a NULL row value rises by at most
@@ -114,13 +115,19 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
     c->seq_words = ((max_len + per - 1) / per + 2 + 1) & ~1;
     c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : red_bytes);
     per_cu = wfa::max_blocks_per_cu(kind, mode, c->lds_ring, c->i16, fmt == WFA_SEQ_PACKED2BITS,
-                                    p.heuristic != WFA_HEURISTIC_NONE, c->nw, c->lds_bytes);
+                                    wfa::heur_kernel(p), c->nw, c->lds_bytes);
   }
   int grid = e->num_cus * per_cu;
   if (long_reads && e->long_grid_cap > 0) grid = std::min(grid, e->long_grid_cap);
   grid = std::max(1, std::min(grid, npairs));
   c->grid = grid;
   int64_t per_slot = c->lds_ring ? 0 : (int64_t)c->ring_rows * c->wcap * 4;
+  // corrected ends-free: one pair's dominance record (components x the
+  // diagonals of [-(plen+scope+2), tlen+scope+2])
+  c->dom_stride = wfa::corrected_endsfree(p)
+                      ? wfa::kind_components(kind) * (2 * max_len + 2 * p.scope + 8)
+                      : 0;
+  per_slot += (int64_t)c->dom_stride * 4;
   if (cig) {
     // worst case: every score up to the slot bound, every row full width
     const int64_t nwf_bound = (int64_t)std::max(2 * max_len, 1000) * 64 + 16;
@@ -211,6 +218,12 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
       if (r) return r;
       p.chain = e->d_chain;
     }
+  }
+  p.dom_stride = c.dom_stride;
+  if (c.dom_stride > 0) {
+    int r = grow((void**)&e->d_dom, &e->dom_bytes, (size_t)c.grid * c.dom_stride * 4);
+    if (r) return r;
+    p.dom = e->d_dom;
   }
   HIPCHK(wfa::launch_reset(e->d_counters, st));
   if (c.window)
@@ -382,6 +395,7 @@ void wfa_engine_free(wfa_engine_t* e) {
   hipFree(e->d_gring);
   hipFree(e->d_chain);
   hipFree(e->d_tab);
+  hipFree(e->d_dom);
   hipFree(e->d_order);
   for (auto b : e->bi_buf) hipFree(b);
   hipHostFree(e->h_pinned);
@@ -675,6 +689,12 @@ int32_t wfa_aligner_set_alignment_scope(wfa_aligner_t* a, int32_t scope) {
 int32_t wfa_aligner_set_memory_mode(wfa_aligner_t* a, int32_t mode) {
   if (!a || mode < WFA_MEMORY_HIGH || mode > WFA_MEMORY_ULTRALOW) return WFA_ERR_INVALID_ARGUMENT;
   a->attr.memory_mode = mode;
+  return WFA_OK;
+}
+int32_t wfa_aligner_set_compat(wfa_aligner_t* a, int32_t compat) {
+  if (!a || (compat != WFA_COMPAT_C && compat != WFA_COMPAT_CORRECTED))
+    return WFA_ERR_INVALID_ARGUMENT;
+  a->attr.compat = compat;
   return WFA_OK;
 }
 int32_t wfa_aligner_set_heuristic(wfa_aligner_t* a, int32_t h, int32_t p0, int32_t p1,

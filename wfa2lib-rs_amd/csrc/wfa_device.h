@@ -53,6 +53,11 @@ struct KParams {
   const int64_t* cig_off;
   int32_t* cig_len;
   int32_t* end_pos;          // optional {score, k, offset} per pair
+  int compat;                // WFA_COMPAT_*
+  // Compat::Corrected begin-free dominance record: dom_stride ints per
+  // workgroup (components x diagonals), or null
+  int32_t* dom;
+  int dom_stride;
   // scheduling
   int32_t* counters;         // [0] work-queue head, [1] retry count
   int32_t* retry_list;
@@ -166,6 +171,17 @@ hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, in
 // Waves per workgroup (one pair each) of the wide kernels, whose rings live
 // in HBM, without heuristics
 constexpr int kWideWaves = 8;
+
+// Compat::Corrected ends-free alignment with a match reward: candidate
+// ends, band and dominance pruning (src/aligner.rs:1104-1360)
+__host__ __device__ inline bool corrected_endsfree(const KParams& p) {
+  return p.compat == WFA_COMPAT_CORRECTED && p.u_match < 0 && p.endsfree;
+}
+// Launches that need the single-wave kernels with the per-score
+// cut-off logic: heuristics, and corrected ends-free alignment
+__host__ __device__ inline bool heur_kernel(const KParams& p) {
+  return p.heuristic != WFA_HEURISTIC_NONE || corrected_endsfree(p);
+}
 
 // Waves per workgroup of a BiWFA launch (`which` 0 = breakpoint search,
 // 1 = base cases): breakpoint searches without a heuristic take kWideWaves

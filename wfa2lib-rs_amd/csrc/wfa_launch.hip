@@ -136,8 +136,7 @@ hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, in
 
 hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, bool i16, int nw,
                         int grid, size_t lds_bytes, hipStream_t stream) {
-  KernelFn fn = pick(kind, mode, lds_ring, i16, p.fmt == WFA_SEQ_PACKED2BITS,
-                     p.heuristic != WFA_HEURISTIC_NONE, nw);
+  KernelFn fn = pick(kind, mode, lds_ring, i16, p.fmt == WFA_SEQ_PACKED2BITS, heur_kernel(p), nw);
   if (!fn) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fn, dim3(grid), dim3(64 * nw), lds_bytes, stream, p);
   return hipGetLastError();
