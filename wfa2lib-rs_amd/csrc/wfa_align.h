@@ -975,7 +975,8 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   const int pe_free = !ef ? 0 : ext ? plen : fe.y;
   const int te_free = !ef ? 0 : ext ? tlen : fe.w;
   // with a match reward the begin-free cells come during compute
-  const bool seedable = p.u_match < 0 && ef && (pb_free > 0 || tb_free > 0);
+  // (the single-wave kernels only: those launches take them)
+  const bool seedable = HEUR && p.u_match < 0 && ef && (pb_free > 0 || tb_free > 0);
   // X-drop and Z-drop leave free leading gaps out of the Smith-Waterman
   // score only with match == 0 (DropParams::new, src/heuristic.rs:92-107)
   const int sw_pb = p.u_match == 0 ? pb_free : 0, sw_tb = p.u_match == 0 ? tb_free : 0;
@@ -1420,7 +1421,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       // begin-free cells to add the computed range decides which stay.
       auto bound = [&](int l, int h, int dlo, int dhi) {
         if (l <= h) any = true;
-        else if (!HEUR && !seedable) return;
+        else if (!HEUR) return;
         else { l = 1; h = -1; }
         lo = imin(lo, l + dlo);
         hi = imax(hi, h + dhi);
@@ -1511,31 +1512,53 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       if (corrected && cand_valid && cand_total <= s) { cand_end(); break; }
       continue;
     }
-    // [clo, chi]: the cells the recurrences compute. The begin-free cells
-    // widen [lo, hi] over NULL cells to reach them (endsfree_init,
+    // [lo, hi]: the cells the recurrences compute; [wlo, whi]: the row,
+    // widened over NULL cells to reach the begin-free cells (endsfree_init,
     // src/aligner.rs:3388-3419): the text cell replaces a computed cell
     // that reaches no further, and is lost below the computed range; the
     // pattern cell likewise above it
-    const int clo = seed_only ? 1 : lo, chi = seed_only ? 0 : hi;
-    bool tseed = false, pseed = false;
+    // The seeding state lives in one scalar across the recurrences' loop
+    // (register pressure): seeds = -1, or d | SEED_ONLY | SEED_TEXT |
+    // SEED_PATTERN.
+    constexpr int SEED_ONLY = 1 << 28, SEED_TEXT = 1 << 29, SEED_PATTERN = 1 << 30;
+    int seeds = -1;
     if (seed_k >= 0) {
       const int d = seed_k;
+      bool tseed, pseed;
       if (seed_only) {
         tseed = tb_free >= d;
         pseed = pb_free >= d;
-        lo = pseed ? -d : d;
-        hi = tseed ? d : -d;
+        lo = 1;
+        hi = 0;
       } else {
         tseed = tb_free >= d && d >= lo;
-        if (tseed) hi = imax(hi, d);
-        pseed = pb_free >= d && -d <= hi;
-        if (pseed) lo = imin(lo, -d);
+        pseed = pb_free >= d && -d <= (tseed ? imax(hi, d) : hi);
       }
+      seeds = d | (seed_only ? SEED_ONLY : 0) | (tseed ? SEED_TEXT : 0) |
+              (pseed ? SEED_PATTERN : 0);
     }
-    if (hi - lo + 1 > R.wcap) { overflow = true; break; }
-    int4 hrow = make_int4(lo, hi, 0, 0);
-    if (MED && !halloc(hi - lo + 1, &hrow, lo, hi)) { overflow = true; break; }
-    const int64_t fbase = MED ? hist_row(H, hrow) : 0;
+    // the row's range from the computed range and the seeds
+    auto widened = [&](int& wl, int& wh) {
+      wl = lo;
+      wh = hi;
+      if (seeds < 0) return;
+      const int d = seeds & (SEED_ONLY - 1);
+      const bool ts = seeds & SEED_TEXT, ps = seeds & SEED_PATTERN;
+      if (seeds & SEED_ONLY) {
+        wl = ps ? -d : d;
+        wh = ts ? d : -d;
+      } else {
+        if (ts) wh = imax(hi, d);
+        if (ps) wl = imin(lo, -d);
+      }
+    };
+    int wlo, whi;
+    widened(wlo, whi);
+    if (whi - wlo + 1 > R.wcap) { overflow = true; break; }
+    int4 hrow = make_int4(wlo, whi, 0, 0);
+    if (MED && !halloc(whi - wlo + 1, &hrow, wlo, whi)) { overflow = true; break; }
+    // winner flags of diagonal k at fbase + (k - lo)
+    const int64_t fbase = MED ? hist_row(H, hrow) + (lo - wlo) : 0;
 
     // ---- compute all chunks; trims by ballot ----
     // U chunks at a time: their input reads are all issued before the
@@ -1667,31 +1690,6 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           vi = in_bounds(i1, k, plen, tlen);
           vd = in_bounds(d1, k, plen, tlen);
         }
-        if (KIND != KIND_EDIT && seed_k >= 0) {
-          if (k < clo || k > chi) {
-            // beyond the computed range: NULL, the gap components too
-            m = WFA_NULL;
-            if (GAPS) {
-              R.set((rI1 + rowE) * R.wcap + (k & R.mask), WFA_NULL);
-              R.set((rD1 + rowE) * R.wcap + (k & R.mask), WFA_NULL);
-              vi = vd = false;
-              if (two) {
-                R.set((rI2 + rowE2) * R.wcap + (k & R.mask), WFA_NULL);
-                R.set((rD2 + rowE2) * R.wcap + (k & R.mask), WFA_NULL);
-                vi2 = vd2 = false;
-              }
-            }
-          }
-          // endsfree_init_offset (src/aligner.rs:3421-3437)
-          if ((tseed && k == seed_k && m <= seed_k) || (pseed && k == -seed_k && m <= 0)) {
-            m = k > 0 ? k : 0;
-            if (MED) fl = (fl & ~7) | FL_M_SEED;
-            // a seed beyond its sequence (a free begin gap longer than it)
-            // is out of bounds, the end cell trim_ends drops
-            // (src/wavefront.rs); a seed-only M keeps it (endsfree_null_m)
-            if (!seed_only && !in_bounds(m, k, plen, tlen)) m = WFA_NULL;
-          }
-        }
         vm = m != WFA_NULL;
         R.set(rowM * R.wcap + (k & R.mask), m);
         if (MED) H.bytes[fbase + (k - lo)] = (uint8_t)fl;
@@ -1764,11 +1762,60 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         d2_lo = imin(d2_lo, x[8]); d2_hi = imax(d2_hi, x[9]);
       }
     }
+    if (KIND != KIND_EDIT && seeds >= 0) {
+      const int seed_k = seeds & (SEED_ONLY - 1);
+      const bool seed_only = seeds & SEED_ONLY;
+      const bool tseed = seeds & SEED_TEXT, pseed = seeds & SEED_PATTERN;
+      int wlo, whi;
+      widened(wlo, whi);
+      // the begin-free cells (endsfree_init / endsfree_null_m,
+      // src/aligner.rs:3388-3465), after the recurrences: the widened cells
+      // are NULL in every component, and a begin-free cell replaces the
+      // cell on its diagonal that reaches no further (endsfree_init_offset,
+      // src/aligner.rs:3421-3437; a medium/low-memory chain starts there)
+      __syncthreads();  // the computed cells are written
+      for (int k = wlo + tid; k <= whi; k += NT) {
+        const int cm = rowM * R.wcap + (k & R.mask);
+        const bool in = k >= lo && k <= hi;
+        int m = in ? R.at(cm) : WFA_NULL;
+        if (!in) {
+          if (GAPS) {
+            R.set((rI1 + rowE) * R.wcap + (k & R.mask), WFA_NULL);
+            R.set((rD1 + rowE) * R.wcap + (k & R.mask), WFA_NULL);
+          }
+          if (two) {
+            R.set((rI2 + rowE2) * R.wcap + (k & R.mask), WFA_NULL);
+            R.set((rD2 + rowE2) * R.wcap + (k & R.mask), WFA_NULL);
+          }
+          if (MED) H.bytes[fbase + (k - lo)] = 0;
+        }
+        const bool seed = (tseed && k == seed_k && m <= seed_k) || (pseed && k == -seed_k && m <= 0);
+        if (seed) {
+          m = k > 0 ? k : 0;
+          if (MED) H.bytes[fbase + (k - lo)] = (H.bytes[fbase + (k - lo)] & ~7) | FL_M_SEED;
+          // a seed beyond its sequence (a free begin gap longer than it)
+          // is out of bounds, the end cell trim_ends drops
+          // (src/wavefront.rs); a seed-only M keeps it (endsfree_null_m)
+          if (!seed_only && !in_bounds(m, k, plen, tlen)) m = WFA_NULL;
+        }
+        if (!in || seed) R.set(cm, m);
+      }
+      __syncthreads();
+      // a begin-free cell widens M's range
+      auto widen = [&](int k) {
+        if (uniform(R.at(rowM * R.wcap + (k & R.mask))) >= 0) {
+          m_lo = imin(m_lo, k);
+          m_hi = imax(m_hi, k);
+        }
+      };
+      if (pseed) widen(-seed_k);
+      if (tseed) widen(seed_k);
+    }
     // trimmed-empty rows: lo > hi (but not META_ABSENT)
     auto fix = [](int& l, int& h) { if (l == INT32_MAX) { l = 1; h = 0; } };
     fix(m_lo, m_hi); fix(i_lo, i_hi); fix(d_lo, d_hi); fix(i2_lo, i2_hi); fix(d2_lo, d2_hi);
     __syncthreads();  // ring rows written before the extension reads them
-    if (dom_on && !seed_only) {
+    if (dom_on && (seeds < 0 || !(seeds & SEED_ONLY))) {
       // Corrected: drop from each component's ends the cells that reach no
       // further than that component did on their diagonal at an earlier
       // score; I/D record what they keep (endsfree_dominance_prune,
@@ -1831,8 +1878,9 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     // iteration; nothing reads M[s] in between) ----
     // In high memory the stored row keeps its computed range: M cells
     // outside the trimmed range are OFFSET_NULL there anyway.
-    const int elo = MODE == 1 ? lo : m_lo, ehi = MODE == 1 ? hi : m_hi;
-    if (MODE == 1 && !halloc(hi - lo + 1, &hrow, elo, ehi)) { overflow = true; break; }
+    int elo = m_lo, ehi = m_hi;
+    if (MODE == 1) widened(elo, ehi);
+    if (MODE == 1 && !halloc(ehi - elo + 1, &hrow, elo, ehi)) { overflow = true; break; }
     if (corrected) {
       if (corrected_step(s, rowM, rowE, rowE2, elo, ehi, hrow)) break;
       continue;

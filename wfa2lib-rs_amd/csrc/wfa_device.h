@@ -178,9 +178,12 @@ __host__ __device__ inline bool corrected_endsfree(const KParams& p) {
   return p.compat == WFA_COMPAT_CORRECTED && p.u_match < 0 && p.endsfree;
 }
 // Launches that need the single-wave kernels with the per-score
-// cut-off logic: heuristics, and corrected ends-free alignment
+// cut-off logic: heuristics, and ends-free alignment with a match reward
+// (its begin-free cells and, in Compat::Corrected, its candidate ends).
+// The other kernels carry no per-score code for them: they are bound by
+// scalar instructions per score.
 __host__ __device__ inline bool heur_kernel(const KParams& p) {
-  return p.heuristic != WFA_HEURISTIC_NONE || corrected_endsfree(p);
+  return p.heuristic != WFA_HEURISTIC_NONE || (p.endsfree && p.u_match < 0);
 }
 
 // Waves per workgroup of a BiWFA launch (`which` 0 = breakpoint search,
