@@ -1343,15 +1343,23 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   int sM = 0, sE1 = 0, sE2 = 0;
   // (s - d) mod ring for 0 <= d < ring
   auto back = [](int cur, int d, int ring) { return cur >= d ? cur - d : cur - d + ring; };
+  // the first score at which one of the per-score limits below applies
+  // (one compare per score: the kernel is bound by scalar instructions)
+  int limit = INT32_MAX;
+  if (!p.modular) limit = imin(limit, nwf);
+  if (CIG) limit = imin(limit, H.hdr_cap);
+  if (sizeof(RT) == 2) limit = imin(limit, 16001);
   while (status == WFA_STATUS_OK && !overflow) {
     score += 1;
     sM = sM + 1 == ringM ? 0 : sM + 1;
     if (GAPS) sE1 = sE1 + 1 == ringE1 ? 0 : sE1 + 1;
     if (two) sE2 = sE2 + 1 == ringE2 ? 0 : sE2 + 1;
-    if (!p.modular && score >= nwf) { status = WFA_STATUS_MAX_STEPS_REACHED; result = score; break; }
-    if (CIG && score >= H.hdr_cap) { overflow = true; break; }
-    // int16 rows hold offsets up to tlen + score: leave long runs to int32
-    if (sizeof(RT) == 2 && score > 16000) { overflow = true; break; }
+    if (score >= limit) {
+      if (!p.modular && score >= nwf) { status = WFA_STATUS_MAX_STEPS_REACHED; result = score; break; }
+      if (CIG && score >= H.hdr_cap) { overflow = true; break; }
+      // int16 rows hold offsets up to tlen + score: leave long runs to int32
+      if (sizeof(RT) == 2 && score > 16000) { overflow = true; break; }
+    }
     const int s = score;
     // ---- input rows ----
     int lo = INT32_MAX, hi = INT32_MIN;
