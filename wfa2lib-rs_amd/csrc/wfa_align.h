@@ -1390,30 +1390,27 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       // gap-linear (compute_linear_step, src/aligner.rs:1866-2002): M at
       // s - x and s - indel; its inputs' ends are NULL-filled only beyond
       // their initialized ranges, as the affine inputs'
+      // An input below score 0 maps to a ring slot this pair has not
+      // written yet (every lookback is shorter than its ring), whose
+      // metadata is still absent: it reads as the null wavefront, so the
+      // loads need no score checks (the kernel is bound by scalar
+      // instructions per score).
       const int go = GAPS ? p.oe1 : p.o1;
-      if (s - p.x >= 0) {
-        row_x = back(sM, p.x, ringM);
-        load(row_x, ml_x, mh_x, gl_x, gh_x);
-      }
-      if (s - go >= 0) {
-        row_o = back(sM, go, ringM);
-        load(row_o, ml_o, mh_o, gl_o, gh_o);
-      }
-      if (GAPS && s - p.e1 >= 0) {
+      row_x = back(sM, p.x, ringM);
+      load(row_x, ml_x, mh_x, gl_x, gh_x);
+      row_o = back(sM, go, ringM);
+      load(row_o, ml_o, mh_o, gl_o, gh_o);
+      if (GAPS) {
         row_e = back(sE1, p.e1, ringE1);
         load(rI1 + row_e, il, ih, gil, gih);
         load(rD1 + row_e, dl, dh, gdl, gdh);
       }
       if (two) {
-        if (s - p.oe2 >= 0) {
-          row_o2 = back(sM, p.oe2, ringM);
-          load(row_o2, ml_o2, mh_o2, gl_o2, gh_o2);
-        }
-        if (s - p.e2 >= 0) {
-          row_e2 = back(sE2, p.e2, ringE2);
-          load(rI2 + row_e2, il2, ih2, gil2, gih2);
-          load(rD2 + row_e2, dl2, dh2, gdl2, gdh2);
-        }
+        row_o2 = back(sM, p.oe2, ringM);
+        load(row_o2, ml_o2, mh_o2, gl_o2, gh_o2);
+        row_e2 = back(sE2, p.e2, ringE2);
+        load(rI2 + row_e2, il2, ih2, gil2, gih2);
+        load(rD2 + row_e2, dl2, dh2, gdl2, gdh2);
       }
       // an input whose limits are empty is the null wavefront; so are its reads
       auto ng = [](int l, int h, int& gl, int& gh) { if (l > h) { gl = 1; gh = 0; } };
