@@ -1412,11 +1412,16 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         load(rI2 + row_e2, il2, ih2, gil2, gih2);
         load(rD2 + row_e2, dl2, dh2, gdl2, gdh2);
       }
-      // an input whose limits are empty is the null wavefront; so are its reads
-      auto ng = [](int l, int h, int& gl, int& gh) { if (l > h) { gl = 1; gh = 0; } };
-      ng(ml_x, mh_x, gl_x, gh_x); ng(ml_o, mh_o, gl_o, gh_o); ng(il, ih, gil, gih);
-      ng(dl, dh, gdl, gdh); ng(ml_o2, mh_o2, gl_o2, gh_o2); ng(il2, ih2, gil2, gih2);
-      ng(dl2, dh2, gdl2, gdh2);
+      // an input whose limits are empty is the null wavefront; so are its
+      // reads. Only a heuristic cut or a Corrected band prune leaves a row
+      // readable beyond empty limits: in the other kernels every row's
+      // readable range equals its limits already.
+      if (HEUR) {
+        auto ng = [](int l, int h, int& gl, int& gh) { if (l > h) { gl = 1; gh = 0; } };
+        ng(ml_x, mh_x, gl_x, gh_x); ng(ml_o, mh_o, gl_o, gh_o); ng(il, ih, gil, gih);
+        ng(dl, dh, gdl, gdh); ng(ml_o2, mh_o2, gl_o2, gh_o2); ng(il2, ih2, gil2, gih2);
+        ng(dl2, dh2, gdl2, gdh2);
+      }
       // Absent rows (META_ABSENT, hi = META_ABSENT-1), missing inputs and
       // trimmed- or cut-empty rows all have lo > hi and read as the null
       // wavefront, whose bounds (1, -1) still enter the range union
