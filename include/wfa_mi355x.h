@@ -94,6 +94,9 @@ extern "C" {
 #define WFA_ERR_OUT_OF_MEMORY      -6
 #define WFA_ERR_REJECTED           -7  /* combination the reference rejects
                                           (src/bin/align_benchmark.rs:531-560) */
+#define WFA_ERR_LIMIT              -8  /* beyond a fixed limit of
+                                          wfa_align_batch_noalloc (pairs,
+                                          length, memory mode) */
 
 /* User penalties, as passed to WavefrontPenalties::new_* (src/penalties.rs:
  * 136-296). `match` <= 0; a negative match is a reward and the engine
@@ -306,6 +309,26 @@ int32_t wfa_engine_align_batch_per_pair(wfa_engine_t* engine, const wfa_attr_t* 
  * reference has no counterpart (its memory is the host's); this is the
  * engine's own limit. */
 int32_t wfa_engine_set_scratch_budget(wfa_engine_t* engine, int64_t bytes);
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code that must not call hipMalloc, such as tight serving loops
+ * or harnesses that forbid it). Aligns `batch` exactly as
+ * wfa_engine_align_batch does, on the current HIP device and `hip_stream`,
+ * but allocates nothing: every scratch buffer comes from one fixed device
+ * array of WFA_NOALLOC_SCRATCH_BYTES that each GPU holds until the process
+ * ends. The caller passes inputs and outputs as for wfa_engine_align_batch.
+ * Limits: at most WFA_NOALLOC_MAX_PAIRS pairs and WFA_NOALLOC_MAX_LENGTH
+ * bases per sequence, memory modes HIGH/MED/LOW (not ULTRALOW); beyond them
+ * WFA_ERR_LIMIT. Within the limits, a pair whose CIGAR history does not fit
+ * the array after the wider passes gets WFA_STATUS_OOM. All calls on one
+ * GPU share its array and leave their contents in it; calls in one process
+ * are serialized per GPU and return after their kernels finish, so they
+ * never use the array at the same time. */
+#define WFA_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+#define WFA_NOALLOC_MAX_PAIRS     (1 << 20)
+#define WFA_NOALLOC_MAX_LENGTH    16384
+int32_t wfa_align_batch_noalloc(const wfa_attr_t* attr, const wfa_batch_t* batch,
+                                void* hip_stream);
 
 /* Launch statistics of the last batch call: number of pairs that needed a
  * wider wavefront or longer history than the first pass provides and were

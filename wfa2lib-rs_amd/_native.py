@@ -28,10 +28,15 @@ SEQ_ASCII, SEQ_PACKED2BITS = 0, 1
 ERRORS = {
     0: "ok", -1: "invalid penalties", -2: "invalid argument", -3: "unsupported on the device path",
     -4: "HIP runtime error", -5: "no HIP device", -6: "out of device memory",
-    -7: "rejected combination",
+    -7: "rejected combination", -8: "beyond a fixed limit of the no-allocation entry point",
 }
 ERR_INVALID_PENALTIES = -1
 ERR_UNSUPPORTED = -3
+ERR_LIMIT = -8
+# limits of wfa_align_batch_noalloc (include/wfa_mi355x.h)
+NOALLOC_SCRATCH_BYTES = 256 << 20
+NOALLOC_MAX_PAIRS = 1 << 20
+NOALLOC_MAX_LENGTH = 16384
 
 
 class Penalties(ctypes.Structure):
@@ -117,6 +122,7 @@ SIGNATURES = {
     "wfa_engine_align_batch": (_I32, [_VP, _P(Attr), _P(Batch), _VP]),
     "wfa_engine_align_batch_per_pair": (_I32, [_VP, _P(Attr), _P(Batch), _VP]),
     "wfa_engine_set_scratch_budget": (_I32, [_VP, ctypes.c_int64]),
+    "wfa_align_batch_noalloc": (_I32, [_P(Attr), _P(Batch), _VP]),
     "wfa_engine_last_retries": (_I32, [_VP]),
     "wfa_engine_last_kernel_ms": (_I32, [_VP, _P(ctypes.c_float), _P(ctypes.c_float)]),
     "wfa_server_config_default": (None, [_P(ServerConfig)]),

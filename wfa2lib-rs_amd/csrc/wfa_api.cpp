@@ -173,7 +173,10 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
     if (pass > 0 || (long_reads && e->long_pool_whole_budget)) elems = std::max(elems, rest / cell);
     elems = std::min(elems, std::max<int64_t>(rest / cell, c->chunk));
     elems = std::min<int64_t>(elems, (int64_t)c->chunk * INT32_MAX);
-    c->pool_chunks = (int)std::max<int64_t>(c->grid, (elems + c->chunk - 1) / c->chunk);
+    // whole chunks, never more than the budget leaves (at least one per slot)
+    const int64_t fit = std::max<int64_t>(rest / cell / c->chunk, 0);
+    c->pool_chunks = (int)std::max<int64_t>(
+        c->grid, std::min<int64_t>((elems + c->chunk - 1) / c->chunk, fit));
   }
   // op chain of one pair: at most every base consumed by a gap op plus one
   // gap-closing mismatch per gap
@@ -191,18 +194,27 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
   p.seq_words = c.seq_words;
   p.nrows_meta = c.ring_rows;
   p.ring_rows = c.ring_rows;
+  if (e->arena) {
+    // no-allocation mode: this launch's buffers start again where the
+    // call's lists end (the previous launch on the stream has finished
+    // with them by the time this one runs)
+    e->arena_top = e->arena_launch;
+    e->gring_bytes = e->hist_bytes = e->hdr_bytes = e->tab_bytes = e->chain_bytes =
+        e->dom_bytes = 0;
+  }
   if (!c.lds_ring) {
-    int r = grow((void**)&e->d_gring, &e->gring_bytes, (size_t)c.grid * c.ring_rows * c.wcap * 4);
+    int r = egrow(e, (void**)&e->d_gring, &e->gring_bytes,
+                  (size_t)c.grid * c.ring_rows * c.wcap * 4);
     if (r) return r;
     p.gring = e->d_gring;
   }
   if (c.cig) {
     const size_t cell = c.mode == 2 ? 1 : 4;
-    int r = grow((void**)&e->d_hist, &e->hist_bytes, (size_t)c.pool_chunks * c.chunk * cell);
+    int r = egrow(e, (void**)&e->d_hist, &e->hist_bytes, (size_t)c.pool_chunks * c.chunk * cell);
     if (r) return r;
-    r = grow((void**)&e->d_hdr, &e->hdr_bytes, (size_t)c.grid * c.hdr_cap * sizeof(int4));
+    r = egrow(e, (void**)&e->d_hdr, &e->hdr_bytes, (size_t)c.grid * c.hdr_cap * sizeof(int4));
     if (r) return r;
-    r = grow((void**)&e->d_tab, &e->tab_bytes, (size_t)c.grid * c.tab_cap * 4);
+    r = egrow(e, (void**)&e->d_tab, &e->tab_bytes, (size_t)c.grid * c.tab_cap * 4);
     if (r) return r;
     p.hist = e->d_hist;
     p.hist_cap = 0;
@@ -214,14 +226,14 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
     p.hdr_cap = c.hdr_cap;
     if (c.mode == 2) {
       p.chain_cap = c.chain_cap;
-      r = grow((void**)&e->d_chain, &e->chain_bytes, (size_t)c.grid * c.chain_cap);
+      r = egrow(e, (void**)&e->d_chain, &e->chain_bytes, (size_t)c.grid * c.chain_cap);
       if (r) return r;
       p.chain = e->d_chain;
     }
   }
   p.dom_stride = c.dom_stride;
   if (c.dom_stride > 0) {
-    int r = grow((void**)&e->d_dom, &e->dom_bytes, (size_t)c.grid * c.dom_stride * 4);
+    int r = egrow(e, (void**)&e->d_dom, &e->dom_bytes, (size_t)c.grid * c.dom_stride * 4);
     if (r) return r;
     p.dom = e->d_dom;
   }
@@ -235,6 +247,16 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
 }
 
 int ensure_lists(wfa_engine_t* e, int n) {
+  if (e->arena) {
+    // no-allocation mode: two lists of n entries from the arena
+    size_t have = 0;
+    for (auto& l : e->d_lists) {
+      int r = egrow(e, (void**)&l, &have, (size_t)std::max(n, 1) * 4);
+      if (r) return r;
+    }
+    e->list_cap = n;
+    return WFA_OK;
+  }
   if ((size_t)n <= e->list_cap) return WFA_OK;
   for (auto& l : e->d_lists) {
     if (l) hipFree(l);
@@ -289,12 +311,15 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
     r = read_counter(e, 2, st, &max_len);
     if (r) return r;
   }
+  if (e->max_len_limit > 0 && max_len > e->max_len_limit) return WFA_ERR_LIMIT;
+  // no-allocation mode: the call's lists and work order follow the
+  // counters; the launches' buffers follow them, within what is left
+  if (e->arena) {
+    e->arena_top = 256;
+    e->order_bytes = 0;
+  }
   r = ensure_lists(e, count);
   if (r) return r;
-  LaunchCfg c;
-  r = plan(e, p, kind, mode, b->seq_format, max_len, count, 0, &c);
-  if (r) return r;
-  p.pair_base = first;
   // Long reads: hand the pairs out longest first, so the longest ones do
   // not start last and stretch the launch (a device radix sort of the
   // lengths; short reads are uniform enough to take in order)
@@ -302,7 +327,7 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
   if (max_len > 1024 && count > 1) {
     const size_t tb = wfa::lpt_temp_bytes(count);
     const size_t nb = ((size_t)count * 4 + 255) & ~(size_t)255;
-    r = grow(&e->d_order, &e->order_bytes, 4 * nb + tb);
+    r = egrow(e, &e->d_order, &e->order_bytes, 4 * nb + tb);
     if (r) return r;
     uint8_t* ob = (uint8_t*)e->d_order;
     int32_t* ord = (int32_t*)(ob + 3 * nb);
@@ -311,10 +336,22 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
                                  ord, ob + 4 * nb, tb, st));
     order = ord;
   }
-  HIPCHK(hipEventRecord(e->ev[0], st));
+  if (e->arena) {
+    // what the launches may take: the rest of the arena, less room for the
+    // 256-byte alignment of their buffers
+    e->arena_launch = (e->arena_top + 255) & ~(size_t)255;
+    e->scratch_budget = (int64_t)(e->arena_cap - e->arena_launch) - (64 << 10);
+  }
+  LaunchCfg c;
+  r = plan(e, p, kind, mode, b->seq_format, max_len, count, 0, &c);
+  if (r) return r;
+  p.pair_base = first;
+  // (no-allocation mode keeps no timing events: creating them allocates)
+  const bool timed = e->ev[0] != nullptr;
+  if (timed) HIPCHK(hipEventRecord(e->ev[0], st));
   r = run_launch(e, p, c, count, order, e->d_lists[0], st);
   if (r) return r;
-  HIPCHK(hipEventRecord(e->ev[1], st));
+  if (timed) HIPCHK(hipEventRecord(e->ev[1], st));
   int32_t nretry = 0;
   r = read_counter(e, 1, st, &nretry);
   if (r) return r;
@@ -336,6 +373,10 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
     r = read_counter(e, 1, st, &nretry);
     if (r) return r;
   }
+  if (!timed) {
+    HIPCHK(hipStreamSynchronize(st));
+    return WFA_OK;
+  }
   HIPCHK(hipEventRecord(e->ev[2], st));
   HIPCHK(hipEventSynchronize(e->ev[2]));
   HIPCHK(hipEventElapsedTime(&e->first_ms, e->ev[0], e->ev[1]));
@@ -344,6 +385,18 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
 }
 
 }  // namespace
+
+namespace wfahost {
+
+int align_batch_locked(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
+                       hipStream_t st) {
+  int r = validate_batch(attr, b);
+  if (r) return r;
+  std::lock_guard<std::mutex> g(e->mu);
+  return align_batch_impl(e, attr, b, st, 0, b->num_pairs);
+}
+
+}  // namespace wfahost
 
 extern "C" {
 

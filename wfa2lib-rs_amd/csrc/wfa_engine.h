@@ -164,6 +164,18 @@ struct wfa_engine_t {
   static constexpr int kBiBuffers = 12;
   void* bi_buf[kBiBuffers] = {};
   size_t bi_cap[kBiBuffers] = {};
+  // No-allocation mode (wfa_align_batch_noalloc): every device buffer is a
+  // piece of one fixed device array of `arena_cap` bytes, handed out by a
+  // bump pointer: the work queue counters at its start, then one call's
+  // pair lists and work order, then (from `arena_launch`, again for every
+  // launch of the call) that launch's rings, history and tables. Host
+  // reads of the counters go through `h_pinned` (then ordinary memory).
+  uint8_t* arena = nullptr;
+  size_t arena_cap = 0;
+  size_t arena_top = 0;
+  size_t arena_launch = 0;
+  // longest pair a call may bring (0: no limit), WFA_ERR_LIMIT beyond it
+  int max_len_limit = 0;
   std::mutex mu;
 };
 
@@ -176,6 +188,19 @@ inline int grow(void** ptr, size_t* have, size_t need) {
   *have = 0;
   if (hipMalloc(ptr, need) != hipSuccess) return WFA_ERR_OUT_OF_MEMORY;
   *have = need;
+  return WFA_OK;
+}
+
+// grow() or, in no-allocation mode, the next `need` bytes of the arena
+// (256-byte aligned); the arena never grows: WFA_ERR_OUT_OF_MEMORY when the
+// piece does not fit.
+inline int egrow(wfa_engine_t* e, void** ptr, size_t* have, size_t need) {
+  if (!e->arena) return grow(ptr, have, need);
+  const size_t at = (e->arena_top + 255) & ~(size_t)255;
+  if (at > e->arena_cap || need > e->arena_cap - at) return WFA_ERR_OUT_OF_MEMORY;
+  *ptr = e->arena + at;
+  *have = need;
+  e->arena_top = at + need;
   return WFA_OK;
 }
 
@@ -242,6 +267,11 @@ inline void base_params(const Internal& ip, const wfa_attr_t& attr, const wfa_ba
 // BiWFA batches (memory mode ULTRALOW): wfa_biwfa.cpp
 int biwfa_align_batch(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
                       hipStream_t st, int first, int count);
+
+// wfa_engine_align_batch's work: validate, then align under the engine's
+// lock (wfa_api.cpp)
+int align_batch_locked(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
+                       hipStream_t st);
 
 inline int read_counter(wfa_engine_t* e, int idx, hipStream_t st, int32_t* out) {
   HIPCHK(hipMemcpyAsync(e->h_pinned, e->d_counters + idx, 4, hipMemcpyDeviceToHost, st));
