@@ -299,6 +299,16 @@ static __device__ __forceinline__ bool st_m(const Hist& H, int nscores, int nwf,
   return true;
 }
 
+// The stored M rows of a CIGAR history pool as the Singletrack backtrace
+// reads them (st_m_offset over the slot's headers and pool chunks).
+struct PoolView {
+  const Hist& H;
+  int nscores, nwf;
+  __device__ __forceinline__ bool m(int s, int k, int* out) const {
+    return st_m(H, nscores, nwf, s, k, out);
+  }
+};
+
 // write `n` copies of op backwards ending at out[pos] (pos decreases)
 static __device__ __forceinline__ void write_ops(uint8_t* out, int& pos, uint8_t op, int n) {
   for (int i = threadIdx.x; i < n; i += 64)
@@ -309,10 +319,12 @@ static __device__ __forceinline__ void write_ops(uint8_t* out, int& pos, uint8_t
 // Singletrack backtrace (src/backtrace.rs:751-858), end-to-end, from the
 // end cell (score, k, offset). Writes ops backwards into out[0..cap) and
 // returns the number of ops, which end at out[cap-1].
-template <int BITS>
-static __device__ int singletrack(const KParams& p, const Hist& H, int nscores, int nwf,
-                                  const uint64_t* sP, const uint64_t* sT, int plen, int tlen,
-                                  int score, int k, int offset, uint8_t* out, int cap) {
+// `hv.m(s, k, &offset)` reads the stored M offset of score s at diagonal k
+// (false when the score has no row, k is outside it, or the cell is NULL).
+template <int BITS, class HV>
+static __device__ int singletrack(const KParams& p, const HV& hv, const uint64_t* sP,
+                                  const uint64_t* sT, int plen, int tlen, int score, int k,
+                                  int offset, uint8_t* out, int cap) {
   const int lane = threadIdx.x;
   const bool two = p.metric == WFA_METRIC_GAP_AFFINE_2P;
   int pos = cap - 1;
@@ -328,7 +340,7 @@ static __device__ int singletrack(const KParams& p, const Hist& H, int nscores, 
       offset_orig = offset - nm;
       const int mis = score - p.x;
       int mo;
-      if (st_m(H, nscores, nwf, mis, k, &mo) && mo == offset_orig - 1) {
+      if (hv.m(mis, k, &mo) && mo == offset_orig - 1) {
         write_ops(out, pos, 'M', nm);
         write_ops(out, pos, 'X', 1);
         offset = offset_orig - 1;
@@ -345,15 +357,15 @@ static __device__ int singletrack(const KParams& p, const Hist& H, int nscores, 
       const int ind1 = score - p.o1 - L * p.e1;
       const int ind2 = two ? score - p.o2 - L * p.e2 : -1;
       int m1i = 0, m1d = 0, m2i = 0, m2d = 0;
-      const bool r1i = st_m(H, nscores, nwf, ind1, k - L, &m1i) && m1i + L >= offset_orig &&
+      const bool r1i = hv.m(ind1, k - L, &m1i) && m1i + L >= offset_orig &&
                        m1i + L <= offset;
-      const bool r1d = st_m(H, nscores, nwf, ind1, k + L, &m1d) && m1d >= offset_orig &&
+      const bool r1d = hv.m(ind1, k + L, &m1d) && m1d >= offset_orig &&
                        m1d <= offset;
       bool r2i = false, r2d = false;
       if (two) {
-        r2i = st_m(H, nscores, nwf, ind2, k - L, &m2i) && m2i + L >= offset_orig &&
+        r2i = hv.m(ind2, k - L, &m2i) && m2i + L >= offset_orig &&
               m2i + L <= offset;
-        r2d = st_m(H, nscores, nwf, ind2, k + L, &m2d) && m2d >= offset_orig && m2d <= offset;
+        r2d = hv.m(ind2, k + L, &m2d) && m2d >= offset_orig && m2d <= offset;
       }
       const bool stop = r1i || r1d || r2i || r2d || (ind1 < 0 && ind2 < 0);
       const uint64_t mask = ballot(stop);
@@ -1943,8 +1955,8 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         ncig = backtrace_linear<BITS>(p, H, score + 1, plen, tlen, pos_s, pos_k, pos_off, out,
                                       cap);
       else
-        ncig = singletrack<BITS>(p, H, score + 1, nwf, sP, sT, plen, tlen, pos_s, pos_k, pos_off,
-                                 out, cap);
+        ncig = singletrack<BITS>(p, PoolView{H, score + 1, nwf}, sP, sT, plen, tlen, pos_s,
+                                 pos_k, pos_off, out, cap);
       if (ncig < 0) {
         // the stored flags do not lead back to the start: never expected;
         // report the pair as unattainable rather than a wrong CIGAR

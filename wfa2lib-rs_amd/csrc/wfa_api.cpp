@@ -31,7 +31,7 @@ struct LaunchCfg {
   bool cig;
   bool lds_ring;
   bool i16;
-  bool window;  // the short-read window kernel
+  bool shortk;  // the short-read kernel (wfa_short.h)
   int nw;       // waves per pair
   int wcap;
   int seq_words;
@@ -98,25 +98,46 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   const size_t red_bytes = c->lds_ring ? 0 : (size_t)64 * c->nw;
   c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : red_bytes);
   if (c->lds_bytes > e->lds_max) return WFA_ERR_UNSUPPORTED;
-  // First pass of gap-affine end-to-end score-only alignments without a
-  // heuristic: the window kernel; pairs that leave its window come back for
-  // the general kernel. (With CIGARs the general kernel measures faster.)
-  c->window = pass == 0 && kind == wfa::KIND_AFFINE && mode == 0 &&
-              p.heuristic == WFA_HEURISTIC_NONE && !p.endsfree && max_len <= 8000;
+  // First pass of gap-affine end-to-end alignments without a heuristic,
+  // score only or with high-memory CIGARs: the short-read kernel (a fixed
+  // 256-diagonal window per pair, 16-bit rows, a per-slot history at a fixed
+  // stride); pairs that leave its window come back for the general kernel.
+  // Its ring-row mask is 64 bits wide.
+  c->shortk = pass == 0 && kind == wfa::KIND_AFFINE && (mode == 0 || mode == 1) &&
+              p.heuristic == WFA_HEURISTIC_NONE && !p.endsfree && max_len <= 8000 &&
+              c->ring_rows <= 64;
   int per_cu;
-  if (c->window) {
+  if (c->shortk) {
     c->seq_words = ((max_len + 31) / 32 + 2 + 1) & ~1;
-    c->lds_bytes = (size_t)2 * c->seq_words * 8 + wfa::window_lds_bytes_host(c->ring_rows);
-    if (c->lds_bytes > e->lds_max) c->window = false;
+    c->lds_bytes = (size_t)2 * c->seq_words * 8 + wfa::short_lds_row_bytes_host(c->ring_rows);
+    if (c->lds_bytes > e->lds_max) c->shortk = false;
   }
-  if (c->window) {
-    per_cu = wfa::window_blocks_per_cu(mode, fmt == WFA_SEQ_PACKED2BITS, c->lds_bytes);
-  } else {
-    c->seq_words = ((max_len + per - 1) / per + 2 + 1) & ~1;
-    c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : red_bytes);
-    per_cu = wfa::max_blocks_per_cu(kind, mode, c->lds_ring, c->i16, fmt == WFA_SEQ_PACKED2BITS,
-                                    wfa::heur_kernel(p), c->nw, c->lds_bytes);
+  if (c->shortk) {
+    per_cu = wfa::short_blocks_per_cu(mode, fmt == WFA_SEQ_PACKED2BITS, c->lds_bytes);
+    int grid = std::max(1, std::min(e->num_cus * per_cu, npairs));
+    c->lds_ring = true;
+    c->i16 = true;
+    c->nw = 1;
+    c->wcap = wfa::kShortWindow;
+    c->dom_stride = 0;
+    c->chunk = c->pool_chunks = c->tab_cap = c->chain_cap = 0;
+    c->hist_cap = 0;
+    c->hdr_cap = 0;
+    if (cig) {
+      // rows for scores below 1024 (a pair scoring more goes to the general
+      // kernel): 16-bit cells over the window, one header per score
+      c->hdr_cap = 1024;
+      const int64_t per_slot = (int64_t)c->hdr_cap * (wfa::kShortWindow * 2 + 4);
+      if ((int64_t)grid * per_slot > e->scratch_budget)
+        grid = (int)std::max<int64_t>(1, e->scratch_budget / per_slot);
+    }
+    c->grid = grid;
+    return WFA_OK;
   }
+  c->seq_words = ((max_len + per - 1) / per + 2 + 1) & ~1;
+  c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : red_bytes);
+  per_cu = wfa::max_blocks_per_cu(kind, mode, c->lds_ring, c->i16, fmt == WFA_SEQ_PACKED2BITS,
+                                  wfa::heur_kernel(p), c->nw, c->lds_bytes);
   int grid = e->num_cus * per_cu;
   if (long_reads && e->long_grid_cap > 0) grid = std::min(grid, e->long_grid_cap);
   grid = std::max(1, std::min(grid, npairs));
@@ -208,6 +229,23 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
     if (r) return r;
     p.gring = e->d_gring;
   }
+  if (c.shortk) {
+    // the short-read kernel: per slot hdr_cap history rows of the window's
+    // 16-bit cells and hdr_cap int32 headers
+    if (c.cig) {
+      int r = egrow(e, (void**)&e->d_hist, &e->hist_bytes,
+                    (size_t)c.grid * c.hdr_cap * wfa::kShortWindow * 2);
+      if (r) return r;
+      r = egrow(e, (void**)&e->d_hdr, &e->hdr_bytes, (size_t)c.grid * c.hdr_cap * 4);
+      if (r) return r;
+      p.hist = e->d_hist;
+      p.hdr = e->d_hdr;
+      p.hdr_cap = c.hdr_cap;
+    }
+    HIPCHK(wfa::launch_reset(e->d_counters, st));
+    HIPCHK(wfa::launch_short(p, c.mode, c.grid, c.lds_bytes, st));
+    return WFA_OK;
+  }
   if (c.cig) {
     const size_t cell = c.mode == 2 ? 1 : 4;
     int r = egrow(e, (void**)&e->d_hist, &e->hist_bytes, (size_t)c.pool_chunks * c.chunk * cell);
@@ -238,11 +276,8 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
     p.dom = e->d_dom;
   }
   HIPCHK(wfa::launch_reset(e->d_counters, st));
-  if (c.window)
-    HIPCHK(wfa::launch_window(p, c.mode, c.grid, c.lds_bytes, st));
-  else
-    HIPCHK(wfa::launch_align(p, c.kind, c.mode, c.lds_ring, c.i16, c.nw, c.grid, c.lds_bytes,
-                             st));
+  HIPCHK(wfa::launch_align(p, c.kind, c.mode, c.lds_ring, c.i16, c.nw, c.grid, c.lds_bytes,
+                           st));
   return WFA_OK;
 }
 

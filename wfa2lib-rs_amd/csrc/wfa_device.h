@@ -158,10 +158,12 @@ KernelFn pick_edit(int mode, bool lds, bool i16, bool packed, bool heur, int nw)
 KernelFn pick_affine(int mode, bool lds, bool i16, bool packed, bool heur, int nw);
 KernelFn pick_affine2p(int mode, bool lds, bool i16, bool packed, bool heur, int nw);
 KernelFn pick_linear(int mode, bool lds, bool i16, bool packed, bool heur, int nw);
-// the short-read window kernel (gap-affine, end-to-end, no heuristic;
-// mode 0 or 1)
-KernelFn pick_affine_window(int mode, bool packed);
-size_t window_lds_bytes_host(int ring_rows);
+// the short-read kernel (gap-affine, end-to-end, no heuristic; mode 0 or
+// 1): its history is p.hdr_cap rows of kShortWindow 16-bit cells and
+// p.hdr_cap int32 headers per slot
+KernelFn pick_affine_short(int mode, bool packed);
+size_t short_lds_row_bytes_host(int ring_rows);
+constexpr int kShortWindow = 256;
 
 hipError_t launch_reset(int32_t* counters, hipStream_t stream);
 hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, int32_t* out,
@@ -207,7 +209,7 @@ hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, boo
                         int grid, size_t lds_bytes, hipStream_t stream);
 int max_blocks_per_cu(int kind, int mode, bool lds_ring, bool i16, bool packed, bool heur, int nw,
                       size_t lds_bytes);
-hipError_t launch_window(const KParams& p, int mode, int grid, size_t lds_bytes, hipStream_t stream);
-int window_blocks_per_cu(int mode, bool packed, size_t lds_bytes);
+hipError_t launch_short(const KParams& p, int mode, int grid, size_t lds_bytes, hipStream_t stream);
+int short_blocks_per_cu(int mode, bool packed, size_t lds_bytes);
 
 }  // namespace wfa

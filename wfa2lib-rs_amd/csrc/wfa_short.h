@@ -1,0 +1,344 @@
+// wfa_short.h — the short-read kernel: gap-affine, end-to-end, no
+// heuristic, score only or high-memory CIGAR, sequences of A/C/G/T.
+//
+// Semantics are those of align_pair (the unialign loop, src/aligner.rs:
+// 638-698, with compute_affine_step, :2006-2193, and the Singletrack
+// backtrace, src/backtrace.rs:751-858); the storage is laid out so that a
+// step costs as few scalar instructions as possible (the general kernel is
+// bound by its per-score scalar bookkeeping on rows of a few dozen
+// diagonals):
+//
+// * Each pair gets a fixed window of 256 diagonals [base, base + 256),
+//   centred on the diagonals its alignment must cross. Every ring row is an
+//   LDS row over the whole window plus one NULL pad cell on each side, all
+//   NULL when the pair starts. A step writes its three output rows over the
+//   64-diagonal chunks it computes, OFFSET_NULL outside their trimmed
+//   ranges, so a row's cells outside its range read as NULL with no range
+//   test: the reference NULL-fills exactly those cells before it reads them
+//   (src/wavefront.rs:422-457).
+// * The chunks a step computes cover [lo - 1, hi + 1] of the union of every
+//   row computed so far. That holds the reference's computed range (the
+//   union of the inputs' ranges, widened by one, src/aligner.rs:2045-2062);
+//   a cell outside the latter has only NULL inputs, so it comes out below
+//   zero, outside the trimmed ranges, and is stored as NULL — as it would
+//   be read in the reference. The union only grows, so no cell outside it
+//   has been written since the pair started.
+// * Which ring rows hold a wavefront (not absent, not trimmed empty) is one
+//   64-bit scalar mask; an absent input row is read from an all-NULL row
+//   instead, and a step whose inputs are all absent is a null step
+//   (src/aligner.rs:2031-2043) that touches no vector registers at all.
+// * In CIGAR scope the extended M row of every score goes to the slot's
+//   history in HBM at a fixed stride (cell k of score s at s * 256 + k -
+//   base, 16-bit), with the computed chunks' first and last diagonal packed
+//   in one int32 header per score; the Singletrack backtrace reads it
+//   through WinView.
+//
+// Offsets are held in 16 bits as in the general kernel's short-read rings
+// (sequences of at most 8000 bases, scores up to 16000; NULL-derived values
+// saturate at -16384 and only their sign matters). A pair whose union of
+// diagonals would leave its window, whose score passes the history's rows,
+// or whose sequences hold other bytes than A/C/G/T is handed back through
+// the retry list to the general kernel, which aligns it from the start.
+#pragma once
+
+#include "wfa_align.h"
+
+namespace wfa {
+
+constexpr int SH_WIN = 256;              // diagonals of a pair's window
+constexpr int SH_CHUNKS = SH_WIN / 64;   // most 64-diagonal chunks a step computes
+constexpr int SH_ROW = SH_WIN + 8;       // LDS cells per row: pad, window, pad (16-byte rows)
+constexpr int SH_NULL = -16384;          // OFFSET_NULL as held in 16 bits
+constexpr int SH_HDR_EMPTY = 0x00000001; // header of an absent M row: lo 1, hi 0
+
+static __device__ __forceinline__ int16_t sh16(int v) { return (int16_t)imax(v, SH_NULL); }
+
+// One slot's high-memory history as the Singletrack backtrace reads it
+// (st_m_offset, src/backtrace.rs:645-663): M[s] at diagonal k, if score s
+// has a row covering k and the cell is not NULL.
+struct WinView {
+  const int16_t* cells;  // row s at cells + s * SH_WIN, diagonal k at k - base
+  const int32_t* hdr;    // per score: first diagonal (low 16 bits), last (high 16 bits)
+  int base, nscores, nwf;
+  __device__ __forceinline__ bool m(int s, int k, int* out) const {
+    if (s < 0 || s >= nscores || s >= nwf) return false;
+    const int h = hdr[s];
+    const int lo = (int)(int16_t)(h & 0xffff), hi = h >> 16;
+    if (k < lo || k > hi) return false;
+    const int v = cells[(int64_t)s * SH_WIN + (k - base)];
+    if (v < 0) return false;
+    *out = v;
+    return true;
+  }
+};
+
+static __device__ __forceinline__ int32_t sh_hdr(int lo, int hi) {
+  return (int32_t)(((uint32_t)hi << 16) | ((uint32_t)lo & 0xffffu));
+}
+
+// Record the first and last set lane of chunk ballot `b` (chunk starting at
+// diagonal kb) into [lo, hi]; chunks come in ascending order.
+static __device__ __forceinline__ void sh_track(int& lo, int& hi, uint64_t b, int kb) {
+  if (b) {
+    if (lo == INT32_MAX) lo = kb + __builtin_ctzll(b);
+    hi = kb + 63 - __builtin_clzll(b);
+  }
+}
+
+template <int MODE>
+static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen,
+                                  const uint64_t* sP, const uint64_t* sT, int16_t* rows,
+                                  int16_t* hcells, int32_t* hhdr) {
+  constexpr bool CIG = MODE == 1;
+  const int lane = threadIdx.x;
+  const int k_end = tlen - plen;
+  const int ak = k_end < 0 ? -k_end : k_end;
+  auto retry = [&]() {
+    if (lane == 0) {
+      const int slot = atomicAdd(&p.counters[1], 1);
+      p.retry_list[slot] = pair;
+      p.status[pair] = ST_RETRY;
+    }
+  };
+  if (ak > SH_WIN - 64 || plen > 8000 || tlen > 8000 || (CIG && p.hdr_cap < 1)) {
+    retry();
+    return;
+  }
+  // the window centred on the diagonals between 0 and k_end
+  const int base = imin(0, k_end) - (SH_WIN - ak) / 2;
+  const int ringM = p.ringM, ringE = p.ringE1;
+  const int rI1 = ringM, rD1 = ringM + ringE, rNull = ringM + 2 * ringE;
+  // every row (and the NULL row) NULL over the window and its pads
+  {
+    const int n16 = (rNull + 1) * SH_ROW / 8;
+    const int nn = (int)(((uint32_t)(uint16_t)SH_NULL << 16) | (uint16_t)SH_NULL);
+    int4* r4 = reinterpret_cast<int4*>(rows);
+    for (int i = lane; i < n16; i += 64) r4[i] = make_int4(nn, nn, nn, nn);
+  }
+  // row r's cell of diagonal k
+  auto rowp = [&](int r) { return rows + r * SH_ROW + 1 - base; };
+  const int nwf = high_mem_nwf(p, plen, tlen);
+  int status = WFA_STATUS_OK, result = 0, score = 0, nulls = 0;
+  bool overflow = false;
+  __syncthreads();
+
+  // score 0: M0 = {k 0: offset 0} (init_alignment, src/aligner.rs:846-1000),
+  // extended
+  {
+    const int off = lcp<2>(sP, sT, 0, 0, plen, tlen);
+    if (lane == 0) {
+      rowp(0)[0] = sh16(off);
+      if (CIG) {
+        hcells[0 - base] = sh16(off);
+        hhdr[0] = sh_hdr(0, 0);
+      }
+    }
+    if (k_end == 0 && off >= tlen) status = WFA_STATUS_END_REACHED;
+  }
+  __syncthreads();
+
+  uint64_t live = 1;      // bit r: ring row r holds a wavefront (M0 in row 0)
+  int ulo = 0, uhi = 0;   // union of the diagonals of every row so far
+  int sM = 0, sE = 0;     // score mod ringM / ringE, kept as the score advances
+  auto back = [](int cur, int d, int ring) { return cur >= d ? cur - d : cur - d + ring; };
+  // the first score at which a per-score limit applies (one compare per score)
+  int limit = imin(16001, p.max_steps);
+  if (!p.modular) limit = imin(limit, nwf);
+  if (CIG) limit = imin(limit, p.hdr_cap);
+  while (status == WFA_STATUS_OK && !overflow) {
+    score += 1;
+    sM = sM + 1 == ringM ? 0 : sM + 1;
+    sE = sE + 1 == ringE ? 0 : sE + 1;
+    const int s = score;
+    bool last = false;  // max steps: this step is computed, then the loop ends
+    if (s >= limit) {
+      if (!p.modular && s >= nwf) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
+      if (CIG && s >= p.hdr_cap) { overflow = true; break; }
+      if (s > 16000) { overflow = true; break; }
+      last = s >= p.max_steps;
+    }
+    // inputs M[s-x], M[s-o-e], I1/D1[s-e]; a lookback below score 0 lands on
+    // a ring row this pair has not written, whose bit is clear
+    const int rx = back(sM, p.x, ringM), ro = back(sM, p.oe1, ringM);
+    const int ri = rI1 + back(sE, p.e1, ringE), rd = ri + ringE;
+    const int wi = rI1 + sE, wd = rD1 + sE;
+    const uint64_t out_bits = (1ull << sM) | (1ull << wi) | (1ull << wd);
+    if (!(live & ((1ull << rx) | (1ull << ro) | (1ull << ri) | (1ull << rd)))) {
+      // null step (src/aligner.rs:2031-2043): the outputs are absent
+      nulls += 1;
+      live &= ~out_bits;
+      if (CIG && lane == 0) hhdr[s] = SH_HDR_EMPTY;
+      if (last) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
+      if (nulls > p.scope) { status = WFA_STATUS_END_UNREACHABLE; result = s; break; }
+      continue;
+    }
+    nulls = 0;
+    const int16_t* Px = rowp((live >> rx) & 1 ? rx : rNull);
+    const int16_t* Po = rowp((live >> ro) & 1 ? ro : rNull);
+    const int16_t* Pi = rowp((live >> ri) & 1 ? ri : rNull);
+    const int16_t* Pd = rowp((live >> rd) & 1 ? rd : rNull);
+    const int clo = ulo - 1, chi = uhi + 1;
+    if (clo < base || chi > base + SH_WIN - 1) { overflow = true; break; }
+    const int nch = ((chi - clo) >> 6) + 1;
+    // first chunk's diagonal: the chunks stay inside the window
+    const int a = imin(clo, base + SH_WIN - 64 * nch);
+
+    // ---- compute_affine_idm (src/compute/affine.rs:318-351) over the chunks,
+    // trims by ballot (trim_ends, src/compute/mod.rs:14-49) ----
+    int mv[SH_CHUNKS], iv[SH_CHUNKS], dv[SH_CHUNKS];
+    int m_lo = INT32_MAX, m_hi = INT32_MIN, i_lo = INT32_MAX, i_hi = INT32_MIN;
+    int d_lo = INT32_MAX, d_hi = INT32_MIN;
+#pragma unroll
+    for (int j = 0; j < SH_CHUNKS; ++j) {
+      if (j < nch) {
+        const int k = a + 64 * j + lane;
+        const int ins = imax(Po[k - 1], Pi[k - 1]) + 1;
+        const int del = imax(Po[k + 1], Pd[k + 1]);
+        const int m = imax(Px[k] + 1, imax(ins, del));
+        const bool vm = in_bounds(m, k, plen, tlen);
+        mv[j] = vm ? m : WFA_NULL;
+        iv[j] = ins;
+        dv[j] = del;
+        sh_track(m_lo, m_hi, ballot(vm), a + 64 * j);
+        sh_track(i_lo, i_hi, ballot(in_bounds(ins, k, plen, tlen)), a + 64 * j);
+        sh_track(d_lo, d_hi, ballot(in_bounds(del, k, plen, tlen)), a + 64 * j);
+      }
+    }
+    const bool mne = m_lo != INT32_MAX, ine = i_lo != INT32_MAX, dne = d_lo != INT32_MAX;
+    // I1 and D1 keep their cells inside their trimmed ranges (out-of-bounds
+    // ones included), NULL elsewhere; M is NULL wherever it is out of bounds
+    const Guard gi = ine ? Guard{i_lo, i_hi - i_lo} : Guard{INT32_MIN / 2, 0};
+    const Guard gd = dne ? Guard{d_lo, d_hi - d_lo} : Guard{INT32_MIN / 2, 0};
+    int16_t* Wm = rowp(sM);
+    int16_t* Wi = rowp(wi);
+    int16_t* Wd = rowp(wd);
+#pragma unroll
+    for (int j = 0; j < SH_CHUNKS; ++j) {
+      if (j < nch) {
+        const int k = a + 64 * j + lane;
+        Wi[k] = (uint32_t)(k - gi.first) <= (uint32_t)gi.span ? sh16(iv[j]) : (int16_t)SH_NULL;
+        Wd[k] = (uint32_t)(k - gd.first) <= (uint32_t)gd.span ? sh16(dv[j]) : (int16_t)SH_NULL;
+      }
+    }
+    live = (live & ~out_bits) | ((uint64_t)mne << sM) | ((uint64_t)ine << wi) |
+           ((uint64_t)dne << wd);
+    if (mne) { ulo = imin(ulo, m_lo); uhi = imax(uhi, m_hi); }
+    if (ine) { ulo = imin(ulo, i_lo); uhi = imax(uhi, i_hi); }
+    if (dne) { ulo = imin(ulo, d_lo); uhi = imax(uhi, d_hi); }
+    if (last) {
+      // the row is computed but not extended (the loop stops first)
+#pragma unroll
+      for (int j = 0; j < SH_CHUNKS; ++j)
+        if (j < nch) Wm[a + 64 * j + lane] = sh16(mv[j]);
+      status = WFA_STATUS_MAX_STEPS_REACHED;
+      result = s;
+      break;
+    }
+    // ---- extend M[s] (extend_matches_packed_end2end, src/extend.rs:47-69;
+    // the reference extends at the top of the next iteration and nothing
+    // reads M[s] in between), keep it, test the end cell
+    // (termination_end2end, src/termination.rs:14-23) ----
+    bool reached = false;
+    int16_t* hrow = CIG ? hcells + (int64_t)s * SH_WIN - base : nullptr;
+#pragma unroll
+    for (int j = 0; j < SH_CHUNKS; ++j) {
+      if (j < nch) {
+        const int k = a + 64 * j + lane;
+        int off = mv[j];
+        if (off >= 0) off += lcp<2>(sP, sT, off - k, off, plen, tlen);
+        const int16_t o16 = sh16(off);
+        Wm[k] = o16;
+        if (CIG) hrow[k] = o16;
+        reached |= k == k_end && off >= tlen;
+      }
+    }
+    if (CIG && lane == 0) hhdr[s] = sh_hdr(a, a + 64 * nch - 1);
+    __syncthreads();  // the step's rows are written before the next step reads them
+    if (ballot(reached)) { status = WFA_STATUS_END_REACHED; result = s; break; }
+  }
+
+  if (overflow) {
+    retry();
+    return;
+  }
+  int out_status = status;
+  int ncig = 0;
+  if (status == WFA_STATUS_END_REACHED) {
+    out_status = WFA_STATUS_COMPLETED;
+    if (CIG) {
+      __threadfence_block();
+      uint8_t* out = p.cig + p.cig_off[pair];
+      const int cap = plen + tlen;
+      ncig = singletrack<2>(p, WinView{hcells, hhdr, base, score + 1, nwf}, sP, sT, plen, tlen,
+                            score, k_end, tlen, out, cap);
+      // move the ops from the tail of the region to its head
+      const int gap = cap - ncig;
+      if (gap > 0) {
+        for (int c = 0; c < ncig; c += 64) {
+          const int i = c + lane;
+          uint8_t v = 0;
+          if (i < ncig) v = out[gap + i];
+          __builtin_amdgcn_wave_barrier();
+          if (i < ncig) out[i] = v;
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+    }
+  }
+  if (lane == 0) {
+    p.score[pair] = result;
+    p.status[pair] = out_status;
+    if (CIG) p.cig_len[pair] = ncig;
+    if (p.end_pos) {
+      // the end cell, or unset (src/aligner.rs:903-909)
+      const bool end = status == WFA_STATUS_END_REACHED;
+      p.end_pos[3 * pair] = end ? score : -1;
+      p.end_pos[3 * pair + 1] = end ? k_end : INT32_MAX;
+      p.end_pos[3 * pair + 2] = end ? tlen : WFA_NULL;
+    }
+  }
+}
+
+// One wave per pair, persistent over the launch's work queue. The slot's
+// history: p.hdr_cap rows of SH_WIN 16-bit cells in p.hist and p.hdr_cap
+// packed headers in p.hdr, both per workgroup.
+template <int MODE, bool PACKED>
+__global__ void __launch_bounds__(64) wfa_short_kernel(KParams p) {
+  extern __shared__ uint64_t smem[];
+  uint64_t* sP = smem;
+  uint64_t* sT = smem + p.seq_words;
+  int16_t* rows = reinterpret_cast<int16_t*>(sT + p.seq_words);
+  int16_t* hcells =
+      MODE == 1 ? reinterpret_cast<int16_t*>(p.hist) + (int64_t)blockIdx.x * p.hdr_cap * SH_WIN
+                : nullptr;
+  int32_t* hhdr =
+      MODE == 1 ? reinterpret_cast<int32_t*>(p.hdr) + (int64_t)blockIdx.x * p.hdr_cap : nullptr;
+  for (;;) {
+    int item = 0;
+    if (threadIdx.x == 0) item = atomicAdd(&p.counters[0], 1);
+    item = uniform(__shfl(item, 0));
+    if (item >= p.num_pairs) break;
+    // pair-level values are uniform: keep them in scalar registers
+    const int pair = uniform(p.pair_list ? p.pair_list[item] : p.pair_base + item);
+    const int plen = uniform(p.plen[pair]), tlen = uniform(p.tlen[pair]);
+    const int bits = stage_sequences<1>(p, pair, plen, tlen, sP, sT, nullptr);
+    if (bits != 2) {
+      // does not fit, or not A/C/G/T: the general kernel takes it
+      if (threadIdx.x == 0) {
+        const int slot = atomicAdd(&p.counters[1], 1);
+        p.retry_list[slot] = pair;
+        p.status[pair] = ST_RETRY;
+      }
+      continue;
+    }
+    short_pair<MODE>(p, pair, plen, tlen, sP, sT, rows, hcells, hhdr);
+    __syncthreads();
+  }
+}
+
+// LDS bytes of the short kernel's rows (the ring rows and the NULL row)
+static inline size_t short_lds_row_bytes(int ring_rows) {
+  return (size_t)(ring_rows + 1) * SH_ROW * 2;
+}
+
+}  // namespace wfa
