@@ -85,13 +85,39 @@ static __device__ __forceinline__ void sh_track(int& lo, int& hi, uint64_t b, in
   }
 }
 
+// Bases [pos, pos + 16) of a 2-bit sequence staged as 32-bit words
+// (16 bases each, base j at bits [2j, 2j + 2)): one funnel shift of two
+// consecutive words.
+static __device__ __forceinline__ uint32_t sh_w16(const uint32_t* s, uint32_t pos) {
+  const uint32_t q = pos >> 4;
+  return __builtin_amdgcn_alignbit(s[q + 1], s[q], (pos & 15u) * 2u);
+}
+
+// Matching prefix length of pattern[v..] and text[h..], bounded by the
+// sequence ends (extend_matches_packed_end2end's per-diagonal LCP,
+// src/extend.rs:15-38): 16 bases per XOR of funnel-shifted words.
+static __device__ __forceinline__ int sh_lcp(const uint32_t* P, const uint32_t* T, uint32_t v,
+                                             uint32_t h, int rem) {
+  int n = 0;
+  while (rem > 0) {
+    const uint32_t x = sh_w16(P, v + n) ^ sh_w16(T, h + n);
+    // matching bases before the first difference (16 when none)
+    const int c = x ? (int)(__builtin_ctz(x) >> 1) : 16;
+    if (c >= rem) { n += rem; break; }
+    n += c;
+    if (c < 16) break;
+    rem -= 16;
+  }
+  return n;
+}
+
 template <int MODE>
 static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen,
                                   const uint64_t* sP, const uint64_t* sT, int16_t* rows,
                                   int16_t* hcells, int32_t* hhdr) {
   constexpr bool CIG = MODE == 1;
   const int lane = threadIdx.x;
-  const int k_end = tlen - plen;
+  const int k_end = uniform(tlen - plen);
   const int ak = k_end < 0 ? -k_end : k_end;
   auto retry = [&]() {
     if (lane == 0) {
@@ -105,7 +131,7 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
     return;
   }
   // the window centred on the diagonals between 0 and k_end
-  const int base = imin(0, k_end) - (SH_WIN - ak) / 2;
+  const int base = uniform(imin(0, k_end) - (SH_WIN - ak) / 2);
   const int ringM = p.ringM, ringE = p.ringE1;
   const int rI1 = ringM, rD1 = ringM + ringE, rNull = ringM + 2 * ringE;
   // every row (and the NULL row) NULL over the window and its pads
@@ -115,9 +141,11 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
     int4* r4 = reinterpret_cast<int4*>(rows);
     for (int i = lane; i < n16; i += 64) r4[i] = make_int4(nn, nn, nn, nn);
   }
+  const uint32_t* P32 = reinterpret_cast<const uint32_t*>(sP);
+  const uint32_t* T32 = reinterpret_cast<const uint32_t*>(sT);
   // row r's cell of diagonal k
   auto rowp = [&](int r) { return rows + r * SH_ROW + 1 - base; };
-  const int nwf = high_mem_nwf(p, plen, tlen);
+  const int nwf = uniform(high_mem_nwf(p, plen, tlen));
   int status = WFA_STATUS_OK, result = 0, score = 0, nulls = 0;
   bool overflow = false;
   __syncthreads();
@@ -125,7 +153,9 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
   // score 0: M0 = {k 0: offset 0} (init_alignment, src/aligner.rs:846-1000),
   // extended
   {
-    const int off = lcp<2>(sP, sT, 0, 0, plen, tlen);
+    // (the same in every lane: keep it scalar, so the loop state stays
+    // scalar)
+    const int off = uniform(lcp<2>(sP, sT, 0, 0, plen, tlen));
     if (lane == 0) {
       rowp(0)[0] = sh16(off);
       if (CIG) {
@@ -137,7 +167,13 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
   }
   __syncthreads();
 
-  uint64_t live = 1;      // bit r: ring row r holds a wavefront (M0 in row 0)
+  // Which earlier rows hold a wavefront (not absent, not trimmed empty), by
+  // age: bit d of agM is M[s - 1 - d] at the top of step s, likewise agI
+  // and agD for I1 and D1; every step shifts them by one.
+  uint64_t agM = 1;  // M0
+  uint32_t agI = 0, agD = 0;
+  const uint64_t needM = (1ull << (p.x - 1)) | (1ull << (p.oe1 - 1));
+  const uint32_t needE = 1u << (p.e1 - 1);
   int ulo = 0, uhi = 0;   // union of the diagonals of every row so far
   int sM = 0, sE = 0;     // score mod ringM / ringE, kept as the score advances
   auto back = [](int cur, int d, int ring) { return cur >= d ? cur - d : cur - d + ring; };
@@ -145,8 +181,22 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
   int limit = imin(16001, p.max_steps);
   if (!p.modular) limit = imin(limit, nwf);
   if (CIG) limit = imin(limit, p.hdr_cap);
-  while (status == WFA_STATUS_OK && !overflow) {
-    score += 1;
+  limit = uniform(limit);
+  // Every exit of the loop is a uniform break that sets the outcome; the
+  // loop-carried state is read back as scalar at the top of each step (the
+  // same in every lane; the compiler cannot tell across the per-lane
+  // extension, and would otherwise run the whole loop under exec masks).
+  for (int step = status == WFA_STATUS_OK ? 1 : 0; step; ) {
+    score = uniform(score) + 1;
+    ulo = uniform(ulo);
+    uhi = uniform(uhi);
+    nulls = uniform(nulls);
+    agM = ((uint64_t)(uint32_t)uniform((int)(agM >> 32)) << 32) |
+          (uint32_t)uniform((int)(uint32_t)agM);
+    agI = (uint32_t)uniform((int)agI);
+    agD = (uint32_t)uniform((int)agD);
+    sM = uniform(sM);
+    sE = uniform(sE);
     sM = sM + 1 == ringM ? 0 : sM + 1;
     sE = sE + 1 == ringE ? 0 : sE + 1;
     const int s = score;
@@ -157,26 +207,27 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
       if (s > 16000) { overflow = true; break; }
       last = s >= p.max_steps;
     }
-    // inputs M[s-x], M[s-o-e], I1/D1[s-e]; a lookback below score 0 lands on
-    // a ring row this pair has not written, whose bit is clear
-    const int rx = back(sM, p.x, ringM), ro = back(sM, p.oe1, ringM);
-    const int ri = rI1 + back(sE, p.e1, ringE), rd = ri + ringE;
-    const int wi = rI1 + sE, wd = rD1 + sE;
-    const uint64_t out_bits = (1ull << sM) | (1ull << wi) | (1ull << wd);
-    if (!(live & ((1ull << rx) | (1ull << ro) | (1ull << ri) | (1ull << rd)))) {
+    // inputs M[s-x], M[s-o-e], I1/D1[s-e]; a lookback below score 0 finds
+    // its age bit clear
+    if (!(agM & needM) && !((agI | agD) & needE)) {
       // null step (src/aligner.rs:2031-2043): the outputs are absent
       nulls += 1;
-      live &= ~out_bits;
+      agM <<= 1;
+      agI <<= 1;
+      agD <<= 1;
       if (CIG && lane == 0) hhdr[s] = SH_HDR_EMPTY;
       if (last) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
       if (nulls > p.scope) { status = WFA_STATUS_END_UNREACHABLE; result = s; break; }
       continue;
     }
     nulls = 0;
-    const int16_t* Px = rowp((live >> rx) & 1 ? rx : rNull);
-    const int16_t* Po = rowp((live >> ro) & 1 ? ro : rNull);
-    const int16_t* Pi = rowp((live >> ri) & 1 ? ri : rNull);
-    const int16_t* Pd = rowp((live >> rd) & 1 ? rd : rNull);
+    const int rx = back(sM, p.x, ringM), ro = back(sM, p.oe1, ringM);
+    const int ri = rI1 + back(sE, p.e1, ringE), rd = ri + ringE;
+    const int wi = rI1 + sE, wd = rD1 + sE;
+    const int16_t* Px = rowp((agM >> (p.x - 1)) & 1 ? rx : rNull);
+    const int16_t* Po = rowp((agM >> (p.oe1 - 1)) & 1 ? ro : rNull);
+    const int16_t* Pi = rowp(agI & needE ? ri : rNull);
+    const int16_t* Pd = rowp(agD & needE ? rd : rNull);
     const int clo = ulo - 1, chi = uhi + 1;
     if (clo < base || chi > base + SH_WIN - 1) { overflow = true; break; }
     const int nch = ((chi - clo) >> 6) + 1;
@@ -195,13 +246,17 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
         const int ins = imax(Po[k - 1], Pi[k - 1]) + 1;
         const int del = imax(Po[k + 1], Pd[k + 1]);
         const int m = imax(Px[k] + 1, imax(ins, del));
-        const bool vm = in_bounds(m, k, plen, tlen);
+        // in bounds: offset h <= tlen and h - k <= plen, both unsigned
+        auto vcol = [&](int h) {
+          return (uint32_t)h <= (uint32_t)tlen ? (uint32_t)(h - k) : 0xffffffffu;
+        };
+        const bool vm = vcol(m) <= (uint32_t)plen;
         mv[j] = vm ? m : WFA_NULL;
         iv[j] = ins;
         dv[j] = del;
         sh_track(m_lo, m_hi, ballot(vm), a + 64 * j);
-        sh_track(i_lo, i_hi, ballot(in_bounds(ins, k, plen, tlen)), a + 64 * j);
-        sh_track(d_lo, d_hi, ballot(in_bounds(del, k, plen, tlen)), a + 64 * j);
+        sh_track(i_lo, i_hi, ballot(vcol(ins) <= (uint32_t)plen), a + 64 * j);
+        sh_track(d_lo, d_hi, ballot(vcol(del) <= (uint32_t)plen), a + 64 * j);
       }
     }
     const bool mne = m_lo != INT32_MAX, ine = i_lo != INT32_MAX, dne = d_lo != INT32_MAX;
@@ -220,8 +275,9 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
         Wd[k] = (uint32_t)(k - gd.first) <= (uint32_t)gd.span ? sh16(dv[j]) : (int16_t)SH_NULL;
       }
     }
-    live = (live & ~out_bits) | ((uint64_t)mne << sM) | ((uint64_t)ine << wi) |
-           ((uint64_t)dne << wd);
+    agM = (agM << 1) | (uint64_t)mne;
+    agI = (agI << 1) | (uint32_t)ine;
+    agD = (agD << 1) | (uint32_t)dne;
     if (mne) { ulo = imin(ulo, m_lo); uhi = imax(uhi, m_hi); }
     if (ine) { ulo = imin(ulo, i_lo); uhi = imax(uhi, i_hi); }
     if (dne) { ulo = imin(ulo, d_lo); uhi = imax(uhi, d_hi); }
@@ -245,7 +301,8 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
       if (j < nch) {
         const int k = a + 64 * j + lane;
         int off = mv[j];
-        if (off >= 0) off += lcp<2>(sP, sT, off - k, off, plen, tlen);
+        if (off >= 0)
+          off += sh_lcp(P32, T32, (uint32_t)(off - k), (uint32_t)off, imin(plen - (off - k), tlen - off));
         const int16_t o16 = sh16(off);
         Wm[k] = o16;
         if (CIG) hrow[k] = o16;
