@@ -158,10 +158,7 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
     const int off = uniform(lcp<2>(sP, sT, 0, 0, plen, tlen));
     if (lane == 0) {
       rowp(0)[0] = sh16(off);
-      if (CIG) {
-        hcells[0 - base] = sh16(off);
-        hhdr[0] = sh_hdr(0, 0);
-      }
+      if (CIG) hcells[0 - base] = sh16(off);
     }
     if (k_end == 0 && off >= tlen) status = WFA_STATUS_END_REACHED;
   }
@@ -175,8 +172,20 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
   const uint64_t needM = (1ull << (p.x - 1)) | (1ull << (p.oe1 - 1));
   const uint32_t needE = 1u << (p.e1 - 1);
   int ulo = 0, uhi = 0;   // union of the diagonals of every row so far
-  int sM = 0, sE = 0;     // score mod ringM / ringE, kept as the score advances
+  // ring rows as LDS byte offsets from `rows`: score s's M row at
+  // (s mod ringM) * RB, its I1 / D1 rows at iB / dB + (s mod ringE) * RB
+  constexpr int RB = SH_ROW * 2;
+  const int ringMB = ringM * RB, ringEB = ringE * RB;
+  const int xB = p.x * RB, oeB = p.oe1 * RB, eB = p.e1 * RB;
+  const int iB = rI1 * RB, dB = rD1 * RB, nullB = rNull * RB;
+  int sMB = 0, sEB = 0;   // (s mod ringM) * RB, (s mod ringE) * RB
   auto back = [](int cur, int d, int ring) { return cur >= d ? cur - d : cur - d + ring; };
+  auto cell = [&](int byte_off) {
+    return reinterpret_cast<int16_t*>(reinterpret_cast<char*>(rows) + byte_off) + 1 - base;
+  };
+  // CIGAR headers of 64 consecutive scores, lane i holding score
+  // (s & ~63) + i, stored once the block is full (and at the end)
+  int hbuf = lane == 0 ? sh_hdr(0, 0) : 0;
   // the first score at which a per-score limit applies (one compare per score)
   int limit = imin(16001, p.max_steps);
   if (!p.modular) limit = imin(limit, nwf);
@@ -195,10 +204,10 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
           (uint32_t)uniform((int)(uint32_t)agM);
     agI = (uint32_t)uniform((int)agI);
     agD = (uint32_t)uniform((int)agD);
-    sM = uniform(sM);
-    sE = uniform(sE);
-    sM = sM + 1 == ringM ? 0 : sM + 1;
-    sE = sE + 1 == ringE ? 0 : sE + 1;
+    sMB = uniform(sMB) + RB;
+    sEB = uniform(sEB) + RB;
+    if (sMB == ringMB) sMB = 0;
+    if (sEB == ringEB) sEB = 0;
     const int s = score;
     bool last = false;  // max steps: this step is computed, then the loop ends
     if (s >= limit) {
@@ -215,19 +224,20 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
       agM <<= 1;
       agI <<= 1;
       agD <<= 1;
-      if (CIG && lane == 0) hhdr[s] = SH_HDR_EMPTY;
+      if (CIG) {
+        hbuf = lane == (s & 63) ? SH_HDR_EMPTY : hbuf;
+        if ((s & 63) == 63) hhdr[(s & ~63) + lane] = hbuf;
+      }
       if (last) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
       if (nulls > p.scope) { status = WFA_STATUS_END_UNREACHABLE; result = s; break; }
       continue;
     }
     nulls = 0;
-    const int rx = back(sM, p.x, ringM), ro = back(sM, p.oe1, ringM);
-    const int ri = rI1 + back(sE, p.e1, ringE), rd = ri + ringE;
-    const int wi = rI1 + sE, wd = rD1 + sE;
-    const int16_t* Px = rowp((agM >> (p.x - 1)) & 1 ? rx : rNull);
-    const int16_t* Po = rowp((agM >> (p.oe1 - 1)) & 1 ? ro : rNull);
-    const int16_t* Pi = rowp(agI & needE ? ri : rNull);
-    const int16_t* Pd = rowp(agD & needE ? rd : rNull);
+    const int eb = back(sEB, eB, ringEB);
+    const int16_t* Px = cell((agM >> (p.x - 1)) & 1 ? back(sMB, xB, ringMB) : nullB);
+    const int16_t* Po = cell((agM >> (p.oe1 - 1)) & 1 ? back(sMB, oeB, ringMB) : nullB);
+    const int16_t* Pi = cell(agI & needE ? iB + eb : nullB);
+    const int16_t* Pd = cell(agD & needE ? dB + eb : nullB);
     const int clo = ulo - 1, chi = uhi + 1;
     if (clo < base || chi > base + SH_WIN - 1) { overflow = true; break; }
     const int nch = ((chi - clo) >> 6) + 1;
@@ -264,9 +274,9 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
     // ones included), NULL elsewhere; M is NULL wherever it is out of bounds
     const Guard gi = ine ? Guard{i_lo, i_hi - i_lo} : Guard{INT32_MIN / 2, 0};
     const Guard gd = dne ? Guard{d_lo, d_hi - d_lo} : Guard{INT32_MIN / 2, 0};
-    int16_t* Wm = rowp(sM);
-    int16_t* Wi = rowp(wi);
-    int16_t* Wd = rowp(wd);
+    int16_t* Wm = cell(sMB);
+    int16_t* Wi = cell(iB + sEB);
+    int16_t* Wd = cell(dB + sEB);
 #pragma unroll
     for (int j = 0; j < SH_CHUNKS; ++j) {
       if (j < nch) {
@@ -309,8 +319,14 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
         reached |= k == k_end && off >= tlen;
       }
     }
-    if (CIG && lane == 0) hhdr[s] = sh_hdr(a, a + 64 * nch - 1);
-    __syncthreads();  // the step's rows are written before the next step reads them
+    if (CIG) {
+      hbuf = lane == (s & 63) ? sh_hdr(a, a + 64 * nch - 1) : hbuf;
+      if ((s & 63) == 63) hhdr[(s & ~63) + lane] = hbuf;
+    }
+    // the step's rows are written before the next step reads them: one
+    // wave's LDS accesses complete in order, so only the compiler must keep
+    // them in program order
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     if (ballot(reached)) { status = WFA_STATUS_END_REACHED; result = s; break; }
   }
 
@@ -318,6 +334,8 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
     retry();
     return;
   }
+  // the headers of the last, partial block
+  if (CIG && lane <= (score & 63)) hhdr[(score & ~63) + lane] = hbuf;
   int out_status = status;
   int ncig = 0;
   if (status == WFA_STATUS_END_REACHED) {
