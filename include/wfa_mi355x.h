@@ -64,9 +64,12 @@ extern "C" {
 #define WFA_HEURISTIC_WFADAPTIVE       1  /* p0 = min_wavefront_length, p1 = max_distance_threshold, p2 = steps_between_cutoffs */
 #define WFA_HEURISTIC_XDROP            2  /* p0 = xdrop, p1 = steps_between_cutoffs */
 #define WFA_HEURISTIC_ZDROP            3  /* p0 = zdrop, p1 = steps_between_cutoffs;
-                                              score-only or high-memory CIGARs
-                                              (a medium/low-memory CIGAR or BiWFA:
-                                              WFA_ERR_UNSUPPORTED) */
+                                              not with BiWFA (WFA_ERR_UNSUPPORTED).
+                                              A medium/low-memory CIGAR in
+                                              WFA_COMPAT_C is read at the best
+                                              cell's diagonal of the stop score,
+                                              as the reference reads it
+                                              (HPCBio/wfa2lib-rs#22) */
 #define WFA_HEURISTIC_BANDED_STATIC    4  /* p0 = min_k, p1 = max_k */
 #define WFA_HEURISTIC_BANDED_ADAPTIVE  5  /* p0 = min_k, p1 = max_k, p2 = steps_between_cutoffs */
 
@@ -265,9 +268,10 @@ typedef struct wfa_batch_t {
  * its heuristic only when attr.heuristic is not WFA_HEURISTIC_NONE (the
  * attribute's own free ends and heuristic parameters are then unused).
  * The caller validates each pair's values as wfa_attr_t values are
- * validated: free ends >= 0; WFA_HEURISTIC_ZDROP only for score-only and
- * high-memory batches (elsewhere the pair runs without a heuristic). Not for
- * WFA_MEMORY_ULTRALOW batches. */
+ * validated: free ends >= 0, a heuristic kind of WFA_HEURISTIC_* (any other
+ * value runs without one). Every heuristic, Z-drop included, runs in every
+ * memory mode and compat of the batch. Not for WFA_MEMORY_ULTRALOW
+ * batches. */
 typedef struct wfa_pair_config_t {
     int32_t span;                   /* WFA_SPAN_* */
     int32_t extension;              /* 1 = extension (span WFA_SPAN_ENDSFREE) */

@@ -436,15 +436,9 @@ wfo_aligner* wfo_aligner_new(const wfa_attr_t* attr, int32_t* err) {
         free(a);
         return NULL;
     }
-    /* Z-drop with a medium/low-memory CIGAR: Compat::C backtraces the best
-     * cell at the current score, reading whatever backtrace chain that
-     * wavefront holds there (HPCBio/wfa2lib-rs#22, src/aligner.rs:1443-1460):
-     * not restated. BiWFA rejects X-drop and Z-drop as the driver does. */
+    /* BiWFA rejects X-drop and Z-drop as the driver does. */
     const int zdrop = attr->heuristic == WFA_HEURISTIC_ZDROP;
-    if ((zdrop && attr->compat != WFA_COMPAT_CORRECTED &&
-         attr->alignment_scope == WFA_SCOPE_ALIGNMENT &&
-         (attr->memory_mode == WFA_MEMORY_MED || attr->memory_mode == WFA_MEMORY_LOW)) ||
-        (attr->memory_mode == WFA_MEMORY_ULTRALOW &&
+    if ((attr->memory_mode == WFA_MEMORY_ULTRALOW &&
          (attr->span != WFA_SPAN_END2END || attr->heuristic == WFA_HEURISTIC_XDROP || zdrop))) {
         if (err) *err = WFA_ERR_UNSUPPORTED;
         free(a);
@@ -1714,8 +1708,11 @@ static void backtrace_from(wfo_aligner* a, int score, int k, int offset) {
 }
 
 /* backtrace_zdrop_end (src/aligner.rs:1393-1406): the best cell, backtraced
- * from the score it was found at (Compat::C: high memory; its
- * medium/low-memory CIGAR is refused at construction. Corrected: in every
+ * from the score it was found at (Compat::C: high memory; medium/low memory
+ * read the best cell's diagonal in the M wavefront of the current score,
+ * whatever it holds there: an unallocated cell gives an empty CIGAR, one
+ * never written the zeroed chain of the first block (backtrace_piggyback,
+ * src/aligner.rs:1443-1460, HPCBio/wfa2lib-rs#22). Corrected: in every
  * memory mode, med/low from the chain kept when the cell was found). */
 static void backtrace_zdrop_end(wfo_aligner* a, int score) {
     if (a->attr.compat == WFA_COMPAT_CORRECTED && a->piggyback) {

@@ -73,8 +73,6 @@ def test_cli_output_lines(args, vecs, pkg, tmp_path):
      "Invalid --wfa-heuristic-parameters"),
     (["--wfa-span", "ends-free,1,2"], "Invalid --wfa-span"),
     (["--wfa-memory", "huge"], "--wfa-memory"),
-    (["--wfa-heuristic", "zdrop", "--wfa-heuristic-parameters", "20,1", "--wfa-memory", "med"],
-     "medium/low-memory CIGAR"),
     (["-g", "1,4,6,2"], "invalid penalties"),
     (["-a", "gap-linear-wfa", "-p", "0,4,0"], "invalid penalties"),
 ])

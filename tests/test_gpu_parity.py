@@ -242,10 +242,7 @@ def test_heuristics_match_oracle(hname, h, sname, span, metric, pen, memory, pkg
     """WF-adaptive, X-drop, banded static/adaptive: cut-offs, band clamp,
     statuses and (gap-affine/2p) CIGAR rescoring, against the oracle."""
     pairs = _heuristic_pairs()
-    # Z-drop with a medium-memory CIGAR is refused (score only runs)
-    scopes = ("score",) if h[0] == N.HEURISTIC_ZDROP and memory == "med" else \
-        ("score", "alignment")
-    for scope in scopes:
+    for scope in ("score", "alignment"):
         attr = attr_for(N, metric, pen, scope, span=span)
         attr.memory_mode = N.MEMORY_HIGH if memory == "high" else N.MEMORY_MED
         attr.heuristic = h[0]
@@ -289,11 +286,14 @@ ZDROP_SPANS = [("end2end", None), ("extension", "extension"), ("endsfree", [20, 
 
 @pytest.mark.parametrize("metric,pen", ZDROP_CONFIGS)
 @pytest.mark.parametrize("span_name,span", ZDROP_SPANS)
-def test_zdrop_end_positions_match_oracle(metric, pen, span_name, span, pkg, N, engine,
+@pytest.mark.parametrize("memory", ["high", "med", "low"])
+def test_zdrop_end_positions_match_oracle(metric, pen, span_name, span, memory, pkg, N, engine,
                                           oracle):
-    """Z-drop (score only and high-memory CIGARs): scores, statuses, CIGARs
-    and each pair's end position (the best cell and the score it was found
-    at) equal the oracle's, on pairs whose unrelated tails make it fire."""
+    """Z-drop (score only and CIGARs, every memory mode, Compat::C): scores,
+    statuses, CIGARs and each pair's end position (the best cell and the
+    score it was found at) equal the oracle's, on pairs whose unrelated
+    tails make it fire. Medium/low memory backtrace the best cell's
+    diagonal at the stop score (src/aligner.rs:1443-1460)."""
     import torch
     from wfa2lib_rs_amd.engine import DeviceBatch
     from wfa2lib_rs_amd.synth import Workload, generate
@@ -303,6 +303,7 @@ def test_zdrop_end_positions_match_oracle(metric, pen, span_name, span, pkg, N, 
     fired = 0
     for scope in ("score", "alignment"):
         attr = attr_for(N, metric, pen, scope, span=span)
+        attr.memory_mode = {"high": N.MEMORY_HIGH, "med": N.MEMORY_MED, "low": N.MEMORY_LOW}[memory]
         attr.heuristic = N.HEURISTIC_ZDROP
         attr.heuristic_p[0], attr.heuristic_p[1] = 5, 1
         b = DeviceBatch(pairs, device="cuda:0", cigar=scope == "alignment", end_position=True)
@@ -319,6 +320,27 @@ def test_zdrop_end_positions_match_oracle(metric, pen, span_name, span, pkg, N, 
             fired += ost == STATUS["PARTIAL"]
     if metric != "edit" and pen[1] >= 4 and span is None:
         assert fired >= 10, "the pairs must make Z-drop fire"
+
+
+def test_zdrop_issue22_c_med_low_on_gpu(pkg, N, engine, oracle):
+    """tests/corrected_zdrop.rs:79-89 on the device: Compat::C, Z-drop 50,10
+    with (-3,5,3,2), medium and low memory: the CIGAR read from the stop
+    score's dead cell scores 4502 and equals the oracle's."""
+    from test_oracle import _issue22_pair, _zdrop_attr
+    from wfa2lib_rs_amd.synth import from_ascii
+    p, t = _issue22_pair()
+    pen = pkg.AffinePenalties(-3, 5, 3, 2)
+    for mem in ("MED", "LOW"):
+        A = pkg.AffineAligner(pkg.WavefrontPenalties.new_affine(pen))
+        A.set_alignment_scope(pkg.AlignmentScope.COMPUTE_ALIGNMENT)
+        A.set_heuristic(pkg.HeuristicStrategy.zdrop(50, 10))
+        A.set_memory_mode(getattr(pkg.MemoryMode, mem))
+        A.align_end2end(p, t)
+        assert A.status() == pkg.AlignStatus.PARTIAL
+        assert A.cigar().score_gap_affine(pen) == 4502, mem
+        a = _zdrop_attr(N, "gap_affine", (-3, 5, 3, 2, 0, 0), "alignment", 50, 10)
+        a.memory_mode = getattr(N, "MEMORY_" + mem)
+        _compare(engine, oracle, a, from_ascii([p], [t]), True, label=f"issue22 {mem}")
 
 
 def test_zdrop_issue22_on_gpu(pkg, N, engine):
@@ -344,7 +366,8 @@ def test_zdrop_issue22_on_gpu(pkg, N, engine):
     assert 0 <= pos.score < s and pos.offset - pos.k >= v and pos.offset >= h
 
 
-def test_per_pair_configs_match_oracle(pkg, N, engine, oracle):
+@pytest.mark.parametrize("memory", ["high", "med"])
+def test_per_pair_configs_match_oracle(memory, pkg, N, engine, oracle):
     """A batch may carry each pair's span and heuristic (the serving loop
     batches mixed requests that way): end-to-end, ends-free with per-pair
     free ends and extension pairs, with and without each heuristic, in one
@@ -365,6 +388,7 @@ def test_per_pair_configs_match_oracle(pkg, N, engine, oracle):
             plen, tlen = int(pairs.plen[i]), int(pairs.tlen[i])
             span = [None, "extension", [plen // 10, plen // 7, tlen // 10, tlen // 5]][i % 3]
             a = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment", span=span)
+            a.memory_mode = N.MEMORY_HIGH if memory == "high" else N.MEMORY_MED
             h = heuristics[(i // 3) % len(heuristics)] if use_heur else None
             if h is not None:
                 a.heuristic = {"wfadaptive": 1, "xdrop": 2, "zdrop": 3, "banded_static": 4,
@@ -373,6 +397,7 @@ def test_per_pair_configs_match_oracle(pkg, N, engine, oracle):
                     a.heuristic_p[j] = v
             attrs.append(a)
         env = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment", span=[0, 0, 0, 0])
+        env.memory_mode = N.MEMORY_HIGH if memory == "high" else N.MEMORY_MED
         if use_heur:
             env.heuristic = N.HEURISTIC_WFADAPTIVE
         cfg = np.array([pair_config_row(a) for a in attrs], dtype=np.int32)

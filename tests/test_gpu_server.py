@@ -134,10 +134,10 @@ def test_serving_rejects_invalid_requests(pkg, N, server):
     bad = attr_for(N, "gap_affine", (1, 4, 6, 2, 0, 0), "alignment")  # match > 0
     with pytest.raises(RuntimeError, match="invalid penalties"):
         server.submit(bad, b"ACGT", b"ACGT", tag=1)
-    # Z-drop with a medium-memory CIGAR is not on the device path
+    # BiWFA takes no Z-drop (src/bin/align_benchmark.rs:529-560)
     z = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment")
     z.heuristic = N.HEURISTIC_ZDROP
-    z.memory_mode = N.MEMORY_MED
+    z.memory_mode = N.MEMORY_ULTRALOW
     with pytest.raises(RuntimeError, match="unsupported"):
         server.submit(z, b"ACGT", b"ACGT", tag=2)
     # an empty pair is served

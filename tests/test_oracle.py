@@ -594,18 +594,36 @@ def test_oracle_corrected_equals_c_with_match0(N, oracle):
 
 
 def test_oracle_zdrop_refusals(N, oracle):
-    """Z-drop with a medium/low-memory CIGAR (the reference's known issue
-    HPCBio/wfa2lib-rs#22) and with BiWFA is refused; score-only medium
-    memory runs."""
-    for mem in (N.MEMORY_MED, N.MEMORY_LOW, N.MEMORY_ULTRALOW):
-        a = _zdrop_attr(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment", 20, 1)
-        a.memory_mode = mem
-        with pytest.raises(ValueError):
-            oracle.OracleAligner(a)
-    a = _zdrop_attr(N, "gap_affine", (0, 4, 6, 2, 0, 0), "score", 20, 1)
-    a.memory_mode = N.MEMORY_MED
-    oracle.OracleAligner(a)
-    # Compat::Corrected backtraces the best cell's own chain: runs
+    """Z-drop with BiWFA is refused (src/bin/align_benchmark.rs:529-560);
+    every other memory mode and scope runs."""
     a = _zdrop_attr(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment", 20, 1)
-    a.memory_mode, a.compat = N.MEMORY_MED, N.COMPAT_CORRECTED
-    oracle.OracleAligner(a)
+    a.memory_mode = N.MEMORY_ULTRALOW
+    with pytest.raises(ValueError):
+        oracle.OracleAligner(a)
+    for scope in ("alignment", "score"):
+        for mem in (N.MEMORY_HIGH, N.MEMORY_MED, N.MEMORY_LOW):
+            for compat in (N.COMPAT_C, N.COMPAT_CORRECTED):
+                a = _zdrop_attr(N, "gap_affine", (0, 4, 6, 2, 0, 0), scope, 20, 1)
+                a.memory_mode, a.compat = mem, compat
+                oracle.OracleAligner(a)
+
+
+def test_oracle_zdrop_issue22_c_med_low_literal(pkg, N, oracle):
+    """tests/corrected_zdrop.rs:79-89 (Compat::C): Z-drop 50,10 with
+    (-3,5,3,2) stops at the same score in every memory mode; medium and low
+    memory backtrace the best cell's diagonal at the stop score, a cell that
+    is no part of the best alignment (HPCBio/wfa2lib-rs#22), and their CIGAR
+    scores 4502 where high memory's scores 4498."""
+    p, t = _issue22_pair()
+    pen = (-3, 5, 3, 2, 0, 0)
+    high = oracle.OracleAligner(_zdrop_attr(N, "gap_affine", pen, "alignment", 50, 10))
+    hs, hst, hops = high.align_end2end(p, t)
+    for mem in (N.MEMORY_MED, N.MEMORY_LOW):
+        a = _zdrop_attr(N, "gap_affine", pen, "alignment", 50, 10)
+        a.memory_mode = mem
+        A = oracle.OracleAligner(a)
+        s, st, ops = A.align_end2end(p, t)
+        assert (s, st) == (hs, hst) == (s, STATUS["PARTIAL"])
+        assert pkg.Cigar(ops).score_gap_affine(pkg.AffinePenalties(-3, 5, 3, 2)) == 4502
+        assert A.end_position() == high.end_position()
+        assert ops != hops

@@ -203,9 +203,6 @@ def main(argv=None, out=None):
         fail("[WFA] BiWFA is not compatible with X-drop/Z-drop heuristics")
     if is_drop and penalties.distance_metric in (DistanceMetric.EDIT, DistanceMetric.INDEL):
         fail("[WFA] Heuristics drops are not compatible with 'edit'/'indel' distance metrics")
-    if (heur is not None and heur[0] == N.HEURISTIC_ZDROP and not score_only and
-            memory in (N.MEMORY_MED, N.MEMORY_LOW) and not a.no_c_compat):
-        fail("[WFA] Z-drop with a medium/low-memory CIGAR is not available on the device path")
 
     attr = N.Attr()
     N.lib().wfa_attr_default(ctypes.byref(attr), int(penalties.distance_metric))

@@ -265,6 +265,7 @@ struct Hist {
   uint8_t* bytes;   // medium/low memory: the pool as winner flags per (score, k)
   uint8_t* chain;   // medium/low memory: op chain scratch
   int chain_cap;
+  uint8_t* signs;   // Compat::C Z-drop: per pool cell, the signs of its offsets (or null)
   int hdr_cap;
   int32_t* tab;     // this slot's chunk ids
   int tab_cap;
@@ -279,9 +280,11 @@ static __device__ __forceinline__ int64_t hist_row(const Hist& H, const int4& h)
 
 // Winner flags of one (score, k) in medium/low memory. Bits 0-2: the
 // source of M (0 I1, 1 I2, 2 D1, 3 D2, 4 mismatch; for edit and gap-linear
-// 0 ins, 1 del, 2 mismatch; 7 a begin-free cell, where a chain starts);
-// bits 3-6: I1, D1, I2, D2 took their extension input.
+// 0 ins, 1 del, 2 mismatch; 7 a begin-free cell, where a chain starts; 6
+// a cell set NULL when M was widened to a begin-free cell, whose chain is
+// empty); bits 3-6: I1, D1, I2, D2 took their extension input.
 #define FL_M_SEED 7
+#define FL_M_NULLED 6
 #define FL_I1_EXT 8
 #define FL_D1_EXT 16
 #define FL_I2_EXT 32
@@ -605,24 +608,74 @@ static __device__ int rescore_cigar(const KParams& p, const uint8_t* ops, int n,
 // kernels push (src/compute/piggyback.rs:195-528), recovered by following
 // the stored winner flags from the end cell back to M at score 0, then
 // unpacked exactly as unpack_piggyback (src/aligner.rs:232-311).
+//
+// `ztr` (Z-drop ends of Compat::C, medium/low memory, else null): the chain
+// starts at M cell (score, k) of the stop score whatever that cell holds
+// (backtrace_piggyback, src/aligner.rs:1443-1460): a cell the step did not
+// compute has the zeroed chain, and a cell whose chain reads a cell outside
+// that component's trimmed range at its score (ztr, per score {M lo, hi, I1
+// lo, hi}, {D1 lo, hi, I2 lo, hi}, {D2 lo, hi}) took that cell's zeroed
+// chain (init_ends, src/wavefront.rs:422-457): both lead to the first
+// backtrace block, the alignment's start (0, 0).
 template <int KIND, int BITS>
 static __device__ int piggyback_backtrace(const KParams& p, const Hist& H, int nscores,
                                           const uint64_t* sP, const uint64_t* sT, int plen,
                                           int tlen, int score, int k, int offset, uint8_t* out,
-                                          int cap) {
+                                          int cap, const int4* ztr = nullptr) {
   const int lane = threadIdx.x;
   enum { CM = 0, CI1, CD1, CI2, CD2 };
   int comp = CM, s = score, kk = k;
   int pos = H.chain_cap - 1;
   bool bad = false;
+  bool root00 = false;  // the chain reached a zeroed cell: it starts at (0, 0)
+  // whether cell kk of component c at score s lies in that component's
+  // trimmed range
+  auto in_trim = [&](int c, int s_, int k_) -> bool {
+    const int4 a = ztr[3 * s_ + (c == CM || c == CI1 ? 0 : c == CD1 || c == CI2 ? 1 : 2)];
+    const int l = c == CM || c == CD1 || c == CD2 ? a.x : a.z;
+    const int h = c == CM || c == CD1 || c == CD2 ? a.y : a.w;
+    return k_ >= l && k_ <= h;
+  };
+  // (an M cell's I/D source at the same score was computed in the same
+  // step, not read through a range: the test applies to earlier scores)
+  int last_s = score;  // the score of the cell whose operation came last
   for (;;) {
+    const bool first = pos == H.chain_cap - 1;
+    const int succ_s = last_s;
+    if (ztr && s != succ_s && (s < 0 || !in_trim(comp, s, kk))) { root00 = true; break; }
+    last_s = s;
     if (comp == CM && s == 0) break;
     if (s < 0 || s >= nscores || pos < 0) { bad = true; break; }
     const int4 hd = H.hdr[s];
-    if (kk < hd.x || kk > hd.y) { bad = true; break; }
+    if (kk < hd.x || kk > hd.y) {
+      // the stop score's cell outside the cells its step computed
+      if (ztr && first) { root00 = true; break; }
+      bad = true;
+      break;
+    }
     const int f = H.bytes[hist_row(H, hd) + (kk - hd.x)];
     // a begin-free cell added during compute starts its chain
     if (comp == CM && (f & 7) == FL_M_SEED) break;
+    // a cell nulled when M was widened holds the zeroed chain
+    // (set_null_cell, src/wavefront.rs:400-408): only a Z-drop end reads one
+    if (comp == CM && (f & 7) == FL_M_NULLED) {
+      if (ztr) root00 = true;
+      else bad = true;
+      break;
+    }
+    // (Compat::C Z-drop ends) whether this cell's packed CIGAR went to the
+    // backtrace buffer before the chain read it on: its wavefront offloaded
+    // at its score (occupancy at the limit; edit every 16 scores), its
+    // offset is not negative (offload_selective, src/compute/piggyback.rs:
+    // 535-556), and the next cell of the chain is of a later score (an M
+    // cell takes its I/D source's chain in the same step) or is the stop
+    // score's cell itself
+    bool offl = false;
+    if (ztr && (first || succ_s != s)) {
+      const int pre = (ztr[3 * s + 2].z >> (5 * comp)) & 31;
+      const bool at = KIND == KIND_EDIT ? s % 16 == 0 : KIND == KIND_LINEAR ? pre >= 16 : pre >= 15;
+      offl = at && ((H.signs[hist_row(H, hd) + (kk - hd.x)] >> comp) & 1);
+    }
     uint8_t op;
     if (KIND == KIND_EDIT || KIND == KIND_LINEAR) {
       // one M component: a mismatch came from M[s - x], a gap from
@@ -649,14 +702,37 @@ static __device__ int piggyback_backtrace(const KParams& p, const Hist& H, int n
       else { s -= comp == CD1 ? p.oe1 : p.oe2; comp = CM; }
       kk += 1;
     }
-    if (lane == 0) H.chain[pos] = op;
+    if (lane == 0) H.chain[pos] = op | (offl ? 0x80 : 0);
     --pos;
   }
   if (bad) return -1;
+  if (ztr && lane == 0) {
+    // The reference packs a cell's operations into 32 bits, 16 at most: a
+    // chain that grows past 16 since its last offload keeps only the last
+    // 16, and an offload starts a new block once the chain holds at least
+    // the mask's count (src/pcigar.rs, offload_selective): replay that
+    // window forward over the chain and drop what it lost (0 bytes).
+    const bool aff = KIND == KIND_AFFINE || KIND == KIND_AFFINE2P;
+    const int need = p.low ? (aff ? 15 : 16) : 9;
+    int cur = 0, wstart = pos + 1;
+    for (int i = pos + 1; i < H.chain_cap; ++i) {
+      const uint8_t c = H.chain[i];
+      ++cur;
+      if (cur > 16) {
+        H.chain[wstart++] = 0;
+        cur = 16;
+      }
+      if ((c & 0x80) && cur >= need) {
+        cur = 0;
+        wstart = i + 1;
+      }
+      H.chain[i] = c & 0x7f;
+    }
+  }
   __threadfence_block();
   // the chain starts at the M cell of score 0 it reached: (0,0) or an
   // ends-free seed (h,0) at k = h, (0,v) at k = -v
-  const int bv = kk < 0 ? -kk : 0, bh = kk > 0 ? kk : 0;
+  const int bv = root00 || kk >= 0 ? 0 : -kk, bh = !root00 && kk > 0 ? kk : 0;
   const int nchain = H.chain_cap - 1 - pos;
   const uint8_t* ch = H.chain + pos + 1;
   const bool affine = KIND == KIND_AFFINE || KIND == KIND_AFFINE2P;
@@ -677,6 +753,7 @@ static __device__ int piggyback_backtrace(const KParams& p, const Hist& H, int n
     const int cnt = imin(64, nchain - b);
     for (int j = 0; j < cnt; ++j) {
       const int op = uniform(__shfl(c, j));
+      if (op == 0) continue;  // lost from its packed CIGAR (above)
       if (in_m || !affine) {
         const int m = lcp<BITS>(sP, sT, v, h, plen, tlen);
         put_run('M', m);
@@ -975,11 +1052,8 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     fe = make_int4(imax(uniform(c0.z), 0), imax(uniform(c0.w), 0), imax(uniform(c1.x), 0),
                    imax(uniform(c1.y), 0));
     hc = HeurCfg{uniform(c1.z), uniform(c1.w), uniform(c2.x), uniform(c2.y)};
-    // (Z-drop with a medium/low-memory CIGAR is refused for Compat::C
-    // launches; a pair asking for it there runs without a heuristic)
-    if (p.heuristic == WFA_HEURISTIC_NONE ||
-        (MED && hc.kind == WFA_HEURISTIC_ZDROP && p.compat != WFA_COMPAT_CORRECTED) ||
-        hc.kind < WFA_HEURISTIC_NONE || hc.kind > WFA_HEURISTIC_BANDED_ADAPTIVE)
+    if (p.heuristic == WFA_HEURISTIC_NONE || hc.kind < WFA_HEURISTIC_NONE ||
+        hc.kind > WFA_HEURISTIC_BANDED_ADAPTIVE)
       hc.kind = WFA_HEURISTIC_NONE;
   }
   const int pb_free = ef && !ext ? fe.x : 0;
@@ -1008,6 +1082,44 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
   const int k_end = tlen - plen;
   int end_k = k_end, end_off = tlen;
   const int hist_lo = -(plen + p.scope + 2), hist_hi = tlen + p.scope + 2;
+  // Z-drop end of a medium/low-memory CIGAR in Compat::C: the backtrace
+  // starts at the best cell's diagonal of the stop score's M wavefront
+  // (backtrace_zdrop_end, src/aligner.rs:1393-1406), empty when that
+  // wavefront's allocation does not hold it. Kept for it: the allocated
+  // diagonals [zalo, zahi] (output_alloc_range, src/aligner.rs:3318-3342:
+  // every computed range widened by scope + 1) and each score's trimmed
+  // ranges (ztr).
+  const bool zc = MED && HEUR && hc.kind == WFA_HEURISTIC_ZDROP &&
+                  p.compat == WFA_COMPAT_C && p.ztr != nullptr;
+  int4* const ztr = zc ? p.ztr + (int64_t)blockIdx.x * p.hdr_cap * 3 : nullptr;
+  const int zpad = p.scope + 1;
+  int zalo = imin(0, lo0 - zpad), zahi = imax(0, hi0 + zpad);
+  // occ: the components' occupancies before offloading (5 bits each, M,
+  // I1, D1, I2, D2; the offload rules of src/compute/piggyback.rs:576-633)
+  auto ztr_put = [&](int s_, int ml, int mh, int il, int ih, int dl, int dh, int i2l, int i2h,
+                     int d2l, int d2h, int occ) {
+    if (zc && tid == 0 && s_ < p.hdr_cap) {
+      ztr[3 * s_] = make_int4(ml, mh, il, ih);
+      ztr[3 * s_ + 1] = make_int4(dl, dh, i2l, i2h);
+      ztr[3 * s_ + 2] = make_int4(d2l, d2h, occ, 0);
+    }
+  };
+  ztr_put(0, lo0, hi0, 1, 0, 1, 0, 1, 0, 1, 0, 0);
+  // the occupancy a component of score s_ passes on (after its offload),
+  // 0 for a missing or null wavefront (occupancy helpers,
+  // src/compute/piggyback.rs:591-620)
+  auto zlive = [&](int s_, int c) -> bool {
+    if (s_ < 0) return false;
+    const int4 a = ztr[3 * s_ + (c == 0 || c == 1 ? 0 : c == 2 || c == 3 ? 1 : 2)];
+    const bool lohi = c == 0 || c == 2 || c == 4;
+    return (lohi ? a.x : a.z) <= (lohi ? a.y : a.w);
+  };
+  auto zocc = [&](int s_, int c) -> int {
+    if (!zlive(s_, c)) return 0;
+    const int pre = (ztr[3 * s_ + 2].z >> (5 * c)) & 31;
+    if (KIND == KIND_LINEAR) return pre >= 16 ? (p.low ? 15 : 8) : pre;
+    return pre >= 15 ? (p.low ? 14 : 8) : pre;
+  };
   // CIGAR history rows go to the slot's pool chunks in order: chunk `cur`
   // (id `cid`) is filled up to `cpos`; a row that does not fit moves to the
   // slot's next chunk, taking a new one from the pool when the slot owns no
@@ -1061,7 +1173,11 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
                       pb_free <= plen && tb_free <= tlen;
   const int dlen = hist_hi - hist_lo + 1;
   int32_t* const dom = dom_on ? p.dom + (int64_t)blockIdx.x * p.dom_stride : nullptr;
-  if (dom_on) {
+  // the record is sized from the batch's stated longest sequence: a longer
+  // pair has no room in it and goes back for retry (the wider passes plan
+  // the same record, so it ends with WFA_STATUS_OOM)
+  if (dom_on && NCOMP * dlen > p.dom_stride) overflow = true;
+  if (dom_on && !overflow) {
     for (int i = tid; i < NCOMP * dlen; i += NT) dom[i] = -1;
     __syncthreads();
   }
@@ -1466,6 +1582,9 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         hi = imin(hi, hc.p1);
       }
     }
+    // affine2p with no second-piece input runs the gap-affine kernel
+    // (src/aligner.rs:2261-2264)
+    const bool use2 = two && (ml_o2 <= mh_o2 || il2 <= ih2 || dl2 <= dh2);
     const int rowM = sM;
     const int rowE = sE1;
     const int rowE2 = two ? sE2 : 0;
@@ -1497,6 +1616,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         }
         if (CIG) H.hdr[s] = make_int4(1, 0, 0, 0);
       }
+      ztr_put(s, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 0);
       __syncthreads();
       if (s >= p.max_steps) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
       // Corrected: with a candidate, null steps do not end the search
@@ -1529,6 +1649,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         }
         if (CIG) H.hdr[s] = make_int4(1, 0, 0, 0);
       }
+      ztr_put(s, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 0);
       __syncthreads();
       if (s >= p.max_steps) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
       if (corrected && cand_valid && cand_total <= s) { cand_end(); break; }
@@ -1576,6 +1697,18 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     };
     int wlo, whi;
     widened(wlo, whi);
+    if (zc) {
+      // the wavefronts of this score are allocated over the computed range,
+      // widened to both begin-free diagonals of a begin-free score
+      // (output_alloc_range_at, endsfree_null_m, src/aligner.rs:3318-3465)
+      int al = wlo, ah = whi;
+      if (seeds >= 0 && !(seeds & SEED_ONLY)) {
+        al = imin(lo, -seed_k);
+        ah = imax(hi, seed_k);
+      }
+      zalo = imin(zalo, al - zpad);
+      zahi = imax(zahi, ah + zpad);
+    }
     if (whi - wlo + 1 > R.wcap) { overflow = true; break; }
     int4 hrow = make_int4(wlo, whi, 0, 0);
     if (MED && !halloc(whi - wlo + 1, &hrow, wlo, whi)) { overflow = true; break; }
@@ -1650,6 +1783,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       if (k <= hi) {
         int m;
         int fl = 0;
+        int gsg = 0;  // signs of the gap components' offsets
         if (KIND == KIND_EDIT) {
           const int ins = in.a;
           const int del = in.b;
@@ -1687,6 +1821,9 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
             const int do2 = in.h, de2 = in.i;
             i2 = imax(io2, ie2) + 1;
             d2 = imax(do2, de2);
+            // with no second-piece input the reference runs the gap-affine
+            // kernel (src/aligner.rs:2261-2264): I2/D2 never win
+            if (zc && !use2) i2 = d2 = INT32_MIN;
             if (MED) fl |= (ie2 >= io2 ? FL_I2_EXT : 0) | (de2 >= do2 ? FL_D2_EXT : 0);
             R.set((rI2 + rowE2) * R.wcap + (k & R.mask), i2);
             R.set((rD2 + rowE2) * R.wcap + (k & R.mask), d2);
@@ -1711,10 +1848,15 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           R.set((rD1 + rowE) * R.wcap + (k & R.mask), d1);
           vi = in_bounds(i1, k, plen, tlen);
           vd = in_bounds(d1, k, plen, tlen);
+          gsg = (i1 >= 0) << 1 | (d1 >= 0) << 2 | (i2 >= 0) << 3 | (d2 >= 0) << 4;
         }
         vm = m != WFA_NULL;
         R.set(rowM * R.wcap + (k & R.mask), m);
         if (MED) H.bytes[fbase + (k - lo)] = (uint8_t)fl;
+        if (MED && zc) {
+          // the offsets' signs (offload_selective moves only offsets >= 0)
+          H.signs[fbase + (k - lo)] = (uint8_t)((m >= 0) | gsg);
+        }
       }
       uint64_t b = ballot(vm);
       if (b) {
@@ -1809,7 +1951,8 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
             R.set((rI2 + rowE2) * R.wcap + (k & R.mask), WFA_NULL);
             R.set((rD2 + rowE2) * R.wcap + (k & R.mask), WFA_NULL);
           }
-          if (MED) H.bytes[fbase + (k - lo)] = 0;
+          if (MED) H.bytes[fbase + (k - lo)] = FL_M_NULLED;
+          if (MED && zc) H.signs[fbase + (k - lo)] = 0;
         }
         const bool seed = (tseed && k == seed_k && m <= seed_k) || (pseed && k == -seed_k && m <= 0);
         if (seed) {
@@ -1884,6 +2027,24 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       __syncthreads();
     }
     if (MED && tid == 0) H.hdr[s] = hrow;
+    int zo = 0;
+    if (zc && !(seeds >= 0 && (seeds & SEED_ONLY))) {
+      // this step's occupancies (a begin-free-only M is a fresh wavefront)
+      if (KIND == KIND_LINEAR) {
+        zo = imax(zocc(s - p.x, 0), zocc(s - p.o1, 0)) + 1;
+      } else if (GAPS) {
+        // occ_gap: one more than the fuller of a gap's open and extend inputs
+        auto gap = [&](int so, int se, int c) {
+          return imax(zlive(so, 0) ? zocc(so, 0) + 1 : 0, zlive(se, c) ? zocc(se, c) + 1 : 0);
+        };
+        const int oi1 = gap(s - p.oe1, s - p.e1, 1), od1 = gap(s - p.oe1, s - p.e1, 2);
+        const int oi2 = two && use2 ? gap(s - p.oe2, s - p.e2, 3) : 0;
+        const int od2 = two && use2 ? gap(s - p.oe2, s - p.e2, 4) : 0;
+        const int om = imax(imax(zocc(s - p.x, 0), oi1), imax(od1, imax(oi2, od2))) + 1;
+        zo = om | oi1 << 5 | od1 << 10 | oi2 << 15 | od2 << 20;
+      }
+    }
+    ztr_put(s, m_lo, m_hi, i_lo, i_hi, d_lo, d_hi, i2_lo, i2_hi, d2_lo, d2_hi, zo);
     if (tid == 0) {
       R.meta[rowM] = make_int4(m_lo, m_hi, m_lo, m_hi);
       if (GAPS) {
@@ -1948,7 +2109,14 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       __threadfence_block();
       uint8_t* out = p.cig + p.cig_off[pair];
       const int cap = plen + tlen;
-      if (MED)
+      if (MED && zc && zdropped) {
+        // Compat::C: the best cell's diagonal at the stop score, as it is
+        // there (src/aligner.rs:1443-1460; HPCBio/wfa2lib-rs#22)
+        ncig = pos_k < zalo || pos_k > zahi
+                   ? 0
+                   : piggyback_backtrace<KIND, BITS>(p, H, score + 1, sP, sT, plen, tlen, score,
+                                                     pos_k, pos_off, out, cap, ztr);
+      } else if (MED)
         ncig = piggyback_backtrace<KIND, BITS>(p, H, score + 1, sP, sT, plen, tlen, pos_s, pos_k,
                                                pos_off, out, cap);
       else if (!GAPS)
@@ -2028,6 +2196,7 @@ __global__ void __launch_bounds__(64 * NW, LDS_RING ? 5 : 4) wfa_align_kernel(KP
   H.bytes = MODE == 2 ? (uint8_t*)p.hist : nullptr;
   H.chain = MODE == 2 ? p.chain + (int64_t)blockIdx.x * p.chain_cap : nullptr;
   H.chain_cap = p.chain_cap;
+  H.signs = MODE == 2 ? p.zsign : nullptr;
   H.hdr_cap = p.hdr_cap;
   H.tab = CIG ? p.chunk_tab + (int64_t)blockIdx.x * p.tab_cap : nullptr;
   H.tab_cap = p.tab_cap;

@@ -48,6 +48,13 @@ struct LaunchCfg {
 };
 
 
+// Launches whose pairs may end by Z-drop with a medium/low-memory CIGAR in
+// Compat::C (a heuristic launch: the heuristic may be per pair) keep every
+// score's trimmed ranges for that backtrace.
+bool zdrop_ranges(const KParams& p, int mode) {
+  return mode == 2 && p.heuristic != WFA_HEURISTIC_NONE && p.compat == WFA_COMPAT_C;
+}
+
 int ring_rows_of(const KParams& p, int kind) {
   if (kind == wfa::KIND_EDIT || kind == wfa::KIND_LINEAR) return p.ringM;
   if (kind == wfa::KIND_AFFINE) return p.ringM + 2 * p.ringE1;
@@ -83,7 +90,9 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   const size_t seq_meta = (size_t)2 * c->seq_words * 8 + (size_t)c->ring_rows * 16;
   // 16-bit offsets for pairs of at most 8000 bases (the kernel re-runs any
   // pair whose score passes 16000 with 32-bit offsets)
-  c->i16 = max_len <= 8000 && pass < 2;
+  // (a Compat::C Z-drop end of a medium/low CIGAR follows the winners of
+  // cells holding values below zero, whose order 16 bits do not keep)
+  c->i16 = max_len <= 8000 && pass < 2 && !zdrop_ranges(p, mode);
   const size_t ring_bytes = (size_t)c->ring_rows * c->wcap * (c->i16 ? 2 : 4);
   c->lds_ring = seq_meta + ring_bytes <= 64 * 1024;
   // rings in HBM are wide: several waves share each pair's rows (heuristic
@@ -94,7 +103,7 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   // and writes whole rows): 16-bit rows for pairs of at most 16000 bases
   // halve it (offsets stay below 2^15; a NULL row value rises by at most
   // one per score, and pairs whose score passes 16000 re-run with 32 bits)
-  if (!c->lds_ring) c->i16 = c->nw > 1 && max_len <= 16000 && pass < 2;
+  if (!c->lds_ring) c->i16 = c->nw > 1 && max_len <= 16000 && pass < 2 && !zdrop_ranges(p, mode);
   const size_t red_bytes = c->lds_ring ? 0 : (size_t)64 * c->nw;
   c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : red_bytes);
   if (c->lds_bytes > e->lds_max) return WFA_ERR_UNSUPPORTED;
@@ -174,6 +183,9 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
     c->tab_cap = (int)std::min<int64_t>((int64_t)c->hdr_cap * c->wcap / c->chunk + 2, 1 << 20);
     per_slot += c->hist_cap * (mode == 2 ? 1 : 4) + (int64_t)c->hdr_cap * 16 +
                 (int64_t)c->tab_cap * 4 + (mode == 2 ? 4 * (int64_t)max_len + 64 : 0);
+    // Compat::C Z-drop ends of medium/low CIGARs: each score's trimmed
+    // ranges, and a sign byte beside every flag byte of the pool
+    if (zdrop_ranges(p, mode)) per_slot += (int64_t)c->hdr_cap * 48 + c->hist_cap;
   } else {
     c->hdr_cap = 0;
     c->hist_cap = 0;
@@ -187,7 +199,8 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   if (cig) {
     int64_t elems = (int64_t)c->grid * c->hist_cap;
     // what the budget leaves after the rings, headers and chunk tables
-    const int64_t cell = mode == 2 ? 1 : 4;
+    // bytes per pool element (and its sign byte when kept)
+    const int64_t cell = mode == 2 ? (zdrop_ranges(p, mode) ? 2 : 1) : 4;
     const int64_t rest = budget - (int64_t)c->grid * (per_slot - c->hist_cap * cell);
     // long reads, or the pairs that outgrew the first pass: the pool takes
     // all of it; never more than it (at least one chunk)
@@ -221,7 +234,7 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
     // with them by the time this one runs)
     e->arena_top = e->arena_launch;
     e->gring_bytes = e->hist_bytes = e->hdr_bytes = e->tab_bytes = e->chain_bytes =
-        e->dom_bytes = 0;
+        e->dom_bytes = e->ztr_bytes = e->zsign_bytes = 0;
   }
   if (!c.lds_ring) {
     int r = egrow(e, (void**)&e->d_gring, &e->gring_bytes,
@@ -268,6 +281,16 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
       if (r) return r;
       p.chain = e->d_chain;
     }
+  }
+  p.ztr = nullptr;
+  p.zsign = nullptr;
+  if (c.cig && zdrop_ranges(p, c.mode)) {
+    int r = egrow(e, (void**)&e->d_ztr, &e->ztr_bytes, (size_t)c.grid * c.hdr_cap * 48);
+    if (r) return r;
+    r = egrow(e, (void**)&e->d_zsign, &e->zsign_bytes, (size_t)c.pool_chunks * c.chunk);
+    if (r) return r;
+    p.ztr = e->d_ztr;
+    p.zsign = e->d_zsign;
   }
   p.dom_stride = c.dom_stride;
   if (c.dom_stride > 0) {
@@ -484,6 +507,8 @@ void wfa_engine_free(wfa_engine_t* e) {
   hipFree(e->d_chain);
   hipFree(e->d_tab);
   hipFree(e->d_dom);
+  hipFree(e->d_ztr);
+  hipFree(e->d_zsign);
   hipFree(e->d_order);
   for (auto b : e->bi_buf) hipFree(b);
   hipHostFree(e->h_pinned);

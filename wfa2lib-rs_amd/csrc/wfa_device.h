@@ -73,6 +73,13 @@ struct KParams {
   int hdr_cap;
   uint8_t* chain;            // medium/low memory op-chain scratch, chain_cap per slot
   int chain_cap;
+  // Z-drop ends of medium/low-memory CIGARs in Compat::C: per slot and
+  // score, the trimmed ranges {M, I1}, {D1, I2}, {D2, occupancies} (3 x
+  // hdr_cap int4), and per cell of the history pool the signs of its five
+  // components' offsets (zsign); else null
+  int4* ztr;
+  uint8_t* zsign;
+  int low;                   // low memory (else medium) in CIGAR mode 2
   // CIGAR history pool (`hist`, pool_chunks chunks of `chunk` elements):
   // every slot takes chunks as its pairs need them and keeps them for its
   // next pairs; chunk_tab holds each slot's chunk ids (tab_cap per slot)

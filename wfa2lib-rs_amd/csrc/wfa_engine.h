@@ -90,15 +90,6 @@ inline int check_supported(const wfa_attr_t& a) {
   if (a.memory_mode < WFA_MEMORY_HIGH || a.memory_mode > WFA_MEMORY_ULTRALOW)
     return WFA_ERR_INVALID_ARGUMENT;
   if (a.compat != WFA_COMPAT_C && a.compat != WFA_COMPAT_CORRECTED) return WFA_ERR_INVALID_ARGUMENT;
-  // Z-drop with a medium/low-memory CIGAR in C mode: the reference
-  // backtraces the best cell at the score the alignment stopped at, from
-  // whatever backtrace chain that wavefront holds there
-  // (src/aligner.rs:1393-1406, 1443-1460; HPCBio/wfa2lib-rs#22): not on
-  // the device path (Compat::Corrected backtraces the cell's own chain)
-  if (a.heuristic == WFA_HEURISTIC_ZDROP && a.compat == WFA_COMPAT_C &&
-      a.alignment_scope == WFA_SCOPE_ALIGNMENT &&
-      (a.memory_mode == WFA_MEMORY_MED || a.memory_mode == WFA_MEMORY_LOW))
-    return WFA_ERR_UNSUPPORTED;
   // BiWFA: end-to-end only, and no X-drop / Z-drop, as the reference's
   // driver checks (src/bin/align_benchmark.rs:529-560)
   if (a.memory_mode == WFA_MEMORY_ULTRALOW &&
@@ -146,6 +137,10 @@ struct wfa_engine_t {
   size_t tab_bytes = 0;
   int32_t* d_dom = nullptr;  // corrected ends-free dominance records per slot
   size_t dom_bytes = 0;
+  int4* d_ztr = nullptr;     // Compat::C Z-drop trimmed ranges per slot and score
+  size_t ztr_bytes = 0;
+  uint8_t* d_zsign = nullptr;  // Compat::C Z-drop offset signs per history cell
+  size_t zsign_bytes = 0;
   void* d_order = nullptr;   // longest-first work order: keys, sorted keys, values, order, temp
   size_t order_bytes = 0;
   int last_retries = 0;
@@ -262,6 +257,7 @@ inline void base_params(const Internal& ip, const wfa_attr_t& attr, const wfa_ba
   p->cig_len = b.cigar_length;
   p->end_pos = b.end_position;
   p->compat = attr.compat;
+  p->low = attr.memory_mode == WFA_MEMORY_LOW;
 }
 
 // BiWFA batches (memory mode ULTRALOW): wfa_biwfa.cpp
