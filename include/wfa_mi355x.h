@@ -177,6 +177,15 @@ int32_t wfa_aligner_set_alignment_extension(wfa_aligner_t* aligner);
 int32_t wfa_aligner_set_alignment_end_to_end(wfa_aligner_t* aligner);
 /* set_max_alignment_steps (src/aligner.rs:533) */
 int32_t wfa_aligner_set_max_alignment_steps(wfa_aligner_t* aligner, int32_t max_steps);
+/* set_max_memory_compact (src/aligner.rs:469): the backtrace-buffer size
+ * above which the reference's medium/low modes compact that buffer, a step
+ * that only frees host memory and changes no result. The device keeps each
+ * pair's CIGAR history in its own planned pool, so the value is stored
+ * (wfa_aligner_max_memory_compact reads it back) and changes nothing. */
+int32_t wfa_aligner_set_max_memory_compact(wfa_aligner_t* aligner, uint64_t bytes);
+/* The stored value: 256 MiB (the reference's MAX_MEMORY_COMPACT,
+ * src/aligner.rs:356) until set. 0 for a null aligner. */
+uint64_t wfa_aligner_max_memory_compact(const wfa_aligner_t* aligner);
 
 /* align_end2end(&[u8], &[u8]) -> i32 (src/aligner.rs:620). Returns the WFA
  * score; INT32_MIN if the call itself failed (see wfa_aligner_error). */

@@ -123,3 +123,12 @@ def test_serving_mix_shape(pkg):
     t = [r.arrival_s for r in a]
     assert all(x < y for x, y in zip(t, t[1:]))
     assert 0.25 < t[-1] < 0.6  # 400 arrivals at 1000/s
+
+
+def test_max_memory_compact_null_aligner(N):
+    """set_max_memory_compact (src/aligner.rs:469) on the C ABI: a null
+    aligner is refused and reads back 0 (the aligner itself needs a GPU;
+    tests/test_gpu_parity.py covers the stored value and unchanged results)."""
+    L = N.lib()
+    assert L.wfa_aligner_set_max_memory_compact(None, 1 << 20) == N.ERR_INVALID_ARGUMENT
+    assert L.wfa_aligner_max_memory_compact(None) == 0

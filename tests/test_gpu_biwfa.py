@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from conftest import attr_for, have_gpu
-from test_gpu_parity import _compare
+from test_gpu_parity import _compare, _run_gpu
 from test_oracle_biwfa import BIWFA_PENALTIES, _recursive_pairs
 
 pytestmark = pytest.mark.gpu
@@ -82,3 +82,24 @@ def test_biwfa_per_pair_equals_batch(pkg, N, engine, oracle):
     pairs = _pairs(6, seed=21)
     attr = _attr(N, "gap_affine_2p", (0, 4, 6, 2, 24, 1), "alignment")
     _compare(engine, oracle, attr, pairs, True, per_pair=True, label="biwfa/per_pair")
+
+
+def test_biwfa_ultralong_matches_fixture(pkg, N, engine):
+    """The 1k x 50-100 kbp workload's first pairs (2-bit packed, as the
+    benchmark feeds them) at full size on the device: edit distance on 4
+    pairs, gap-affine (0,4,6,2) on 2, full CIGAR; score, status and CIGAR
+    digest equal the oracle's (tests/golden/ultralong50k_100k_biwfa.json)."""
+    import hashlib
+    from test_oracle_biwfa import _ultralong_fixture
+    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    d = _ultralong_fixture()
+    n = max(r["pair"] for r in d["rows"]) + 1
+    pairs = generate(WORKLOADS[d["workload"]], num_pairs=n)
+    for metric in ("edit", "gap_affine"):
+        rows = [r for r in d["rows"] if r["metric"] == metric]
+        sub = pairs.subset([r["pair"] for r in rows])
+        s, st, c = _run_gpu(engine, _attr(N, metric, rows[0]["penalties"], "alignment"), sub,
+                            True)
+        for j, r in enumerate(rows):
+            got = (int(s[j]), int(st[j]), len(c[j]), hashlib.sha256(bytes(c[j])).hexdigest())
+            assert got == (r["score"], r["status"], r["cigar_len"], r["sha256"]), (metric, j)

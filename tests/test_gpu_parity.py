@@ -343,6 +343,29 @@ def test_zdrop_issue22_c_med_low_on_gpu(pkg, N, engine, oracle):
         _compare(engine, oracle, a, from_ascii([p], [t]), True, label=f"issue22 {mem}")
 
 
+def test_max_memory_compact_changes_nothing(pkg):
+    """set_max_memory_compact (src/aligner.rs:466-471): compaction only
+    frees memory, so a 1-byte threshold (compact at every probe) gives the
+    default's results in medium and low memory; the value reads back."""
+    from test_oracle import _issue22_pair
+    p, t = _issue22_pair()
+    pen = pkg.AffinePenalties(0, 4, 6, 2)
+    out = {}
+    for mem in ("MED", "LOW"):
+        for compact in (None, 1):
+            A = pkg.AffineAligner(pkg.WavefrontPenalties.new_affine(pen))
+            A.set_alignment_scope(pkg.AlignmentScope.COMPUTE_ALIGNMENT)
+            A.set_memory_mode(getattr(pkg.MemoryMode, mem))
+            assert A.max_memory_compact == 256 << 20
+            if compact is not None:
+                A.set_max_memory_compact(compact)
+                assert A.max_memory_compact == compact
+            s = A.align_end2end(p, t)
+            out[mem, compact] = (s, A.status(), bytes(A.cigar().operations))
+        assert out[mem, None] == out[mem, 1]
+        assert out[mem, None][1] == pkg.AlignStatus.COMPLETED
+
+
 def test_zdrop_issue22_on_gpu(pkg, N, engine):
     """tests/corrected_zdrop.rs:77-89 through the aligner API: Z-drop 50,10
     with (-3,5,3,2), high memory: a valid partial alignment scoring 4498;

@@ -165,3 +165,76 @@ def test_serving_poisson_stream(pkg, N, server, oracle):
     _check(results, _oracle(oracle, [(r.pattern, r.text) for r in mix], attrs), "poisson")
     lat = np.array([x.latency_us for x in got])
     assert np.isfinite(lat).all() and (lat > 0).all()
+
+
+def _endsfree_penalty(ops, ends, x=4, o=6, e=2):
+    """A gap-affine (0,x,o,e) CIGAR's ends-free penalty: a leading or
+    trailing gap run is free up to its end's limit (tests/corrected_endsfree.rs:
+    109-146); ends = (pattern begin, pattern end, text begin, text end)."""
+    runs = []
+    for op in ops:
+        if runs and runs[-1][0] == op:
+            runs[-1][1] += 1
+        else:
+            runs.append([op, 1])
+    pb, pe, tb, te = ends
+    total = 0
+    for i, (op, n) in enumerate(runs):
+        if op == ord("X"):
+            total += x * n
+        elif op != ord("M"):
+            ch = n
+            if i == 0:
+                ch -= min(ch, pb if op == ord("D") else tb)
+            if i == len(runs) - 1:
+                ch -= min(ch, pe if op == ord("D") else te)
+            if ch > 0:
+                total += o + e * ch
+    return total
+
+
+def test_serving_full_mix(pkg, N, server, oracle):
+    """The full serving mix (70% 150 bp, 25% 1-10 kbp, 5% 10-100 kbp; half
+    end-to-end, a quarter ends-free, a quarter extension; heuristics none /
+    WF-adaptive / X-drop / banded-adaptive), half submitted while the other
+    half runs. Requests up to 10 kbp equal the oracle; the 10-100 kbp ones
+    are checked at full size by properties: a valid CIGAR (spanning both
+    sequences unless extension or cut short), its penalty equal to the score
+    for completed end-to-end and ends-free requests, and the same result
+    when the request is aligned alone."""
+    from test_oracle import valid_prefix
+    from wfa2lib_rs_amd.engine import BatchEngine
+    from wfa2lib_rs_amd.server import request_attr
+    from wfa2lib_rs_amd.synth import SERVING_LENGTHS, serving_mix
+    mix = serving_mix(120, 1e9, seed=21, lengths=SERVING_LENGTHS)
+    reqs = [(r.pattern, r.text) for r in mix]
+    attrs = [request_attr(r.span, r.heuristic, len(r.pattern), len(r.text)) for r in mix]
+    long_idx = [i for i, r in enumerate(mix) if max(len(r.pattern), len(r.text)) > 10_000]
+    assert len(long_idx) >= 4, "the mix must hold several 10-100 kbp requests"
+    half = len(reqs) // 2
+    for i in range(half):
+        server.submit(attrs[i], reqs[i][0], reqs[i][1], tag=i)
+    got = server.poll(timeout_us=0)
+    for i in range(half, len(reqs)):
+        server.submit(attrs[i], reqs[i][0], reqs[i][1], tag=i)
+    got += server.drain(len(reqs) - len(got), timeout_s=600)
+    assert sorted(r.tag for r in got) == list(range(len(reqs)))
+    assert all(r.error == 0 for r in got), [r for r in got if r.error][:3]
+    results = {r.tag: (r.score, r.status, r.cigar) for r in got}
+    short = [i for i in range(len(reqs)) if i not in long_idx]
+    want = _oracle(oracle, [reqs[i] for i in short], [attrs[i] for i in short])
+    _check({i: results[i] for i in short}, {i: want[j] for j, i in enumerate(short)},
+           "full mix vs oracle")
+    for i in long_idx:
+        p, t = reqs[i]
+        s, st, c = results[i]
+        assert st != N.STATUS_OOM, i
+        assert valid_prefix(c, p, t), i
+        a = attrs[i]
+        whole = pkg.Cigar(c).check_alignment(p, t) is None
+        if mix[i].span != "extension" and st == N.STATUS_COMPLETED:
+            assert whole, i
+            ends = (a.pattern_begin_free, a.pattern_end_free, a.text_begin_free,
+                    a.text_end_free)
+            assert _endsfree_penalty(c, ends) == s, i
+    _check(results, _alone(BatchEngine(0), reqs, attrs, long_idx), "long requests alone")

@@ -31,6 +31,7 @@ ERRORS = {
     -7: "rejected combination", -8: "beyond a fixed limit of the no-allocation entry point",
 }
 ERR_INVALID_PENALTIES = -1
+ERR_INVALID_ARGUMENT = -2
 ERR_UNSUPPORTED = -3
 ERR_LIMIT = -8
 # limits of wfa_align_batch_noalloc (include/wfa_mi355x.h)
@@ -107,6 +108,8 @@ SIGNATURES = {
     "wfa_aligner_set_alignment_extension": (_I32, [_VP]),
     "wfa_aligner_set_alignment_end_to_end": (_I32, [_VP]),
     "wfa_aligner_set_max_alignment_steps": (_I32, [_VP, _I32]),
+    "wfa_aligner_set_max_memory_compact": (_I32, [_VP, ctypes.c_uint64]),
+    "wfa_aligner_max_memory_compact": (ctypes.c_uint64, [_VP]),
     "wfa_align_end2end": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
     "wfa_align_packed2bits": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
     "wfa_align_endsfree": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
@@ -146,6 +149,15 @@ def lib():
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                 "(the engine has no CPU fallback)")
+        # PyTorch-ROCm ships its own HIP runtime beside the system one this
+        # library links; in one process torch's must start first, or torch
+        # finds no device once this library has initialised the GPU. Device
+        # batches are torch tensors, so start it here when it is installed.
+        try:
+            import torch
+            torch.cuda.is_available()
+        except ImportError:
+            pass
         l = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(l, name)

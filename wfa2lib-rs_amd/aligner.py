@@ -332,6 +332,16 @@ class WavefrontAligner:
     def set_max_alignment_steps(self, max_steps: int):
         N.check(N.lib().wfa_aligner_set_max_alignment_steps(self._h, max_steps))
 
+    def set_max_memory_compact(self, nbytes: int):
+        """src/aligner.rs:469: the reference compacts its medium/low-memory
+        backtrace buffer above this size, which changes no result. Stored
+        only: the device plans each pair's CIGAR history pool itself."""
+        N.check(N.lib().wfa_aligner_set_max_memory_compact(self._h, nbytes))
+
+    @property
+    def max_memory_compact(self) -> int:
+        return int(N.lib().wfa_aligner_max_memory_compact(self._h))
+
     # ---- align (src/aligner.rs:620, 2755) ----
     def _finish(self, score):
         L = N.lib()

@@ -603,6 +603,8 @@ struct wfa_aligner_t {
   std::vector<uint8_t> cigar;
   int32_t cigar_score = INT32_MIN;
   int32_t end_pos[3] = {-1, INT32_MAX, INT32_MIN / 2};
+  // set_max_memory_compact's value (stored only: no buffer here compacts)
+  uint64_t max_memory_compact = uint64_t(256) << 20;
   // device staging for one pair: [pattern | text | cigar], plus arrays
   uint8_t* d_buf = nullptr;
   size_t d_cap = 0;
@@ -848,6 +850,16 @@ int32_t wfa_aligner_set_max_alignment_steps(wfa_aligner_t* a, int32_t max_steps)
   if (!a) return WFA_ERR_INVALID_ARGUMENT;
   a->attr.max_alignment_steps = max_steps;
   return WFA_OK;
+}
+
+int32_t wfa_aligner_set_max_memory_compact(wfa_aligner_t* a, uint64_t bytes) {
+  if (!a) return WFA_ERR_INVALID_ARGUMENT;
+  a->max_memory_compact = bytes;
+  return WFA_OK;
+}
+
+uint64_t wfa_aligner_max_memory_compact(const wfa_aligner_t* a) {
+  return a ? a->max_memory_compact : 0;
 }
 
 int32_t wfa_align_end2end(wfa_aligner_t* a, const uint8_t* P, int32_t plen, const uint8_t* T,
