@@ -32,6 +32,8 @@ struct LaunchCfg {
   bool lds_ring;
   bool i16;
   bool shortk;  // the short-read kernel (wfa_short.h)
+  int short_rc;  // its ring class
+  int short_gsh; // its score step, 2^short_gsh
   int nw;       // waves per pair
   int wcap;
   int seq_words;
@@ -111,18 +113,22 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   // score only or with high-memory CIGARs: the short-read kernel (a fixed
   // 256-diagonal window per pair, 16-bit rows, a per-slot history at a fixed
   // stride); pairs that leave its window come back for the general kernel.
-  // Its ring-row mask is 64 bits wide.
+  // Its rings hold a fixed number of rows (the ring class), enough for the
+  // lookbacks of most penalty sets once scores step by their common power
+  // of two.
+  c->short_rc = wfa::short_ring_class(p.x, p.oe1, p.e1, &c->short_gsh);
   c->shortk = pass == 0 && kind == wfa::KIND_AFFINE && (mode == 0 || mode == 1) &&
               p.heuristic == WFA_HEURISTIC_NONE && !p.endsfree && max_len <= 8000 &&
-              c->ring_rows <= 64;
+              c->short_rc >= 0;
   int per_cu;
   if (c->shortk) {
     c->seq_words = ((max_len + 31) / 32 + 2 + 1) & ~1;
-    c->lds_bytes = (size_t)2 * c->seq_words * 8 + wfa::short_lds_row_bytes_host(c->ring_rows);
+    c->lds_bytes = (size_t)2 * c->seq_words * 8 + wfa::short_lds_row_bytes_host(c->short_rc);
     if (c->lds_bytes > e->lds_max) c->shortk = false;
   }
   if (c->shortk) {
-    per_cu = wfa::short_blocks_per_cu(mode, fmt == WFA_SEQ_PACKED2BITS, c->lds_bytes);
+    per_cu = wfa::short_blocks_per_cu(mode, fmt == WFA_SEQ_PACKED2BITS, c->short_rc,
+                                      c->lds_bytes);
     int grid = std::max(1, std::min(e->num_cus * per_cu, npairs));
     c->lds_ring = true;
     c->i16 = true;
@@ -133,8 +139,9 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
     c->hist_cap = 0;
     c->hdr_cap = 0;
     if (cig) {
-      // rows for scores below 1024 (a pair scoring more goes to the general
-      // kernel): 16-bit cells over the window, one header per score
+      // rows for the first 1024 scores of the kernel's score step (a pair
+      // scoring more goes to the general kernel): 16-bit cells over the
+      // window, one header per row
       c->hdr_cap = 1024;
       const int64_t per_slot = (int64_t)c->hdr_cap * (wfa::kShortWindow * 2 + 4);
       if ((int64_t)grid * per_slot > e->scratch_budget)
@@ -255,8 +262,9 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
       p.hdr = e->d_hdr;
       p.hdr_cap = c.hdr_cap;
     }
+    p.sh_gsh = c.short_gsh;
     HIPCHK(wfa::launch_reset(e->d_counters, st));
-    HIPCHK(wfa::launch_short(p, c.mode, c.grid, c.lds_bytes, st));
+    HIPCHK(wfa::launch_short(p, c.mode, c.short_rc, c.grid, c.lds_bytes, st));
     return WFA_OK;
   }
   if (c.cig) {

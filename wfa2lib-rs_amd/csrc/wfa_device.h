@@ -80,6 +80,7 @@ struct KParams {
   int4* ztr;
   uint8_t* zsign;
   int low;                   // low memory (else medium) in CIGAR mode 2
+  int sh_gsh;                // short-read kernel: scores in steps of 2^sh_gsh
   // CIGAR history pool (`hist`, pool_chunks chunks of `chunk` elements):
   // every slot takes chunks as its pairs need them and keeps them for its
   // next pairs; chunk_tab holds each slot's chunk ids (tab_cap per slot)
@@ -168,8 +169,11 @@ KernelFn pick_linear(int mode, bool lds, bool i16, bool packed, bool heur, int n
 // the short-read kernel (gap-affine, end-to-end, no heuristic; mode 0 or
 // 1): its history is p.hdr_cap rows of kShortWindow 16-bit cells and
 // p.hdr_cap int32 headers per slot
-KernelFn pick_affine_short(int mode, bool packed);
-size_t short_lds_row_bytes_host(int ring_rows);
+KernelFn pick_affine_short(int mode, bool packed, int rc);
+size_t short_lds_row_bytes_host(int rc);
+// the short kernel's ring class for penalties x, o + e, e (the score step
+// 2^gsh in *gsh), or -1 when their reduced lookbacks exceed every class
+int short_ring_class(int x, int oe, int e, int* gsh);
 constexpr int kShortWindow = 256;
 
 hipError_t launch_reset(int32_t* counters, hipStream_t stream);
@@ -216,7 +220,8 @@ hipError_t launch_align(const KParams& p, int kind, int mode, bool lds_ring, boo
                         int grid, size_t lds_bytes, hipStream_t stream);
 int max_blocks_per_cu(int kind, int mode, bool lds_ring, bool i16, bool packed, bool heur, int nw,
                       size_t lds_bytes);
-hipError_t launch_short(const KParams& p, int mode, int grid, size_t lds_bytes, hipStream_t stream);
-int short_blocks_per_cu(int mode, bool packed, size_t lds_bytes);
+hipError_t launch_short(const KParams& p, int mode, int rc, int grid, size_t lds_bytes,
+                        hipStream_t stream);
+int short_blocks_per_cu(int mode, bool packed, int rc, size_t lds_bytes);
 
 }  // namespace wfa

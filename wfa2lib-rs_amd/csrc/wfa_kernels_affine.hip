@@ -11,14 +11,30 @@ KernelFn pick_affine(int mode, bool lds, bool i16, bool packed, bool heur, int n
 }
 
 // The short-read kernel (gap-affine, end-to-end, no heuristic): score only
-// or high-memory CIGAR
-KernelFn pick_affine_short(int mode, bool packed) {
-  if (mode == 0) return packed ? wfa_short_kernel<0, true> : wfa_short_kernel<0, false>;
-  if (mode == 1) return packed ? wfa_short_kernel<1, true> : wfa_short_kernel<1, false>;
+// or high-memory CIGAR, for ring class `rc`
+KernelFn pick_affine_short(int mode, bool packed, int rc) {
+  if (rc == 0) {
+    if (mode == 0) return packed ? wfa_short_kernel<0, true, 0> : wfa_short_kernel<0, false, 0>;
+    if (mode == 1) return packed ? wfa_short_kernel<1, true, 0> : wfa_short_kernel<1, false, 0>;
+  } else if (rc == 1) {
+    if (mode == 0) return packed ? wfa_short_kernel<0, true, 1> : wfa_short_kernel<0, false, 1>;
+    if (mode == 1) return packed ? wfa_short_kernel<1, true, 1> : wfa_short_kernel<1, false, 1>;
+  }
   return nullptr;
 }
 
-size_t short_lds_row_bytes_host(int ring_rows) { return short_lds_row_bytes(ring_rows); }
+size_t short_lds_row_bytes_host(int rc) { return short_lds_row_bytes(rc); }
+
+int short_ring_class(int x, int oe, int e, int* gsh) {
+  // g: the largest power of two dividing all three penalties
+  int g = 0;
+  while (g < 8 && ((x | oe | e) & ((2 << g) - 1)) == 0) ++g;
+  *gsh = g;
+  const int xr = x >> g, oer = oe >> g, er = e >> g;
+  if (xr < ShRing<0>::M && oer < ShRing<0>::M && er < ShRing<0>::E) return 0;
+  if (xr < ShRing<1>::M && oer < ShRing<1>::M && er < ShRing<1>::E) return 1;
+  return -1;
+}
 
 KernelFn2 pick_biwfa_affine(int which, bool heur) {
   return pick_biwfa_kind<KIND_AFFINE>(which, heur);

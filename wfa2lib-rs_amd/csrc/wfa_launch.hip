@@ -183,16 +183,16 @@ hipError_t launch_biwfa_rescore(const KParams& p, const int32_t* pairs, int n,
   return hipGetLastError();
 }
 
-hipError_t launch_short(const KParams& p, int mode, int grid, size_t lds_bytes,
+hipError_t launch_short(const KParams& p, int mode, int rc, int grid, size_t lds_bytes,
                         hipStream_t stream) {
-  KernelFn fn = pick_affine_short(mode, p.fmt == WFA_SEQ_PACKED2BITS);
+  KernelFn fn = pick_affine_short(mode, p.fmt == WFA_SEQ_PACKED2BITS, rc);
   if (!fn) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fn, dim3(grid), dim3(64), lds_bytes, stream, p);
   return hipGetLastError();
 }
 
-int short_blocks_per_cu(int mode, bool packed, size_t lds_bytes) {
-  KernelFn fn = pick_affine_short(mode, packed);
+int short_blocks_per_cu(int mode, bool packed, int rc, size_t lds_bytes) {
+  KernelFn fn = pick_affine_short(mode, packed, rc);
   int n = 0;
   if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, lds_bytes) != hipSuccess)
     return 1;

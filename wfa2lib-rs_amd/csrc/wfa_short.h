@@ -8,30 +8,41 @@
 // bound by its per-score scalar bookkeeping on rows of a few dozen
 // diagonals):
 //
+// * Scores in steps of g, the largest power of two that divides the
+//   mismatch, gap-open + gap-extend and gap-extend penalties: every
+//   wavefront of a score that g does not divide is absent (each one derives
+//   from M0 by those penalties), so the reference's steps there are null
+//   steps. The loop runs over reduced scores t (score t * g) with the
+//   penalties divided by g, and accounts for the real scores it passes over
+//   exactly: each counts towards the run of null steps (src/aligner.rs:
+//   2031-2043) and meets the per-score limits in order.
 // * Each pair gets a fixed window of 256 diagonals [base, base + 256),
 //   centred on the diagonals its alignment must cross. Every ring row is an
-//   LDS row over the whole window plus one NULL pad cell on each side, all
-//   NULL when the pair starts. A step writes its three output rows over the
+//   LDS row over the whole window plus pad cells on each side, all NULL when
+//   the pair starts. A step writes its three output rows over the
 //   64-diagonal chunks it computes, OFFSET_NULL outside their trimmed
 //   ranges, so a row's cells outside its range read as NULL with no range
 //   test: the reference NULL-fills exactly those cells before it reads them
-//   (src/wavefront.rs:422-457).
+//   (src/wavefront.rs:422-457). An absent wavefront is a row of NULL cells
+//   as well: a null step writes its rows NULL, and the rows of scores below
+//   zero are the initial fill (a ring holds more rows than any lookback), so
+//   a step reads its inputs from their ring slots with no presence test.
 // * The chunks a step computes cover [lo - 1, hi + 1] of the union of every
 //   row computed so far. That holds the reference's computed range (the
 //   union of the inputs' ranges, widened by one, src/aligner.rs:2045-2062);
 //   a cell outside the latter has only NULL inputs, so it comes out below
 //   zero, outside the trimmed ranges, and is stored as NULL — as it would
 //   be read in the reference. The union only grows, so no cell outside it
-//   has been written since the pair started.
-// * Which ring rows hold a wavefront (not absent, not trimmed empty) is one
-//   64-bit scalar mask; an absent input row is read from an all-NULL row
-//   instead, and a step whose inputs are all absent is a null step
-//   (src/aligner.rs:2031-2043) that touches no vector registers at all.
-// * In CIGAR scope the extended M row of every score goes to the slot's
-//   history in HBM at a fixed stride (cell k of score s at s * 256 + k -
-//   base, 16-bit), with the computed chunks' first and last diagonal packed
-//   in one int32 header per score; the Singletrack backtrace reads it
-//   through WinView.
+//   has been written since the pair started, and every write of a slot
+//   covers what the slot's previous score wrote.
+// * Which ring rows hold a wavefront (not absent, not trimmed empty) is kept
+//   by age in two scalar masks, only to detect null steps.
+// * In CIGAR scope the extended M row of every reduced score goes to the
+//   slot's history in HBM at a fixed stride (cell k of reduced score t at
+//   t * 256 + k - base, 16-bit), with the computed chunks' first and last
+//   diagonal packed in one int32 header per reduced score; the Singletrack
+//   backtrace reads it through WinView, which finds no wavefront at the
+//   scores in between.
 //
 // Offsets are held in 16 bits as in the general kernel's short-read rings
 // (sequences of at most 8000 bases, scores up to 16000; NULL-derived values
@@ -51,21 +62,30 @@ constexpr int SH_ROW = SH_WIN + 8;       // LDS cells per row: pad, window, pad 
 constexpr int SH_NULL = -16384;          // OFFSET_NULL as held in 16 bits
 constexpr int SH_HDR_EMPTY = 0x00000001; // header of an absent M row: lo 1, hi 0
 
+// Ring classes: M rows and I1/D1 rows per ring (powers of two, so a slot is
+// a mask of the reduced score), for reduced lookbacks x, o + e < M and
+// e < E.
+template <int RC> struct ShRing;
+template <> struct ShRing<0> { static constexpr int M = 8, E = 2; };
+template <> struct ShRing<1> { static constexpr int M = 16, E = 8; };
+
 static __device__ __forceinline__ int16_t sh16(int v) { return (int16_t)imax(v, SH_NULL); }
 
 // One slot's high-memory history as the Singletrack backtrace reads it
 // (st_m_offset, src/backtrace.rs:645-663): M[s] at diagonal k, if score s
-// has a row covering k and the cell is not NULL.
+// has a row covering k and the cell is not NULL. Rows exist for multiples
+// of 2^gsh only.
 struct WinView {
-  const int16_t* cells;  // row s at cells + s * SH_WIN, diagonal k at k - base
-  const int32_t* hdr;    // per score: first diagonal (low 16 bits), last (high 16 bits)
-  int base, nscores, nwf;
+  const int16_t* cells;  // row t (score t << gsh) at cells + t * SH_WIN, diagonal k at k - base
+  const int32_t* hdr;    // per row: first diagonal (low 16 bits), last (high 16 bits)
+  int base, nscores, nwf, gsh;
   __device__ __forceinline__ bool m(int s, int k, int* out) const {
-    if (s < 0 || s >= nscores || s >= nwf) return false;
-    const int h = hdr[s];
+    if (s < 0 || s >= nscores || s >= nwf || (s & ((1 << gsh) - 1))) return false;
+    const int t = s >> gsh;
+    const int h = hdr[t];
     const int lo = (int)(int16_t)(h & 0xffff), hi = h >> 16;
     if (k < lo || k > hi) return false;
-    const int v = cells[(int64_t)s * SH_WIN + (k - base)];
+    const int v = cells[(int64_t)t * SH_WIN + (k - base)];
     if (v < 0) return false;
     *out = v;
     return true;
@@ -93,29 +113,46 @@ static __device__ __forceinline__ uint32_t sh_w16(const uint32_t* s, uint32_t po
   return __builtin_amdgcn_alignbit(s[q + 1], s[q], (pos & 15u) * 2u);
 }
 
-// Matching prefix length of pattern[v..] and text[h..], bounded by the
-// sequence ends (extend_matches_packed_end2end's per-diagonal LCP,
-// src/extend.rs:15-38): 16 bases per XOR of funnel-shifted words.
-static __device__ __forceinline__ int sh_lcp(const uint32_t* P, const uint32_t* T, uint32_t v,
-                                             uint32_t h, int rem) {
-  int n = 0;
-  while (rem > 0) {
-    const uint32_t x = sh_w16(P, v + n) ^ sh_w16(T, h + n);
-    // matching bases before the first difference (16 when none)
-    const int c = x ? (int)(__builtin_ctz(x) >> 1) : 16;
-    if (c >= rem) { n += rem; break; }
-    n += c;
-    if (c < 16) break;
-    rem -= 16;
-  }
-  return n;
+// Matching bases before the first difference of two 16-base words (16 when
+// they are equal).
+static __device__ __forceinline__ int sh_match16(uint32_t x) {
+  return x ? (int)(__builtin_ctz(x) >> 1) : 16;
 }
 
-template <int MODE>
+// Extend every lane's M cell `off` along its diagonal k (lanes with a NULL
+// cell keep it): the matching prefix of pattern[off - k..] and text[off..]
+// bounded by the sequence ends (extend_matches_packed_end2end's per-diagonal
+// LCP, src/extend.rs:15-38). The first 16 bases are compared for every lane
+// at once; only lanes still matching after them loop, 16 bases at a time.
+static __device__ __forceinline__ int sh_extend(const uint32_t* P, const uint32_t* T, int off,
+                                                int k, int plen, int tlen) {
+  const bool act = off >= 0;
+  const uint32_t h = act ? (uint32_t)off : 0u, v = act ? (uint32_t)(off - k) : 0u;
+  const int rem = act ? imin(plen - (int)v, tlen - (int)h) : 0;
+  const int c = sh_match16(sh_w16(P, v) ^ sh_w16(T, h));
+  int n = imin(c, rem);
+  bool more = c == 16 && rem > 16;
+  if (ballot(more)) {
+    while (more) {
+      const int left = rem - n;
+      const int d = sh_match16(sh_w16(P, v + n) ^ sh_w16(T, h + n));
+      n += imin(d, left);
+      more = d == 16 && left > 16;
+    }
+  }
+  return off + n;
+}
+
+template <int MODE, int RC>
 static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen,
                                   const uint64_t* sP, const uint64_t* sT, int16_t* rows,
                                   int16_t* hcells, int32_t* hhdr) {
   constexpr bool CIG = MODE == 1;
+  constexpr int RM = ShRing<RC>::M, RE = ShRing<RC>::E;
+  // ring rows as LDS byte offsets from `rows`: reduced score t's M row at
+  // (t mod RM) * RB, its I1 / D1 rows at IB0 / DB0 + (t mod RE) * RB
+  constexpr int RB = SH_ROW * 2;
+  constexpr int IB0 = RM * RB, DB0 = (RM + RE) * RB;
   const int lane = threadIdx.x;
   const int k_end = uniform(tlen - plen);
   const int ak = k_end < 0 ? -k_end : k_end;
@@ -132,21 +169,23 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
   }
   // the window centred on the diagonals between 0 and k_end
   const int base = uniform(imin(0, k_end) - (SH_WIN - ak) / 2);
-  const int ringM = p.ringM, ringE = p.ringE1;
-  const int rI1 = ringM, rD1 = ringM + ringE, rNull = ringM + 2 * ringE;
-  // every row (and the NULL row) NULL over the window and its pads
+  // reduced penalties (score = t << gsh)
+  const int gsh = p.sh_gsh;
+  const int xr = p.x >> gsh, oer = p.oe1 >> gsh, er = p.e1 >> gsh;
+  // every row NULL over the window and its pads
   {
-    const int n16 = (rNull + 1) * SH_ROW / 8;
+    constexpr int n16 = (RM + 2 * RE) * SH_ROW / 8;
     const int nn = (int)(((uint32_t)(uint16_t)SH_NULL << 16) | (uint16_t)SH_NULL);
     int4* r4 = reinterpret_cast<int4*>(rows);
     for (int i = lane; i < n16; i += 64) r4[i] = make_int4(nn, nn, nn, nn);
   }
   const uint32_t* P32 = reinterpret_cast<const uint32_t*>(sP);
   const uint32_t* T32 = reinterpret_cast<const uint32_t*>(sT);
-  // row r's cell of diagonal k
-  auto rowp = [&](int r) { return rows + r * SH_ROW + 1 - base; };
+  auto cell = [&](int byte_off) {
+    return reinterpret_cast<int16_t*>(reinterpret_cast<char*>(rows) + byte_off) + 1 - base;
+  };
   const int nwf = uniform(high_mem_nwf(p, plen, tlen));
-  int status = WFA_STATUS_OK, result = 0, score = 0, nulls = 0;
+  int status = WFA_STATUS_OK, result = 0, t = 0;
   bool overflow = false;
   __syncthreads();
 
@@ -157,92 +196,101 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
     // scalar)
     const int off = uniform(lcp<2>(sP, sT, 0, 0, plen, tlen));
     if (lane == 0) {
-      rowp(0)[0] = sh16(off);
+      cell(0)[0] = sh16(off);
       if (CIG) hcells[0 - base] = sh16(off);
     }
     if (k_end == 0 && off >= tlen) status = WFA_STATUS_END_REACHED;
   }
   __syncthreads();
 
-  // Which earlier rows hold a wavefront (not absent, not trimmed empty), by
-  // age: bit d of agM is M[s - 1 - d] at the top of step s, likewise agI
-  // and agD for I1 and D1; every step shifts them by one.
-  uint64_t agM = 1;  // M0
-  uint32_t agI = 0, agD = 0;
-  const uint64_t needM = (1ull << (p.x - 1)) | (1ull << (p.oe1 - 1));
-  const uint32_t needE = 1u << (p.e1 - 1);
-  int ulo = 0, uhi = 0;   // union of the diagonals of every row so far
-  // ring rows as LDS byte offsets from `rows`: score s's M row at
-  // (s mod ringM) * RB, its I1 / D1 rows at iB / dB + (s mod ringE) * RB
-  constexpr int RB = SH_ROW * 2;
-  const int ringMB = ringM * RB, ringEB = ringE * RB;
-  const int xB = p.x * RB, oeB = p.oe1 * RB, eB = p.e1 * RB;
-  const int iB = rI1 * RB, dB = rD1 * RB, nullB = rNull * RB;
-  int sMB = 0, sEB = 0;   // (s mod ringM) * RB, (s mod ringE) * RB
-  auto back = [](int cur, int d, int ring) { return cur >= d ? cur - d : cur - d + ring; };
-  auto cell = [&](int byte_off) {
-    return reinterpret_cast<int16_t*>(reinterpret_cast<char*>(rows) + byte_off) + 1 - base;
-  };
-  // CIGAR headers of 64 consecutive scores, lane i holding score
-  // (s & ~63) + i, stored once the block is full (and at the end)
+  // Which earlier rows hold a wavefront, by age: bit d of agM is M[t - 1 - d]
+  // at the top of step t, bit d of agE is I1 or D1 of t - 1 - d.
+  uint32_t agM = 1;  // M0
+  uint32_t agE = 0;
+  const uint32_t needM = (1u << (xr - 1)) | (1u << (oer - 1));
+  const uint32_t needE = 1u << (er - 1);
+  int ulo = 0, uhi = 0;  // union of the diagonals of every row so far
+  // CIGAR headers of 64 consecutive reduced scores, lane i holding
+  // (t & ~63) + i, stored once the block is full (and at the end)
   int hbuf = lane == 0 ? sh_hdr(0, 0) : 0;
-  // the first score at which a per-score limit applies (one compare per score)
+  // the first score at which a per-score limit applies
   int limit = imin(16001, p.max_steps);
   if (!p.modular) limit = imin(limit, nwf);
-  if (CIG) limit = imin(limit, p.hdr_cap);
+  if (CIG) limit = imin(limit, p.hdr_cap << gsh);
   limit = uniform(limit);
+  // last non-null score, and the first score at which a check can fire: a
+  // limit, or a run of null steps longer than the scope
+  int lastnn = 0;
+  int ulim = imin(limit, p.scope + 1);
   // Every exit of the loop is a uniform break that sets the outcome; the
   // loop-carried state is read back as scalar at the top of each step (the
   // same in every lane; the compiler cannot tell across the per-lane
   // extension, and would otherwise run the whole loop under exec masks).
-  for (int step = status == WFA_STATUS_OK ? 1 : 0; step; ) {
-    score = uniform(score) + 1;
+  for (int step = status == WFA_STATUS_OK ? 1 : 0; step;) {
+    t = uniform(t) + 1;
     ulo = uniform(ulo);
     uhi = uniform(uhi);
-    nulls = uniform(nulls);
-    agM = ((uint64_t)(uint32_t)uniform((int)(agM >> 32)) << 32) |
-          (uint32_t)uniform((int)(uint32_t)agM);
-    agI = (uint32_t)uniform((int)agI);
-    agD = (uint32_t)uniform((int)agD);
-    sMB = uniform(sMB) + RB;
-    sEB = uniform(sEB) + RB;
-    if (sMB == ringMB) sMB = 0;
-    if (sEB == ringEB) sEB = 0;
-    const int s = score;
-    bool last = false;  // max steps: this step is computed, then the loop ends
-    if (s >= limit) {
-      if (!p.modular && s >= nwf) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
-      if (CIG && s >= p.hdr_cap) { overflow = true; break; }
-      if (s > 16000) { overflow = true; break; }
-      last = s >= p.max_steps;
-    }
-    // inputs M[s-x], M[s-o-e], I1/D1[s-e]; a lookback below score 0 finds
-    // its age bit clear
-    if (!(agM & needM) && !((agI | agD) & needE)) {
-      // null step (src/aligner.rs:2031-2043): the outputs are absent
-      nulls += 1;
-      agM <<= 1;
-      agI <<= 1;
-      agD <<= 1;
-      if (CIG) {
-        hbuf = lane == (s & 63) ? SH_HDR_EMPTY : hbuf;
-        if ((s & 63) == 63) hhdr[(s & ~63) + lane] = hbuf;
-      }
-      if (last) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; break; }
-      if (nulls > p.scope) { status = WFA_STATUS_END_UNREACHABLE; result = s; break; }
-      continue;
-    }
-    nulls = 0;
-    const int eb = back(sEB, eB, ringEB);
-    const int16_t* Px = cell((agM >> (p.x - 1)) & 1 ? back(sMB, xB, ringMB) : nullB);
-    const int16_t* Po = cell((agM >> (p.oe1 - 1)) & 1 ? back(sMB, oeB, ringMB) : nullB);
-    const int16_t* Pi = cell(agI & needE ? iB + eb : nullB);
-    const int16_t* Pd = cell(agD & needE ? dB + eb : nullB);
+    agM = (uint32_t)uniform((int)agM);
+    agE = (uint32_t)uniform((int)agE);
+    lastnn = uniform(lastnn);
+    ulim = uniform(ulim);
+    const int r = t << gsh;
+    // inputs M[t-x], M[t-o-e], I1/D1[t-e]; all absent: a null step
+    const bool nul = !(agM & needM) && !(agE & needE);
+    // the chunks: [ulo - 1, uhi + 1], inside the window
     const int clo = ulo - 1, chi = uhi + 1;
     if (clo < base || chi > base + SH_WIN - 1) { overflow = true; break; }
     const int nch = ((chi - clo) >> 6) + 1;
-    // first chunk's diagonal: the chunks stay inside the window
     const int a = imin(clo, base + SH_WIN - 64 * nch);
+    const int tm = (t & (RM - 1)) * RB, te = (t & (RE - 1)) * RB;
+    int16_t* Wm = cell(tm);
+    int16_t* Wi = cell(IB0 + te);
+    int16_t* Wd = cell(DB0 + te);
+    bool last = false;  // max steps: this step is computed, then the loop ends
+    if (r >= ulim || nul) {
+      // The real scores of this step one at a time, as the reference meets
+      // them: those since the previous step's are null steps, and so is r
+      // when its inputs are all absent.
+      bool stop = false;
+      for (int q = r - (1 << gsh) + 1; q <= r; ++q) {
+        if (q >= limit) {
+          if (!p.modular && q >= nwf) { status = WFA_STATUS_MAX_STEPS_REACHED; result = q; stop = true; break; }
+          if (CIG && q >= (p.hdr_cap << gsh)) { overflow = true; stop = true; break; }
+          if (q > 16000) { overflow = true; stop = true; break; }
+          last = q >= p.max_steps;
+        }
+        if (q < r || nul) {
+          if (last) { status = WFA_STATUS_MAX_STEPS_REACHED; result = q; stop = true; break; }
+          if (q - lastnn > p.scope) { status = WFA_STATUS_END_UNREACHABLE; result = q; stop = true; break; }
+        }
+      }
+      if (stop) break;
+      if (nul) {
+        // null step (src/aligner.rs:2031-2043): the outputs are absent
+#pragma unroll
+        for (int j = 0; j < SH_CHUNKS; ++j) {
+          if (j < nch) {
+            const int k = a + 64 * j + lane;
+            Wm[k] = (int16_t)SH_NULL;
+            Wi[k] = (int16_t)SH_NULL;
+            Wd[k] = (int16_t)SH_NULL;
+          }
+        }
+        agM <<= 1;
+        agE <<= 1;
+        if (CIG) {
+          hbuf = lane == (t & 63) ? SH_HDR_EMPTY : hbuf;
+          if ((t & 63) == 63) hhdr[(t & ~63) + lane] = hbuf;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        continue;
+      }
+    }
+    const int16_t* Px = cell(((t - xr) & (RM - 1)) * RB);
+    const int16_t* Po = cell(((t - oer) & (RM - 1)) * RB);
+    const int eb = ((t - er) & (RE - 1)) * RB;
+    const int16_t* Pi = cell(IB0 + eb);
+    const int16_t* Pd = cell(DB0 + eb);
 
     // ---- compute_affine_idm (src/compute/affine.rs:318-351) over the chunks,
     // trims by ballot (trim_ends, src/compute/mod.rs:14-49) ----
@@ -274,9 +322,6 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
     // ones included), NULL elsewhere; M is NULL wherever it is out of bounds
     const Guard gi = ine ? Guard{i_lo, i_hi - i_lo} : Guard{INT32_MIN / 2, 0};
     const Guard gd = dne ? Guard{d_lo, d_hi - d_lo} : Guard{INT32_MIN / 2, 0};
-    int16_t* Wm = cell(sMB);
-    int16_t* Wi = cell(iB + sEB);
-    int16_t* Wd = cell(dB + sEB);
 #pragma unroll
     for (int j = 0; j < SH_CHUNKS; ++j) {
       if (j < nch) {
@@ -285,34 +330,30 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
         Wd[k] = (uint32_t)(k - gd.first) <= (uint32_t)gd.span ? sh16(dv[j]) : (int16_t)SH_NULL;
       }
     }
-    agM = (agM << 1) | (uint64_t)mne;
-    agI = (agI << 1) | (uint32_t)ine;
-    agD = (agD << 1) | (uint32_t)dne;
+    agM = (agM << 1) | (uint32_t)mne;
+    agE = (agE << 1) | (uint32_t)(ine | dne);
     if (mne) { ulo = imin(ulo, m_lo); uhi = imax(uhi, m_hi); }
     if (ine) { ulo = imin(ulo, i_lo); uhi = imax(uhi, i_hi); }
     if (dne) { ulo = imin(ulo, d_lo); uhi = imax(uhi, d_hi); }
+    lastnn = r;
+    ulim = imin(limit, r + p.scope + 1);
     if (last) {
       // the row is computed but not extended (the loop stops first)
-#pragma unroll
-      for (int j = 0; j < SH_CHUNKS; ++j)
-        if (j < nch) Wm[a + 64 * j + lane] = sh16(mv[j]);
       status = WFA_STATUS_MAX_STEPS_REACHED;
-      result = s;
+      result = r;
       break;
     }
-    // ---- extend M[s] (extend_matches_packed_end2end, src/extend.rs:47-69;
+    // ---- extend M[t] (extend_matches_packed_end2end, src/extend.rs:47-69;
     // the reference extends at the top of the next iteration and nothing
-    // reads M[s] in between), keep it, test the end cell
+    // reads M[t] in between), keep it, test the end cell
     // (termination_end2end, src/termination.rs:14-23) ----
     bool reached = false;
-    int16_t* hrow = CIG ? hcells + (int64_t)s * SH_WIN - base : nullptr;
+    int16_t* hrow = CIG ? hcells + (int64_t)t * SH_WIN - base : nullptr;
 #pragma unroll
     for (int j = 0; j < SH_CHUNKS; ++j) {
       if (j < nch) {
         const int k = a + 64 * j + lane;
-        int off = mv[j];
-        if (off >= 0)
-          off += sh_lcp(P32, T32, (uint32_t)(off - k), (uint32_t)off, imin(plen - (off - k), tlen - off));
+        const int off = sh_extend(P32, T32, mv[j], k, plen, tlen);
         const int16_t o16 = sh16(off);
         Wm[k] = o16;
         if (CIG) hrow[k] = o16;
@@ -320,32 +361,32 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
       }
     }
     if (CIG) {
-      hbuf = lane == (s & 63) ? sh_hdr(a, a + 64 * nch - 1) : hbuf;
-      if ((s & 63) == 63) hhdr[(s & ~63) + lane] = hbuf;
+      hbuf = lane == (t & 63) ? sh_hdr(a, a + 64 * nch - 1) : hbuf;
+      if ((t & 63) == 63) hhdr[(t & ~63) + lane] = hbuf;
     }
     // the step's rows are written before the next step reads them: one
     // wave's LDS accesses complete in order, so only the compiler must keep
     // them in program order
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    if (ballot(reached)) { status = WFA_STATUS_END_REACHED; result = s; break; }
+    if (ballot(reached)) { status = WFA_STATUS_END_REACHED; result = r; break; }
   }
 
   if (overflow) {
     retry();
     return;
   }
-  // the headers of the last, partial block
-  if (CIG && lane <= (score & 63)) hhdr[(score & ~63) + lane] = hbuf;
   int out_status = status;
   int ncig = 0;
   if (status == WFA_STATUS_END_REACHED) {
     out_status = WFA_STATUS_COMPLETED;
     if (CIG) {
+      // the headers of the last, partial block
+      if (lane <= (t & 63)) hhdr[(t & ~63) + lane] = hbuf;
       __threadfence_block();
       uint8_t* out = p.cig + p.cig_off[pair];
       const int cap = plen + tlen;
-      ncig = singletrack<2>(p, WinView{hcells, hhdr, base, score + 1, nwf}, sP, sT, plen, tlen,
-                            score, k_end, tlen, out, cap);
+      ncig = singletrack<2>(p, WinView{hcells, hhdr, base, result + 1, nwf, gsh}, sP, sT, plen,
+                            tlen, result, k_end, tlen, out, cap);
       // move the ops from the tail of the region to its head
       const int gap = cap - ncig;
       if (gap > 0) {
@@ -367,7 +408,7 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
     if (p.end_pos) {
       // the end cell, or unset (src/aligner.rs:903-909)
       const bool end = status == WFA_STATUS_END_REACHED;
-      p.end_pos[3 * pair] = end ? score : -1;
+      p.end_pos[3 * pair] = end ? result : -1;
       p.end_pos[3 * pair + 1] = end ? k_end : INT32_MAX;
       p.end_pos[3 * pair + 2] = end ? tlen : WFA_NULL;
     }
@@ -377,7 +418,7 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
 // One wave per pair, persistent over the launch's work queue. The slot's
 // history: p.hdr_cap rows of SH_WIN 16-bit cells in p.hist and p.hdr_cap
 // packed headers in p.hdr, both per workgroup.
-template <int MODE, bool PACKED>
+template <int MODE, bool PACKED, int RC>
 __global__ void __launch_bounds__(64) wfa_short_kernel(KParams p) {
   extern __shared__ uint64_t smem[];
   uint64_t* sP = smem;
@@ -406,14 +447,15 @@ __global__ void __launch_bounds__(64) wfa_short_kernel(KParams p) {
       }
       continue;
     }
-    short_pair<MODE>(p, pair, plen, tlen, sP, sT, rows, hcells, hhdr);
+    short_pair<MODE, RC>(p, pair, plen, tlen, sP, sT, rows, hcells, hhdr);
     __syncthreads();
   }
 }
 
-// LDS bytes of the short kernel's rows (the ring rows and the NULL row)
-static inline size_t short_lds_row_bytes(int ring_rows) {
-  return (size_t)(ring_rows + 1) * SH_ROW * 2;
+// LDS bytes of the short kernel's ring rows for ring class `rc`
+static inline size_t short_lds_row_bytes(int rc) {
+  const int rows = rc == 0 ? ShRing<0>::M + 2 * ShRing<0>::E : ShRing<1>::M + 2 * ShRing<1>::E;
+  return (size_t)rows * SH_ROW * 2;
 }
 
 }  // namespace wfa
