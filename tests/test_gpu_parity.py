@@ -91,6 +91,21 @@ def test_reads100_300_packed_match_oracle(scope, pkg, N, engine, oracle):
     _compare(engine, oracle, attr, pairs, scope == "alignment", label=scope)
 
 
+@pytest.mark.parametrize("pen", [(0, 6, 4, 2, 0, 0), (0, 8, 12, 4, 0, 0), (0, 5, 3, 2, 0, 0),
+                                 (0, 3, 1, 1, 0, 0), (0, 4, 20, 1, 0, 0)],
+                         ids=["step2", "step4", "step1_wide_ring", "step1", "beyond_rings"])
+@pytest.mark.parametrize("scope", ["score", "alignment"])
+def test_short_kernel_penalty_sets(pen, scope, pkg, N, engine, oracle):
+    """The short-read kernel steps scores by the penalties' common power of
+    two and keeps rings of a fixed size per class: penalty sets with steps
+    2 and 4, the wider ring class, step 1, and lookbacks beyond every class
+    (the general kernel) all match the oracle on 100-300 bp pairs."""
+    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    pairs = generate(WORKLOADS["reads100_300"], num_pairs=1500)
+    attr = attr_for(N, "gap_affine", pen, scope)
+    _compare(engine, oracle, attr, pairs, scope == "alignment", label=f"{pen}/{scope}")
+
+
 def test_per_pair_path_equals_batch(pkg, N, engine, oracle):
     from wfa2lib_rs_amd.synth import WORKLOADS, generate
     pairs = generate(WORKLOADS["reads100_300"], num_pairs=64)

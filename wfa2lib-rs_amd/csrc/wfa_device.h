@@ -172,7 +172,8 @@ KernelFn pick_linear(int mode, bool lds, bool i16, bool packed, bool heur, int n
 KernelFn pick_affine_short(int mode, bool packed, int rc);
 size_t short_lds_row_bytes_host(int rc);
 // the short kernel's ring class for penalties x, o + e, e (the score step
-// 2^gsh in *gsh), or -1 when their reduced lookbacks exceed every class
+// 2^gsh in *gsh): 2 for the built-in default penalties, else 0 or 1, or -1
+// when their reduced lookbacks exceed every class
 int short_ring_class(int x, int oe, int e, int* gsh);
 constexpr int kShortWindow = 256;
 

@@ -11,19 +11,23 @@ KernelFn pick_affine(int mode, bool lds, bool i16, bool packed, bool heur, int n
 }
 
 // The short-read kernel (gap-affine, end-to-end, no heuristic): score only
-// or high-memory CIGAR, for ring class `rc`
+// or high-memory CIGAR, for ring class `rc` (2: ring class 0 with the
+// default penalties built in)
 KernelFn pick_affine_short(int mode, bool packed, int rc) {
-  if (rc == 0) {
-    if (mode == 0) return packed ? wfa_short_kernel<0, true, 0> : wfa_short_kernel<0, false, 0>;
-    if (mode == 1) return packed ? wfa_short_kernel<1, true, 0> : wfa_short_kernel<1, false, 0>;
+  if (rc == 2) {
+    if (mode == 0) return packed ? wfa_short_kernel<0, true, 0, 1> : wfa_short_kernel<0, false, 0, 1>;
+    if (mode == 1) return packed ? wfa_short_kernel<1, true, 0, 1> : wfa_short_kernel<1, false, 0, 1>;
+  } else if (rc == 0) {
+    if (mode == 0) return packed ? wfa_short_kernel<0, true, 0, 0> : wfa_short_kernel<0, false, 0, 0>;
+    if (mode == 1) return packed ? wfa_short_kernel<1, true, 0, 0> : wfa_short_kernel<1, false, 0, 0>;
   } else if (rc == 1) {
-    if (mode == 0) return packed ? wfa_short_kernel<0, true, 1> : wfa_short_kernel<0, false, 1>;
-    if (mode == 1) return packed ? wfa_short_kernel<1, true, 1> : wfa_short_kernel<1, false, 1>;
+    if (mode == 0) return packed ? wfa_short_kernel<0, true, 1, 0> : wfa_short_kernel<0, false, 1, 0>;
+    if (mode == 1) return packed ? wfa_short_kernel<1, true, 1, 0> : wfa_short_kernel<1, false, 1, 0>;
   }
   return nullptr;
 }
 
-size_t short_lds_row_bytes_host(int rc) { return short_lds_row_bytes(rc); }
+size_t short_lds_row_bytes_host(int rc) { return short_lds_row_bytes(rc == 2 ? 0 : rc); }
 
 int short_ring_class(int x, int oe, int e, int* gsh) {
   // g: the largest power of two dividing all three penalties
@@ -31,6 +35,8 @@ int short_ring_class(int x, int oe, int e, int* gsh) {
   while (g < 8 && ((x | oe | e) & ((2 << g) - 1)) == 0) ++g;
   *gsh = g;
   const int xr = x >> g, oer = oe >> g, er = e >> g;
+  if (g == ShPen<1>::gsh && xr == ShPen<1>::x && oer == ShPen<1>::oe && er == ShPen<1>::e)
+    return 2;
   if (xr < ShRing<0>::M && oer < ShRing<0>::M && er < ShRing<0>::E) return 0;
   if (xr < ShRing<1>::M && oer < ShRing<1>::M && er < ShRing<1>::E) return 1;
   return -1;

@@ -143,7 +143,14 @@ static __device__ __forceinline__ int sh_extend(const uint32_t* P, const uint32_
   return off + n;
 }
 
-template <int MODE, int RC>
+// Penalty sets built in at compile time (lookbacks and masks become
+// immediates): 0 none (the kernel reads them from KParams), 1 the default
+// gap-affine penalties (mismatch 4, gap open 6, gap extend 2, the driver's
+// default, src/bin/align_benchmark.rs:177-183; ring class 0, score step 2).
+template <int PEN> struct ShPen;
+template <> struct ShPen<1> { static constexpr int gsh = 1, x = 2, oe = 4, e = 1; };
+
+template <int MODE, int RC, int PEN>
 static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen,
                                   const uint64_t* sP, const uint64_t* sT, int16_t* rows,
                                   int16_t* hcells, int32_t* hhdr) {
@@ -170,8 +177,18 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
   // the window centred on the diagonals between 0 and k_end
   const int base = uniform(imin(0, k_end) - (SH_WIN - ak) / 2);
   // reduced penalties (score = t << gsh)
-  const int gsh = p.sh_gsh;
-  const int xr = p.x >> gsh, oer = p.oe1 >> gsh, er = p.e1 >> gsh;
+  int gsh, xr, oer, er;
+  if constexpr (PEN != 0) {
+    gsh = ShPen<PEN>::gsh;
+    xr = ShPen<PEN>::x;
+    oer = ShPen<PEN>::oe;
+    er = ShPen<PEN>::e;
+  } else {
+    gsh = p.sh_gsh;
+    xr = p.x >> gsh;
+    oer = p.oe1 >> gsh;
+    er = p.e1 >> gsh;
+  }
   // every row NULL over the window and its pads
   {
     constexpr int n16 = (RM + 2 * RE) * SH_ROW / 8;
@@ -332,9 +349,9 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
     }
     agM = (agM << 1) | (uint32_t)mne;
     agE = (agE << 1) | (uint32_t)(ine | dne);
-    if (mne) { ulo = imin(ulo, m_lo); uhi = imax(uhi, m_hi); }
-    if (ine) { ulo = imin(ulo, i_lo); uhi = imax(uhi, i_hi); }
-    if (dne) { ulo = imin(ulo, d_lo); uhi = imax(uhi, d_hi); }
+    // (an empty range is [INT32_MAX, INT32_MIN]: no effect)
+    ulo = imin(ulo, imin(m_lo, imin(i_lo, d_lo)));
+    uhi = imax(uhi, imax(m_hi, imax(i_hi, d_hi)));
     lastnn = r;
     ulim = imin(limit, r + p.scope + 1);
     if (last) {
@@ -418,7 +435,7 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
 // One wave per pair, persistent over the launch's work queue. The slot's
 // history: p.hdr_cap rows of SH_WIN 16-bit cells in p.hist and p.hdr_cap
 // packed headers in p.hdr, both per workgroup.
-template <int MODE, bool PACKED, int RC>
+template <int MODE, bool PACKED, int RC, int PEN>
 __global__ void __launch_bounds__(64) wfa_short_kernel(KParams p) {
   extern __shared__ uint64_t smem[];
   uint64_t* sP = smem;
@@ -447,7 +464,7 @@ __global__ void __launch_bounds__(64) wfa_short_kernel(KParams p) {
       }
       continue;
     }
-    short_pair<MODE, RC>(p, pair, plen, tlen, sP, sT, rows, hcells, hhdr);
+    short_pair<MODE, RC, PEN>(p, pair, plen, tlen, sP, sT, rows, hcells, hhdr);
     __syncthreads();
   }
 }
