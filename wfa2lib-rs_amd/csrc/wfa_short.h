@@ -52,6 +52,8 @@
 // the retry list to the general kernel, which aligns it from the start.
 #pragma once
 
+#include <type_traits>
+
 #include "wfa_align.h"
 
 namespace wfa {
@@ -309,14 +311,21 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
     const int16_t* Pi = cell(IB0 + eb);
     const int16_t* Pd = cell(DB0 + eb);
 
-    // ---- compute_affine_idm (src/compute/affine.rs:318-351) over the chunks,
-    // trims by ballot (trim_ends, src/compute/mod.rs:14-49) ----
-    int mv[SH_CHUNKS], iv[SH_CHUNKS], dv[SH_CHUNKS];
-    int m_lo = INT32_MAX, m_hi = INT32_MIN, i_lo = INT32_MAX, i_hi = INT32_MIN;
-    int d_lo = INT32_MAX, d_hi = INT32_MIN;
+    // The step over NCH chunks (one instance per chunk count, so the chunk
+    // loops carry no guards); true when the alignment ends here.
+    auto compute = [&](auto nch_c) -> bool {
+      constexpr int NCH = decltype(nch_c)::value;
+      // ---- compute_affine_idm (src/compute/affine.rs:318-351) over the
+      // chunks, trims by ballot (trim_ends, src/compute/mod.rs:14-49) ----
+      // M's trimmed range needs no tracking: a cell of M in bounds takes its
+      // offset from an in-bounds I1 or D1 cell of this score or from a cell
+      // of M[t - x], so it lies in their ranges or the union already; only
+      // whether M is empty matters (null steps)
+      int mv[NCH], iv[NCH], dv[NCH];
+      uint64_t m_any = 0;
+      int i_lo = INT32_MAX, i_hi = INT32_MIN, d_lo = INT32_MAX, d_hi = INT32_MIN;
 #pragma unroll
-    for (int j = 0; j < SH_CHUNKS; ++j) {
-      if (j < nch) {
+      for (int j = 0; j < NCH; ++j) {
         const int k = a + 64 * j + lane;
         const int ins = imax(Po[k - 1], Pi[k - 1]) + 1;
         const int del = imax(Po[k + 1], Pd[k + 1]);
@@ -329,46 +338,42 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
         mv[j] = vm ? m : WFA_NULL;
         iv[j] = ins;
         dv[j] = del;
-        sh_track(m_lo, m_hi, ballot(vm), a + 64 * j);
+        m_any |= ballot(vm);
         sh_track(i_lo, i_hi, ballot(vcol(ins) <= (uint32_t)plen), a + 64 * j);
         sh_track(d_lo, d_hi, ballot(vcol(del) <= (uint32_t)plen), a + 64 * j);
       }
-    }
-    const bool mne = m_lo != INT32_MAX, ine = i_lo != INT32_MAX, dne = d_lo != INT32_MAX;
-    // I1 and D1 keep their cells inside their trimmed ranges (out-of-bounds
-    // ones included), NULL elsewhere; M is NULL wherever it is out of bounds
-    const Guard gi = ine ? Guard{i_lo, i_hi - i_lo} : Guard{INT32_MIN / 2, 0};
-    const Guard gd = dne ? Guard{d_lo, d_hi - d_lo} : Guard{INT32_MIN / 2, 0};
+      const bool mne = m_any != 0, ine = i_lo != INT32_MAX, dne = d_lo != INT32_MAX;
+      // I1 and D1 keep their cells inside their trimmed ranges (out-of-bounds
+      // ones included), NULL elsewhere; M is NULL wherever it is out of bounds
+      const Guard gi = ine ? Guard{i_lo, i_hi - i_lo} : Guard{INT32_MIN / 2, 0};
+      const Guard gd = dne ? Guard{d_lo, d_hi - d_lo} : Guard{INT32_MIN / 2, 0};
 #pragma unroll
-    for (int j = 0; j < SH_CHUNKS; ++j) {
-      if (j < nch) {
+      for (int j = 0; j < NCH; ++j) {
         const int k = a + 64 * j + lane;
         Wi[k] = (uint32_t)(k - gi.first) <= (uint32_t)gi.span ? sh16(iv[j]) : (int16_t)SH_NULL;
         Wd[k] = (uint32_t)(k - gd.first) <= (uint32_t)gd.span ? sh16(dv[j]) : (int16_t)SH_NULL;
       }
-    }
-    agM = (agM << 1) | (uint32_t)mne;
-    agE = (agE << 1) | (uint32_t)(ine | dne);
-    // (an empty range is [INT32_MAX, INT32_MIN]: no effect)
-    ulo = imin(ulo, imin(m_lo, imin(i_lo, d_lo)));
-    uhi = imax(uhi, imax(m_hi, imax(i_hi, d_hi)));
-    lastnn = r;
-    ulim = imin(limit, r + p.scope + 1);
-    if (last) {
-      // the row is computed but not extended (the loop stops first)
-      status = WFA_STATUS_MAX_STEPS_REACHED;
-      result = r;
-      break;
-    }
-    // ---- extend M[t] (extend_matches_packed_end2end, src/extend.rs:47-69;
-    // the reference extends at the top of the next iteration and nothing
-    // reads M[t] in between), keep it, test the end cell
-    // (termination_end2end, src/termination.rs:14-23) ----
-    bool reached = false;
-    int16_t* hrow = CIG ? hcells + (int64_t)t * SH_WIN - base : nullptr;
+      agM = (agM << 1) | (uint32_t)mne;
+      agE = (agE << 1) | (uint32_t)(ine | dne);
+      // (an empty range is [INT32_MAX, INT32_MIN]: no effect)
+      ulo = imin(ulo, imin(i_lo, d_lo));
+      uhi = imax(uhi, imax(i_hi, d_hi));
+      lastnn = r;
+      ulim = imin(limit, r + p.scope + 1);
+      if (last) {
+        // the row is computed but not extended (the loop stops first)
+        status = WFA_STATUS_MAX_STEPS_REACHED;
+        result = r;
+        return true;
+      }
+      // ---- extend M[t] (extend_matches_packed_end2end, src/extend.rs:47-69;
+      // the reference extends at the top of the next iteration and nothing
+      // reads M[t] in between), keep it, test the end cell
+      // (termination_end2end, src/termination.rs:14-23) ----
+      bool reached = false;
+      int16_t* hrow = CIG ? hcells + (int64_t)t * SH_WIN - base : nullptr;
 #pragma unroll
-    for (int j = 0; j < SH_CHUNKS; ++j) {
-      if (j < nch) {
+      for (int j = 0; j < NCH; ++j) {
         const int k = a + 64 * j + lane;
         const int off = sh_extend(P32, T32, mv[j], k, plen, tlen);
         const int16_t o16 = sh16(off);
@@ -376,16 +381,29 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
         if (CIG) hrow[k] = o16;
         reached |= k == k_end && off >= tlen;
       }
+      if (CIG) {
+        hbuf = lane == (t & 63) ? sh_hdr(a, a + 64 * NCH - 1) : hbuf;
+        if ((t & 63) == 63) hhdr[(t & ~63) + lane] = hbuf;
+      }
+      // the step's rows are written before the next step reads them: one
+      // wave's LDS accesses complete in order, so only the compiler must
+      // keep them in program order
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      if (ballot(reached)) {
+        status = WFA_STATUS_END_REACHED;
+        result = r;
+        return true;
+      }
+      return false;
+    };
+    bool done;
+    switch (nch) {
+      case 1: done = compute(std::integral_constant<int, 1>{}); break;
+      case 2: done = compute(std::integral_constant<int, 2>{}); break;
+      case 3: done = compute(std::integral_constant<int, 3>{}); break;
+      default: done = compute(std::integral_constant<int, 4>{}); break;
     }
-    if (CIG) {
-      hbuf = lane == (t & 63) ? sh_hdr(a, a + 64 * nch - 1) : hbuf;
-      if ((t & 63) == 63) hhdr[(t & ~63) + lane] = hbuf;
-    }
-    // the step's rows are written before the next step reads them: one
-    // wave's LDS accesses complete in order, so only the compiler must keep
-    // them in program order
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    if (ballot(reached)) { status = WFA_STATUS_END_REACHED; result = r; break; }
+    if (done) break;
   }
 
   if (overflow) {
