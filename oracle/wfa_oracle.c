@@ -209,6 +209,12 @@ static void trim_ends(int plen, int tlen, wf_t* w) {
 /* ------------------------------------------------------------------ */
 enum { C_M = 0, C_I1, C_D1, C_I2, C_D2, NCOMP };
 
+/* HeuristicState (src/heuristic.rs:60-143): the fields in this order, six
+ * int32 (the test hooks below take them as an array) */
+typedef struct {
+    int32_t steps_wait, max_sw_score, max_sw_k, max_sw_off, max_wf_score, zlast;
+} hstate_t;
+
 struct wfo_aligner {
     wfa_attr_t attr;
     pen_t pen;
@@ -236,10 +242,7 @@ struct wfo_aligner {
      * (src/bt_buffer.rs:34-209) as flat arrays */
     int piggyback, low;
     /* heuristic state (src/heuristic.rs:60-143) */
-    int h_steps_wait, h_max_sw_score, h_max_sw_k;
-    /* Z-drop: the best cell's offset and WFA score, the last score with no
-     * drop (HeuristicState, src/heuristic.rs:60-69) */
-    int h_max_sw_off, h_max_wf_score, h_zlast;
+    hstate_t hs;
     int32_t* dist; int cap_dist;
     /* Compat::Corrected (src/aligner.rs:53-94, 1111-1360): the best ends-free
      * end candidate so far (EndsfreeCandidate) with its piggyback chain; the
@@ -436,10 +439,13 @@ wfo_aligner* wfo_aligner_new(const wfa_attr_t* attr, int32_t* err) {
         free(a);
         return NULL;
     }
-    /* BiWFA rejects X-drop and Z-drop as the driver does. */
+    /* BiWFA is end-to-end; Z-drop is refused with it (C since 7110ee3; the
+     * reference's driver refuses X-drop too, src/bin/align_benchmark.rs:
+     * 529-560, but its aligner runs X-drop sub-alignments,
+     * tests/xdrop_steps.rs:57-92). */
     const int zdrop = attr->heuristic == WFA_HEURISTIC_ZDROP;
     if ((attr->memory_mode == WFA_MEMORY_ULTRALOW &&
-         (attr->span != WFA_SPAN_END2END || attr->heuristic == WFA_HEURISTIC_XDROP || zdrop))) {
+         (attr->span != WFA_SPAN_END2END || zdrop))) {
         if (err) *err = WFA_ERR_UNSUPPORTED;
         free(a);
         return NULL;
@@ -672,18 +678,33 @@ static inline int lcp(const wfo_aligner* a, int v, int h) {
 /* ------------------------------------------------------------------ */
 /* Heuristics (src/heuristic.rs, src/aligner.rs:1644-1766)              */
 /* ------------------------------------------------------------------ */
+/* DropParams::new (src/heuristic.rs:75-105) */
+typedef struct { int swg_match, pbf, tbf, max_penalty; } drop_t;
+
+static drop_t drop_params(int match, int endsfree, int pbf, int tbf, int scope) {
+    const int begin_free = match == 0 && endsfree;
+    drop_t d;
+    d.swg_match = match != 0 ? -match : 1;
+    d.pbf = begin_free ? pbf : 0;
+    d.tbf = begin_free ? tbf : 0;
+    d.max_penalty = scope - 1;
+    return d;
+}
+/* HeuristicState::reset (src/heuristic.rs:122-143) */
+static void hstate_reset(hstate_t* st, int heuristic, const int* hp) {
+    st->steps_wait = 0;
+    if (heuristic == WFA_HEURISTIC_WFADAPTIVE || heuristic == WFA_HEURISTIC_BANDED_ADAPTIVE)
+        st->steps_wait = hp[2];
+    else if (heuristic == WFA_HEURISTIC_XDROP || heuristic == WFA_HEURISTIC_ZDROP)
+        st->steps_wait = hp[1];
+    st->max_sw_score = 0;
+    st->max_sw_k = DIAG_NULL;
+    st->max_sw_off = OFFSET_NULL;
+    st->max_wf_score = 0;
+    st->zlast = 0;
+}
 static void heuristic_reset(wfo_aligner* a) {
-    int h = a->attr.heuristic;
-    a->h_steps_wait = 0;
-    if (h == WFA_HEURISTIC_WFADAPTIVE || h == WFA_HEURISTIC_BANDED_ADAPTIVE)
-        a->h_steps_wait = a->attr.heuristic_p[2];
-    else if (h == WFA_HEURISTIC_XDROP || h == WFA_HEURISTIC_ZDROP)
-        a->h_steps_wait = a->attr.heuristic_p[1];
-    a->h_max_sw_score = 0;
-    a->h_max_sw_k = DIAG_NULL;
-    a->h_max_sw_off = OFFSET_NULL;
-    a->h_max_wf_score = 0;
-    a->h_zlast = 0;
+    hstate_reset(&a->hs, a->attr.heuristic, a->attr.heuristic_p);
 }
 /* wf_distance_end2end (src/heuristic.rs:150-160) */
 static inline int32_t wf_distance(int32_t off, int k, int plen, int tlen) {
@@ -699,139 +720,165 @@ static int32_t* dist_at(wfo_aligner* a, const wf_t* w) {
     }
     return a->dist;
 }
-/* heuristic_cutoff (src/heuristic.rs:167-255); narrows m->lo/hi only (cut
+/* wfadaptive_reduce (src/heuristic.rs:303-339): drop diagonals from each
+ * end while their distance is more than `thr` above the minimum, keeping
+ * [min_k, max_k] */
+static void wfadaptive_reduce(wf_t* wf, const int32_t* d, int dbase, int min_d, int thr,
+                              int min_k, int max_k) {
+    int top = max_k < wf->hi ? max_k : wf->hi;
+    int lo_r = wf->lo;
+    for (int k = wf->lo; k < top; ++k) {
+        if ((int64_t)d[k - dbase] - min_d <= thr) break;
+        lo_r++;
+    }
+    wf->lo = lo_r;
+    int bottom = min_k > wf->lo ? min_k : wf->lo;
+    int hi_r = wf->hi;
+    for (int k = wf->hi; k > bottom; --k) {
+        if ((int64_t)d[k - dbase] - min_d <= thr) break;
+        hi_r--;
+    }
+    wf->hi = hi_r;
+}
+/* wfadaptive_cutoff (src/heuristic.rs:258-300) */
+static void wfadaptive_cutoff(hstate_t* st, wf_t* wf, int32_t* d, int dbase, int plen, int tlen,
+                              int min_len, int thr, int steps) {
+    if (st->steps_wait > 0) return;
+    if (wf->hi - wf->lo + 1 < min_len) return;
+    int min_d = plen > tlen ? plen : tlen;
+    for (int k = wf->lo; k <= wf->hi; ++k) {
+        int32_t dd = wf_distance(wf_get(wf, k), k, plen, tlen);
+        d[k - dbase] = dd;
+        if (dd < min_d) min_d = dd;
+    }
+    const int ak = tlen - plen;
+    wfadaptive_reduce(wf, d, dbase, min_d, thr, ak, ak);
+    st->steps_wait = steps;
+}
+/* banded_adaptive_cutoff (src/heuristic.rs:355-394) */
+static void banded_adaptive_cutoff(hstate_t* st, wf_t* wf, int plen, int tlen, int min_k,
+                                   int max_k, int steps) {
+    if (st->steps_wait > 0) return;
+    int lo = wf->lo, hi = wf->hi, len = hi - lo + 1;
+    if (len < 4) return;
+    int maxlen = max_k - min_k + 1;
+    if (len > maxlen) {
+        int leeway = (len - maxlen) / 2, q = len / 4;
+        int32_t d0 = wf_distance(wf_get(wf, lo), lo, plen, tlen);
+        int32_t d1 = wf_distance(wf_get(wf, lo + q), lo + q, plen, tlen);
+        int32_t d2 = wf_distance(wf_get(wf, lo + 2 * q), lo + 2 * q, plen, tlen);
+        int32_t d3 = wf_distance(wf_get(wf, hi), hi, plen, tlen);
+        int nlo = lo;
+        if (d0 > d3) nlo += leeway;
+        if (d1 > d2) nlo += leeway;
+        wf->lo = nlo > lo ? nlo : lo;
+        int nhi = nlo + maxlen - 1;
+        wf->hi = nhi < hi ? nhi : hi;
+    }
+    st->steps_wait = steps;
+}
+/* banded_static_cutoff (src/heuristic.rs:396-403) */
+static void banded_static_cutoff(wf_t* wf, int min_k, int max_k) {
+    if (wf->lo < min_k) wf->lo = min_k;
+    if (wf->hi > max_k) wf->hi = max_k;
+}
+/* compute_sw_scores (src/heuristic.rs:491-525): the Smith-Waterman score
+ * of every non-null cell, free leading gaps left out; returns the first
+ * cell with the largest one */
+static void compute_sw_scores(const wf_t* wf, int32_t* sw, int swbase, const drop_t* dp, int score,
+                              int* cmax, int* cmax_k, int* cmax_off) {
+    int mx = INT32_MIN, mk = 0, mo = 0;
+    for (int k = wf->lo; k <= wf->hi; ++k) {
+        int32_t off = wf_get(wf, k);
+        if (off < 0) continue;
+        int fv = k < 0 ? ((-k) < dp->pbf ? -k : dp->pbf) : 0;
+        int fh = k > 0 ? (k < dp->tbf ? k : dp->tbf) : 0;
+        int v = off - k - fv, h = off - fh;
+        int sws = (dp->swg_match * (v + h) - score) / 2;
+        sw[k - swbase] = sws;
+        if (mx < sws) { mx = sws; mk = k; mo = off; }
+    }
+    *cmax = mx; *cmax_k = mk; *cmax_off = mo;
+}
+/* xdrop_cutoff (src/heuristic.rs:410-448) */
+static void xdrop_cutoff(hstate_t* st, wf_t* wf, int32_t* sw, int swbase, const drop_t* dp,
+                         int score, int xdrop, int steps) {
+    if (st->steps_wait > 0) return;
+    int cmax, cmax_k, cmax_off;
+    compute_sw_scores(wf, sw, swbase, dp, score, &cmax, &cmax_k, &cmax_off);
+    if (st->max_sw_k != DIAG_NULL) {
+        int k = wf->lo;
+        while (k <= wf->hi && (wf_get(wf, k) < 0 || st->max_sw_score - sw[k - swbase] >= xdrop))
+            k++;
+        wf->lo = k;
+        k = wf->hi;
+        while (k >= wf->lo && (wf_get(wf, k) < 0 || st->max_sw_score - sw[k - swbase] >= xdrop))
+            k--;
+        wf->hi = k;
+        if (cmax > st->max_sw_score) { st->max_sw_score = cmax; st->max_sw_k = cmax_k; }
+    } else {
+        st->max_sw_score = cmax;
+        st->max_sw_k = cmax_k;
+    }
+    st->steps_wait = steps;
+}
+/* zdrop_cutoff (src/heuristic.rs:449-489): returns 1 when the drop has
+ * lasted the largest single penalty */
+static int zdrop_cutoff(hstate_t* st, wf_t* wf, int32_t* sw, int swbase, const drop_t* dp,
+                        int score, int zdrop, int steps) {
+    if (st->steps_wait > 0) return 0;
+    int cmax, cmax_k, cmax_off;
+    compute_sw_scores(wf, sw, swbase, dp, score, &cmax, &cmax_k, &cmax_off);
+    if (st->max_sw_k == DIAG_NULL || cmax > st->max_sw_score) {
+        /* a new best: its cell and the score it was reached at */
+        st->max_sw_score = cmax;
+        st->max_wf_score = score;
+        st->max_sw_k = cmax_k;
+        st->max_sw_off = cmax_off;
+        st->zlast = score;
+    } else if ((int64_t)st->max_sw_score - cmax <= zdrop) {
+        st->zlast = score;
+    } else if (score - st->zlast >= dp->max_penalty) {
+        return 1;
+    }
+    st->steps_wait = steps;
+    return 0;
+}
+/* heuristic_cutoff (src/heuristic.rs:167-255); narrows wf->lo/hi only (cut
  * cells keep their offsets and stay readable). Returns 1 when Z-drop ends
  * the alignment. */
 static int heuristic_cutoff(wfo_aligner* a, wf_t* wf, int score) {
     if (wf->lo > wf->hi) return 0;
-    a->h_steps_wait -= 1;
+    a->hs.steps_wait -= 1;
     const int plen = a->plen, tlen = a->tlen;
     const int* hp = a->attr.heuristic_p;
+    const drop_t dp = drop_params(a->pen.match, a->endsfree, a->pb_free, a->tb_free, a->scope);
     switch (a->attr.heuristic) {
-    case WFA_HEURISTIC_WFADAPTIVE: { /* src/heuristic.rs:258-346 */
-        if (a->h_steps_wait > 0) return 0;
-        int lo = wf->lo, hi = wf->hi;
-        if (hi - lo + 1 < hp[0]) return 0;
-        int32_t* d = dist_at(a, wf);
-        int min_d = plen > tlen ? plen : tlen;
-        for (int k = lo; k <= hi; ++k) {
-            int32_t dd = wf_distance(wf_get(wf, k), k, plen, tlen);
-            d[k - wf->base] = dd;
-            if (dd < min_d) min_d = dd;
-        }
-        int ak = tlen - plen;
-        int top = ak < wf->hi ? ak : wf->hi;
-        int lo_r = wf->lo;
-        for (int k = wf->lo; k < top; ++k) {
-            if ((int64_t)d[k - wf->base] - min_d <= hp[1]) break;
-            lo_r++;
-        }
-        wf->lo = lo_r;
-        int bottom = ak > wf->lo ? ak : wf->lo;
-        int hi_r = wf->hi;
-        for (int k = wf->hi; k > bottom; --k) {
-            if ((int64_t)d[k - wf->base] - min_d <= hp[1]) break;
-            hi_r--;
-        }
-        wf->hi = hi_r;
-        a->h_steps_wait = hp[2];
+    case WFA_HEURISTIC_WFADAPTIVE:
+        wfadaptive_cutoff(&a->hs, wf, dist_at(a, wf), wf->base, plen, tlen, hp[0], hp[1], hp[2]);
         return 0;
-    }
-    case WFA_HEURISTIC_BANDED_STATIC: /* src/heuristic.rs:400-407 */
-        if (wf->lo < hp[0]) wf->lo = hp[0];
-        if (wf->hi > hp[1]) wf->hi = hp[1];
+    case WFA_HEURISTIC_BANDED_STATIC:
+        banded_static_cutoff(wf, hp[0], hp[1]);
         return 0;
-    case WFA_HEURISTIC_BANDED_ADAPTIVE: { /* src/heuristic.rs:355-394 */
-        if (a->h_steps_wait > 0) return 0;
-        int lo = wf->lo, hi = wf->hi, len = hi - lo + 1;
-        if (len < 4) return 0;
-        int maxlen = hp[1] - hp[0] + 1;
-        if (len > maxlen) {
-            int leeway = (len - maxlen) / 2, q = len / 4;
-            int32_t d0 = wf_distance(wf_get(wf, lo), lo, plen, tlen);
-            int32_t d1 = wf_distance(wf_get(wf, lo + q), lo + q, plen, tlen);
-            int32_t d2 = wf_distance(wf_get(wf, lo + 2 * q), lo + 2 * q, plen, tlen);
-            int32_t d3 = wf_distance(wf_get(wf, hi), hi, plen, tlen);
-            int nlo = lo;
-            if (d0 > d3) nlo += leeway;
-            if (d1 > d2) nlo += leeway;
-            wf->lo = nlo > lo ? nlo : lo;
-            int nhi = nlo + maxlen - 1;
-            wf->hi = nhi < hi ? nhi : hi;
-        }
-        a->h_steps_wait = hp[2];
+    case WFA_HEURISTIC_BANDED_ADAPTIVE:
+        banded_adaptive_cutoff(&a->hs, wf, plen, tlen, hp[0], hp[1], hp[2]);
         return 0;
-    }
-    case WFA_HEURISTIC_XDROP: { /* src/heuristic.rs:410-448, 491-525 */
-        if (a->h_steps_wait > 0) return 0;
-        const int swg_match = a->pen.match != 0 ? -a->pen.match : 1;
-        const int bfree = a->pen.match == 0 && a->endsfree;
-        const int pbf = bfree ? a->pb_free : 0, tbf = bfree ? a->tb_free : 0;
-        int32_t* sw = dist_at(a, wf);
-        int cmax = INT32_MIN, cmax_k = 0;
-        for (int k = wf->lo; k <= wf->hi; ++k) {
-            int32_t off = wf_get(wf, k);
-            if (off < 0) continue;
-            int fv = k < 0 ? ((-k) < pbf ? -k : pbf) : 0;
-            int fh = k > 0 ? (k < tbf ? k : tbf) : 0;
-            int v = off - k - fv, h = off - fh;
-            int sws = (swg_match * (v + h) - score) / 2;
-            sw[k - wf->base] = sws;
-            if (cmax < sws) { cmax = sws; cmax_k = k; }
-        }
-        if (a->h_max_sw_k != DIAG_NULL) {
-            int k = wf->lo;
-            while (k <= wf->hi &&
-                   (wf_get(wf, k) < 0 || a->h_max_sw_score - sw[k - wf->base] >= hp[0]))
-                k++;
-            wf->lo = k;
-            k = wf->hi;
-            while (k >= wf->lo &&
-                   (wf_get(wf, k) < 0 || a->h_max_sw_score - sw[k - wf->base] >= hp[0]))
-                k--;
-            wf->hi = k;
-            if (cmax > a->h_max_sw_score) { a->h_max_sw_score = cmax; a->h_max_sw_k = cmax_k; }
-        } else {
-            a->h_max_sw_score = cmax;
-            a->h_max_sw_k = cmax_k;
-        }
-        a->h_steps_wait = hp[1];
+    case WFA_HEURISTIC_XDROP:
+        xdrop_cutoff(&a->hs, wf, dist_at(a, wf), wf->base, &dp, score, hp[0], hp[1]);
         return 0;
-    }
-    case WFA_HEURISTIC_ZDROP: { /* zdrop_cutoff + compute_sw_scores, src/heuristic.rs:449-525 */
-        if (a->h_steps_wait > 0) return 0;
-        const int swg_match = a->pen.match != 0 ? -a->pen.match : 1;
-        const int bfree = a->pen.match == 0 && a->endsfree;
-        const int pbf = bfree ? a->pb_free : 0, tbf = bfree ? a->tb_free : 0;
-        /* the wavefront's best cell: the first k with the largest score */
-        int cmax = INT32_MIN, cmax_k = 0, cmax_off = 0;
-        for (int k = wf->lo; k <= wf->hi; ++k) {
-            int32_t off = wf_get(wf, k);
-            if (off < 0) continue;
-            int fv = k < 0 ? ((-k) < pbf ? -k : pbf) : 0;
-            int fh = k > 0 ? (k < tbf ? k : tbf) : 0;
-            int sws = (swg_match * ((off - k - fv) + (off - fh)) - score) / 2;
-            if (cmax < sws) { cmax = sws; cmax_k = k; cmax_off = off; }
-        }
-        if (a->h_max_sw_k == DIAG_NULL || cmax > a->h_max_sw_score) {
-            /* a new best: its cell and the score it was reached at */
-            a->h_max_sw_score = cmax;
-            a->h_max_wf_score = score;
-            a->h_max_sw_k = cmax_k;
-            a->h_max_sw_off = cmax_off;
-            a->h_zlast = score;
-        } else if ((int64_t)a->h_max_sw_score - cmax <= hp[0]) {
-            a->h_zlast = score;
-        } else if (score - a->h_zlast >= a->scope - 1) {
-            /* dropped for at least the largest single penalty */
-            return 1;
-        }
-        a->h_steps_wait = hp[1];
-        return 0;
-    }
+    case WFA_HEURISTIC_ZDROP:
+        return zdrop_cutoff(&a->hs, wf, dist_at(a, wf), wf->base, &dp, score, hp[0], hp[1]);
     default:
         return 0;
     }
+}
+
+/* heuristic_equate (src/heuristic.rs:526-538): an I/D row takes the
+ * narrowed M bounds */
+static void heuristic_equate(wf_t* d, const wf_t* wf) {
+    if (wf->lo > d->lo) d->lo = wf->lo;
+    if (wf->hi < d->hi) d->hi = wf->hi;
+    if (d->lo > d->hi) { d->null = 1; d->init_min = d->lo; d->init_max = d->hi; }
 }
 
 /* apply_heuristic_cutoff (src/aligner.rs:1644-1766). Returns 1 when Z-drop
@@ -841,21 +888,21 @@ static int apply_heuristic_cutoff(wfo_aligner* a, int score, wf_t* wf) {
         wf->lo >= a->attr.heuristic_p[0] && wf->hi <= a->attr.heuristic_p[1])
         return 0;
     int lo_base = wf->lo, hi_base = wf->hi;
-    const int max_wf_base = a->h_max_wf_score;
+    const int max_wf_base = a->hs.max_wf_score;
     const int zdropped = heuristic_cutoff(a, wf, score);
     /* Corrected: keep the backtrace of a new Z-drop best cell, which the
      * end backtrace needs after its wavefront is gone (med/low,
      * src/aligner.rs:1676-1688) */
     if (a->attr.compat == WFA_COMPAT_CORRECTED && a->piggyback &&
-        a->h_max_wf_score == score && max_wf_base != score) {
-        const int k = a->h_max_sw_k;
+        a->hs.max_wf_score == score && max_wf_base != score) {
+        const int k = a->hs.max_sw_k;
         a->zbest_pc = wf->pc[k - wf->base];
         a->zbest_pv = wf->pv[k - wf->base];
     }
     if (zdropped) {
-        a->end_score = a->h_max_wf_score;
-        a->end_k = a->h_max_sw_k;
-        a->end_offset = a->h_max_sw_off;
+        a->end_score = a->hs.max_wf_score;
+        a->end_k = a->hs.max_sw_k;
+        a->end_offset = a->hs.max_sw_off;
         return 1;
     }
     if (lo_base > hi_base || (wf->lo == lo_base && wf->hi == hi_base)) return 0;
@@ -866,10 +913,7 @@ static int apply_heuristic_cutoff(wfo_aligner* a, int score, wf_t* wf) {
     int nc = a->pen.metric == WFA_METRIC_GAP_AFFINE ? 3 : a->pen.metric == WFA_METRIC_GAP_AFFINE_2P ? 5 : 1;
     for (int c = 1; c < nc; ++c) {
         wf_t* d = get(a, c, score);
-        if (!d) continue;
-        if (wf->lo > d->lo) d->lo = wf->lo;
-        if (wf->hi < d->hi) d->hi = wf->hi;
-        if (d->lo > d->hi) { d->null = 1; d->init_min = d->lo; d->init_max = d->hi; }
+        if (d) heuristic_equate(d, wf);
     }
     return 0;
 }
@@ -1104,6 +1148,39 @@ static void free_outputs(wfo_aligner* a, int score, int ncomp) {
     for (int c = 0; c < ncomp; ++c) put(a, c, score, NULL);
 }
 
+/* compute_affine_idm / compute_affine2p_idm (scalar semantics,
+ * src/compute/affine.rs:318-351, src/compute/affine2p.rs:205-236):
+ * I1 = max(Mo1[k-1], I1e[k-1]) + 1, D1 = max(Mo1[k+1], D1e[k+1]) (I2/D2
+ * likewise from Mo2, I2e, D2e when `use2`), M = max(Mx[k] + 1, I*, D*),
+ * NULL when out of the sequences (only M is bounds-checked). An absent
+ * input reads OFFSET_NULL everywhere. */
+static void affine_kernel(int plen, int tlen, const wf_t* mx, const wf_t* mo1, const wf_t* i1e,
+                          const wf_t* d1e, const wf_t* mo2, const wf_t* i2e, const wf_t* d2e,
+                          int use2, wf_t* m, wf_t* i1, wf_t* d1, wf_t* i2, wf_t* d2, int lo,
+                          int hi) {
+    for (int k = lo; k <= hi; ++k) {
+        int32_t vi1 = rd(mo1, k - 1) > rd(i1e, k - 1) ? rd(mo1, k - 1) : rd(i1e, k - 1);
+        vi1 += 1;
+        int32_t vd1 = rd(mo1, k + 1) > rd(d1e, k + 1) ? rd(mo1, k + 1) : rd(d1e, k + 1);
+        int32_t mm = rd(mx, k) + 1;
+        int32_t ins = vi1, del = vd1;
+        if (use2) {
+            int32_t vi2 = (rd(mo2, k - 1) > rd(i2e, k - 1) ? rd(mo2, k - 1) : rd(i2e, k - 1)) + 1;
+            int32_t vd2 = rd(mo2, k + 1) > rd(d2e, k + 1) ? rd(mo2, k + 1) : rd(d2e, k + 1);
+            wf_set(i2, k, vi2);
+            wf_set(d2, k, vd2);
+            if (vi2 > ins) ins = vi2;
+            if (vd2 > del) del = vd2;
+        }
+        int32_t mv = ins > del ? ins : del;
+        if (mm > mv) mv = mm;
+        if ((uint32_t)mv > (uint32_t)tlen || (uint32_t)(mv - k) > (uint32_t)plen) mv = OFFSET_NULL;
+        wf_set(m, k, mv);
+        wf_set(i1, k, vi1);
+        wf_set(d1, k, vd1);
+    }
+}
+
 /* compute_affine_step + compute_affine2p_step (src/aligner.rs:2006-2193,
  * 2201-2553) with the kernels of src/compute/affine.rs:318-351 and
  * src/compute/affine2p.rs:205-236. */
@@ -1261,27 +1338,9 @@ static void compute_affine_step(wfo_aligner* a, int score) {
             offload_affine_component(a, i2, lo, hi);
             offload_affine_component(a, d2, lo, hi);
         }
-    } else
-    for (int k = lo; k <= hi; ++k) {
-        int32_t vi1 = rd(mo1, k - 1) > rd(i1e, k - 1) ? rd(mo1, k - 1) : rd(i1e, k - 1);
-        vi1 += 1;
-        int32_t vd1 = rd(mo1, k + 1) > rd(d1e, k + 1) ? rd(mo1, k + 1) : rd(d1e, k + 1);
-        int32_t mm = rd(mx, k) + 1;
-        int32_t ins = vi1, del = vd1;
-        if (use2) {
-            int32_t vi2 = (rd(mo2, k - 1) > rd(i2e, k - 1) ? rd(mo2, k - 1) : rd(i2e, k - 1)) + 1;
-            int32_t vd2 = rd(mo2, k + 1) > rd(d2e, k + 1) ? rd(mo2, k + 1) : rd(d2e, k + 1);
-            wf_set(i2, k, vi2);
-            wf_set(d2, k, vd2);
-            if (vi2 > ins) ins = vi2;
-            if (vd2 > del) del = vd2;
-        }
-        int32_t mv = ins > del ? ins : del;
-        if (mm > mv) mv = mm;
-        if ((uint32_t)mv > (uint32_t)tlen || (uint32_t)(mv - k) > (uint32_t)plen) mv = OFFSET_NULL;
-        wf_set(m, k, mv);
-        wf_set(i1, k, vi1);
-        wf_set(d1, k, vd1);
+    } else {
+        affine_kernel(plen, tlen, mx, mo1, i1e, d1e, mo2, i2e, d2e, use2, m, i1, d1, i2, d2, lo,
+                      hi);
     }
     put(a, C_M, score, m); put(a, C_I1, score, i1); put(a, C_D1, score, d1);
     if (two) { put(a, C_I2, score, i2); put(a, C_D2, score, d2); }
@@ -1292,6 +1351,26 @@ static void compute_affine_step(wfo_aligner* a, int score) {
     if (i2) trim_ends(plen, tlen, i2);
     if (d2) trim_ends(plen, tlen, d2);
     dominance_prune_step(a, score);
+}
+
+/* compute_edit_idm / compute_indel_idm (src/compute/edit.rs:104-118,
+ * 218-231): edit M = max(prev[k+1], max(prev[k-1], prev[k]) + 1); indel
+ * M = max(prev[k+1], prev[k-1] + 1); NULL when out of the sequences. */
+static void edit_kernel(int plen, int tlen, const wf_t* prev, wf_t* cur, int lo, int hi,
+                        int indel) {
+    for (int k = lo; k <= hi; ++k) {
+        int32_t ins = wf_get(prev, k - 1), del = wf_get(prev, k + 1), mx;
+        if (indel) {
+            ins += 1;
+            mx = del > ins ? del : ins;
+        } else {
+            int32_t mis = wf_get(prev, k);
+            int32_t t = (ins > mis ? ins : mis) + 1;
+            mx = del > t ? del : t;
+        }
+        if ((uint32_t)mx > (uint32_t)tlen || (uint32_t)(mx - k) > (uint32_t)plen) mx = OFFSET_NULL;
+        wf_set(cur, k, mx);
+    }
 }
 
 /* compute_edit_step (src/aligner.rs:1769-1865) with compute_edit_idm /
@@ -1353,19 +1432,7 @@ static void compute_edit_step(wfo_aligner* a, int score) {
         if (score % 16 == 0)
             offload_selective(a, cur, lo, hi, a->low ? PC_FULL_MASK : PC_HALF_FULL_MASK);
     } else
-    for (int k = lo; k <= hi; ++k) {
-        int32_t ins = wf_get(prev, k - 1), del = wf_get(prev, k + 1), mx;
-        if (indel) {
-            ins += 1;
-            mx = del > ins ? del : ins;
-        } else {
-            int32_t mis = wf_get(prev, k);
-            int32_t t = (ins > mis ? ins : mis) + 1;
-            mx = del > t ? del : t;
-        }
-        if ((uint32_t)mx > (uint32_t)tlen || (uint32_t)(mx - k) > (uint32_t)plen) mx = OFFSET_NULL;
-        wf_set(cur, k, mx);
-    }
+        edit_kernel(plen, tlen, prev, cur, lo, hi, indel);
     put(a, C_M, score, cur);
     trim_ends(plen, tlen, cur);
 }
@@ -2673,4 +2740,145 @@ int32_t wfo_align_batch(const wfa_attr_t* attr, int32_t n, int32_t seq_format,
     free(jobs);
     free(th);
     return err;
+}
+
+/* ------------------------------------------------------------------ */
+/* Test hooks: the step primitives above on caller arrays, so the      */
+/* reference's unit literals (cells, heuristic decisions, breakpoints) */
+/* can be checked against them directly. A row is `n` cells for the    */
+/* diagonals [base, base + n - 1], with [lo, hi] its range and every   */
+/* allocated cell readable.                                            */
+/* ------------------------------------------------------------------ */
+static wf_t row_view(int32_t* mem, int base, int n, int lo, int hi) {
+    wf_t w;
+    memset(&w, 0, sizeof(w));
+    w.mem = mem;
+    w.base = base;
+    w.nalloc = n;
+    w.lo = lo;
+    w.hi = hi;
+    w.init_min = base;
+    w.init_max = base + n - 1;
+    return w;
+}
+
+int32_t wfo_t_wf_distance(int32_t off, int32_t k, int32_t plen, int32_t tlen) {
+    return wf_distance(off, k, plen, tlen);
+}
+
+void wfo_t_wfadaptive_reduce(int32_t* lohi, const int32_t* d, int32_t dbase, int32_t min_d,
+                             int32_t thr, int32_t min_k, int32_t max_k) {
+    wf_t w = row_view(NULL, 0, 0, lohi[0], lohi[1]);
+    wfadaptive_reduce(&w, d, dbase, min_d, thr, min_k, max_k);
+    lohi[0] = w.lo;
+    lohi[1] = w.hi;
+}
+
+void wfo_t_wfadaptive_cutoff(int32_t* st, int32_t* mem, int32_t base, int32_t n, int32_t* lohi,
+                             int32_t* d, int32_t plen, int32_t tlen, int32_t min_len,
+                             int32_t thr, int32_t steps) {
+    wf_t w = row_view(mem, base, n, lohi[0], lohi[1]);
+    wfadaptive_cutoff((hstate_t*)st, &w, d, base, plen, tlen, min_len, thr, steps);
+    lohi[0] = w.lo;
+    lohi[1] = w.hi;
+}
+
+void wfo_t_banded_static(int32_t* lohi, int32_t min_k, int32_t max_k) {
+    wf_t w = row_view(NULL, 0, 0, lohi[0], lohi[1]);
+    banded_static_cutoff(&w, min_k, max_k);
+    lohi[0] = w.lo;
+    lohi[1] = w.hi;
+}
+
+void wfo_t_banded_adaptive(int32_t* st, int32_t* mem, int32_t base, int32_t n, int32_t* lohi,
+                           int32_t plen, int32_t tlen, int32_t min_k, int32_t max_k,
+                           int32_t steps) {
+    wf_t w = row_view(mem, base, n, lohi[0], lohi[1]);
+    banded_adaptive_cutoff((hstate_t*)st, &w, plen, tlen, min_k, max_k, steps);
+    lohi[0] = w.lo;
+    lohi[1] = w.hi;
+}
+
+void wfo_t_hstate_reset(int32_t* st, int32_t heuristic, const int32_t* hp) {
+    hstate_reset((hstate_t*)st, heuristic, hp);
+}
+
+/* drop = {match, endsfree, pattern_begin_free, text_begin_free, scope};
+ * out = {max, k, offset} */
+void wfo_t_compute_sw_scores(int32_t* mem, int32_t base, int32_t n, int32_t lo, int32_t hi,
+                             int32_t* sw, const int32_t* drop, int32_t score, int32_t* out) {
+    wf_t w = row_view(mem, base, n, lo, hi);
+    drop_t dp = drop_params(drop[0], drop[1], drop[2], drop[3], drop[4]);
+    compute_sw_scores(&w, sw, base, &dp, score, &out[0], &out[1], &out[2]);
+}
+
+void wfo_t_xdrop_cutoff(int32_t* st, int32_t* mem, int32_t base, int32_t n, int32_t* lohi,
+                        int32_t* sw, const int32_t* drop, int32_t score, int32_t xdrop,
+                        int32_t steps) {
+    wf_t w = row_view(mem, base, n, lohi[0], lohi[1]);
+    drop_t dp = drop_params(drop[0], drop[1], drop[2], drop[3], drop[4]);
+    xdrop_cutoff((hstate_t*)st, &w, sw, base, &dp, score, xdrop, steps);
+    lohi[0] = w.lo;
+    lohi[1] = w.hi;
+}
+
+int32_t wfo_t_zdrop_cutoff(int32_t* st, int32_t* mem, int32_t base, int32_t n, int32_t* lohi,
+                           int32_t* sw, const int32_t* drop, int32_t score, int32_t zdrop,
+                           int32_t steps) {
+    wf_t w = row_view(mem, base, n, lohi[0], lohi[1]);
+    drop_t dp = drop_params(drop[0], drop[1], drop[2], drop[3], drop[4]);
+    int r = zdrop_cutoff((hstate_t*)st, &w, sw, base, &dp, score, zdrop, steps);
+    lohi[0] = w.lo;
+    lohi[1] = w.hi;
+    return r;
+}
+
+/* rows = {Mx, Mo1, I1e, D1e, Mo2, I2e, D2e} inputs (NULL = absent),
+ * then {M, I1, D1, I2, D2} outputs, all over the same n diagonals from
+ * `base`; cells [lo, hi] are computed */
+void wfo_t_affine_cells(int32_t plen, int32_t tlen, int32_t* const* rows, int32_t base,
+                        int32_t n, int32_t lo, int32_t hi, int32_t use2) {
+    wf_t v[12];
+    wf_t* p[12];
+    for (int i = 0; i < 12; ++i) {
+        v[i] = row_view(rows[i], base, n, lo, hi);
+        p[i] = rows[i] ? &v[i] : NULL;
+    }
+    affine_kernel(plen, tlen, p[0], p[1], p[2], p[3], p[4], p[5], p[6], use2, p[7], p[8], p[9],
+                  p[10], p[11], lo, hi);
+}
+
+void wfo_t_edit_cells(int32_t plen, int32_t tlen, int32_t* prev, int32_t* cur, int32_t base,
+                      int32_t n, int32_t lo, int32_t hi, int32_t indel) {
+    wf_t vp = row_view(prev, base, n, lo, hi), vc = row_view(cur, base, n, lo, hi);
+    edit_kernel(plen, tlen, &vp, &vc, lo, hi, indel);
+}
+
+/* One breakpoint scan of a forward row (w0) against a reverse row (w1);
+ * bp (in/out) = {score, score_forward, score_reverse, k_forward,
+ * k_reverse, offset_forward, offset_reverse, component}: pass score =
+ * INT32_MAX for none found yet */
+void wfo_t_breakpoint_scan(int32_t* m0, int32_t base0, int32_t n0, int32_t lo0, int32_t hi0,
+                           int32_t* m1, int32_t base1, int32_t n1, int32_t lo1, int32_t hi1,
+                           int32_t plen, int32_t tlen, int32_t s0, int32_t s1, int32_t gap_open,
+                           int32_t comp, int32_t fwd, int32_t* bp) {
+    wf_t w0 = row_view(m0, base0, n0, lo0, hi0), w1 = row_view(m1, base1, n1, lo1, hi1);
+    bi_bp_t b;
+    b.score = bp[0]; b.score_forward = bp[1]; b.score_reverse = bp[2];
+    b.k_forward = bp[3]; b.k_reverse = bp[4]; b.offset_forward = bp[5];
+    b.offset_reverse = bp[6]; b.component = bp[7];
+    bi_breakpoint_scan(&w0, &w1, plen, tlen, s0, s1, gap_open, comp, fwd, &b);
+    bp[0] = b.score; bp[1] = b.score_forward; bp[2] = b.score_reverse;
+    bp[3] = b.k_forward; bp[4] = b.k_reverse; bp[5] = b.offset_forward;
+    bp[6] = b.offset_reverse; bp[7] = b.component;
+}
+
+/* lohi (in/out) = {lo, hi, null} of the I/D row; src = {lo, hi} of M */
+void wfo_t_heuristic_equate(int32_t* dst, const int32_t* src) {
+    wf_t d = row_view(NULL, 0, 0, dst[0], dst[1]), m = row_view(NULL, 0, 0, src[0], src[1]);
+    d.null = dst[2];
+    heuristic_equate(&d, &m);
+    dst[0] = d.lo;
+    dst[1] = d.hi;
+    dst[2] = d.null;
 }

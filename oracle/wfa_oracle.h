@@ -79,6 +79,42 @@ int32_t wfo_align_batch(const wfa_attr_t* attr, int32_t n, int32_t seq_format,
                         uint8_t* ops, const int64_t* ops_off, int32_t* ops_len,
                         int32_t nthreads);
 
+
+/* Test hooks: the step primitives on caller arrays (tests/ only), so the
+ * reference's unit literals can be checked against them. A row is n cells
+ * for the diagonals [base, base + n - 1]; lohi = {lo, hi} in/out; st = the
+ * six HeuristicState fields {steps_wait, max_sw_score, max_sw_k,
+ * max_sw_off, max_wf_score, zlast}; drop = {match, endsfree,
+ * pattern_begin_free, text_begin_free, scope}. */
+int32_t wfo_t_wf_distance(int32_t off, int32_t k, int32_t plen, int32_t tlen);
+void wfo_t_wfadaptive_reduce(int32_t* lohi, const int32_t* d, int32_t dbase, int32_t min_d,
+                             int32_t thr, int32_t min_k, int32_t max_k);
+void wfo_t_wfadaptive_cutoff(int32_t* st, int32_t* mem, int32_t base, int32_t n, int32_t* lohi,
+                             int32_t* d, int32_t plen, int32_t tlen, int32_t min_len,
+                             int32_t thr, int32_t steps);
+void wfo_t_banded_static(int32_t* lohi, int32_t min_k, int32_t max_k);
+void wfo_t_banded_adaptive(int32_t* st, int32_t* mem, int32_t base, int32_t n, int32_t* lohi,
+                           int32_t plen, int32_t tlen, int32_t min_k, int32_t max_k,
+                           int32_t steps);
+void wfo_t_hstate_reset(int32_t* st, int32_t heuristic, const int32_t* hp);
+void wfo_t_compute_sw_scores(int32_t* mem, int32_t base, int32_t n, int32_t lo, int32_t hi,
+                             int32_t* sw, const int32_t* drop, int32_t score, int32_t* out);
+void wfo_t_xdrop_cutoff(int32_t* st, int32_t* mem, int32_t base, int32_t n, int32_t* lohi,
+                        int32_t* sw, const int32_t* drop, int32_t score, int32_t xdrop,
+                        int32_t steps);
+int32_t wfo_t_zdrop_cutoff(int32_t* st, int32_t* mem, int32_t base, int32_t n, int32_t* lohi,
+                           int32_t* sw, const int32_t* drop, int32_t score, int32_t zdrop,
+                           int32_t steps);
+void wfo_t_heuristic_equate(int32_t* dst, const int32_t* src);
+void wfo_t_affine_cells(int32_t plen, int32_t tlen, int32_t* const* rows, int32_t base,
+                        int32_t n, int32_t lo, int32_t hi, int32_t use2);
+void wfo_t_edit_cells(int32_t plen, int32_t tlen, int32_t* prev, int32_t* cur, int32_t base,
+                      int32_t n, int32_t lo, int32_t hi, int32_t indel);
+void wfo_t_breakpoint_scan(int32_t* m0, int32_t base0, int32_t n0, int32_t lo0, int32_t hi0,
+                           int32_t* m1, int32_t base1, int32_t n1, int32_t lo1, int32_t hi1,
+                           int32_t plen, int32_t tlen, int32_t s0, int32_t s1, int32_t gap_open,
+                           int32_t comp, int32_t fwd, int32_t* bp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -126,17 +126,21 @@ def test_biwfa_recursion_optimal_and_valid(metric, pen, pkg, N, oracle):
     assert split >= 4, "the pairs must exercise the breakpoint search"
 
 
-def test_biwfa_rejects_ends_free_and_xdrop(N, oracle):
-    """The driver's BiWFA checks (src/bin/align_benchmark.rs:529-560)."""
+def test_biwfa_rejects_ends_free_and_zdrop(N, oracle):
+    """BiWFA is end-to-end, and Z-drop is refused with it (C since 7110ee3,
+    and the driver's checks, src/bin/align_benchmark.rs:529-560); X-drop
+    sub-alignments run (tests/xdrop_steps.rs:57-92)."""
     attr = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment", span="extension")
     attr.memory_mode = N.MEMORY_ULTRALOW
     with pytest.raises(ValueError):
         oracle.OracleAligner(attr)
     attr = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment")
     attr.memory_mode = N.MEMORY_ULTRALOW
-    attr.heuristic = N.HEURISTIC_XDROP
+    attr.heuristic = N.HEURISTIC_ZDROP
     with pytest.raises(ValueError):
         oracle.OracleAligner(attr)
+    attr.heuristic = N.HEURISTIC_XDROP
+    oracle.OracleAligner(attr)
 
 
 def _ultralong_fixture():
@@ -178,3 +182,24 @@ def test_ultralong_fixture_first_pair_on_oracle(pkg, N, oracle):
     s, st, ops = oracle.OracleAligner(a).align(pairs.pattern(0), pairs.text(0))
     assert (s, st, len(ops)) == (r["score"], r["status"], r["cigar_len"])
     assert hashlib.sha256(ops).hexdigest() == r["sha256"]
+
+
+@pytest.mark.parametrize("metric,pen", [BIWFA_PENALTIES[2], BIWFA_PENALTIES[3], BIWFA_PENALTIES[4],
+                                        BIWFA_PENALTIES[7]])
+def test_biwfa_xdrop_reports_cigar_score(metric, pen, pkg, N, oracle):
+    """X-drop sub-alignments in BiWFA (tests/xdrop_steps.rs:57-92 runs
+    align_biwfa with X-drop): a completed alignment has a valid CIGAR and
+    reports that CIGAR's score (C 5713de8, src/aligner.rs:2607-2616);
+    score-only BiWFA ends with the same status."""
+    pairs = _recursive_pairs(12, seed=3)
+    bi = _biwfa(N, oracle, metric, pen, heuristic=(N.HEURISTIC_XDROP, 50, 1))
+    done = 0
+    for p, t in pairs:
+        score, status, ops = bi.align(p, t)
+        if status != N.STATUS_COMPLETED:
+            continue
+        cig = pkg.Cigar(ops)
+        assert cig.check_alignment(p, t) is None
+        assert _cigar_wf_score(pkg, metric, pen, cig, len(p), len(t)) == score
+        done += 1
+    assert done >= 6
