@@ -323,6 +323,53 @@ int32_t wfa_engine_align_batch_per_pair(wfa_engine_t* engine, const wfa_attr_t* 
  * engine's own limit. */
 int32_t wfa_engine_set_scratch_budget(wfa_engine_t* engine, int64_t bytes);
 
+/* ---- Launch shapes of long-read launches and their autotuner ----
+ * No counterpart in the reference (its aligner runs one pair on one core).
+ * A long-read launch (sequences over 1024 bases, no heuristic, not BiWFA)
+ * gives each pair a workgroup of several waves whose wavefront rings live
+ * in HBM; how many waves per pair (4, 8 or 16; 4 and 16 for score-only and
+ * medium/low-memory launches of pairs up to 16 kbp) and how many
+ * workgroups per CU run together decide how the rings share the caches.
+ * The engine keeps one shape per (metric, scope and memory mode, length
+ * bucket = ceil(log2(longest sequence))); without one it uses a fixed
+ * plan (8 waves, as many workgroups as fit). Results never depend on the
+ * shape. */
+#define WFA_TUNE_MAX_TRIALS 16
+typedef struct {
+    int32_t waves_per_pair;
+    int32_t groups_per_cu;
+    float ms;                    /* the trial launch on the sample (HIP events) */
+} wfa_tune_trial_t;
+typedef struct {
+    int32_t sample_pairs;        /* pairs each trial aligned */
+    int32_t num_trials;
+    wfa_tune_trial_t trials[WFA_TUNE_MAX_TRIALS];
+    int32_t waves_per_pair;      /* the shape kept (0: nothing to tune) */
+    int32_t groups_per_cu;
+    int32_t default_waves_per_pair;  /* the fixed plan's shape */
+    int32_t default_groups_per_cu;
+} wfa_tune_report_t;
+
+/* Time the candidate shapes for `batch` under `attr` on a strided sample of
+ * `sample_pairs` of its pairs (0: 16 per CU), longest first as a batch
+ * runs, on `hip_stream`, and keep the fastest in the engine for the
+ * batch's key. The trials write the sampled pairs' results into the
+ * batch's outputs (a later wfa_engine_align_batch rewrites them). Batches
+ * with nothing to choose (short reads, heuristics, BiWFA) return WFA_OK
+ * with waves_per_pair 0. `report` may be NULL. */
+int32_t wfa_engine_autotune(wfa_engine_t* engine, const wfa_attr_t* attr,
+                            const wfa_batch_t* batch, void* hip_stream, int32_t sample_pairs,
+                            wfa_tune_report_t* report);
+/* Set (or, with 0 and 0, clear) the shape for launches under `attr` whose
+ * longest sequence falls in the bucket of `max_length`. */
+int32_t wfa_engine_set_launch_shape(wfa_engine_t* engine, const wfa_attr_t* attr,
+                                    int32_t max_length, int32_t waves_per_pair,
+                                    int32_t groups_per_cu);
+/* The shape kept for that key, 0 and 0 when the fixed plan applies. */
+int32_t wfa_engine_get_launch_shape(wfa_engine_t* engine, const wfa_attr_t* attr,
+                                    int32_t max_length, int32_t* waves_per_pair,
+                                    int32_t* groups_per_cu);
+
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code that must not call hipMalloc, such as tight serving loops
  * or harnesses that forbid it). Aligns `batch` exactly as

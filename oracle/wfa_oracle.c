@@ -1628,7 +1628,7 @@ static void backtrace_singletrack(wfo_aligner* a, int score, int k, int offset) 
 #define BT_I1_OPEN 1
 #define BT_D1_OPEN 5
 #define BT_M 9
-static int64_t bt_pack(int32_t off, int64_t t) { return ((int64_t)off << 4) | t; }
+static int64_t bt_pack(int32_t off, int64_t t) { return (int64_t)off * 16 + t; }
 static int64_t bt_misms(const wfo_aligner* a, int score, int k) {
     if (score < 0) return OFFSET_NULL;
     const wf_t* w = get(a, C_M, score);
@@ -1933,7 +1933,10 @@ static int rescore_heuristic_cigar(wfo_aligner* a, int score, int end_v, int end
     int b = a->cig_begin, e = a->cig_end;
     if (a->endsfree) {
         int trailing = (a->plen - end_v) + (a->tlen - end_h);
-        e = e - trailing < 0 ? 0 : e - trailing;
+        /* end_offset.saturating_sub(trailing as usize): a negative trailing
+         * count (an end beyond the sequences, after free begin gaps longer
+         * than them) wraps to a huge one and leaves nothing */
+        e = trailing < 0 || e - trailing < 0 ? 0 : e - trailing;
         uint8_t op = b < a->cig_end ? a->cig[b] : 0;
         int fr = op == 'I' ? a->tb_free : op == 'D' ? a->pb_free : 0;
         int len = 0;

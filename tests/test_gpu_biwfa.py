@@ -66,6 +66,23 @@ def test_biwfa_heuristics_match_oracle(h, metric, pen, pkg, N, engine, oracle):
                  label=f"biwfa/{metric}/{h}/{scope}")
 
 
+@pytest.mark.parametrize("h", [(2, 50, 1, 0), (2, 20, 2, 0), (2, 200, 1, 0)],
+                         ids=["xdrop_50_1", "xdrop_20_2", "xdrop_200_1"])
+@pytest.mark.parametrize("metric,pen", [BIWFA_PENALTIES[2], BIWFA_PENALTIES[3],
+                                        BIWFA_PENALTIES[4], BIWFA_PENALTIES[7]])
+def test_biwfa_xdrop_matches_oracle(h, metric, pen, pkg, N, engine, oracle):
+    """X-drop sub-alignments (tests/xdrop_steps.rs:57-92 runs align_biwfa
+    with X-drop), with the pair of tests/data/issue21_xdrop.seq riding
+    along."""
+    from wfa2lib_rs_amd.synth import from_ascii
+    from test_oracle_heuristics import _issue21_pair
+    ps = _recursive_pairs(12, seed=13) + [_issue21_pair()]
+    pairs = from_ascii([p for p, _ in ps], [t for _, t in ps], N.SEQ_ASCII)
+    for scope in ("score", "alignment"):
+        _compare(engine, oracle, _attr(N, metric, pen, scope, h), pairs, scope == "alignment",
+                 label=f"biwfa/{metric}/{h}/{scope}")
+
+
 def test_biwfa_wide_rows_take_the_retry_path(pkg, N, engine, oracle):
     """Unrelated 40 kbp sequences: rows wider than the first launch's ring."""
     from wfa2lib_rs_amd.synth import from_ascii

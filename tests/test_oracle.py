@@ -627,3 +627,36 @@ def test_oracle_zdrop_issue22_c_med_low_literal(pkg, N, oracle):
         assert pkg.Cigar(ops).score_gap_affine(pkg.AffinePenalties(-3, 5, 3, 2)) == 4502
         assert A.end_position() == high.end_position()
         assert ops != hops
+
+
+def test_oracle_sentinel_bytes_are_ordinary_symbols(pkg, N, oracle):
+    """Sequence ends are hard bounds here; the reference pads the pattern
+    with '!' and the text with '?' (src/sequences.rs:16-19, 178-191), so a
+    text holding '!' or a pattern holding '?' can let its extension run past
+    a sequence end. In the oracle those bytes are ordinary symbols: pairs
+    with '!' and '?' at and near both ends score the exact gap-affine
+    optimum, with valid CIGARs."""
+    from wfa2lib_rs_amd.synth import from_ascii
+    rng = np.random.default_rng(11)
+    alpha = np.frombuffer(b"ACGT!?", dtype=np.uint8)
+    pats, txts = [], []
+    for i in range(200):
+        n = int(rng.integers(1, 120))
+        p = rng.choice(alpha, n).tobytes()
+        t = bytearray(p)
+        for _ in range(int(rng.integers(0, 6))):
+            j = int(rng.integers(0, len(t) + 1))
+            t[j:j] = rng.choice(alpha, 1).tobytes()
+        t = bytes(t)
+        if i % 3 == 0:
+            p, t = p + b"?", t + b"!"
+        elif i % 3 == 1:
+            p, t = p + b"!" * 3, t + b"?" * 2
+        pats.append(p)
+        txts.append(t)
+    attr = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), "alignment")
+    s, st, c = oracle.align_batch(attr, from_ascii(pats, txts, N.SEQ_ASCII), cigar=True,
+                                  threads=8)
+    for i in range(len(pats)):
+        assert s[i] == oracle.dp_affine(pats[i], txts[i], 4, 6, 2)
+        assert pkg.Cigar(c[i]).check_alignment(pats[i], txts[i]) is None

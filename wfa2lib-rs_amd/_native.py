@@ -9,7 +9,10 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
-LIB_PATH = os.path.join(LIB_DIR, "libwfa_mi355x.so")
+# WFA_MI355X_LIB names another engine build in lib/ (libwfa_mi355x_checks.so:
+# the debug build with device index checks, build.py --checks)
+LIB_PATH = os.path.join(LIB_DIR, os.path.basename(os.environ.get("WFA_MI355X_LIB",
+                                                                 "libwfa_mi355x.so")))
 SYNTH_PATH = os.path.join(LIB_DIR, "libwfa_synth.so")
 
 # ---- constants (same values as include/wfa_mi355x.h) ----
@@ -93,6 +96,21 @@ class SynthCfg(ctypes.Structure):
                 ("run_mean", ctypes.c_double)]
 
 
+TUNE_MAX_TRIALS = 16
+
+
+class TuneTrial(ctypes.Structure):
+    _fields_ = [("waves_per_pair", ctypes.c_int32), ("groups_per_cu", ctypes.c_int32),
+                ("ms", ctypes.c_float)]
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("sample_pairs", ctypes.c_int32), ("num_trials", ctypes.c_int32),
+                ("trials", TuneTrial * TUNE_MAX_TRIALS), ("waves_per_pair", ctypes.c_int32),
+                ("groups_per_cu", ctypes.c_int32), ("default_waves_per_pair", ctypes.c_int32),
+                ("default_groups_per_cu", ctypes.c_int32)]
+
+
 # Every entry point the header declares, with its ctypes signature.
 _P, _I32, _VP = ctypes.POINTER, ctypes.c_int32, ctypes.c_void_p
 SIGNATURES = {
@@ -125,6 +143,9 @@ SIGNATURES = {
     "wfa_engine_align_batch": (_I32, [_VP, _P(Attr), _P(Batch), _VP]),
     "wfa_engine_align_batch_per_pair": (_I32, [_VP, _P(Attr), _P(Batch), _VP]),
     "wfa_engine_set_scratch_budget": (_I32, [_VP, ctypes.c_int64]),
+    "wfa_engine_autotune": (_I32, [_VP, _P(Attr), _P(Batch), _VP, _I32, _P(TuneReport)]),
+    "wfa_engine_set_launch_shape": (_I32, [_VP, _P(Attr), _I32, _I32, _I32]),
+    "wfa_engine_get_launch_shape": (_I32, [_VP, _P(Attr), _I32, _P(_I32), _P(_I32)]),
     "wfa_align_batch_noalloc": (_I32, [_P(Attr), _P(Batch), _VP]),
     "wfa_engine_last_retries": (_I32, [_VP]),
     "wfa_engine_last_kernel_ms": (_I32, [_VP, _P(ctypes.c_float), _P(ctypes.c_float)]),

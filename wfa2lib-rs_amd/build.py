@@ -1,9 +1,14 @@
 """Build the engine (gfx950) and the synthetic-workload generator in-tree.
 
-    python wfa2lib-rs_amd/build.py
+    python wfa2lib-rs_amd/build.py            # the release engine and the generator
+    python wfa2lib-rs_amd/build.py --checks   # the debug engine (device index checks)
 
 ``libwfa_mi355x.so`` is built with hipcc for gfx950 from csrc/*.hip and
 csrc/*.cpp; ``libwfa_synth.so`` with gcc. Outputs go to ``lib/``.
+``--checks`` builds ``libwfa_mi355x_checks.so`` with WFA_DEVICE_CHECKS: every
+ring and history index is tested on the device and a call that found one
+out of range fails with WFA_ERR_DEVICE; set WFA_MI355X_LIB to its file name
+to load it in place of the release library.
 """
 import os
 import subprocess
@@ -32,14 +37,15 @@ def _run(cmd):
         raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
 
 
-def build_engine():
+def build_engine(checks=False):
     """Compile each .hip translation unit to an object in parallel (the
     per-kind kernel sets are the slow part), then link with the host API."""
     os.makedirs(LIB, exist_ok=True)
-    obj_dir = os.path.join(LIB, "obj")
+    obj_dir = os.path.join(LIB, "obj_checks" if checks else "obj")
     os.makedirs(obj_dir, exist_ok=True)
     hips = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
-    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    flags = HIP_FLAGS + (["-DWFA_DEVICE_CHECKS"] if checks else [])
+    compile_flags = [f for f in flags if f != "-shared"]
     procs = []
     for f in hips:
         obj = os.path.join(obj_dir, f[:-4] + ".o")
@@ -58,9 +64,9 @@ def build_engine():
         objs.append(obj)
     if failed:
         raise RuntimeError("build step failed: " + "; ".join(failed))
-    out = os.path.join(LIB, "libwfa_mi355x.so")
+    out = os.path.join(LIB, "libwfa_mi355x_checks.so" if checks else "libwfa_mi355x.so")
     tmp = out + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + objs +
+    _run([HIPCC] + flags + ["-o", tmp] + objs +
          [os.path.join(CSRC, f) for f in ("wfa_api.cpp", "wfa_biwfa.cpp", "wfa_server.cpp")])
     os.replace(tmp, out)
     return out
@@ -81,5 +87,8 @@ def build_all():
 
 
 if __name__ == "__main__":
-    build_all()
+    if "--checks" in sys.argv[1:]:
+        build_engine(checks=True)
+    else:
+        build_all()
     sys.exit(0)

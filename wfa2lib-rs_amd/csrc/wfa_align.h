@@ -134,6 +134,38 @@ static __device__ __forceinline__ int backward_matches(const uint64_t* P, const 
 // extension skips it), so int16 keeps valid offsets exactly and saturates
 // the NULL-derived ones at -16384, where adding the few +1s of later steps
 // keeps them negative.
+// Debug builds (WFA_DEVICE_CHECKS: wfa2lib-rs_amd/build.py --checks) test
+// every ring and history index against its buffer. An index outside it is
+// counted in the engine's counters[kCheckCounter] and replaced by 0 (the
+// access stays inside the buffer), and the host reports a call with a
+// nonzero count as WFA_ERR_DEVICE. Release builds carry no test.
+template <typename T>
+static __device__ __forceinline__ int64_t checked_index(int64_t i, int64_t n, int32_t* chk) {
+#ifdef WFA_DEVICE_CHECKS
+  if ((uint64_t)i >= (uint64_t)n) {
+    atomicAdd(chk + kCheckCounter, 1);
+    return 0;
+  }
+#else
+  (void)n;
+  (void)chk;
+#endif
+  return i;
+}
+
+// A history buffer (pool, headers, signs): element access, tested against
+// the buffer's size in WFA_DEVICE_CHECKS builds.
+template <typename T>
+struct HBuf {
+  T* p = nullptr;
+  int64_t n = 0;
+  int32_t* chk = nullptr;
+  __device__ __forceinline__ T& operator[](int64_t i) const {
+    return p[checked_index<T>(i, n, chk)];
+  }
+  __device__ __forceinline__ explicit operator bool() const { return p != nullptr; }
+};
+
 template <typename RT>
 struct Rows {
   RT* ring;       // LDS or global
@@ -144,10 +176,15 @@ struct Rows {
   int4* meta;
   int mask;       // wcap - 1
   int wcap;
-  __device__ __forceinline__ int at(int idx) const { return (int)ring[idx]; }
+  int cap = 0;            // cells in the ring (tested in WFA_DEVICE_CHECKS builds)
+  int32_t* chk = nullptr;  // the launch's counters
+  __device__ __forceinline__ int at(int idx) const {
+    return (int)ring[checked_index<RT>(idx, cap, chk)];
+  }
   __device__ __forceinline__ void set(int idx, int v) const {
-    if constexpr (sizeof(RT) == 2) ring[idx] = (RT)imax(v, -16384);
-    else ring[idx] = v;
+    const int64_t i = checked_index<RT>(idx, cap, chk);
+    if constexpr (sizeof(RT) == 2) ring[i] = (RT)imax(v, -16384);
+    else ring[i] = v;
   }
 };
 
@@ -260,12 +297,12 @@ static __device__ int stage_sequences(const KParams& p, int pair, int plen, int 
 // (high memory) or winner flags (medium/low memory).
 // ---------------------------------------------------------------------
 struct Hist {
-  int4* hdr;
-  int32_t* cells;   // high memory: the pool as extended M offsets
-  uint8_t* bytes;   // medium/low memory: the pool as winner flags per (score, k)
+  HBuf<int4> hdr;
+  HBuf<int32_t> cells;   // high memory: the pool as extended M offsets
+  HBuf<uint8_t> bytes;   // medium/low memory: the pool as winner flags per (score, k)
   uint8_t* chain;   // medium/low memory: op chain scratch
   int chain_cap;
-  uint8_t* signs;   // Compat::C Z-drop: per pool cell, the signs of its offsets (or null)
+  HBuf<uint8_t> signs;   // Compat::C Z-drop: per pool cell, the signs of its offsets (or null)
   int hdr_cap;
   int32_t* tab;     // this slot's chunk ids
   int tab_cap;
@@ -438,11 +475,11 @@ static __device__ int backtrace_linear(const KParams& p, const Hist& H, int nsco
     int64_t mis = WFA_NULL, ins = WFA_NULL, del = WFA_NULL;
     int c;
     // bt_misms: init_min <= k <= init_max
-    if (!indel && cell(mis_s, k, 0, &c)) mis = ((int64_t)(c + 1) << 4) | 9;
+    if (!indel && cell(mis_s, k, 0, &c)) mis = (int64_t)(c + 1) * 16 + 9;
     // bt_ins1_open: init_min < k && k-1 <= init_max  -> cell k-1 in range
-    if (cell(gap_s, k - 1, 0, &c)) ins = ((int64_t)(c + 1) << 4) | 1;
+    if (cell(gap_s, k - 1, 0, &c)) ins = (int64_t)(c + 1) * 16 + 1;
     // bt_del1_open: init_min <= k+1 && k < init_max -> cell k+1 in range
-    if (cell(gap_s, k + 1, 0, &c)) del = ((int64_t)c << 4) | 5;
+    if (cell(gap_s, k + 1, 0, &c)) del = (int64_t)c * 16 + 5;
     int64_t mx = ins > del ? ins : del;
     if (mis > mx) mx = mis;
     if (mx < 0) break;
@@ -544,7 +581,10 @@ static __device__ int rescore_cigar(const KParams& p, const uint8_t* ops, int n,
   const int lane = threadIdx.x;
   int b = 0, e = n;
   if (ef) {
-    e = imax(0, e - trailing);
+    // end_offset.saturating_sub(trailing as usize): a negative count (an
+    // end beyond the sequences, after free begin gaps longer than them)
+    // wraps to a huge one and leaves nothing
+    e = trailing < 0 ? 0 : imax(0, e - trailing);
     const int op = n > 0 ? ops[0] : 0;
     const int fr = op == 'I' ? tb_free : op == 'D' ? pb_free : 0;
     while (b < fr && b < e && ops[b] == op) ++b;
@@ -1715,6 +1755,14 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     // winner flags of diagonal k at fbase + (k - lo)
     const int64_t fbase = MED ? hist_row(H, hrow) + (lo - wlo) : 0;
 
+    // Several waves, end to end, score only or medium/low memory: each lane
+    // extends the M cell it computes, and the row is written once (the
+    // extension of a cell reads only that cell and the sequences; the
+    // trims test the unextended offsets, which the extension keeps within
+    // the sequences). The end test is the end diagonal's cell.
+    constexpr bool FUSE = NW > 1 && MODE != 1;
+    const bool fuse = FUSE && !ef;
+    bool freach = false;
     // ---- compute all chunks; trims by ballot ----
     // U chunks at a time: their input reads are all issued before the
     // first chunk's outputs are written (outputs go to other ring rows), so
@@ -1851,6 +1899,12 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           gsg = (i1 >= 0) << 1 | (d1 >= 0) << 2 | (i2 >= 0) << 3 | (d2 >= 0) << 4;
         }
         vm = m != WFA_NULL;
+        if (fuse && vm) {
+          // the extension of M[s] fused into its computation (a valid cell
+          // is in the sequences, so m >= 0)
+          m += lcp<BITS>(sP, sT, m - k, m, plen, tlen);
+          if (k == k_end && m >= tlen) freach = true;
+        }
         R.set(rowM * R.wcap + (k & R.mask), m);
         if (MED) H.bytes[fbase + (k - lo)] = (uint8_t)fl;
         if (MED && zc) {
@@ -1908,13 +1962,20 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
         gather(kb + lane, in);
         chunk(kb, in);
       }
-      // every wave saw its own chunks: combine the trims
+      // every wave saw its own chunks: combine the trims (and, fused, the
+      // end test)
+      const bool wreach = FUSE && ballot(freach) != 0;
       if (lane == 0) {
         int* x = red + 10 * wave;
         x[0] = m_lo; x[1] = m_hi; x[2] = i_lo; x[3] = i_hi; x[4] = d_lo;
         x[5] = d_hi; x[6] = i2_lo; x[7] = i2_hi; x[8] = d2_lo; x[9] = d2_hi;
+        if (FUSE) red[10 * NW + 3 * wave] = wreach;
       }
       __syncthreads();
+      if (FUSE) {
+        freach = false;
+        for (int w = 0; w < NW; ++w) freach |= red[10 * NW + 3 * w] != 0;
+      }
       m_lo = i_lo = d_lo = i2_lo = d2_lo = INT32_MAX;
       m_hi = i_hi = d_hi = i2_hi = d2_hi = INT32_MIN;
       for (int w = 0; w < NW; ++w) {
@@ -2057,6 +2118,12 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       }
     }
     if (s >= p.max_steps) { status = WFA_STATUS_MAX_STEPS_REACHED; result = s; __syncthreads(); break; }
+    if (FUSE && fuse) {
+      // M[s] is extended already
+      __syncthreads();
+      if (freach) { status = WFA_STATUS_END_REACHED; result = s; break; }
+      continue;
+    }
     // ---- extend M[s] (the reference extends at the top of the next
     // iteration; nothing reads M[s] in between) ----
     // In high memory the stored row keeps its computed range: M cells
@@ -2173,7 +2240,8 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
 }
 
 template <int KIND, int MODE, bool LDS_RING, bool I16, bool PACKED, bool HEUR, int NW>
-__global__ void __launch_bounds__(64 * NW, LDS_RING ? 5 : 4) wfa_align_kernel(KParams p) {
+__global__ void __launch_bounds__(64 * NW, LDS_RING ? 5 : NW > 1 ? 32 / NW : 4)
+    wfa_align_kernel(KParams p) {
   using RT = typename std::conditional<I16, int16_t, int32_t>::type;
   constexpr bool CIG = MODE != 0;
   static_assert(NW == 1 || !LDS_RING, "several waves per pair only with rings in HBM");
@@ -2186,17 +2254,21 @@ __global__ void __launch_bounds__(64 * NW, LDS_RING ? 5 : 4) wfa_align_kernel(KP
   R.meta = meta;
   R.wcap = p.wcap;
   R.mask = p.wcap - 1;
+  R.cap = p.ring_rows * p.wcap;
+  R.chk = p.counters;
   if constexpr (LDS_RING) R.ring = (RT*)(meta + p.nrows_meta);
   else R.ring = (RT*)(p.gring + (int64_t)blockIdx.x * p.ring_rows * p.wcap);
   // cross-wave reductions (NW > 1): 16 ints per wave after the metadata
   int* red = LDS_RING ? nullptr : (int*)(meta + p.nrows_meta);
   Hist H;
-  H.hdr = CIG ? p.hdr + (int64_t)blockIdx.x * p.hdr_cap : nullptr;
-  H.cells = MODE == 1 ? p.hist : nullptr;
-  H.bytes = MODE == 2 ? (uint8_t*)p.hist : nullptr;
+  const int64_t pool = (int64_t)p.pool_chunks * p.chunk;
+  H.hdr = HBuf<int4>{CIG ? p.hdr + (int64_t)blockIdx.x * p.hdr_cap : nullptr, p.hdr_cap,
+                     p.counters};
+  H.cells = HBuf<int32_t>{MODE == 1 ? p.hist : nullptr, pool, p.counters};
+  H.bytes = HBuf<uint8_t>{MODE == 2 ? (uint8_t*)p.hist : nullptr, pool, p.counters};
   H.chain = MODE == 2 ? p.chain + (int64_t)blockIdx.x * p.chain_cap : nullptr;
   H.chain_cap = p.chain_cap;
-  H.signs = MODE == 2 ? p.zsign : nullptr;
+  H.signs = HBuf<uint8_t>{MODE == 2 ? p.zsign : nullptr, pool, p.counters};
   H.hdr_cap = p.hdr_cap;
   H.tab = CIG ? p.chunk_tab + (int64_t)blockIdx.x * p.tab_cap : nullptr;
   H.tab_cap = p.tab_cap;
@@ -2245,13 +2317,23 @@ __global__ void __launch_bounds__(64 * NW, LDS_RING ? 5 : 4) wfa_align_kernel(KP
 // ---------------------------------------------------------------------
 // Instantiation: every storage/format/heuristic variant of one kind
 // ---------------------------------------------------------------------
+template <int K, int M, int NW>
+static KernelFn pick_wide16(bool packed) {
+  return packed ? wfa_align_kernel<K, M, false, true, true, false, NW>
+                : wfa_align_kernel<K, M, false, true, false, false, NW>;
+}
+
 template <int K, int M, bool HEUR>
 static KernelFn pick_kmh(bool lds, bool i16, bool packed, int nw) {
   if (!lds) {
     if constexpr (!HEUR) {
-      if (nw > 1 && i16)
-        return packed ? wfa_align_kernel<K, M, false, true, true, false, kWideWaves>
-                      : wfa_align_kernel<K, M, false, true, false, false, kWideWaves>;
+      // the launch shapes the autotuner tries for long reads (16-bit rows,
+      // score only and medium/low memory): 4, 8 or 16 waves per pair
+      if constexpr (M != 1) {
+        if (nw == 4 && i16) return pick_wide16<K, M, 4>(packed);
+        if (nw == 16 && i16) return pick_wide16<K, M, 16>(packed);
+      }
+      if (nw > 1 && i16) return pick_wide16<K, M, kWideWaves>(packed);
       if (nw > 1)
         return packed ? wfa_align_kernel<K, M, false, false, true, false, kWideWaves>
                       : wfa_align_kernel<K, M, false, false, false, false, kWideWaves>;

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -55,6 +56,22 @@ struct LaunchCfg {
 // score's trimmed ranges for that backtrace.
 bool zdrop_ranges(const KParams& p, int mode) {
   return mode == 2 && p.heuristic != WFA_HEURISTIC_NONE && p.compat == WFA_COMPAT_C;
+}
+
+// The length bucket of a launch: ceil(log2(longest sequence))
+int length_bucket(int max_len) {
+  int b = 0;
+  while ((1 << b) < max_len && b < 30) ++b;
+  return b;
+}
+
+int shape_key(int kind, int mode, int max_len) {
+  return (kind * 4 + mode) * 64 + length_bucket(max_len);
+}
+
+// Waves per pair a wide kernel is built for (mode, 16-bit rows)
+bool wide_waves_built(int nw, int mode, bool i16) {
+  return nw == wfa::kWideWaves || (i16 && mode != 1 && (nw == 4 || nw == 16));
 }
 
 int ring_rows_of(const KParams& p, int kind) {
@@ -106,6 +123,16 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   // halve it (offsets stay below 2^15; a NULL row value rises by at most
   // one per score, and pairs whose score passes 16000 re-run with 32 bits)
   if (!c->lds_ring) c->i16 = c->nw > 1 && max_len <= 16000 && pass < 2 && !zdrop_ranges(p, mode);
+  // a tuned shape (wfa_engine_autotune) for long reads: waves per pair and
+  // workgroups per CU
+  int tuned_per_cu = 0;
+  if (long_reads && !c->lds_ring && c->nw > 1) {
+    auto it = e->shapes.find(shape_key(kind, mode, max_len));
+    if (it != e->shapes.end()) {
+      if (wide_waves_built(it->second.first, mode, c->i16)) c->nw = it->second.first;
+      tuned_per_cu = it->second.second;
+    }
+  }
   const size_t red_bytes = c->lds_ring ? 0 : (size_t)64 * c->nw;
   c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : red_bytes);
   if (c->lds_bytes > e->lds_max) return WFA_ERR_UNSUPPORTED;
@@ -154,6 +181,7 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   c->lds_bytes = seq_meta + (c->lds_ring ? ring_bytes : red_bytes);
   per_cu = wfa::max_blocks_per_cu(kind, mode, c->lds_ring, c->i16, fmt == WFA_SEQ_PACKED2BITS,
                                   wfa::heur_kernel(p), c->nw, c->lds_bytes);
+  if (tuned_per_cu > 0) per_cu = std::min(per_cu, tuned_per_cu);
   int grid = e->num_cus * per_cu;
   if (long_reads && e->long_grid_cap > 0) grid = std::min(grid, e->long_grid_cap);
   grid = std::max(1, std::min(grid, npairs));
@@ -324,14 +352,16 @@ int ensure_lists(wfa_engine_t* e, int n) {
     return WFA_OK;
   }
   if ((size_t)n <= e->list_cap) return WFA_OK;
-  for (auto& l : e->d_lists) {
-    if (l) hipFree(l);
-    l = nullptr;
+  const size_t cap = std::max<size_t>(n, 1024);
+  size_t have[2] = {e->list_cap * 4, e->list_cap * 4};
+  for (int i = 0; i < 2; ++i) {
+    int r = grow((void**)&e->d_lists[i], &have[i], cap * 4);
+    if (r) {
+      e->list_cap = 0;
+      return r;
+    }
   }
-  size_t cap = std::max<size_t>(n, 1024);
-  for (auto& l : e->d_lists)
-    if (hipMalloc((void**)&l, cap * 4) != hipSuccess) return WFA_ERR_OUT_OF_MEMORY;
-  e->list_cap = cap;
+  e->list_cap = std::min(have[0], have[1]) / 4;
   return WFA_OK;
 }
 
@@ -350,10 +380,36 @@ int validate_batch(const wfa_attr_t* attr, const wfa_batch_t* b) {
   return WFA_OK;
 }
 
-// One batch: first pass over all pairs (or `first..first+count` when
-// `per_pair`), then wider passes over the pairs that asked for retry.
+int align_batch_run(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
+                    hipStream_t st, int first, int count);
+
+// One batch (see align_batch_run). A WFA_DEVICE_CHECKS build counts the
+// out-of-range ring and history indexes of the call's launches and fails
+// the call when there were any.
 int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
                      hipStream_t st, int first, int count) {
+#ifdef WFA_DEVICE_CHECKS
+  HIPCHK(hipSetDevice(e->device));
+  HIPCHK(hipMemsetAsync(e->d_counters + wfa::kCheckCounter, 0, 4, st));
+  int r = align_batch_run(e, attr, b, st, first, count);
+  if (r) return r;
+  int32_t bad = 0;
+  r = read_counter(e, wfa::kCheckCounter, st, &bad);
+  if (r) return r;
+  if (bad) {
+    std::fprintf(stderr, "wfa_mi355x: %d ring/history index(es) out of range\n", bad);
+    return WFA_ERR_DEVICE;
+  }
+  return WFA_OK;
+#else
+  return align_batch_run(e, attr, b, st, first, count);
+#endif
+}
+
+// One batch: first pass over all pairs (or `first..first+count` when
+// `per_pair`), then wider passes over the pairs that asked for retry.
+int align_batch_run(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
+                    hipStream_t st, int first, int count) {
   Internal ip;
   int r = make_internal(attr->penalties, &ip);
   if (r) return r;
@@ -450,6 +506,117 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
   return WFA_OK;
 }
 
+// 0 score only, 1 high-memory CIGAR, 2 medium/low-memory CIGAR
+int mode_of(const wfa_attr_t& a) {
+  if (a.alignment_scope != WFA_SCOPE_ALIGNMENT) return 0;
+  return a.memory_mode == WFA_MEMORY_HIGH ? 1 : 2;
+}
+
+// The autotuner: time candidate launch shapes of a long-read launch on a
+// strided sample of the batch (longest first, as the batch runs) and keep
+// the fastest for the batch's (kind, mode, length bucket).
+int autotune_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b, hipStream_t st,
+                  int sample_pairs, wfa_tune_report_t* rep) {
+  Internal ip;
+  int r = make_internal(attr->penalties, &ip);
+  if (r) return r;
+  r = check_supported(*attr);
+  if (r) return r;
+  if (b->num_pairs == 0 || attr->memory_mode == WFA_MEMORY_ULTRALOW) return WFA_OK;
+  HIPCHK(hipSetDevice(e->device));
+  KParams p;
+  base_params(ip, *attr, *b, &p);
+  // heuristic launches run the single-wave kernels: nothing to tune
+  if (wfa::heur_kernel(p)) return WFA_OK;
+  const int kind = kind_of(ip.metric);
+  const int mode = mode_of(*attr);
+  const int n = b->num_pairs;
+  int max_len = b->max_length;
+  if (max_len <= 0) {
+    HIPCHK(wfa::launch_max_length(b->pattern_length, b->text_length, n, e->d_counters + 2, st));
+    r = read_counter(e, 2, st, &max_len);
+    if (r) return r;
+  }
+  // short reads start on the narrow kernels: no shape to choose
+  if (max_len <= 1024) return WFA_OK;
+  const int m = std::max(1, std::min(n, sample_pairs > 0 ? sample_pairs : e->num_cus * 16));
+  std::vector<int32_t> pl(n), tl(n), idx(m);
+  HIPCHK(hipMemcpy(pl.data(), b->pattern_length, (size_t)n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(tl.data(), b->text_length, (size_t)n * 4, hipMemcpyDeviceToHost));
+  for (int i = 0; i < m; ++i) idx[i] = (int)((int64_t)i * n / m);
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) {
+    return (int64_t)pl[a] + tl[a] > (int64_t)pl[c] + tl[c];
+  });
+  r = ensure_lists(e, m);
+  if (r) return r;
+  HIPCHK(hipMemcpy(e->d_lists[0], idx.data(), (size_t)m * 4, hipMemcpyHostToDevice));
+  p.pair_base = 0;
+  const int key = shape_key(kind, mode, max_len);
+  const auto saved = e->shapes.find(key);
+  const bool had = saved != e->shapes.end();
+  const std::pair<int, int> old = had ? saved->second : std::make_pair(0, 0);
+  // the fixed plan's shape
+  e->shapes.erase(key);
+  LaunchCfg c;
+  r = plan(e, p, kind, mode, b->seq_format, max_len, m, 0, &c);
+  if (r) return r;
+  rep->default_waves_per_pair = c.nw;
+  rep->default_groups_per_cu = std::max(1, c.grid / e->num_cus);
+  rep->sample_pairs = m;
+  if (c.lds_ring || c.nw == 1) {
+    if (had) e->shapes[key] = old;
+    return WFA_OK;
+  }
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  HIPCHK(hipEventCreate(&ev[0]));
+  HIPCHK(hipEventCreate(&ev[1]));
+  float best_ms = 0.f;
+  std::pair<int, int> best(0, 0);
+  std::vector<std::pair<int, int>> seen;
+  bool warmed = false;
+  for (int nw : {4, wfa::kWideWaves, 16}) {
+    for (int g = 1; g <= 4 && rep->num_trials < WFA_TUNE_MAX_TRIALS; ++g) {
+      e->shapes[key] = std::make_pair(nw, g);
+      if (plan(e, p, kind, mode, b->seq_format, max_len, m, 0, &c) != WFA_OK || c.nw != nw)
+        continue;
+      const std::pair<int, int> shape(c.nw, c.grid);
+      if (std::find(seen.begin(), seen.end(), shape) != seen.end()) continue;
+      seen.push_back(shape);
+      float ms = 0.f;
+      for (int rep_i = warmed ? 1 : 0; rep_i < 2; ++rep_i) {
+        r = hipEventRecord(ev[0], st) == hipSuccess ? WFA_OK : WFA_ERR_DEVICE;
+        if (!r) r = run_launch(e, p, c, m, e->d_lists[0], e->d_lists[1], st);
+        if (!r && hipEventRecord(ev[1], st) != hipSuccess) r = WFA_ERR_DEVICE;
+        if (!r && hipEventSynchronize(ev[1]) != hipSuccess) r = WFA_ERR_DEVICE;
+        if (!r && hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) r = WFA_ERR_DEVICE;
+        if (r) break;
+      }
+      if (r) break;
+      warmed = true;
+      wfa_tune_trial_t& t = rep->trials[rep->num_trials++];
+      t.waves_per_pair = c.nw;
+      t.groups_per_cu = std::max(1, c.grid / e->num_cus);
+      t.ms = ms;
+      if (best.first == 0 || ms < best_ms) {
+        best_ms = ms;
+        best = std::make_pair(c.nw, g);
+      }
+    }
+    if (r) break;
+  }
+  hipEventDestroy(ev[0]);
+  hipEventDestroy(ev[1]);
+  if (r || best.first == 0) {
+    if (had) e->shapes[key] = old;
+    else e->shapes.erase(key);
+    return r;
+  }
+  e->shapes[key] = best;
+  rep->waves_per_pair = best.first;
+  rep->groups_per_cu = best.second;
+  return WFA_OK;
+}
+
 }  // namespace
 
 namespace wfahost {
@@ -508,17 +675,14 @@ void wfa_engine_free(wfa_engine_t* e) {
   hipSetDevice(e->device);
   hipDeviceSynchronize();
   hipFree(e->d_counters);
-  for (auto l : e->d_lists) hipFree(l);
-  hipFree(e->d_hist);
-  hipFree(e->d_hdr);
-  hipFree(e->d_gring);
-  hipFree(e->d_chain);
-  hipFree(e->d_tab);
-  hipFree(e->d_dom);
-  hipFree(e->d_ztr);
-  hipFree(e->d_zsign);
-  hipFree(e->d_order);
-  for (auto b : e->bi_buf) hipFree(b);
+  if (e->own_arena) {
+    hipFree(e->arena);
+  } else if (!e->arena) {
+    void* bufs[] = {e->d_lists[0], e->d_lists[1], e->d_hist, e->d_hdr, e->d_gring,
+                    e->d_chain, e->d_tab, e->d_dom, e->d_ztr, e->d_zsign, e->d_order};
+    for (void* b : bufs) hipFree(b);
+  }
+  for (void* b : e->bi_buf) hipFree(b);
   hipHostFree(e->h_pinned);
   for (auto ev : e->ev) hipEventDestroy(ev);
   delete e;
@@ -544,6 +708,50 @@ int32_t wfa_engine_align_batch_per_pair(wfa_engine_t* e, const wfa_attr_t* attr,
     r = align_batch_impl(e, attr, &one, (hipStream_t)stream, i, 1);
     if (r) return r;
   }
+  return WFA_OK;
+}
+
+int32_t wfa_engine_autotune(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
+                            void* stream, int32_t sample_pairs, wfa_tune_report_t* report) {
+  if (!e || e->arena) return WFA_ERR_INVALID_ARGUMENT;
+  int r = validate_batch(attr, b);
+  if (r) return r;
+  wfa_tune_report_t local;
+  wfa_tune_report_t* rep = report ? report : &local;
+  std::memset(rep, 0, sizeof(*rep));
+  std::lock_guard<std::mutex> g(e->mu);
+  return autotune_impl(e, attr, b, (hipStream_t)stream, sample_pairs, rep);
+}
+
+int32_t wfa_engine_set_launch_shape(wfa_engine_t* e, const wfa_attr_t* attr, int32_t max_length,
+                                    int32_t waves_per_pair, int32_t groups_per_cu) {
+  if (!e || !attr || max_length <= 0) return WFA_ERR_INVALID_ARGUMENT;
+  const bool clear = waves_per_pair == 0 && groups_per_cu == 0;
+  if (!clear && ((waves_per_pair != 4 && waves_per_pair != wfa::kWideWaves &&
+                  waves_per_pair != 16) ||
+                 groups_per_cu < 1 || groups_per_cu > 16))
+    return WFA_ERR_INVALID_ARGUMENT;
+  Internal ip;
+  int r = make_internal(attr->penalties, &ip);
+  if (r) return r;
+  std::lock_guard<std::mutex> g(e->mu);
+  const int key = shape_key(kind_of(ip.metric), mode_of(*attr), max_length);
+  if (clear) e->shapes.erase(key);
+  else e->shapes[key] = std::make_pair(waves_per_pair, groups_per_cu);
+  return WFA_OK;
+}
+
+int32_t wfa_engine_get_launch_shape(wfa_engine_t* e, const wfa_attr_t* attr, int32_t max_length,
+                                    int32_t* waves_per_pair, int32_t* groups_per_cu) {
+  if (!e || !attr || max_length <= 0 || !waves_per_pair || !groups_per_cu)
+    return WFA_ERR_INVALID_ARGUMENT;
+  Internal ip;
+  int r = make_internal(attr->penalties, &ip);
+  if (r) return r;
+  std::lock_guard<std::mutex> g(e->mu);
+  const auto it = e->shapes.find(shape_key(kind_of(ip.metric), mode_of(*attr), max_length));
+  *waves_per_pair = it == e->shapes.end() ? 0 : it->second.first;
+  *groups_per_cu = it == e->shapes.end() ? 0 : it->second.second;
   return WFA_OK;
 }
 

@@ -599,6 +599,8 @@ static __device__ BiResult find_breakpoint(const BiParams& bp, const BiJob& J, u
   auto setup = [&](Dir& d, int4* meta, int32_t* ring, bool rev) {
     d.p = &p;
     d.R.ring = ring; d.R.meta = meta; d.R.wcap = bp.wcap; d.R.mask = bp.wcap - 1;
+    d.R.cap = kind_components(KIND) * p.scope * bp.wcap;
+    d.R.chk = bp.counters;
     d.scope = p.scope;
     d.sP = sP; d.sT = sT; d.plen = plen; d.tlen = tlen; d.rev = rev;
     d.heur = HEUR ? p.heuristic : WFA_HEURISTIC_NONE;
@@ -777,11 +779,11 @@ static __device__ int backtrace_classic(const KParams& p, const int4* hdr, const
   };
   auto ins = [&](int c, int s, int kk, int64_t t) -> int64_t {
     int v;
-    return get(c, s, kk - 1, &v) ? (((int64_t)(v + 1) << 4) | t) : NEG;
+    return get(c, s, kk - 1, &v) ? ((int64_t)(v + 1) * 16 + t) : NEG;
   };
   auto del = [&](int c, int s, int kk, int64_t t) -> int64_t {
     int v;
-    return get(c, s, kk + 1, &v) ? (((int64_t)v << 4) | t) : NEG;
+    return get(c, s, kk + 1, &v) ? ((int64_t)v * 16 + t) : NEG;
   };
   auto mx2 = [](int64_t a, int64_t b) { return a > b ? a : b; };
   int pos = cap - 1;
@@ -798,7 +800,7 @@ static __device__ int backtrace_classic(const KParams& p, const int4* hdr, const
     int64_t best;
     if (matrix == C_M) {
       int mv;
-      const int64_t misms = get(C_M, mis, k, &mv) ? (((int64_t)(mv + 1) << 4) | BT_M) : NEG;
+      const int64_t misms = get(C_M, mis, k, &mv) ? ((int64_t)(mv + 1) * 16 + BT_M) : NEG;
       const int64_t i1 = mx2(ins(C_M, go1, k, BT_I1_OPEN), ins(C_I1, ge1, k, BT_I1_EXT));
       const int64_t d1 = mx2(del(C_M, go1, k, BT_D1_OPEN), del(C_D1, ge1, k, BT_D1_EXT));
       if (KIND == KIND_AFFINE2P) {
@@ -875,6 +877,8 @@ static __device__ void base_case(const BiParams& bp, int item, const BiJob& J, u
   Dir d;
   d.p = &p;
   d.R.ring = ring; d.R.meta = meta; d.R.wcap = bp.wcap; d.R.mask = bp.wcap - 1;
+  d.R.cap = kind_components(KIND) * p.scope * bp.wcap;
+  d.R.chk = bp.counters;
   d.scope = p.scope;
   d.sP = sP; d.sT = sT; d.plen = plen; d.tlen = tlen; d.rev = false;
   d.heur = HEUR ? p.heuristic : WFA_HEURISTIC_NONE;
@@ -915,7 +919,9 @@ static __device__ void base_case(const BiParams& bp, int item, const BiJob& J, u
       if constexpr (KIND == KIND_EDIT || KIND == KIND_LINEAR) {
         // one contiguous history per sub-problem: chunk 0 of size 0
         Hist H;
-        H.hdr = hdr; H.cells = cells; H.chunk = 0;
+        H.hdr = HBuf<int4>{hdr, (int64_t)bp.hdr_cap * kind_components(KIND), bp.counters};
+        H.cells = HBuf<int32_t>{cells, bp.hist_cap, bp.counters};
+        H.chunk = 0;
         n = backtrace_linear<BITS>(p, H, score + 1, plen, tlen, from_score, from_k, from_off,
                                    out, cap);
       } else {

@@ -186,6 +186,12 @@ hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, in
 // in HBM, without heuristics
 constexpr int kWideWaves = 8;
 
+// The engine's counters: [0] work queue head, [1] retries, [2] scratch,
+// [3] next history pool chunk (all reset before each launch), and
+// [kCheckCounter] the out-of-range indexes a WFA_DEVICE_CHECKS build found
+// during one call (reset at the call's start)
+constexpr int kCheckCounter = 4;
+
 // Compat::Corrected ends-free alignment with a match reward: candidate
 // ends, band and dominance pruning (src/aligner.rs:1104-1360)
 __host__ __device__ inline bool corrected_endsfree(const KParams& p) {

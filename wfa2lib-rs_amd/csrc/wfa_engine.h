@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <map>
 #include <mutex>
 
 #include "wfa_device.h"
@@ -90,11 +91,12 @@ inline int check_supported(const wfa_attr_t& a) {
   if (a.memory_mode < WFA_MEMORY_HIGH || a.memory_mode > WFA_MEMORY_ULTRALOW)
     return WFA_ERR_INVALID_ARGUMENT;
   if (a.compat != WFA_COMPAT_C && a.compat != WFA_COMPAT_CORRECTED) return WFA_ERR_INVALID_ARGUMENT;
-  // BiWFA: end-to-end only, and no X-drop / Z-drop, as the reference's
-  // driver checks (src/bin/align_benchmark.rs:529-560)
+  // BiWFA: end-to-end only, and no Z-drop (C refuses it since 7110ee3, and
+  // the reference's driver refuses X-drop and Z-drop,
+  // src/bin/align_benchmark.rs:529-560); X-drop sub-alignments run, as in
+  // the reference's aligner (tests/xdrop_steps.rs:57-92)
   if (a.memory_mode == WFA_MEMORY_ULTRALOW &&
-      (a.span != WFA_SPAN_END2END || a.heuristic == WFA_HEURISTIC_XDROP ||
-       a.heuristic == WFA_HEURISTIC_ZDROP))
+      (a.span != WFA_SPAN_END2END || a.heuristic == WFA_HEURISTIC_ZDROP))
     return WFA_ERR_UNSUPPORTED;
   return WFA_OK;
 }
@@ -167,6 +169,14 @@ struct wfa_engine_t {
   // reads of the counters go through `h_pinned` (then ordinary memory).
   uint8_t* arena = nullptr;
   size_t arena_cap = 0;
+  // the arena is the engine's own allocation (a serving loop's engines:
+  // one allocation up front, none while launches of other engines run)
+  bool own_arena = false;
+  // Launch shapes of long-read launches (pairs over 1024 bases, rings in
+  // HBM, no heuristic) by (kind, mode, length bucket): {waves per pair,
+  // workgroups per CU}; from wfa_engine_autotune or
+  // wfa_engine_set_launch_shape, else the fixed plan
+  std::map<int, std::pair<int, int>> shapes;
   size_t arena_top = 0;
   size_t arena_launch = 0;
   // longest pair a call may bring (0: no limit), WFA_ERR_LIMIT beyond it
@@ -176,13 +186,27 @@ struct wfa_engine_t {
 
 namespace wfahost {
 
+// Device scratch grows geometrically (to at most twice the old size and
+// 1 GiB beyond the need), so a batch engine reallocates a buffer only a
+// few times. hipFree waits for the whole device: engines that must not
+// stall other streams' work (a serving loop's) take their buffers from a
+// fixed arena instead (egrow).
 inline int grow(void** ptr, size_t* have, size_t need) {
   if (*have >= need) return WFA_OK;
+  size_t cap = std::max(need, std::min(*have * 2, need + ((size_t)1 << 30)));
   if (*ptr) hipFree(*ptr);
   *ptr = nullptr;
   *have = 0;
-  if (hipMalloc(ptr, need) != hipSuccess) return WFA_ERR_OUT_OF_MEMORY;
-  *have = need;
+  if (hipMalloc(ptr, cap) != hipSuccess) {
+    (void)hipGetLastError();
+    cap = need;
+    if (hipMalloc(ptr, cap) != hipSuccess) {
+      (void)hipGetLastError();
+      *ptr = nullptr;
+      return WFA_ERR_OUT_OF_MEMORY;
+    }
+  }
+  *have = cap;
   return WFA_OK;
 }
 

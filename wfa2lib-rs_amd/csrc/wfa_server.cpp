@@ -207,6 +207,9 @@ bool pick(wfa_server_t* s, wfa_attr_t* attr_out, int* fmt_out, bool* per_pair_ou
   return true;
 }
 
+// staging each worker holds from the start (host pinned and device)
+constexpr size_t kStagingBytes = size_t(32) << 20;
+
 int grow_pinned(uint8_t** p, size_t* cap, size_t need) {
   if (need <= *cap) return WFA_OK;
   const size_t n = std::max(need, *cap * 2);
@@ -219,14 +222,7 @@ int grow_pinned(uint8_t** p, size_t* cap, size_t need) {
 }
 
 int grow_device(uint8_t** p, size_t* cap, size_t need) {
-  if (need <= *cap) return WFA_OK;
-  const size_t n = std::max(need, *cap * 2);
-  if (*p) hipFree(*p);
-  *p = nullptr;
-  *cap = 0;
-  if (hipMalloc((void**)p, n) != hipSuccess) return WFA_ERR_OUT_OF_MEMORY;
-  *cap = n;
-  return WFA_OK;
+  return grow((void**)p, cap, need);
 }
 
 // One launch: stage the batch, align it on the worker's stream, read back.
@@ -432,13 +428,39 @@ wfa_server_t* wfa_server_new(const wfa_server_config_t* cfg, int32_t* err) {
     // a long-read launch takes at most half a workgroup per CU on average,
     // so short requests' launches always find room on the CUs
     w->e->long_grid_cap = std::max(1, w->e->num_cus / 2);
-    // and sizes its history pool by the slots' average: multi-GiB
-    // allocations while requests run would stall the other streams
-    w->e->long_pool_whole_budget = false;
+    // (its history pool may take the whole arena: no allocation is made
+    // for it, so a long launch needs fewer retry passes)
+    w->e->long_pool_whole_budget = true;
     if (hipStreamCreateWithFlags(&w->st, hipStreamNonBlocking) != hipSuccess) {
       s->workers.push_back(std::move(w));
       destroy(s);
       if (err) *err = WFA_ERR_DEVICE;
+      return nullptr;
+    }
+    // Allocations wait for the whole device (hipMalloc / hipFree, and the
+    // stream-ordered allocator as well), so while one engine's long launch
+    // runs, another engine that grows a buffer would stall for as long.
+    // Every engine therefore takes its scratch from one arena allocated
+    // here (its launch plans fit the arena, the pairs that do not end as
+    // WFA_STATUS_OOM), and the staging buffers start at a size that holds
+    // a typical launch.
+    void* arena = nullptr;
+    const size_t arena_bytes = (size_t)w->e->scratch_budget;
+    if (hipMalloc(&arena, arena_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      s->workers.push_back(std::move(w));
+      destroy(s);
+      if (err) *err = WFA_ERR_OUT_OF_MEMORY;
+      return nullptr;
+    }
+    w->e->arena = static_cast<uint8_t*>(arena);
+    w->e->arena_cap = arena_bytes;
+    w->e->own_arena = true;
+    if (grow_pinned(&w->h, &w->h_cap, kStagingBytes) ||
+        grow_device(&w->d, &w->d_cap, kStagingBytes)) {
+      s->workers.push_back(std::move(w));
+      destroy(s);
+      if (err) *err = WFA_ERR_OUT_OF_MEMORY;
       return nullptr;
     }
     s->workers.push_back(std::move(w));

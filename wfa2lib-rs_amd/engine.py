@@ -128,6 +128,41 @@ class BatchEngine:
         N.check(fn(self._h, ctypes.byref(attr), ctypes.byref(b), ctypes.c_void_p(stream)),
                 "wfa_engine_align_batch")
 
+    def autotune(self, attr, batch: DeviceBatch, sample_pairs=0, stream=None):
+        """Time the candidate launch shapes of a long-read batch on a
+        sample and keep the fastest for its key (wfa_engine_autotune).
+        Returns {"waves_per_pair", "groups_per_cu", "default", "trials",
+        "sample_pairs"}; waves_per_pair 0 when there is nothing to tune."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        rep = N.TuneReport()
+        b = batch.c_batch()
+        N.check(N.lib().wfa_engine_autotune(self._h, ctypes.byref(attr), ctypes.byref(b),
+                                            ctypes.c_void_p(stream), int(sample_pairs),
+                                            ctypes.byref(rep)), "wfa_engine_autotune")
+        return {
+            "waves_per_pair": rep.waves_per_pair, "groups_per_cu": rep.groups_per_cu,
+            "default": (rep.default_waves_per_pair, rep.default_groups_per_cu),
+            "sample_pairs": rep.sample_pairs,
+            "trials": [(t.waves_per_pair, t.groups_per_cu, round(t.ms, 3))
+                       for t in rep.trials[:rep.num_trials]],
+        }
+
+    def set_launch_shape(self, attr, max_length, waves_per_pair, groups_per_cu):
+        """Fix the long-read launch shape for attr's key and max_length's
+        bucket (0, 0: back to the fixed plan)."""
+        N.check(N.lib().wfa_engine_set_launch_shape(self._h, ctypes.byref(attr), int(max_length),
+                                                    int(waves_per_pair), int(groups_per_cu)),
+                "wfa_engine_set_launch_shape")
+
+    def launch_shape(self, attr, max_length):
+        w, g = ctypes.c_int32(0), ctypes.c_int32(0)
+        N.check(N.lib().wfa_engine_get_launch_shape(self._h, ctypes.byref(attr), int(max_length),
+                                                    ctypes.byref(w), ctypes.byref(g)),
+                "wfa_engine_get_launch_shape")
+        return w.value, g.value
+
     def set_scratch_budget(self, nbytes):
         """Cap one launch's device scratch (pairs that do not fit: OOM)."""
         N.check(N.lib().wfa_engine_set_scratch_budget(self._h, int(nbytes)),
