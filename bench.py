@@ -53,6 +53,7 @@ WORKLOADS = {
                           "end-to-end, CIGAR high memory"),
     "long1k_10k": dict(metric="gap_affine_2p", pen=(0, 4, 6, 2, 24, 1), memory=1,
                        pairs=100_000, steps=2, cpu_sample=256, parity=256, plain=20, latency=20,
+                       autotune=True,
                        desc="{n} pairs gap-affine 2-piece (0,4,6,2,24,1), 1-10 kbp log-uniform, "
                             "divergence 1-10%, 2-bit packed, end-to-end, CIGAR medium memory"),
     "ultralong50k_100k": dict(metric="edit", pen=(0, 0, 0, 0, 0, 0), memory=3, pairs=1000,
@@ -202,6 +203,18 @@ def cpu_model():
     return None
 
 
+def extend_phase(workload):
+    """The extend phase of the dominant kernel, measured by its phase-clock
+    build (profiles/extend_phase_<workload>.json): the HBM bytes it moves
+    (the sequences staged into LDS once per pair) and their rate over the
+    phase's share of the launch."""
+    p = os.path.join(ROOT, "profiles", f"extend_phase_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
 def issue_limit(workload):
     """The limit that binds the dominant kernel, as measured by rocprofv3 SQ
     counters (profiles/issue_<workload>.json): the kernel moves a few bytes
@@ -245,6 +258,18 @@ def run_leg(name, npairs, parity, log):
     eng.align(a, warm)
     batch = DeviceBatch(pairs, device="cuda:0", cigar=True)
     torch.cuda.synchronize()
+    tune, fixed_sec = None, None
+    if cfg.get("autotune"):
+        # the fixed launch plan first, then the autotuner's choice (the
+        # tuning trials run on a sample before the timed step)
+        t = time.perf_counter()
+        eng.align(a, batch)
+        torch.cuda.synchronize()
+        fixed_sec = time.perf_counter() - t
+        t = time.perf_counter()
+        tune = eng.autotune(a, batch)
+        torch.cuda.synchronize()
+        tune["seconds"] = round(time.perf_counter() - t, 3)
     t = time.perf_counter()
     eng.align(a, batch)
     torch.cuda.synchronize()
@@ -276,8 +301,14 @@ def run_leg(name, npairs, parity, log):
            "retried_pairs": int(eng.last_retries()),
            "completed": int((np.asarray(status) == N.STATUS_COMPLETED).sum()),
            "parity": f"{same}/{checked} vs the {how}"}
+    if tune is not None:
+        out["fixed_plan_pairs_per_s"] = round(npairs / fixed_sec, 3)
+        out["autotune"] = tune
     log(f"{name}: {npairs / sec:,.1f} pairs/s ({sec * 1e3:.0f} ms for {npairs} pairs), "
-        f"parity {same}/{checked}")
+        f"parity {same}/{checked}" +
+        (f"; fixed plan {npairs / fixed_sec:,.1f} pairs/s, tuned shape "
+         f"{tune['waves_per_pair']} waves x {tune['groups_per_cu']} per CU "
+         f"(tuning {tune['seconds']} s)" if tune is not None else ""))
     del batch, warm, eng
     torch.cuda.empty_cache()
     return out
@@ -300,6 +331,8 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=None)
     ap.add_argument("--serving-rate", type=float, default=None, help="requests per second")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the fixed launch plan for long reads (profiling runs)")
     ap.add_argument("--no-legs", action="store_true",
                     help="skip the long1k_10k and ultralong50k_100k legs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -378,6 +411,24 @@ def main():
             kms.append(eng.last_kernel_ms())
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / n_steps, kms
+
+    # ---- long reads: the fixed launch plan, then the autotuner's shape ----
+    tune = None
+    if cfg.get("autotune") and not args.no_autotune:
+        eng.align(attr_cig, batch)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        eng.align(attr_cig, batch)
+        torch.cuda.synchronize()
+        fixed_sec = time.perf_counter() - t
+        t = time.perf_counter()
+        tune = {"cigar": eng.autotune(attr_cig, batch), "score_only": eng.autotune(attr_score, batch)}
+        torch.cuda.synchronize()
+        tune["seconds"] = round(time.perf_counter() - t, 3)
+        tune["fixed_plan_pairs_per_s"] = round(len(pairs) / fixed_sec, 3)
+        log(f"fixed plan: {len(pairs) / fixed_sec:,.1f} pairs/s; tuned shape "
+            f"{tune['cigar']['waves_per_pair']} waves x {tune['cigar']['groups_per_cu']} per CU "
+            f"(tuning {tune['seconds']} s)")
 
     # ---- CIGAR mode: the headline ----
     sec, kms = timed(attr_cig, steps, warmup)
@@ -515,6 +566,7 @@ def main():
             "kernel_ms_all_passes": round(all_ms, 3),
             "serving": serving,
             "legs": legs,
+            **({"autotune": tune} if tune else {}),
         },
         "roofline": {
             "bound": "hbm",
@@ -524,6 +576,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 6),
             "traffic": traffic,
             "binding": issue_limit(args.workload),
+            "extend_phase": extend_phase(args.workload),
         },
         "cpu_baseline": cpu,
     }

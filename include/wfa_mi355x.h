@@ -197,6 +197,22 @@ int32_t wfa_align_end2end(wfa_aligner_t* aligner,
 int32_t wfa_align_packed2bits(wfa_aligner_t* aligner,
                               const uint8_t* pattern, int32_t pattern_length,
                               const uint8_t* text, int32_t text_length);
+/* align_lambda(match_funct, pattern_length, text_length) -> i32
+ * (src/aligner.rs:2743-2752, src/sequences.rs:225-257) for the match
+ * functions that compare symbols: pattern position i matches text position j
+ * iff pattern_class[pattern[i]] == text_class[text[j]] (two 256-entry
+ * tables). This covers every relation that is an equivalence between the
+ * pattern's and the text's symbols (byte equality, the reference's own
+ * tests/singletrack.rs:123-136, with identity tables; case-insensitive or
+ * RNA U = T matching; masked symbols mapped to a class of their own). A
+ * device cannot call a host closure, so closures over positions, and
+ * symbol relations that are not equivalences (an N matching every base),
+ * have no counterpart here. End-to-end, as align_lambda; the CIGAR's M and X
+ * follow the classes. */
+int32_t wfa_align_lambda_classes(wfa_aligner_t* aligner,
+                                 const uint8_t* pattern, int32_t pattern_length,
+                                 const uint8_t* text, int32_t text_length,
+                                 const uint8_t* pattern_class, const uint8_t* text_class);
 /* align_endsfree(&[u8], &[u8]) -> i32 (src/aligner.rs:707): ends-free with
  * the free ends of wfa_aligner_set_alignment_free_ends, or extension after
  * wfa_aligner_set_alignment_extension (all ends zero otherwise). */
@@ -304,7 +320,8 @@ void wfa_engine_free(wfa_engine_t* engine);
  * a retry counter after the launch, and pairs whose wavefronts outgrew the
  * first launch's per-pair storage are re-run by wider launches before it
  * returns. Scratch grows on demand (allocations happen only when a batch
- * needs more than any earlier one), up to 48 GiB per launch; a pair whose
+ * needs more than any earlier one), up to the scratch cap per launch
+ * (wfa_engine_set_scratch_budget); a pair whose
  * CIGAR history still does not fit after six ever wider passes gets
  * WFA_STATUS_OOM (score 0, no CIGAR) instead of failing the batch. */
 int32_t wfa_engine_align_batch(wfa_engine_t* engine, const wfa_attr_t* attr,
@@ -317,10 +334,13 @@ int32_t wfa_engine_align_batch_per_pair(wfa_engine_t* engine, const wfa_attr_t* 
                                         const wfa_batch_t* batch, void* hip_stream);
 
 /* Cap the device scratch (wavefront rings, CIGAR history) one launch of
- * this engine may use, in bytes (default 48 GiB; at least 64 MiB). Pairs
- * whose CIGAR history does not fit the cap get WFA_STATUS_OOM. The
- * reference has no counterpart (its memory is the host's); this is the
- * engine's own limit. */
+ * this engine may use, in bytes (at least 64 MiB). Until set, the cap is
+ * 48 GiB for short reads and, for long reads (over 1024 bases) and BiWFA,
+ * half of the device's free memory plus what the engine already holds,
+ * between 48 GiB (or 90% of that, when less is free) and 128 GiB, taken
+ * again at most once a second. Pairs whose CIGAR history does not fit the cap get
+ * WFA_STATUS_OOM. The reference has no counterpart (its memory is the
+ * host's); this is the engine's own limit. */
 int32_t wfa_engine_set_scratch_budget(wfa_engine_t* engine, int64_t bytes);
 
 /* ---- Launch shapes of long-read launches and their autotuner ----
@@ -350,11 +370,12 @@ typedef struct {
     int32_t default_groups_per_cu;
 } wfa_tune_report_t;
 
-/* Time the candidate shapes for `batch` under `attr` on a strided sample of
- * `sample_pairs` of its pairs (0: 16 per CU), longest first as a batch
- * runs, on `hip_stream`, and keep the fastest in the engine for the
- * batch's key. The trials write the sampled pairs' results into the
- * batch's outputs (a later wfa_engine_align_batch rewrites them). Batches
+/* Time the candidate shapes for `batch` under `attr` on its
+ * `sample_pairs` longest pairs (0: 32 per CU), longest first as a batch
+ * runs, each trial with its retry passes, on `hip_stream`, and keep the
+ * fastest in the engine for the batch's key. The trials write the sampled
+ * pairs' results into the batch's outputs (a later wfa_engine_align_batch
+ * rewrites them). Batches
  * with nothing to choose (short reads, heuristics, BiWFA) return WFA_OK
  * with waves_per_pair 0. `report` may be NULL. */
 int32_t wfa_engine_autotune(wfa_engine_t* engine, const wfa_attr_t* attr,
@@ -400,6 +421,15 @@ int32_t wfa_engine_last_retries(const wfa_engine_t* engine);
  * retry passes, in milliseconds. */
 int32_t wfa_engine_last_kernel_ms(const wfa_engine_t* engine, float* first_pass_ms,
                                   float* all_passes_ms);
+
+/* No counterpart in the reference: the phase clocks of a profiling build
+ * (python wfa2lib-rs_amd/build.py --phase-clocks, libwfa_mi355x_phases.so).
+ * After a batch call whose first pass ran the short-read kernel: out[0]
+ * pairs it aligned, out[1] cycles its waves spent staging sequences from
+ * HBM into LDS, out[2] cycles extending M rows (the LCP over the staged
+ * sequences), out[3] cycles per pair in all, summed over the waves.
+ * WFA_ERR_UNSUPPORTED in a release build. */
+int32_t wfa_engine_last_phase_cycles(wfa_engine_t* engine, uint64_t* out);
 
 /* ======================================================================
  * Serving loop: continuous batching of independent requests

@@ -522,3 +522,31 @@ def test_sentinel_bytes_are_ordinary_symbols(pkg, N, engine, oracle):
     for i in range(len(pats)):
         assert s[i] == oracle.dp_affine(pats[i], txts[i], 4, 6, 2)
         assert pkg.Cigar(c[i]).check_alignment(pats[i], txts[i]) is None
+
+
+def test_lambda_classes(pkg, N, engine, oracle):
+    """align_lambda for symbol comparisons (src/aligner.rs:2743-2752): with
+    identity classes (byte equality) it equals the ASCII alignment, score
+    and CIGAR, as tests/singletrack.rs:123-136 checks; with case-folding
+    classes a lower-case text aligns as its upper-case copy; with
+    U = T for an RNA pattern as its T copy."""
+    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    pairs = generate(WORKLOADS["short150"], num_pairs=24)
+    A = pkg.AffineAligner(pkg.WavefrontPenalties.new_affine(pkg.AffinePenalties(0, 4, 6, 2)))
+    A.set_alignment_scope(pkg.AlignmentScope.COMPUTE_ALIGNMENT)
+    B = pkg.AffineAligner(pkg.WavefrontPenalties.new_affine(pkg.AffinePenalties(0, 4, 6, 2)))
+    B.set_alignment_scope(pkg.AlignmentScope.COMPUTE_ALIGNMENT)
+    fold = bytes(c - 32 if 97 <= c <= 122 else c for c in range(256))
+    rna = bytes(ord("T") if c == ord("U") else c for c in range(256))
+    for i in range(len(pairs)):
+        p, t = pairs.pattern(i), pairs.text(i)
+        s = A.align_end2end(p, t)
+        cig = A.cigar().to_string_rle(True)
+        assert B.align_lambda_classes(p, t) == s
+        assert B.cigar().to_string_rle(True) == cig
+        assert B.align_lambda_classes(p, t.lower(), fold, fold) == s
+        assert B.cigar().to_string_rle(True) == cig
+        assert B.align_lambda_classes(p.replace(b"T", b"U"), t, rna, rna) == s
+        assert B.cigar().to_string_rle(True) == cig
+    with pytest.raises(NotImplementedError):
+        B.align_lambda(lambda i, j: True, 4, 4)

@@ -130,6 +130,8 @@ SIGNATURES = {
     "wfa_aligner_max_memory_compact": (ctypes.c_uint64, [_VP]),
     "wfa_align_end2end": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
     "wfa_align_packed2bits": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
+    "wfa_align_lambda_classes": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32,
+                                        ctypes.c_char_p, ctypes.c_char_p]),
     "wfa_align_endsfree": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
     "wfa_align_biwfa": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
     "wfa_align_biwfa_score": (_I32, [_VP, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32]),
@@ -143,6 +145,7 @@ SIGNATURES = {
     "wfa_engine_align_batch": (_I32, [_VP, _P(Attr), _P(Batch), _VP]),
     "wfa_engine_align_batch_per_pair": (_I32, [_VP, _P(Attr), _P(Batch), _VP]),
     "wfa_engine_set_scratch_budget": (_I32, [_VP, ctypes.c_int64]),
+    "wfa_engine_last_phase_cycles": (_I32, [_VP, _P(ctypes.c_uint64)]),
     "wfa_engine_autotune": (_I32, [_VP, _P(Attr), _P(Batch), _VP, _I32, _P(TuneReport)]),
     "wfa_engine_set_launch_shape": (_I32, [_VP, _P(Attr), _I32, _I32, _I32]),
     "wfa_engine_get_launch_shape": (_I32, [_VP, _P(Attr), _I32, _P(_I32), _P(_I32)]),

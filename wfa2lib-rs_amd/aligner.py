@@ -382,6 +382,31 @@ class WavefrontAligner:
         return self._finish(N.lib().wfa_align_packed2bits(
             self._h, bytes(pattern), pattern_length, bytes(text), text_length))
 
+    def align_lambda_classes(self, pattern: bytes, text: bytes, pattern_class=None,
+                             text_class=None) -> int:
+        """align_lambda (src/aligner.rs:2743-2752) for match functions that
+        compare symbols: pattern[i] matches text[j] iff
+        pattern_class[pattern[i]] == text_class[text[j]] (256-entry tables;
+        identity when omitted, i.e. byte equality).
+
+        A device cannot call a host closure: closures over positions and
+        symbol relations that are not equivalences have no counterpart."""
+        ident = bytes(range(256))
+        pc = bytes(pattern_class) if pattern_class is not None else ident
+        tc = bytes(text_class) if text_class is not None else ident
+        if len(pc) != 256 or len(tc) != 256:
+            raise ValueError("class tables have 256 entries")
+        pattern, text = bytes(pattern), bytes(text)
+        return self._finish(N.lib().wfa_align_lambda_classes(self._h, pattern, len(pattern),
+                                                             text, len(text), pc, tc))
+
+    def align_lambda(self, match_funct, pattern_length: int, text_length: int) -> int:
+        """align_lambda with a Python closure over positions: a device cannot
+        call it. Use align_lambda_classes for symbol comparisons."""
+        raise NotImplementedError(
+            "align_lambda with a closure needs a host call per compared position; the device "
+            "path takes symbol classes instead (align_lambda_classes)")
+
     def status(self) -> AlignStatus:
         return AlignStatus(N.lib().wfa_aligner_status(self._h))
 

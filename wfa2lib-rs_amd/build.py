@@ -5,6 +5,9 @@
 
 ``libwfa_mi355x.so`` is built with hipcc for gfx950 from csrc/*.hip and
 csrc/*.cpp; ``libwfa_synth.so`` with gcc. Outputs go to ``lib/``.
+``--phase-clocks`` builds ``libwfa_mi355x_phases.so`` with WFA_PHASE_CLOCKS:
+the short-read kernel sums its waves' cycles for sequence staging, the
+extension and whole pairs (wfa_engine_last_phase_cycles).
 ``--checks`` builds ``libwfa_mi355x_checks.so`` with WFA_DEVICE_CHECKS: every
 ring and history index is tested on the device and a call that found one
 out of range fails with WFA_ERR_DEVICE; set WFA_MI355X_LIB to its file name
@@ -37,14 +40,16 @@ def _run(cmd):
         raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
 
 
-def build_engine(checks=False):
+def build_engine(checks=False, phases=False):
     """Compile each .hip translation unit to an object in parallel (the
     per-kind kernel sets are the slow part), then link with the host API."""
     os.makedirs(LIB, exist_ok=True)
-    obj_dir = os.path.join(LIB, "obj_checks" if checks else "obj")
+    variant = "checks" if checks else "phases" if phases else ""
+    obj_dir = os.path.join(LIB, "obj_" + variant if variant else "obj")
     os.makedirs(obj_dir, exist_ok=True)
     hips = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
-    flags = HIP_FLAGS + (["-DWFA_DEVICE_CHECKS"] if checks else [])
+    flags = HIP_FLAGS + (["-DWFA_DEVICE_CHECKS"] if checks else []) + (
+        ["-DWFA_PHASE_CLOCKS"] if phases else [])
     compile_flags = [f for f in flags if f != "-shared"]
     procs = []
     for f in hips:
@@ -64,7 +69,7 @@ def build_engine(checks=False):
         objs.append(obj)
     if failed:
         raise RuntimeError("build step failed: " + "; ".join(failed))
-    out = os.path.join(LIB, "libwfa_mi355x_checks.so" if checks else "libwfa_mi355x.so")
+    out = os.path.join(LIB, f"libwfa_mi355x_{variant}.so" if variant else "libwfa_mi355x.so")
     tmp = out + ".tmp"
     _run([HIPCC] + flags + ["-o", tmp] + objs +
          [os.path.join(CSRC, f) for f in ("wfa_api.cpp", "wfa_biwfa.cpp", "wfa_server.cpp")])
@@ -89,6 +94,8 @@ def build_all():
 if __name__ == "__main__":
     if "--checks" in sys.argv[1:]:
         build_engine(checks=True)
+    elif "--phase-clocks" in sys.argv[1:]:
+        build_engine(phases=True)
     else:
         build_all()
     sys.exit(0)

@@ -2240,8 +2240,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
 }
 
 template <int KIND, int MODE, bool LDS_RING, bool I16, bool PACKED, bool HEUR, int NW>
-__global__ void __launch_bounds__(64 * NW, LDS_RING ? 5 : NW > 1 ? 32 / NW : 4)
-    wfa_align_kernel(KParams p) {
+__global__ void __launch_bounds__(64 * NW, LDS_RING ? 5 : 4) wfa_align_kernel(KParams p) {
   using RT = typename std::conditional<I16, int16_t, int32_t>::type;
   constexpr bool CIG = MODE != 0;
   static_assert(NW == 1 || !LDS_RING, "several waves per pair only with rings in HBM");

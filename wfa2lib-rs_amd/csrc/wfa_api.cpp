@@ -291,6 +291,7 @@ int run_launch(wfa_engine_t* e, KParams p, const LaunchCfg& c, int num_items,
       p.hdr_cap = c.hdr_cap;
     }
     p.sh_gsh = c.short_gsh;
+    p.phase = e->d_phase;
     HIPCHK(wfa::launch_reset(e->d_counters, st));
     HIPCHK(wfa::launch_short(p, c.mode, c.short_rc, c.grid, c.lds_bytes, st));
     return WFA_OK;
@@ -383,11 +384,42 @@ int validate_batch(const wfa_attr_t* attr, const wfa_batch_t* b) {
 int align_batch_run(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
                     hipStream_t st, int first, int count);
 
+// The passes after a first pass that left `nretry` pairs in d_lists[0]:
+// ever wider launches over the pairs that asked for retry, then, after
+// six, the out-of-memory status for what still does not fit.
+int retry_passes(wfa_engine_t* e, KParams p, int kind, int mode, const wfa_batch_t* b,
+                 int max_len, hipStream_t st, int32_t nretry) {
+  int cur = 0;
+  LaunchCfg c;
+  for (int pass = 1; nretry > 0; ++pass) {
+    if (pass > 6) {
+      // what still does not fit the scratch budget gets the reference's
+      // out-of-memory status (AlignStatus::Oom, src/aligner.rs:120-135)
+      HIPCHK(wfa::launch_mark_oom(e->d_lists[cur], nretry, b->score, b->status, b->cigar_length,
+                                  b->end_position, st));
+      break;
+    }
+    int r = plan(e, p, kind, mode, b->seq_format, max_len, nretry, pass, &c);
+    if (r) return r;
+    r = run_launch(e, p, c, nretry, e->d_lists[cur], e->d_lists[cur ^ 1], st);
+    if (r) return r;
+    cur ^= 1;
+    r = read_counter(e, 1, st, &nretry);
+    if (r) return r;
+  }
+  return WFA_OK;
+}
+
 // One batch (see align_batch_run). A WFA_DEVICE_CHECKS build counts the
 // out-of-range ring and history indexes of the call's launches and fails
 // the call when there were any.
 int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
                      hipStream_t st, int first, int count) {
+#ifdef WFA_PHASE_CLOCKS
+  HIPCHK(hipSetDevice(e->device));
+  if (!e->d_phase) HIPCHK(hipMalloc((void**)&e->d_phase, 4 * sizeof(unsigned long long)));
+  HIPCHK(hipMemsetAsync(e->d_phase, 0, 4 * sizeof(unsigned long long), st));
+#endif
 #ifdef WFA_DEVICE_CHECKS
   HIPCHK(hipSetDevice(e->device));
   HIPCHK(hipMemsetAsync(e->d_counters + wfa::kCheckCounter, 0, 4, st));
@@ -419,6 +451,7 @@ int align_batch_run(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* 
   HIPCHK(hipSetDevice(e->device));
   if (attr->memory_mode == WFA_MEMORY_ULTRALOW) {
     if (b->pair_config) return WFA_ERR_UNSUPPORTED;
+    refresh_budget(e, true);
     return biwfa_align_batch(e, attr, b, st, first, count);
   }
   KParams p;
@@ -463,6 +496,8 @@ int align_batch_run(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* 
     // 256-byte alignment of their buffers
     e->arena_launch = (e->arena_top + 255) & ~(size_t)255;
     e->scratch_budget = (int64_t)(e->arena_cap - e->arena_launch) - (64 << 10);
+  } else {
+    refresh_budget(e, max_len > 1024);
   }
   LaunchCfg c;
   r = plan(e, p, kind, mode, b->seq_format, max_len, count, 0, &c);
@@ -478,23 +513,8 @@ int align_batch_run(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* 
   r = read_counter(e, 1, st, &nretry);
   if (r) return r;
   e->last_retries = nretry;
-  int cur = 0;
-  for (int pass = 1; nretry > 0; ++pass) {
-    if (pass > 6) {
-      // what still does not fit the scratch budget gets the reference's
-      // out-of-memory status (AlignStatus::Oom, src/aligner.rs:120-135)
-      HIPCHK(wfa::launch_mark_oom(e->d_lists[cur], nretry, b->score, b->status, b->cigar_length,
-                                  b->end_position, st));
-      break;
-    }
-    r = plan(e, p, kind, mode, b->seq_format, max_len, nretry, pass, &c);
-    if (r) return r;
-    r = run_launch(e, p, c, nretry, e->d_lists[cur], e->d_lists[cur ^ 1], st);
-    if (r) return r;
-    cur ^= 1;
-    r = read_counter(e, 1, st, &nretry);
-    if (r) return r;
-  }
+  r = retry_passes(e, p, kind, mode, b, max_len, st, nretry);
+  if (r) return r;
   if (!timed) {
     HIPCHK(hipStreamSynchronize(st));
     return WFA_OK;
@@ -539,18 +559,30 @@ int autotune_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
   }
   // short reads start on the narrow kernels: no shape to choose
   if (max_len <= 1024) return WFA_OK;
-  const int m = std::max(1, std::min(n, sample_pairs > 0 ? sample_pairs : e->num_cus * 16));
+  // the sample: enough pairs that the widest candidate grid (4 workgroups
+  // per CU) runs 16 pairs per slot, so a trial measures the shape's rate
+  // more than its tail
+  const int m = std::max(1, std::min(n, sample_pairs > 0 ? sample_pairs : e->num_cus * 64));
   std::vector<int32_t> pl(n), tl(n), idx(m);
   HIPCHK(hipMemcpy(pl.data(), b->pattern_length, (size_t)n * 4, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(tl.data(), b->text_length, (size_t)n * 4, hipMemcpyDeviceToHost));
-  for (int i = 0; i < m; ++i) idx[i] = (int)((int64_t)i * n / m);
-  std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) {
+  // the m longest pairs, longest first: a long-read batch's cost is
+  // dominated by its longest pairs (cells grow with the square of the
+  // score), and they are what every slot takes first
+  std::vector<int32_t> all(n);
+  for (int i = 0; i < n; ++i) all[i] = i;
+  std::stable_sort(all.begin(), all.end(), [&](int a, int c) {
     return (int64_t)pl[a] + tl[a] > (int64_t)pl[c] + tl[c];
   });
+  std::copy(all.begin(), all.begin() + m, idx.begin());
   r = ensure_lists(e, m);
   if (r) return r;
-  HIPCHK(hipMemcpy(e->d_lists[0], idx.data(), (size_t)m * 4, hipMemcpyHostToDevice));
+  r = egrow(e, &e->d_order, &e->order_bytes, (size_t)m * 4);
+  if (r) return r;
+  int32_t* sample = (int32_t*)e->d_order;
+  HIPCHK(hipMemcpy(sample, idx.data(), (size_t)m * 4, hipMemcpyHostToDevice));
   p.pair_base = 0;
+  refresh_budget(e, true);
   const int key = shape_key(kind, mode, max_len);
   const auto saved = e->shapes.find(key);
   const bool had = saved != e->shapes.end();
@@ -572,37 +604,40 @@ int autotune_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t* b,
   HIPCHK(hipEventCreate(&ev[1]));
   float best_ms = 0.f;
   std::pair<int, int> best(0, 0);
-  std::vector<std::pair<int, int>> seen;
   bool warmed = false;
+  // The candidates split the same resident waves differently: 4, 8 or 16
+  // waves per pair, each at as many workgroups per CU as occupancy allows
+  // (a shape with fewer leaves wave slots idle, and measured slower on
+  // every long-read workload tried)
   for (int nw : {4, wfa::kWideWaves, 16}) {
-    for (int g = 1; g <= 4 && rep->num_trials < WFA_TUNE_MAX_TRIALS; ++g) {
-      e->shapes[key] = std::make_pair(nw, g);
-      if (plan(e, p, kind, mode, b->seq_format, max_len, m, 0, &c) != WFA_OK || c.nw != nw)
-        continue;
-      const std::pair<int, int> shape(c.nw, c.grid);
-      if (std::find(seen.begin(), seen.end(), shape) != seen.end()) continue;
-      seen.push_back(shape);
-      float ms = 0.f;
-      for (int rep_i = warmed ? 1 : 0; rep_i < 2; ++rep_i) {
-        r = hipEventRecord(ev[0], st) == hipSuccess ? WFA_OK : WFA_ERR_DEVICE;
-        if (!r) r = run_launch(e, p, c, m, e->d_lists[0], e->d_lists[1], st);
-        if (!r && hipEventRecord(ev[1], st) != hipSuccess) r = WFA_ERR_DEVICE;
-        if (!r && hipEventSynchronize(ev[1]) != hipSuccess) r = WFA_ERR_DEVICE;
-        if (!r && hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) r = WFA_ERR_DEVICE;
-        if (r) break;
-      }
+    e->shapes[key] = std::make_pair(nw, 16);
+    if (plan(e, p, kind, mode, b->seq_format, max_len, m, 0, &c) != WFA_OK || c.nw != nw)
+      continue;
+    const int g = std::max(1, c.grid / e->num_cus);
+    float ms = 0.f;
+    for (int rep_i = warmed ? 1 : 0; rep_i < 2; ++rep_i) {
+      // the first pass over the sample and its retry passes, as a batch
+      // runs: a shape whose slots exhaust the history pool pays for it
+      r = hipEventRecord(ev[0], st) == hipSuccess ? WFA_OK : WFA_ERR_DEVICE;
+      if (!r) r = run_launch(e, p, c, m, sample, e->d_lists[0], st);
+      int32_t nretry = 0;
+      if (!r) r = read_counter(e, 1, st, &nretry);
+      if (!r) r = retry_passes(e, p, kind, mode, b, max_len, st, nretry);
+      if (!r && hipEventRecord(ev[1], st) != hipSuccess) r = WFA_ERR_DEVICE;
+      if (!r && hipEventSynchronize(ev[1]) != hipSuccess) r = WFA_ERR_DEVICE;
+      if (!r && hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) r = WFA_ERR_DEVICE;
       if (r) break;
-      warmed = true;
-      wfa_tune_trial_t& t = rep->trials[rep->num_trials++];
-      t.waves_per_pair = c.nw;
-      t.groups_per_cu = std::max(1, c.grid / e->num_cus);
-      t.ms = ms;
-      if (best.first == 0 || ms < best_ms) {
-        best_ms = ms;
-        best = std::make_pair(c.nw, g);
-      }
     }
     if (r) break;
+    warmed = true;
+    wfa_tune_trial_t& t = rep->trials[rep->num_trials++];
+    t.waves_per_pair = c.nw;
+    t.groups_per_cu = g;
+    t.ms = ms;
+    if (best.first == 0 || ms < best_ms) {
+      best_ms = ms;
+      best = std::make_pair(c.nw, g);
+    }
   }
   hipEventDestroy(ev[0]);
   hipEventDestroy(ev[1]);
@@ -675,6 +710,7 @@ void wfa_engine_free(wfa_engine_t* e) {
   hipSetDevice(e->device);
   hipDeviceSynchronize();
   hipFree(e->d_counters);
+  if (e->d_phase) hipFree(e->d_phase);
   if (e->own_arena) {
     hipFree(e->arena);
   } else if (!e->arena) {
@@ -755,10 +791,28 @@ int32_t wfa_engine_get_launch_shape(wfa_engine_t* e, const wfa_attr_t* attr, int
   return WFA_OK;
 }
 
+int32_t wfa_engine_last_phase_cycles(wfa_engine_t* e, uint64_t* out) {
+  if (!e || !out) return WFA_ERR_INVALID_ARGUMENT;
+#ifdef WFA_PHASE_CLOCKS
+  std::lock_guard<std::mutex> g(e->mu);
+  unsigned long long v[4] = {0, 0, 0, 0};
+  if (e->d_phase) {
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipMemcpy(v, e->d_phase, sizeof(v), hipMemcpyDeviceToHost));
+  }
+  for (int i = 0; i < 4; ++i) out[i] = v[i];
+  return WFA_OK;
+#else
+  (void)out;
+  return WFA_ERR_UNSUPPORTED;
+#endif
+}
+
 int32_t wfa_engine_set_scratch_budget(wfa_engine_t* e, int64_t bytes) {
   if (!e || bytes < ((int64_t)64 << 20)) return WFA_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> g(e->mu);
   e->scratch_budget = bytes;
+  e->auto_budget = false;
   return WFA_OK;
 }
 
@@ -1088,6 +1142,23 @@ int32_t wfa_align_end2end(wfa_aligner_t* a, const uint8_t* P, int32_t plen, cons
   a->err = aligner_run(a, P, plen, T, tlen, WFA_SEQ_ASCII);
   a->attr = saved;
   return a->err ? INT32_MIN : a->score;
+}
+
+int32_t wfa_align_lambda_classes(wfa_aligner_t* a, const uint8_t* P, int32_t plen,
+                                 const uint8_t* T, int32_t tlen, const uint8_t* pattern_class,
+                                 const uint8_t* text_class) {
+  if (!a) return INT32_MIN;
+  if (!pattern_class || !text_class || plen < 0 || tlen < 0 || (plen > 0 && !P) ||
+      (tlen > 0 && !T)) {
+    a->err = WFA_ERR_INVALID_ARGUMENT;
+    return INT32_MIN;
+  }
+  // pattern[i] matches text[j] iff their classes are equal: align the class
+  // sequences, whose bytes the extension compares
+  std::vector<uint8_t> pc((size_t)plen), tc((size_t)tlen);
+  for (int32_t i = 0; i < plen; ++i) pc[i] = pattern_class[P[i]];
+  for (int32_t j = 0; j < tlen; ++j) tc[j] = text_class[T[j]];
+  return wfa_align_end2end(a, pc.data(), plen, tc.data(), tlen);
 }
 
 int32_t wfa_align_endsfree(wfa_aligner_t* a, const uint8_t* P, int32_t plen, const uint8_t* T,

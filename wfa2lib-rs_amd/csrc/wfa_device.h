@@ -88,6 +88,10 @@ struct KParams {
   int tab_cap;
   int chunk;
   int pool_chunks;
+  // phase-clock builds (WFA_PHASE_CLOCKS) of the short-read kernel: cycles
+  // summed over the launch, {pairs, sequence staging, extension, whole
+  // pairs}; null otherwise
+  unsigned long long* phase;
 };
 
 // ---------------------------------------------------------------------

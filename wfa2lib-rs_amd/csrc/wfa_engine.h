@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cstring>
 #include <map>
@@ -146,8 +147,13 @@ struct wfa_engine_t {
   void* d_order = nullptr;   // longest-first work order: keys, sorted keys, values, order, temp
   size_t order_bytes = 0;
   int last_retries = 0;
-  // device scratch one launch may take (wavefront rings, CIGAR history)
+  // device scratch one launch may take (wavefront rings, CIGAR history);
+  // `auto_budget`: recomputed before each call from the device's free
+  // memory (refresh_budget) until wfa_engine_set_scratch_budget fixes it
   int64_t scratch_budget = (int64_t)48 << 30;
+  bool auto_budget = true;
+  int64_t long_budget = 0;  // the last long-read budget and when it was taken
+  std::chrono::steady_clock::time_point long_budget_at;
   // most workgroups of a long-read launch (0 = as many as fit): a serving
   // loop leaves room on every CU for the short requests' launches
   int long_grid_cap = 0;
@@ -172,6 +178,8 @@ struct wfa_engine_t {
   // the arena is the engine's own allocation (a serving loop's engines:
   // one allocation up front, none while launches of other engines run)
   bool own_arena = false;
+  // phase-clock builds: the short-read kernel's cycle sums of the last call
+  unsigned long long* d_phase = nullptr;
   // Launch shapes of long-read launches (pairs over 1024 bases, rings in
   // HBM, no heuristic) by (kind, mode, length bucket): {waves per pair,
   // workgroups per CU}; from wfa_engine_autotune or
@@ -221,6 +229,51 @@ inline int egrow(wfa_engine_t* e, void** ptr, size_t* have, size_t need) {
   *have = need;
   e->arena_top = at + need;
   return WFA_OK;
+}
+
+// Device bytes this engine holds in its scratch buffers
+inline int64_t held_bytes(const wfa_engine_t* e) {
+  if (e->arena) return 0;
+  int64_t h = (int64_t)(e->hist_bytes + e->hdr_bytes + e->gring_bytes + e->chain_bytes +
+                        e->tab_bytes + e->dom_bytes + e->ztr_bytes + e->zsign_bytes +
+                        e->order_bytes) +
+              (int64_t)e->list_cap * 8;
+  for (size_t c : e->bi_cap) h += (int64_t)c;
+  return h;
+}
+
+// The automatic scratch budget of a call. Short reads: 48 GiB. Long reads
+// (`long_reads`: sequences over 1024 bases, and BiWFA): half of what this
+// engine could reach (the device's free memory and what it already
+// holds), between 48 GiB (or 90% of that reach, when less is free) and
+// 128 GiB: on an idle 288 GB device a long-read launch keeps twice the
+// workgroups' rows and history a fixed 48 GiB allowed, and engines that
+// start in a process already holding memory get less. The device query
+// (hipMemGetInfo, about a millisecond) runs at most once a second.
+inline void refresh_budget(wfa_engine_t* e, bool long_reads) {
+  if (!e->auto_budget || e->arena) return;
+  constexpr int64_t kShort = (int64_t)48 << 30;
+  if (!long_reads) {
+    e->scratch_budget = kShort;
+    return;
+  }
+  const auto now = std::chrono::steady_clock::now();
+  if (e->long_budget > 0 && now - e->long_budget_at < std::chrono::seconds(1)) {
+    e->scratch_budget = e->long_budget;
+    return;
+  }
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    e->scratch_budget = kShort;
+    return;
+  }
+  const int64_t reach = (int64_t)free_b + held_bytes(e);
+  int64_t b = std::min<int64_t>(reach / 2, (int64_t)128 << 30);
+  b = std::max<int64_t>(b, std::min<int64_t>(kShort, reach / 10 * 9));
+  e->long_budget = std::max<int64_t>(b, (int64_t)64 << 20);
+  e->long_budget_at = now;
+  e->scratch_budget = e->long_budget;
 }
 
 inline int kind_of(int metric) {

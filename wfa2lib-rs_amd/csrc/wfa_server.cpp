@@ -423,6 +423,7 @@ wfa_server_t* wfa_server_new(const wfa_server_config_t* cfg, int32_t* err) {
     // Admission keeps the predicted total within the budget; an engine's
     // own launches may use up to twice its even share (at most the whole
     // budget), so one large request fits in one engine
+    w->e->auto_budget = false;
     w->e->scratch_budget = std::max<int64_t>(
         std::min(c.memory_budget, 2 * c.memory_budget / c.num_engines), (int64_t)1 << 30);
     // a long-read launch takes at most half a workgroup per CU on average,

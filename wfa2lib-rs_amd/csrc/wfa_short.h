@@ -155,7 +155,7 @@ template <> struct ShPen<1> { static constexpr int gsh = 1, x = 2, oe = 4, e = 1
 template <int MODE, int RC, int PEN>
 static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen,
                                   const uint64_t* sP, const uint64_t* sT, int16_t* rows,
-                                  int16_t* hcells, int32_t* hhdr) {
+                                  int16_t* hcells, int32_t* hhdr, uint64_t& ext_cycles) {
   constexpr bool CIG = MODE == 1;
   constexpr int RM = ShRing<RC>::M, RE = ShRing<RC>::E;
   // ring rows as LDS byte offsets from `rows`: reduced score t's M row at
@@ -372,6 +372,9 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
       // (termination_end2end, src/termination.rs:14-23) ----
       bool reached = false;
       int16_t* hrow = CIG ? hcells + (int64_t)t * SH_WIN - base : nullptr;
+#ifdef WFA_PHASE_CLOCKS
+      const uint64_t c0 = __builtin_readcyclecounter();
+#endif
 #pragma unroll
       for (int j = 0; j < NCH; ++j) {
         const int k = a + 64 * j + lane;
@@ -381,6 +384,9 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
         if (CIG) hrow[k] = o16;
         reached |= k == k_end && off >= tlen;
       }
+#ifdef WFA_PHASE_CLOCKS
+      ext_cycles += __builtin_readcyclecounter() - c0;
+#endif
       if (CIG) {
         hbuf = lane == (t & 63) ? sh_hdr(a, a + 64 * NCH - 1) : hbuf;
         if ((t & 63) == 63) hhdr[(t & ~63) + lane] = hbuf;
@@ -464,15 +470,25 @@ __global__ void __launch_bounds__(64) wfa_short_kernel(KParams p) {
                 : nullptr;
   int32_t* hhdr =
       MODE == 1 ? reinterpret_cast<int32_t*>(p.hdr) + (int64_t)blockIdx.x * p.hdr_cap : nullptr;
+  // phase clocks (WFA_PHASE_CLOCKS builds): pairs, staging, extension and
+  // whole-pair cycles of this wave
+  uint64_t npairs = 0, stage_cycles = 0, ext_cycles = 0, pair_cycles = 0;
   for (;;) {
     int item = 0;
     if (threadIdx.x == 0) item = atomicAdd(&p.counters[0], 1);
     item = uniform(__shfl(item, 0));
     if (item >= p.num_pairs) break;
+#ifdef WFA_PHASE_CLOCKS
+    const uint64_t c0 = __builtin_readcyclecounter();
+#endif
     // pair-level values are uniform: keep them in scalar registers
     const int pair = uniform(p.pair_list ? p.pair_list[item] : p.pair_base + item);
     const int plen = uniform(p.plen[pair]), tlen = uniform(p.tlen[pair]);
     const int bits = stage_sequences<1>(p, pair, plen, tlen, sP, sT, nullptr);
+#ifdef WFA_PHASE_CLOCKS
+    const uint64_t c1 = __builtin_readcyclecounter();
+    stage_cycles += c1 - c0;
+#endif
     if (bits != 2) {
       // does not fit, or not A/C/G/T: the general kernel takes it
       if (threadIdx.x == 0) {
@@ -482,9 +498,26 @@ __global__ void __launch_bounds__(64) wfa_short_kernel(KParams p) {
       }
       continue;
     }
-    short_pair<MODE, RC, PEN>(p, pair, plen, tlen, sP, sT, rows, hcells, hhdr);
+    short_pair<MODE, RC, PEN>(p, pair, plen, tlen, sP, sT, rows, hcells, hhdr, ext_cycles);
     __syncthreads();
+#ifdef WFA_PHASE_CLOCKS
+    pair_cycles += __builtin_readcyclecounter() - c0;
+    npairs += 1;
+#endif
   }
+#ifdef WFA_PHASE_CLOCKS
+  if (p.phase && threadIdx.x == 0) {
+    atomicAdd(&p.phase[0], (unsigned long long)npairs);
+    atomicAdd(&p.phase[1], (unsigned long long)stage_cycles);
+    atomicAdd(&p.phase[2], (unsigned long long)ext_cycles);
+    atomicAdd(&p.phase[3], (unsigned long long)pair_cycles);
+  }
+#else
+  (void)npairs;
+  (void)stage_cycles;
+  (void)ext_cycles;
+  (void)pair_cycles;
+#endif
 }
 
 // LDS bytes of the short kernel's ring rows for ring class `rc`
