@@ -575,6 +575,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6),
             "traffic": traffic,
+            # the measured HBM bytes of one launch over this run's kernel time
+            "traffic_GBps": round(traffic / (first_ms * 1e-3) / 1e9, 1) if traffic else None,
             "binding": issue_limit(args.workload),
             "extend_phase": extend_phase(args.workload),
         },

@@ -427,8 +427,9 @@ int32_t wfa_engine_last_kernel_ms(const wfa_engine_t* engine, float* first_pass_
  * After a batch call whose first pass ran the short-read kernel: out[0]
  * pairs it aligned, out[1] cycles its waves spent staging sequences from
  * HBM into LDS, out[2] cycles extending M rows (the LCP over the staged
- * sequences), out[3] cycles per pair in all, summed over the waves.
- * WFA_ERR_UNSUPPORTED in a release build. */
+ * sequences), out[3] cycles per pair in all, out[4] cycles in the CIGAR
+ * backtrace, summed over the waves (five values). WFA_ERR_UNSUPPORTED in a
+ * release build. */
 int32_t wfa_engine_last_phase_cycles(wfa_engine_t* engine, uint64_t* out);
 
 /* ======================================================================

@@ -417,8 +417,8 @@ int align_batch_impl(wfa_engine_t* e, const wfa_attr_t* attr, const wfa_batch_t*
                      hipStream_t st, int first, int count) {
 #ifdef WFA_PHASE_CLOCKS
   HIPCHK(hipSetDevice(e->device));
-  if (!e->d_phase) HIPCHK(hipMalloc((void**)&e->d_phase, 4 * sizeof(unsigned long long)));
-  HIPCHK(hipMemsetAsync(e->d_phase, 0, 4 * sizeof(unsigned long long), st));
+  if (!e->d_phase) HIPCHK(hipMalloc((void**)&e->d_phase, 5 * sizeof(unsigned long long)));
+  HIPCHK(hipMemsetAsync(e->d_phase, 0, 5 * sizeof(unsigned long long), st));
 #endif
 #ifdef WFA_DEVICE_CHECKS
   HIPCHK(hipSetDevice(e->device));
@@ -795,12 +795,12 @@ int32_t wfa_engine_last_phase_cycles(wfa_engine_t* e, uint64_t* out) {
   if (!e || !out) return WFA_ERR_INVALID_ARGUMENT;
 #ifdef WFA_PHASE_CLOCKS
   std::lock_guard<std::mutex> g(e->mu);
-  unsigned long long v[4] = {0, 0, 0, 0};
+  unsigned long long v[5] = {0, 0, 0, 0, 0};
   if (e->d_phase) {
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipMemcpy(v, e->d_phase, sizeof(v), hipMemcpyDeviceToHost));
   }
-  for (int i = 0; i < 4; ++i) out[i] = v[i];
+  for (int i = 0; i < 5; ++i) out[i] = v[i];
   return WFA_OK;
 #else
   (void)out;

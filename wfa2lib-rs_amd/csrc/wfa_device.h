@@ -90,7 +90,7 @@ struct KParams {
   int pool_chunks;
   // phase-clock builds (WFA_PHASE_CLOCKS) of the short-read kernel: cycles
   // summed over the launch, {pairs, sequence staging, extension, whole
-  // pairs}; null otherwise
+  // pairs, backtrace}; null otherwise
   unsigned long long* phase;
 };
 

@@ -94,6 +94,107 @@ struct WinView {
   }
 };
 
+// M[s] at diagonals k1 and k2 of one row (WinView::m twice), or -1 where
+// absent. The cells' addresses follow from the window alone, so their loads
+// and the header's go out together.
+static __device__ __forceinline__ void win_cells(const WinView& hv, int s, int k1, int k2,
+                                                 int& v1, int& v2) {
+  v1 = v2 = -1;
+  if (s < 0 || s >= hv.nscores || s >= hv.nwf || (s & ((1 << hv.gsh) - 1))) return;
+  const int t = s >> hv.gsh;
+  const int w1 = k1 - hv.base, w2 = k2 - hv.base;
+  const bool in1 = (uint32_t)w1 < (uint32_t)SH_WIN, in2 = (uint32_t)w2 < (uint32_t)SH_WIN;
+  if (!in1 && !in2) return;
+  const int16_t* row = hv.cells + (int64_t)t * SH_WIN;
+  const int h = hv.hdr[t];
+  const int c1 = in1 ? row[w1] : -1;
+  const int c2 = in2 ? row[w2] : -1;
+  const int lo = (int)(int16_t)(h & 0xffff), hi = h >> 16;
+  v1 = k1 >= lo && k1 <= hi ? c1 : -1;
+  v2 = k2 >= lo && k2 <= hi ? c2 : -1;
+}
+
+// Singletrack backtrace (src/backtrace.rs:751-858) of the short kernel's
+// end-to-end gap-affine alignments: the steps of `singletrack` for one gap
+// piece, arranged for latency. At each M cell the wave loads the mismatch
+// source and the sources of gap lengths 1..64 (one lane per length,
+// insertion and deletion) at once, and counts the matches before the cell
+// in the staged sequences while those loads are in flight; a longer gap
+// loads 64 more lengths at a time. The ops are written backwards into `ops`
+// (LDS), ending at ops[cap - 1]; returns their number.
+static __device__ __forceinline__ int short_backtrace(const KParams& p, const WinView& hv,
+                                                      const uint64_t* sP, const uint64_t* sT,
+                                                      int plen, int tlen, int score, int k,
+                                                      int offset, uint8_t* ops, int cap) {
+  const int lane = threadIdx.x;
+  const int x = p.x, o = p.o1, e = p.e1;
+  int pos = cap - 1;
+  write_ops(ops, pos, 'D', plen - (offset - k));
+  write_ops(ops, pos, 'I', tlen - offset);
+  while (score != 0) {
+    score = uniform(score);
+    k = uniform(k);
+    offset = uniform(offset);
+    int mo, unused;
+    win_cells(hv, score - x, k, k, mo, unused);
+    int L = 1 + lane;
+    int sg = score - o - L * e;
+    int mi, md;
+    win_cells(hv, sg, k - L, k + L, mi, md);
+    const int nm = backward_matches<2>(sP, sT, offset - k, offset);
+    const int orig = offset - nm;
+    if (mo >= 0 && mo == orig - 1) {
+      write_ops(ops, pos, 'M', nm);
+      write_ops(ops, pos, 'X', 1);
+      offset = orig - 1;
+      score -= x;
+      continue;
+    }
+    // the lowest gap length with a source (or past the score) decides,
+    // insertion before deletion
+    bool found = false;
+    for (int l = 0;; l += 64) {
+      if (l > 0) {
+        L = l + 1 + lane;
+        sg = score - o - L * e;
+        win_cells(hv, sg, k - L, k + L, mi, md);
+      }
+      const bool ri = mi >= 0 && mi + L >= orig && mi + L <= offset;
+      const bool rd = md >= 0 && md >= orig && md <= offset;
+      const uint64_t mask = ballot(ri || rd || sg < 0);
+      if (!mask) continue;
+      const int j = __builtin_ctzll(mask);
+      const int Lj = l + 1 + j;
+      const int fj = uniform(__shfl((ri ? 1 : 0) | (rd ? 2 : 0), j));
+      const int m = uniform(__shfl(ri ? mi : md, j));
+      if (fj & 1) {
+        write_ops(ops, pos, 'M', offset - (m + Lj));
+        write_ops(ops, pos, 'I', Lj);
+        k -= Lj;
+      } else if (fj & 2) {
+        write_ops(ops, pos, 'M', offset - m);
+        write_ops(ops, pos, 'D', Lj);
+        k += Lj;
+      } else {
+        break;  // no source
+      }
+      offset = m;
+      score -= o + Lj * e;
+      found = true;
+      break;
+    }
+    if (!found) break;
+  }
+  int v = offset - k, h = offset;
+  const int nm = imin(v, h);
+  v -= nm;
+  h -= nm;
+  write_ops(ops, pos, 'M', nm);
+  write_ops(ops, pos, 'D', v);
+  write_ops(ops, pos, 'I', h);
+  return cap - 1 - pos;
+}
+
 static __device__ __forceinline__ int32_t sh_hdr(int lo, int hi) {
   return (int32_t)(((uint32_t)hi << 16) | ((uint32_t)lo & 0xffffu));
 }
@@ -155,7 +256,8 @@ template <> struct ShPen<1> { static constexpr int gsh = 1, x = 2, oe = 4, e = 1
 template <int MODE, int RC, int PEN>
 static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen,
                                   const uint64_t* sP, const uint64_t* sT, int16_t* rows,
-                                  int16_t* hcells, int32_t* hhdr, uint64_t& ext_cycles) {
+                                  int16_t* hcells, int32_t* hhdr, uint64_t& ext_cycles,
+                                  uint64_t& bt_cycles) {
   constexpr bool CIG = MODE == 1;
   constexpr int RM = ShRing<RC>::M, RE = ShRing<RC>::E;
   // ring rows as LDS byte offsets from `rows`: reduced score t's M row at
@@ -421,16 +523,31 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
   if (status == WFA_STATUS_END_REACHED) {
     out_status = WFA_STATUS_COMPLETED;
     if (CIG) {
+#ifdef WFA_PHASE_CLOCKS
+      const uint64_t b0 = __builtin_readcyclecounter();
+#endif
       // the headers of the last, partial block
       if (lane <= (t & 63)) hhdr[(t & ~63) + lane] = hbuf;
       __threadfence_block();
       uint8_t* out = p.cig + p.cig_off[pair];
       const int cap = plen + tlen;
-      ncig = singletrack<2>(p, WinView{hcells, hhdr, base, result + 1, nwf, gsh}, sP, sT, plen,
-                            tlen, result, k_end, tlen, out, cap);
+      const WinView hv{hcells, hhdr, base, result + 1, nwf, gsh};
+      // the ring rows are done with: the ops go there when they fit, and
+      // from there to the pair's region in order
+      constexpr int OPS_CAP = (RM + 2 * RE) * SH_ROW * 2;
+      if (cap <= OPS_CAP) {
+        uint8_t* ops = reinterpret_cast<uint8_t*>(rows);
+        ncig = short_backtrace(p, hv, sP, sT, plen, tlen, result, k_end, tlen, ops, cap);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const uint8_t* src = ops + (cap - ncig);
+        for (int i = lane; i < ncig; i += 64) out[i] = src[i];
+      } else {
+        ncig = singletrack<2>(p, hv, sP, sT, plen, tlen, result, k_end, tlen, out, cap);
+      }
       // move the ops from the tail of the region to its head
       const int gap = cap - ncig;
-      if (gap > 0) {
+      if (cap > OPS_CAP && gap > 0) {
         for (int c = 0; c < ncig; c += 64) {
           const int i = c + lane;
           uint8_t v = 0;
@@ -440,6 +557,9 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
           __builtin_amdgcn_wave_barrier();
         }
       }
+#ifdef WFA_PHASE_CLOCKS
+      bt_cycles += __builtin_readcyclecounter() - b0;
+#endif
     }
   }
   if (lane == 0) {
@@ -459,8 +579,10 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
 // One wave per pair, persistent over the launch's work queue. The slot's
 // history: p.hdr_cap rows of SH_WIN 16-bit cells in p.hist and p.hdr_cap
 // packed headers in p.hdr, both per workgroup.
+// The CIGAR kernel is held to 80 VGPRs (6 waves per SIMD, the 24 waves per
+// CU that its LDS allows); unbounded it takes 83 and runs 20 waves per CU.
 template <int MODE, bool PACKED, int RC, int PEN>
-__global__ void __launch_bounds__(64) wfa_short_kernel(KParams p) {
+__global__ void __launch_bounds__(64, MODE == 1 ? 6 : 1) wfa_short_kernel(KParams p) {
   extern __shared__ uint64_t smem[];
   uint64_t* sP = smem;
   uint64_t* sT = smem + p.seq_words;
@@ -472,7 +594,7 @@ __global__ void __launch_bounds__(64) wfa_short_kernel(KParams p) {
       MODE == 1 ? reinterpret_cast<int32_t*>(p.hdr) + (int64_t)blockIdx.x * p.hdr_cap : nullptr;
   // phase clocks (WFA_PHASE_CLOCKS builds): pairs, staging, extension and
   // whole-pair cycles of this wave
-  uint64_t npairs = 0, stage_cycles = 0, ext_cycles = 0, pair_cycles = 0;
+  uint64_t npairs = 0, stage_cycles = 0, ext_cycles = 0, pair_cycles = 0, bt_cycles = 0;
   for (;;) {
     int item = 0;
     if (threadIdx.x == 0) item = atomicAdd(&p.counters[0], 1);
@@ -498,7 +620,8 @@ __global__ void __launch_bounds__(64) wfa_short_kernel(KParams p) {
       }
       continue;
     }
-    short_pair<MODE, RC, PEN>(p, pair, plen, tlen, sP, sT, rows, hcells, hhdr, ext_cycles);
+    short_pair<MODE, RC, PEN>(p, pair, plen, tlen, sP, sT, rows, hcells, hhdr, ext_cycles,
+                              bt_cycles);
     __syncthreads();
 #ifdef WFA_PHASE_CLOCKS
     pair_cycles += __builtin_readcyclecounter() - c0;
@@ -511,8 +634,10 @@ __global__ void __launch_bounds__(64) wfa_short_kernel(KParams p) {
     atomicAdd(&p.phase[1], (unsigned long long)stage_cycles);
     atomicAdd(&p.phase[2], (unsigned long long)ext_cycles);
     atomicAdd(&p.phase[3], (unsigned long long)pair_cycles);
+    atomicAdd(&p.phase[4], (unsigned long long)bt_cycles);
   }
 #else
+  (void)bt_cycles;
   (void)npairs;
   (void)stage_cycles;
   (void)ext_cycles;
