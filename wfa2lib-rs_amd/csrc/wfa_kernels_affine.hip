@@ -11,12 +11,12 @@ KernelFn pick_affine(int mode, bool lds, bool i16, bool packed, bool heur, int n
 }
 
 // The short-read kernel (gap-affine, end-to-end, no heuristic): score only
-// or high-memory CIGAR, for ring class `rc` (2: ring class 0 with the
-// default penalties built in)
+// or high-memory CIGAR, for ring class `rc` (2: the default penalties built
+// in, with the smallest ring they allow)
 KernelFn pick_affine_short(int mode, bool packed, int rc) {
   if (rc == 2) {
-    if (mode == 0) return packed ? wfa_short_kernel<0, true, 0, 1> : wfa_short_kernel<0, false, 0, 1>;
-    if (mode == 1) return packed ? wfa_short_kernel<1, true, 0, 1> : wfa_short_kernel<1, false, 0, 1>;
+    if (mode == 0) return packed ? wfa_short_kernel<0, true, 2, 1> : wfa_short_kernel<0, false, 2, 1>;
+    if (mode == 1) return packed ? wfa_short_kernel<1, true, 2, 1> : wfa_short_kernel<1, false, 2, 1>;
   } else if (rc == 0) {
     if (mode == 0) return packed ? wfa_short_kernel<0, true, 0, 0> : wfa_short_kernel<0, false, 0, 0>;
     if (mode == 1) return packed ? wfa_short_kernel<1, true, 0, 0> : wfa_short_kernel<1, false, 0, 0>;
@@ -27,7 +27,7 @@ KernelFn pick_affine_short(int mode, bool packed, int rc) {
   return nullptr;
 }
 
-size_t short_lds_row_bytes_host(int rc) { return short_lds_row_bytes(rc == 2 ? 0 : rc); }
+size_t short_lds_row_bytes_host(int rc) { return short_lds_row_bytes(rc); }
 
 int short_ring_class(int x, int oe, int e, int* gsh) {
   // g: the largest power of two dividing all three penalties

@@ -25,8 +25,10 @@
 //   test: the reference NULL-fills exactly those cells before it reads them
 //   (src/wavefront.rs:422-457). An absent wavefront is a row of NULL cells
 //   as well: a null step writes its rows NULL, and the rows of scores below
-//   zero are the initial fill (a ring holds more rows than any lookback), so
-//   a step reads its inputs from their ring slots with no presence test.
+//   zero are the initial fill (a ring holds as many rows as the longest
+//   lookback: a step reads all its inputs before it writes an output, so
+//   its output may take the slot of its oldest input), so a step reads its
+//   inputs from their ring slots with no presence test.
 // * The chunks a step computes cover [lo - 1, hi + 1] of the union of every
 //   row computed so far. That holds the reference's computed range (the
 //   union of the inputs' ranges, widened by one, src/aligner.rs:2045-2062);
@@ -66,10 +68,12 @@ constexpr int SH_HDR_EMPTY = 0x00000001; // header of an absent M row: lo 1, hi 
 
 // Ring classes: M rows and I1/D1 rows per ring (powers of two, so a slot is
 // a mask of the reduced score), for reduced lookbacks x, o + e < M and
-// e < E.
+// e < E; class 2 is sized for the built-in penalties alone (x 2, o + e 4,
+// e 1 reduced: lookbacks of exactly M and E rows).
 template <int RC> struct ShRing;
 template <> struct ShRing<0> { static constexpr int M = 8, E = 2; };
 template <> struct ShRing<1> { static constexpr int M = 16, E = 8; };
+template <> struct ShRing<2> { static constexpr int M = 4, E = 1; };
 
 static __device__ __forceinline__ int16_t sh16(int v) { return (int16_t)imax(v, SH_NULL); }
 
@@ -647,7 +651,9 @@ __global__ void __launch_bounds__(64, MODE == 1 ? 6 : 1) wfa_short_kernel(KParam
 
 // LDS bytes of the short kernel's ring rows for ring class `rc`
 static inline size_t short_lds_row_bytes(int rc) {
-  const int rows = rc == 0 ? ShRing<0>::M + 2 * ShRing<0>::E : ShRing<1>::M + 2 * ShRing<1>::E;
+  const int rows = rc == 0   ? ShRing<0>::M + 2 * ShRing<0>::E
+                   : rc == 1 ? ShRing<1>::M + 2 * ShRing<1>::E
+                             : ShRing<2>::M + 2 * ShRing<2>::E;
   return (size_t)rows * SH_ROW * 2;
 }
 
