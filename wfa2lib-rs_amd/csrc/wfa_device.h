@@ -179,7 +179,7 @@ size_t short_lds_row_bytes_host(int rc);
 // 2^gsh in *gsh): 2 for the built-in default penalties, else 0 or 1, or -1
 // when their reduced lookbacks exceed every class
 int short_ring_class(int x, int oe, int e, int* gsh);
-constexpr int kShortWindow = 256;
+constexpr int kShortWindow = 512;
 
 hipError_t launch_reset(int32_t* counters, hipStream_t stream);
 hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, int32_t* out,

@@ -16,7 +16,7 @@
 //   penalties divided by g, and accounts for the real scores it passes over
 //   exactly: each counts towards the run of null steps (src/aligner.rs:
 //   2031-2043) and meets the per-score limits in order.
-// * Each pair gets a fixed window of 256 diagonals [base, base + 256),
+// * Each pair gets a fixed window of 512 diagonals [base, base + 512),
 //   centred on the diagonals its alignment must cross. Every ring row is an
 //   LDS row over the whole window plus pad cells on each side, all NULL when
 //   the pair starts. A step writes its three output rows over the
@@ -60,7 +60,7 @@
 
 namespace wfa {
 
-constexpr int SH_WIN = 256;              // diagonals of a pair's window
+constexpr int SH_WIN = kShortWindow;     // diagonals of a pair's window
 constexpr int SH_CHUNKS = SH_WIN / 64;   // most 64-diagonal chunks a step computes
 constexpr int SH_ROW = SH_WIN + 8;       // LDS cells per row: pad, window, pad (16-byte rows)
 constexpr int SH_NULL = -16384;          // OFFSET_NULL as held in 16 bits
@@ -513,7 +513,11 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
       case 1: done = compute(std::integral_constant<int, 1>{}); break;
       case 2: done = compute(std::integral_constant<int, 2>{}); break;
       case 3: done = compute(std::integral_constant<int, 3>{}); break;
-      default: done = compute(std::integral_constant<int, 4>{}); break;
+      case 4: done = compute(std::integral_constant<int, 4>{}); break;
+      case 5: done = compute(std::integral_constant<int, 5>{}); break;
+      case 6: done = compute(std::integral_constant<int, 6>{}); break;
+      case 7: done = compute(std::integral_constant<int, 7>{}); break;
+      default: done = compute(std::integral_constant<int, 8>{}); break;
     }
     if (done) break;
   }
@@ -583,8 +587,8 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
 // One wave per pair, persistent over the launch's work queue. The slot's
 // history: p.hdr_cap rows of SH_WIN 16-bit cells in p.hist and p.hdr_cap
 // packed headers in p.hdr, both per workgroup.
-// The CIGAR kernel is held to 80 VGPRs (6 waves per SIMD, the 24 waves per
-// CU that its LDS allows); unbounded it takes 83 and runs 20 waves per CU.
+// The CIGAR kernel is held to 80 VGPRs (6 waves per SIMD: 24 per CU, about
+// what its LDS allows); unbounded it takes 89 and runs 20 waves per CU.
 template <int MODE, bool PACKED, int RC, int PEN>
 __global__ void __launch_bounds__(64, MODE == 1 ? 6 : 1) wfa_short_kernel(KParams p) {
   extern __shared__ uint64_t smem[];
