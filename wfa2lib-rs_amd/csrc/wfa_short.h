@@ -40,11 +40,11 @@
 // * Which ring rows hold a wavefront (not absent, not trimmed empty) is kept
 //   by age in two scalar masks, only to detect null steps.
 // * In CIGAR scope the extended M row of every reduced score goes to the
-//   slot's history in HBM at a fixed stride (cell k of reduced score t at
-//   t * 256 + k - base, 16-bit), with the computed chunks' first and last
-//   diagonal packed in one int32 header per reduced score; the Singletrack
-//   backtrace reads it through WinView, which finds no wavefront at the
-//   scores in between.
+//   slot's history in HBM, its computed chunks packed one after another
+//   (16-bit cells, whole 128-byte lines), with one int32 header per reduced
+//   score: the row's first diagonal, its chunk count and its first chunk.
+//   The Singletrack backtrace reads it through WinView, which finds no
+//   wavefront at the scores in between.
 //
 // Offsets are held in 16 bits as in the general kernel's short-read rings
 // (sequences of at most 8000 bases, scores up to 16000; NULL-derived values
@@ -64,7 +64,7 @@ constexpr int SH_WIN = kShortWindow;     // diagonals of a pair's window
 constexpr int SH_CHUNKS = SH_WIN / 64;   // most 64-diagonal chunks a step computes
 constexpr int SH_ROW = SH_WIN + 8;       // LDS cells per row: pad, window, pad (16-byte rows)
 constexpr int SH_NULL = -16384;          // OFFSET_NULL as held in 16 bits
-constexpr int SH_HDR_EMPTY = 0x00000001; // header of an absent M row: lo 1, hi 0
+constexpr int SH_HDR_EMPTY = 0;          // header of an absent M row: no chunks
 
 // Ring classes: M rows and I1/D1 rows per ring (powers of two, so a slot is
 // a mask of the reduced score), for reduced lookbacks x, o + e < M and
@@ -77,21 +77,34 @@ template <> struct ShRing<2> { static constexpr int M = 4, E = 1; };
 
 static __device__ __forceinline__ int16_t sh16(int v) { return (int16_t)imax(v, SH_NULL); }
 
+// A history row's header: its first diagonal as an offset into the window
+// (9 bits), its chunk count (4 bits; none for a score without an M row) and
+// its first chunk in the slot's history (19 bits).
+static __device__ __forceinline__ int32_t sh_hdr(int a_rel, int nch, int chunk) {
+  return (int32_t)((uint32_t)a_rel | ((uint32_t)nch << 9) | ((uint32_t)chunk << 13));
+}
+
+// The history cell of diagonal k in the row of header h, or -1 outside the
+// row's chunks.
+static __device__ __forceinline__ int sh_cell(int32_t h, int base, int k) {
+  const uint32_t u = (uint32_t)h;
+  const int w = k - base - (int)(u & 511u);
+  return (uint32_t)w < 64u * ((u >> 9) & 15u) ? (int)(u >> 13) * 64 + w : -1;
+}
+
 // One slot's high-memory history as the Singletrack backtrace reads it
 // (st_m_offset, src/backtrace.rs:645-663): M[s] at diagonal k, if score s
 // has a row covering k and the cell is not NULL. Rows exist for multiples
 // of 2^gsh only.
 struct WinView {
-  const int16_t* cells;  // row t (score t << gsh) at cells + t * SH_WIN, diagonal k at k - base
-  const int32_t* hdr;    // per row: first diagonal (low 16 bits), last (high 16 bits)
+  const int16_t* cells;  // the slot's history: rows of packed 64-cell chunks
+  const int32_t* hdr;    // per reduced score t (score t << gsh): the row's header (sh_hdr)
   int base, nscores, nwf, gsh;
   __device__ __forceinline__ bool m(int s, int k, int* out) const {
     if (s < 0 || s >= nscores || s >= nwf || (s & ((1 << gsh) - 1))) return false;
-    const int t = s >> gsh;
-    const int h = hdr[t];
-    const int lo = (int)(int16_t)(h & 0xffff), hi = h >> 16;
-    if (k < lo || k > hi) return false;
-    const int v = cells[(int64_t)t * SH_WIN + (k - base)];
+    const int i = sh_cell(hdr[s >> gsh], base, k);
+    if (i < 0) return false;
+    const int v = cells[i];
     if (v < 0) return false;
     *out = v;
     return true;
@@ -99,23 +112,15 @@ struct WinView {
 };
 
 // M[s] at diagonals k1 and k2 of one row (WinView::m twice), or -1 where
-// absent. The cells' addresses follow from the window alone, so their loads
-// and the header's go out together.
-static __device__ __forceinline__ void win_cells(const WinView& hv, int s, int k1, int k2,
-                                                 int& v1, int& v2) {
+// absent, with the row headers read from `lhdr` (a copy in LDS).
+static __device__ __forceinline__ void win_cells(const WinView& hv, const int32_t* lhdr, int s,
+                                                 int k1, int k2, int& v1, int& v2) {
   v1 = v2 = -1;
   if (s < 0 || s >= hv.nscores || s >= hv.nwf || (s & ((1 << hv.gsh) - 1))) return;
-  const int t = s >> hv.gsh;
-  const int w1 = k1 - hv.base, w2 = k2 - hv.base;
-  const bool in1 = (uint32_t)w1 < (uint32_t)SH_WIN, in2 = (uint32_t)w2 < (uint32_t)SH_WIN;
-  if (!in1 && !in2) return;
-  const int16_t* row = hv.cells + (int64_t)t * SH_WIN;
-  const int h = hv.hdr[t];
-  const int c1 = in1 ? row[w1] : -1;
-  const int c2 = in2 ? row[w2] : -1;
-  const int lo = (int)(int16_t)(h & 0xffff), hi = h >> 16;
-  v1 = k1 >= lo && k1 <= hi ? c1 : -1;
-  v2 = k2 >= lo && k2 <= hi ? c2 : -1;
+  const int32_t h = lhdr[s >> hv.gsh];
+  const int i1 = sh_cell(h, hv.base, k1), i2 = sh_cell(h, hv.base, k2);
+  if (i1 >= 0) v1 = hv.cells[i1];
+  if (i2 >= 0) v2 = hv.cells[i2];
 }
 
 // Singletrack backtrace (src/backtrace.rs:751-858) of the short kernel's
@@ -124,9 +129,11 @@ static __device__ __forceinline__ void win_cells(const WinView& hv, int s, int k
 // source and the sources of gap lengths 1..64 (one lane per length,
 // insertion and deletion) at once, and counts the matches before the cell
 // in the staged sequences while those loads are in flight; a longer gap
-// loads 64 more lengths at a time. The ops are written backwards into `ops`
-// (LDS), ending at ops[cap - 1]; returns their number.
+// loads 64 more lengths at a time. The row headers come from `lhdr` (LDS).
+// The ops are written backwards into `ops` (LDS), ending at ops[cap - 1];
+// returns their number.
 static __device__ __forceinline__ int short_backtrace(const KParams& p, const WinView& hv,
+                                                      const int32_t* lhdr,
                                                       const uint64_t* sP, const uint64_t* sT,
                                                       int plen, int tlen, int score, int k,
                                                       int offset, uint8_t* ops, int cap) {
@@ -140,11 +147,11 @@ static __device__ __forceinline__ int short_backtrace(const KParams& p, const Wi
     k = uniform(k);
     offset = uniform(offset);
     int mo, unused;
-    win_cells(hv, score - x, k, k, mo, unused);
+    win_cells(hv, lhdr, score - x, k, k, mo, unused);
     int L = 1 + lane;
     int sg = score - o - L * e;
     int mi, md;
-    win_cells(hv, sg, k - L, k + L, mi, md);
+    win_cells(hv, lhdr, sg, k - L, k + L, mi, md);
     const int nm = backward_matches<2>(sP, sT, offset - k, offset);
     const int orig = offset - nm;
     if (mo >= 0 && mo == orig - 1) {
@@ -161,7 +168,7 @@ static __device__ __forceinline__ int short_backtrace(const KParams& p, const Wi
       if (l > 0) {
         L = l + 1 + lane;
         sg = score - o - L * e;
-        win_cells(hv, sg, k - L, k + L, mi, md);
+        win_cells(hv, lhdr, sg, k - L, k + L, mi, md);
       }
       const bool ri = mi >= 0 && mi + L >= orig && mi + L <= offset;
       const bool rd = md >= 0 && md >= orig && md <= offset;
@@ -197,10 +204,6 @@ static __device__ __forceinline__ int short_backtrace(const KParams& p, const Wi
   write_ops(ops, pos, 'D', v);
   write_ops(ops, pos, 'I', h);
   return cap - 1 - pos;
-}
-
-static __device__ __forceinline__ int32_t sh_hdr(int lo, int hi) {
-  return (int32_t)(((uint32_t)hi << 16) | ((uint32_t)lo & 0xffffu));
 }
 
 // Record the first and last set lane of chunk ballot `b` (chunk starting at
@@ -320,10 +323,9 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
     // (the same in every lane: keep it scalar, so the loop state stays
     // scalar)
     const int off = uniform(lcp<2>(sP, sT, 0, 0, plen, tlen));
-    if (lane == 0) {
-      cell(0)[0] = sh16(off);
-      if (CIG) hcells[0 - base] = sh16(off);
-    }
+    if (lane == 0) cell(0)[0] = sh16(off);
+    // the history's first chunk: M0, then NULL cells
+    if (CIG) hcells[lane] = lane == 0 ? sh16(off) : (int16_t)SH_NULL;
     if (k_end == 0 && off >= tlen) status = WFA_STATUS_END_REACHED;
   }
   __syncthreads();
@@ -337,7 +339,8 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
   int ulo = 0, uhi = 0;  // union of the diagonals of every row so far
   // CIGAR headers of 64 consecutive reduced scores, lane i holding
   // (t & ~63) + i, stored once the block is full (and at the end)
-  int hbuf = lane == 0 ? sh_hdr(0, 0) : 0;
+  int hbuf = lane == 0 ? sh_hdr(-base, 1, 0) : 0;
+  int hnext = 1;  // the history's next free chunk
   // the first score at which a per-score limit applies
   int limit = imin(16001, p.max_steps);
   if (!p.modular) limit = imin(limit, nwf);
@@ -359,6 +362,7 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
     agE = (uint32_t)uniform((int)agE);
     lastnn = uniform(lastnn);
     ulim = uniform(ulim);
+    if (CIG) hnext = uniform(hnext);
     const int r = t << gsh;
     // inputs M[t-x], M[t-o-e], I1/D1[t-e]; all absent: a null step
     const bool nul = !(agM & needM) && !(agE & needE);
@@ -477,7 +481,7 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
       // reads M[t] in between), keep it, test the end cell
       // (termination_end2end, src/termination.rs:14-23) ----
       bool reached = false;
-      int16_t* hrow = CIG ? hcells + (int64_t)t * SH_WIN - base : nullptr;
+      int16_t* hrow = CIG ? hcells + (int64_t)hnext * 64 : nullptr;
 #ifdef WFA_PHASE_CLOCKS
       const uint64_t c0 = __builtin_readcyclecounter();
 #endif
@@ -487,15 +491,16 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
         const int off = sh_extend(P32, T32, mv[j], k, plen, tlen);
         const int16_t o16 = sh16(off);
         Wm[k] = o16;
-        if (CIG) hrow[k] = o16;
+        if (CIG) hrow[64 * j + lane] = o16;
         reached |= k == k_end && off >= tlen;
       }
 #ifdef WFA_PHASE_CLOCKS
       ext_cycles += __builtin_readcyclecounter() - c0;
 #endif
       if (CIG) {
-        hbuf = lane == (t & 63) ? sh_hdr(a, a + 64 * NCH - 1) : hbuf;
+        hbuf = lane == (t & 63) ? sh_hdr(a - base, NCH, hnext) : hbuf;
         if ((t & 63) == 63) hhdr[(t & ~63) + lane] = hbuf;
+        hnext += NCH;
       }
       // the step's rows are written before the next step reads them: one
       // wave's LDS accesses complete in order, so only the compiler must
@@ -534,28 +539,38 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
 #ifdef WFA_PHASE_CLOCKS
       const uint64_t b0 = __builtin_readcyclecounter();
 #endif
-      // the headers of the last, partial block
-      if (lane <= (t & 63)) hhdr[(t & ~63) + lane] = hbuf;
+      // (this wave's history writes land before it reads them back)
       __threadfence_block();
       uint8_t* out = p.cig + p.cig_off[pair];
       const int cap = plen + tlen;
       const WinView hv{hcells, hhdr, base, result + 1, nwf, gsh};
-      // the ring rows are done with: the ops go there when they fit, and
-      // from there to the pair's region in order
-      constexpr int OPS_CAP = (RM + 2 * RE) * SH_ROW * 2;
-      if (cap <= OPS_CAP) {
-        uint8_t* ops = reinterpret_cast<uint8_t*>(rows);
-        ncig = short_backtrace(p, hv, sP, sT, plen, tlen, result, k_end, tlen, ops, cap);
+      // The ring rows are done with: the row headers (rows 0..t) and the ops
+      // go there when they fit, and the ops from there to the pair's region
+      // in order.
+      constexpr int LDS_ROWS = (RM + 2 * RE) * SH_ROW * 2;
+      const int hbytes = (t + 1) * 4;
+      const bool in_lds = hbytes + cap <= LDS_ROWS;
+      if (in_lds) {
+        int32_t* lhdr = reinterpret_cast<int32_t*>(rows);
+        for (int b = 0; b < (t & ~63); b += 64) lhdr[b + lane] = hhdr[b + lane];
+        if (lane <= (t & 63)) lhdr[(t & ~63) + lane] = hbuf;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        uint8_t* ops = reinterpret_cast<uint8_t*>(rows) + hbytes;
+        ncig = short_backtrace(p, hv, lhdr, sP, sT, plen, tlen, result, k_end, tlen, ops, cap);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         const uint8_t* src = ops + (cap - ncig);
         for (int i = lane; i < ncig; i += 64) out[i] = src[i];
       } else {
+        // the headers of the last, partial block
+        if (lane <= (t & 63)) hhdr[(t & ~63) + lane] = hbuf;
+        __threadfence_block();
         ncig = singletrack<2>(p, hv, sP, sT, plen, tlen, result, k_end, tlen, out, cap);
       }
       // move the ops from the tail of the region to its head
       const int gap = cap - ncig;
-      if (cap > OPS_CAP && gap > 0) {
+      if (!in_lds && gap > 0) {
         for (int c = 0; c < ncig; c += 64) {
           const int i = c + lane;
           uint8_t v = 0;
@@ -585,8 +600,9 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
 }
 
 // One wave per pair, persistent over the launch's work queue. The slot's
-// history: p.hdr_cap rows of SH_WIN 16-bit cells in p.hist and p.hdr_cap
-// packed headers in p.hdr, both per workgroup.
+// history: room for p.hdr_cap rows of SH_WIN 16-bit cells in p.hist (the
+// rows' chunks packed from its start) and p.hdr_cap row headers in p.hdr,
+// both per workgroup.
 // The CIGAR kernel is held to 80 VGPRs (6 waves per SIMD: 24 per CU, about
 // what its LDS allows); unbounded it takes 89 and runs 20 waves per CU.
 template <int MODE, bool PACKED, int RC, int PEN>
