@@ -36,6 +36,14 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# The serving leg runs one HIP stream per server engine beside this
+# process's other streams. HIP maps streams onto at most GPU_MAX_HW_QUEUES
+# hardware queues (4 unless set), and streams that share a queue run their
+# kernels in order: a 150 bp launch queued behind a 1-10 kbp launch waits
+# for it (p99 of the 150 bp requests 11-18 ms with 4 queues, 0.47 ms with
+# 8). Set before HIP starts; an explicit setting wins.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 METRIC = "aligned pairs/sec (gap-affine, 150bp + 1-10kbp, bit-exact CIGAR) + p99 latency"
 HBM_PEAK_GBS = 8000.0
 
@@ -128,6 +136,16 @@ def serving_leg(n, rate, lengths, seed, oracle_sample, log):
         done.set()
 
     th = threading.Thread(target=poller, daemon=True)
+    # The client's own pauses are not the server's latency: Python's cyclic
+    # collector walking this process's million-pair results, and the
+    # interpreter's 5 ms thread switch between the submitting thread and the
+    # poller, would otherwise land in the measured tail.
+    import gc
+    gc.collect()
+    gc.freeze()
+    gc.disable()
+    switch = sys.getswitchinterval()
+    sys.setswitchinterval(1e-4)
     t0 = time.perf_counter()
     th.start()
     for i, r in enumerate(mix):
@@ -136,11 +154,16 @@ def serving_leg(n, rate, lengths, seed, oracle_sample, log):
             time.sleep(wait)
         srv.submit(attrs[i], r.pattern, r.text, tag=i)
     done.wait(timeout=1200)
+    sys.setswitchinterval(switch)
+    gc.enable()
+    gc.unfreeze()
     t_end = max(recv.values())
     st = srv.stats()
     srv.close()
     assert len(got) == n, f"{len(got)}/{n} results"
     lat = np.array([(recv[i] - (t0 + mix[i].arrival_s)) * 1e3 for i in range(n)])
+    # the server's own clock: submission to result ready (wfa_server_poll)
+    slat = np.array([got[i].latency_us * 1e-3 for i in range(n)])
     short = np.array([len(r.pattern) <= 256 for r in mix])
     errors = sum(1 for r in got.values() if r.error != 0)
     oom = sum(1 for r in got.values() if r.status == N.STATUS_OOM)
@@ -167,6 +190,9 @@ def serving_leg(n, rate, lengths, seed, oracle_sample, log):
         "latency_ms_p50_150bp": round(float(np.percentile(lat[short], 50)), 3) if short.any() else None,
         "latency_ms_p99_150bp": round(float(np.percentile(lat[short], 99)), 3) if short.any() else None,
         "latency_ms_p99_long": round(float(np.percentile(lat[~short], 99)), 3) if (~short).any() else None,
+        "server_latency_ms_p50": round(float(np.percentile(slat, 50)), 3),
+        "server_latency_ms_p99": round(float(np.percentile(slat, 99)), 3),
+        "server_latency_ms_p99_150bp": round(float(np.percentile(slat[short], 99)), 3) if short.any() else None,
         "launches": int(st["launches"]), "errors": errors, "oom": oom,
         "parity_sample": f"{same}/{len(idx)}",
         "lengths": [list(x) for x in lengths],

@@ -474,7 +474,12 @@ typedef struct wfa_server_result_t {
 
 /* Defaults: device 0, 4 engines, 65536 requests per launch, 64 GiB. */
 void wfa_server_config_default(wfa_server_config_t* config);
-/* Start the engines and their worker threads; NULL and *err on failure. */
+/* Start the engines and their worker threads; NULL and *err on failure.
+ * Each engine launches on its own HIP stream. HIP runs a process's streams
+ * on at most GPU_MAX_HW_QUEUES hardware queues (4 unless the environment
+ * sets it before HIP starts), and streams that share a queue run their
+ * kernels in order, so a short request's launch can wait behind a long
+ * one's: give a serving process at least num_engines + 4 queues. */
 wfa_server_t* wfa_server_new(const wfa_server_config_t* config, int32_t* err);
 /* Queue one request (validated as the aligner validates its configuration:
  * WFA_ERR_INVALID_PENALTIES, WFA_ERR_UNSUPPORTED, WFA_ERR_INVALID_ARGUMENT).
