@@ -41,8 +41,9 @@ sys.path.insert(0, ROOT)
 # hardware queues (4 unless set), and streams that share a queue run their
 # kernels in order: a 150 bp launch queued behind a 1-10 kbp launch waits
 # for it (p99 of the 150 bp requests 11-18 ms with 4 queues, 0.47 ms with
-# 8). Set before HIP starts; an explicit setting wins.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+# 8). At least 8, set before HIP starts.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 METRIC = "aligned pairs/sec (gap-affine, 150bp + 1-10kbp, bit-exact CIGAR) + p99 latency"
 HBM_PEAK_GBS = 8000.0
