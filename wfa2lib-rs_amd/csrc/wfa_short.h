@@ -64,6 +64,8 @@ constexpr int SH_WIN = kShortWindow;     // diagonals of a pair's window
 constexpr int SH_CHUNKS = SH_WIN / 64;   // most 64-diagonal chunks a step computes
 constexpr int SH_ROW = SH_WIN + 8;       // LDS cells per row: pad, window, pad (16-byte rows)
 constexpr int SH_NULL = -16384;          // OFFSET_NULL as held in 16 bits
+constexpr int SH_GRAB = 4;               // score-only work items a wave takes at once
+constexpr int SH_GRAB_TAIL = 2;          // ... until this many runs per workgroup are left
 constexpr int SH_HDR_EMPTY = 0;          // header of an absent M row: no chunks
 
 // Ring classes: M rows and I1/D1 rows per ring (powers of two, so a slot is
@@ -619,10 +621,29 @@ __global__ void __launch_bounds__(64, MODE == 1 ? 6 : 1) wfa_short_kernel(KParam
   // phase clocks (WFA_PHASE_CLOCKS builds): pairs, staging, extension and
   // whole-pair cycles of this wave
   uint64_t npairs = 0, stage_cycles = 0, ext_cycles = 0, pair_cycles = 0, bt_cycles = 0;
+  // Score-only pairs take a few microseconds each, and one atomic per pair
+  // on the launch's single queue counter limits them: they are taken
+  // SH_GRAB at a time, one at a time again once the queue is nearly empty
+  // (the last SH_GRAB_TAIL runs per workgroup), so the waves still finish
+  // together. CIGAR pairs take one at a time.
+  const int tail = p.num_pairs - SH_GRAB_TAIL * SH_GRAB * (int)gridDim.x;
+  int next = 0, end = 0;
   for (;;) {
     int item = 0;
-    if (threadIdx.x == 0) item = atomicAdd(&p.counters[0], 1);
-    item = uniform(__shfl(item, 0));
+    if constexpr (MODE == 0) {
+      if (uniform(next) == uniform(end)) {
+        const int g = uniform(end) < tail ? SH_GRAB : 1;
+        int got = 0;
+        if (threadIdx.x == 0) got = atomicAdd(&p.counters[0], g);
+        next = uniform(__shfl(got, 0));
+        end = next + g;
+      }
+      item = uniform(next);
+      next = item + 1;
+    } else {
+      if (threadIdx.x == 0) item = atomicAdd(&p.counters[0], 1);
+      item = uniform(__shfl(item, 0));
+    }
     if (item >= p.num_pairs) break;
 #ifdef WFA_PHASE_CLOCKS
     const uint64_t c0 = __builtin_readcyclecounter();
