@@ -36,14 +36,15 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# The serving leg runs one HIP stream per server engine beside this
-# process's other streams. HIP maps streams onto at most GPU_MAX_HW_QUEUES
-# hardware queues (4 unless set), and streams that share a queue run their
-# kernels in order: a 150 bp launch queued behind a 1-10 kbp launch waits
-# for it (p99 of the 150 bp requests 11-18 ms with 4 queues, 0.47 ms with
-# 8). At least 8, set before HIP starts.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+# The serving leg runs one HIP stream per server engine (SERVING_ENGINES)
+# beside this process's other streams. HIP maps streams onto at most
+# GPU_MAX_HW_QUEUES hardware queues (4 unless set), and streams that share a
+# queue run their kernels in order: a 150 bp launch queued behind a 1-10 kbp
+# launch waits for it (with 4 engines, p99 of the 150 bp requests 11-18 ms
+# on 4 queues, 0.47 ms on 8). At least 16, set before HIP starts.
+SERVING_ENGINES = 8
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 METRIC = "aligned pairs/sec (gap-affine, 150bp + 1-10kbp, bit-exact CIGAR) + p99 latency"
 HBM_PEAK_GBS = 8000.0
@@ -118,7 +119,10 @@ def serving_leg(n, rate, lengths, seed, oracle_sample, log):
 
     mix = serving_mix(n, rate, seed=seed, lengths=lengths)
     attrs = [request_attr(r.span, r.heuristic, len(r.pattern), len(r.text)) for r in mix]
-    srv = AlignmentServer(device=0)
+    # 8 engines: up to 4 long-read launches at once (half the engines), so
+    # 1-10 kbp requests queue less (3000-request mix p99 49 ms with 4
+    # engines, 21 ms with 8; 150 bp p99 0.48 / 0.60 ms)
+    srv = AlignmentServer(device=0, num_engines=SERVING_ENGINES)
     # warm the engines (device buffers, code objects) on requests of the
     # same mix (another seed)
     warm = serving_mix(min(400, n), 1e9, seed=seed + 1000, lengths=lengths)
@@ -184,7 +188,7 @@ def serving_leg(n, rate, lengths, seed, oracle_sample, log):
             same += int(r.score == sc[j] and r.status == sts[j] and r.cigar == cg[j])
     cpu_sec = time.perf_counter() - t
     out = {
-        "requests": n, "rate_per_s": rate,
+        "requests": n, "rate_per_s": rate, "engines": SERVING_ENGINES,
         "pairs_per_s": round(n / (t_end - t0), 3),
         "latency_ms_p50": round(float(np.percentile(lat, 50)), 3),
         "latency_ms_p99": round(float(np.percentile(lat, 99)), 3),

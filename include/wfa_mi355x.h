@@ -479,7 +479,8 @@ void wfa_server_config_default(wfa_server_config_t* config);
  * on at most GPU_MAX_HW_QUEUES hardware queues (4 unless the environment
  * sets it before HIP starts), and streams that share a queue run their
  * kernels in order, so a short request's launch can wait behind a long
- * one's: give a serving process at least num_engines + 4 queues. */
+ * one's: give a serving process at least num_engines + 4 queues. Long-read
+ * launches (sequences over 1024 bases) may use half the engines at once. */
 wfa_server_t* wfa_server_new(const wfa_server_config_t* config, int32_t* err);
 /* Queue one request (validated as the aligner validates its configuration:
  * WFA_ERR_INVALID_PENALTIES, WFA_ERR_UNSUPPORTED, WFA_ERR_INVALID_ARGUMENT).
