@@ -91,6 +91,20 @@ def test_reads100_300_packed_match_oracle(scope, pkg, N, engine, oracle):
     _compare(engine, oracle, attr, pairs, scope == "alignment", label=scope)
 
 
+def test_reads100_300_large_launch_match_oracle(pkg, N, engine, oracle):
+    """A launch large enough that score-only waves take their pairs several
+    at a time from the work queue (until its last runs, then one at a
+    time): 200,000 pairs, score only and CIGAR, equal to the oracle, every
+    pair aligned once."""
+    from wfa2lib_rs_amd.synth import WORKLOADS, generate
+    pairs = generate(WORKLOADS["reads100_300"], num_pairs=200_000)
+    for scope in ("score", "alignment"):
+        attr = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), scope)
+        s, st, _ = _compare(engine, oracle, attr, pairs, scope == "alignment",
+                            label=f"large/{scope}")
+        assert (st == N.STATUS_COMPLETED).all(), scope
+
+
 @pytest.mark.parametrize("pen", [(0, 6, 4, 2, 0, 0), (0, 8, 12, 4, 0, 0), (0, 5, 3, 2, 0, 0),
                                  (0, 3, 1, 1, 0, 0), (0, 4, 20, 1, 0, 0)],
                          ids=["step2", "step4", "step1_wide_ring", "step1", "beyond_rings"])
