@@ -396,6 +396,107 @@ struct WfDir {
     }
   }
 
+  // One score of a search: compute(s), then extend(s); the caller checks
+  // `overflow`. Edit and indel searches without a heuristic or a history
+  // extend each M cell as they compute it, and write the row once: the
+  // extension of a cell reads only that cell and the sequences, and the
+  // trims test the unextended offsets, which the extension keeps inside
+  // the sequences.
+  __device__ bool step(int s, int comp_end, int* max_ak) {
+    if constexpr (KIND == KIND_EDIT && !HEUR && !HISTORY) {
+      return edit_step(s, comp_end, max_ak);
+    } else {
+      compute(s);
+      if (overflow) return false;
+      return extend(s, comp_end, max_ak);
+    }
+  }
+
+  // compute(s) and extend(s) of the edit/indel step in one pass
+  // (src/aligner.rs:1769-1865, 3516-3554).
+  __device__ bool edit_step(int s, int comp_end, int* max_ak) {
+    const int lane = threadIdx.x;
+    *max_ak = 0;
+    const int xrow = row(C_M, s - 1);
+    const int4 xm = R.meta[xrow];
+    const int xl = uniform(xm.x), xh = uniform(xm.y);
+    __syncthreads();
+    if (xl > xh) {
+      nulls += 1;
+      set_absent(s);
+      return nulls > scope ? (status = WFA_STATUS_END_UNREACHABLE, true) : false;
+    }
+    nulls = 0;
+    // the two cells beyond each end of the previous row are NULL for this
+    // step (src/aligner.rs:1796-1806): reads are guarded by [xl, xh]
+    const int lo = imax(xl - 1, hist_lo + 1);
+    const int hi = imin(xh + 1, hist_hi - 1);
+    if (lo > hi) {
+      set_absent(s);
+      return false;
+    }
+    if (hi - lo + 1 > R.wcap) { overflow = true; return false; }
+    const int rM = row(C_M, s);
+    int m_lo = INT32_MAX, m_hi = INT32_MIN, mx = 0;
+    for (int kb = lo + 64 * (lane >> 6); kb <= hi; kb += NT) {
+      const int k = kb + (lane & 63);
+      bool vm = false;
+      if (k <= hi) {
+        const int ins = rd(R, xrow, xl, xh, k - 1);
+        const int del = rd(R, xrow, xl, xh, k + 1);
+        int m;
+        if (p->metric == WFA_METRIC_INDEL) {
+          m = imax(del, ins + 1);
+        } else {
+          const int mis = rd(R, xrow, xl, xh, k);
+          m = imax(del, imax(ins, mis) + 1);
+        }
+        vm = in_bounds(m, k, plen, tlen);
+        if (vm) {
+          const int v = m - k;
+          m += rev ? backward_matches<BITS>(sP, sT, plen - v, tlen - m)
+                   : lcp<BITS>(sP, sT, v, m, plen, tlen);
+          mx = imax(mx, 2 * m - k);
+        }
+        R.set(rM * R.wcap + (k & R.mask), vm ? m : WFA_NULL);
+      }
+      const uint64_t b = ballot(vm);
+      if (b) {
+        if (m_lo == INT32_MAX) m_lo = kb + __builtin_ctzll(b);
+        m_hi = kb + 63 - __builtin_clzll(b);
+      }
+    }
+    int amax = uniform(wave_max(mx));
+    if (NW > 1) {
+      if ((lane & 63) == 0) {
+        red[2 * (lane >> 6)] = m_lo;
+        red[2 * (lane >> 6) + 1] = m_hi;
+        red[10 * NW + (lane >> 6)] = amax;
+      }
+      __syncthreads();
+      m_lo = INT32_MAX; m_hi = INT32_MIN;
+      for (int w = 0; w < NW; ++w) {
+        m_lo = imin(m_lo, red[2 * w]);
+        m_hi = imax(m_hi, red[2 * w + 1]);
+        amax = imax(amax, red[10 * NW + w]);
+      }
+    }
+    if (m_lo == INT32_MAX) { m_lo = 1; m_hi = 0; }
+    __syncthreads();
+    if (lane == 0) R.meta[rM] = make_int4(m_lo, m_hi, m_lo, m_hi);
+    __syncthreads();
+    *max_ak = amax;
+    // check_component_termination (src/aligner.rs:3269-3302)
+    const int k_end = tlen - plen;
+    if (comp_end == C_M && k_end >= m_lo && k_end <= m_hi &&
+        uniform(cell(C_M, s, k_end)) >= tlen) {
+      status = WFA_STATUS_END_REACHED;
+      end_score = s; end_k = k_end; end_off = tlen;
+      return true;
+    }
+    return false;
+  }
+
   // extend_biwfa_step (src/aligner.rs:3516-3554). Returns true when the
   // search is done (end reached on `comp_end`, or unreachable); *max_ak is
   // the largest antidiagonal 2*offset - k of an extended cell (0 if none).
@@ -649,17 +750,15 @@ static __device__ BiResult find_breakpoint(const BiParams& bp, const BiJob& J, u
   if (r.extend(0, rev_end, &rmax)) return quit(r);
   while (fmax + rmax < max_ad) {
     s_fwd += 1;
-    f.compute(s_fwd);
+    bool done = f.step(s_fwd, fwd_end, &ak);
     if (f.overflow) return retry();
-    bool done = f.extend(s_fwd, fwd_end, &ak);
     fmax = imax(fmax, ak);
     last_fwd = true;
     if (done) return quit(f);
     if (fmax + rmax >= max_ad) break;
     s_rev += 1;
-    r.compute(s_rev);
+    done = r.step(s_rev, rev_end, &ak);
     if (r.overflow) return retry();
-    done = r.extend(s_rev, rev_end, &ak);
     rmax = imax(rmax, ak);
     last_fwd = false;
     if (done) return quit(r);
@@ -677,17 +776,17 @@ static __device__ BiResult find_breakpoint(const BiParams& bp, const BiJob& J, u
       if (s_fwd + min_rev - gap_open >= b.score) break;
       bi_overlap<KIND>(f, r, s_fwd, s_rev, true, b);
       s_rev += 1;
-      r.compute(s_rev);
+      const bool rdone = r.step(s_rev, rev_end, &ak);
       if (r.overflow) return retry();
-      if (r.extend(s_rev, rev_end, &ak)) return quit(r);
+      if (rdone) return quit(r);
     }
     const int min_fwd = s_fwd > scope - 1 ? s_fwd - (scope - 1) : 0;
     if (min_fwd + s_rev - gap_open >= b.score) break;
     bi_overlap<KIND>(r, f, s_rev, s_fwd, false, b);
     s_fwd += 1;
-    f.compute(s_fwd);
+    const bool fdone = f.step(s_fwd, fwd_end, &ak);
     if (f.overflow) return retry();
-    if (f.extend(s_fwd, fwd_end, &ak)) return quit(f);
+    if (fdone) return quit(f);
     if (s_fwd + s_rev >= p.max_steps) { res.status = BI_MAX_STEPS; return res; }
     if (runaway()) { res.status = BI_UNREACHABLE; return res; }
     last_fwd = true;
