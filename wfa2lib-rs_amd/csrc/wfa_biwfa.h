@@ -140,6 +140,10 @@ struct WfDir {
   int hdr_cap;      // scores
   int32_t* cells;
   int64_t cap, hpos;
+  // searches without a heuristic or a history extend M as they compute it
+  // (step); the largest antidiagonal of the step's extended cells
+  static constexpr bool FUSED = !HEUR && !HISTORY;
+  int step_amax;
 
   __device__ int row(int c, int s) const { return c * scope + s % scope; }
   __device__ int cell(int c, int s, int k) const {
@@ -325,6 +329,7 @@ struct WfDir {
       const int rI2 = two ? row(C_I2, s) : 0, rD2 = two ? row(C_D2, s) : 0;
       int lim[5][2];
       for (int c = 0; c < 5; ++c) { lim[c][0] = INT32_MAX; lim[c][1] = INT32_MIN; }
+      int ext_max = 0;  // (FUSED) antidiagonals of the extended M cells
       for (int kb = lo + 64 * (lane >> 6); kb <= hi; kb += NT) {
         const int k = kb + (lane & 63);
         bool v[5] = {false, false, false, false, false};
@@ -352,7 +357,14 @@ struct WfDir {
           }
           int m = imax(mis, imax(ins, del));
           v[C_M] = in_bounds(m, k, plen, tlen);
-          if (!v[C_M]) m = WFA_NULL;
+          if (!v[C_M]) {
+            m = WFA_NULL;
+          } else if constexpr (FUSED) {
+            const int vv = m - k;
+            m += rev ? backward_matches<BITS>(sP, sT, plen - vv, tlen - m)
+                     : lcp<BITS>(sP, sT, vv, m, plen, tlen);
+            ext_max = imax(ext_max, 2 * m - k);
+          }
           R.set(rM * R.wcap + (k & R.mask), m);
           if (GAPS) {
             R.set(rI1 * R.wcap + (k & R.mask), iv);
@@ -369,12 +381,15 @@ struct WfDir {
           }
         }
       }
+      int amax = FUSED ? uniform(wave_max(ext_max)) : 0;
       if (NW > 1) {
-        if ((lane & 63) == 0)
+        if ((lane & 63) == 0) {
           for (int c = 0; c < NC; ++c) {
             red[10 * (lane >> 6) + 2 * c] = lim[c][0];
             red[10 * (lane >> 6) + 2 * c + 1] = lim[c][1];
           }
+          if (FUSED) red[10 * NW + (lane >> 6)] = amax;
+        }
         __syncthreads();
         for (int c = 0; c < NC; ++c) {
           lim[c][0] = INT32_MAX; lim[c][1] = INT32_MIN;
@@ -383,7 +398,10 @@ struct WfDir {
             lim[c][1] = imax(lim[c][1], red[10 * w + 2 * c + 1]);
           }
         }
+        if (FUSED)
+          for (int w = 0; w < NW; ++w) amax = imax(amax, red[10 * NW + w]);
       }
+      step_amax = amax;
       for (int c = 0; c < NC; ++c)
         if (lim[c][0] == INT32_MAX) { lim[c][0] = 1; lim[c][1] = 0; }
       __syncthreads();
@@ -403,13 +421,43 @@ struct WfDir {
   // trims test the unextended offsets, which the extension keeps inside
   // the sequences.
   __device__ bool step(int s, int comp_end, int* max_ak) {
-    if constexpr (KIND == KIND_EDIT && !HEUR && !HISTORY) {
+    if constexpr (KIND == KIND_EDIT && FUSED) {
       return edit_step(s, comp_end, max_ak);
+    } else if constexpr (FUSED) {
+      // compute(s) extended M already; what extend(s) does after that
+      step_amax = 0;
+      compute(s);
+      if (overflow) return false;
+      *max_ak = 0;
+      const int4 mm = R.meta[row(C_M, s)];
+      if (uniform(mm.x) == META_ABSENT) {
+        if (nulls > scope) {
+          status = WFA_STATUS_END_UNREACHABLE;
+          return true;
+        }
+        return false;
+      }
+      *max_ak = step_amax;
+      return end_reached(s, comp_end);
     } else {
       compute(s);
       if (overflow) return false;
       return extend(s, comp_end, max_ak);
     }
+  }
+
+  // check_component_termination (src/aligner.rs:3269-3302) at score s
+  __device__ bool end_reached(int s, int comp_end) {
+    const int4 ce = R.meta[row(comp_end, s)];
+    const int clo = uniform(ce.x), chi = uniform(ce.y);
+    const int k_end = tlen - plen;
+    if (clo != META_ABSENT && k_end >= clo && k_end <= chi &&
+        uniform(cell(comp_end, s, k_end)) >= tlen) {
+      status = WFA_STATUS_END_REACHED;
+      end_score = s; end_k = k_end; end_off = tlen;
+      return true;
+    }
+    return false;
   }
 
   // compute(s) and extend(s) of the edit/indel step in one pass
@@ -537,16 +585,7 @@ struct WfDir {
     *max_ak = amax;
     __syncthreads();
     store(C_M, s, lo, hi);
-    // check_component_termination (src/aligner.rs:3269-3302)
-    const int4 ce = R.meta[row(comp_end, s)];
-    const int clo = uniform(ce.x), chi = uniform(ce.y);
-    const int k_end = tlen - plen;
-    if (clo != META_ABSENT && k_end >= clo && k_end <= chi &&
-        uniform(cell(comp_end, s, k_end)) >= tlen) {
-      status = WFA_STATUS_END_REACHED;
-      end_score = s; end_k = k_end; end_off = tlen;
-      return true;
-    }
+    if (end_reached(s, comp_end)) return true;
     if (HEUR && heur != WFA_HEURISTIC_NONE) cutoff(s, lo, hi, mm);
     return false;
   }
