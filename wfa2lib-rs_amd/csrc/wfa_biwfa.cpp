@@ -118,14 +118,24 @@ int run_jobs(wfa_engine_t* e, const KParams& kp, int kind, bool heur, int which,
     const size_t lds = (size_t)2 * bp.seq_words * 8 + meta_bytes +
                        (which == 0 ? (size_t)64 * wfa::kWideWaves : 0);
     if (lds > e->lds_max) return WFA_ERR_UNSUPPORTED;
-    // rows: no row is wider than the sub-problem's diagonal range
-    const int full = ceil_pow2(max_sum + 2 * kp.scope + 4);
-    const int first_cap = which == 0 ? 16384 : 4096;
-    bp.wcap = pass == 0 ? std::min(full, first_cap) : full;
-    const size_t ring_slot = (size_t)dirs * NC * kp.scope * bp.wcap * 4;
     int grid = e->num_cus * std::max(1, wfa::biwfa_blocks_per_cu(kind, which, heur, lds));
     grid = std::min(grid, m);
     const size_t ring_budget = std::min<size_t>(kRingBudget, (size_t)e->scratch_budget / 2);
+    // Rows: no row is wider than the sub-problem's diagonal range. A first
+    // breakpoint pass takes the widest power of two whose rings fit the
+    // budget for the whole grid (at least 16384 diagonals): a sub-problem
+    // whose range outgrows its rows starts again at full width, after the
+    // work it did. Base cases start at 4096. Later passes take the full
+    // width.
+    const int full = ceil_pow2(max_sum + 2 * kp.scope + 4);
+    int first_cap = 4096;
+    if (which == 0) {
+      const size_t per_diag = (size_t)dirs * NC * kp.scope * 4 * grid;
+      first_cap = 16384;
+      while (first_cap < full && (size_t)first_cap * 2 * per_diag <= ring_budget) first_cap *= 2;
+    }
+    bp.wcap = pass == 0 ? std::min(full, first_cap) : full;
+    const size_t ring_slot = (size_t)dirs * NC * kp.scope * bp.wcap * 4;
     const size_t hist_budget = std::min<size_t>(kHistBudget, (size_t)e->scratch_budget / 4);
     grid = (int)std::max<size_t>(1, std::min<size_t>(grid, ring_budget / ring_slot));
     if (which == 1) {
