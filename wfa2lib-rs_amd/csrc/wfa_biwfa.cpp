@@ -87,8 +87,13 @@ int run_jobs(wfa_engine_t* e, const KParams& kp, int kind, bool heur, int which,
   if (n == 0) return WFA_OK;
   const int NC = nc_of(kind);
   const int dirs = which == 0 ? 2 : 1;
+  // Largest sub-problems first (the launch's workgroups take them from one
+  // queue in this order), so the longest searches do not start last and
+  // stretch the launch
   std::vector<int> todo(n);
   for (int i = 0; i < n; ++i) todo[i] = i;
+  auto size_of = [&](int i) { return (int64_t)(jobs[i].pe - jobs[i].pb) + (jobs[i].te - jobs[i].tb); };
+  std::stable_sort(todo.begin(), todo.end(), [&](int a, int b) { return size_of(a) > size_of(b); });
   if (results) results->assign(n, BiResult{});
   if (frag_len) {
     frag_len->assign(n, 0);
