@@ -1754,13 +1754,20 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
     if (MED && !halloc(whi - wlo + 1, &hrow, wlo, whi)) { overflow = true; break; }
     // winner flags of diagonal k at fbase + (k - lo)
     const int64_t fbase = MED ? hist_row(H, hrow) + (lo - wlo) : 0;
+    // Several waves, end to end, high memory: the extended M row goes to
+    // the history as computed, over the computed range (its cells outside
+    // the trimmed range are NULL, and the backtrace reads NULL and missing
+    // cells alike)
+    const bool hfused = MODE == 1 && NW > 1 && !ef;
+    if (hfused && !halloc(whi - wlo + 1, &hrow, wlo, whi)) { overflow = true; break; }
+    const int64_t hbase = hfused ? hist_row(H, hrow) - wlo : 0;
 
-    // Several waves, end to end, score only or medium/low memory: each lane
-    // extends the M cell it computes, and the row is written once (the
-    // extension of a cell reads only that cell and the sequences; the
-    // trims test the unextended offsets, which the extension keeps within
-    // the sequences). The end test is the end diagonal's cell.
-    constexpr bool FUSE = NW > 1 && MODE != 1;
+    // Several waves, end to end: each lane extends the M cell it computes,
+    // and the row is written once (the extension of a cell reads only that
+    // cell and the sequences; the trims test the unextended offsets, which
+    // the extension keeps within the sequences). The end test is the end
+    // diagonal's cell.
+    constexpr bool FUSE = NW > 1;
     const bool fuse = FUSE && !ef;
     bool freach = false;
     // ---- compute all chunks; trims by ballot ----
@@ -1906,6 +1913,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
           if (k == k_end && m >= tlen) freach = true;
         }
         R.set(rowM * R.wcap + (k & R.mask), m);
+        if (MODE == 1 && FUSE && fuse) H.cells[hbase + k] = m;
         if (MED) H.bytes[fbase + (k - lo)] = (uint8_t)fl;
         if (MED && zc) {
           // the offsets' signs (offload_selective moves only offsets >= 0)
@@ -2087,7 +2095,7 @@ static __device__ void align_pair(const KParams& p, int pair, int plen, int tlen
       }
       __syncthreads();
     }
-    if (MED && tid == 0) H.hdr[s] = hrow;
+    if ((MED || (MODE == 1 && FUSE && fuse)) && tid == 0) H.hdr[s] = hrow;
     int zo = 0;
     if (zc && !(seeds >= 0 && (seeds & SEED_ONLY))) {
       // this step's occupancies (a begin-free-only M is a fresh wavefront)
