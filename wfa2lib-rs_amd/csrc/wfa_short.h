@@ -102,11 +102,16 @@ struct WinView {
   const int16_t* cells;  // the slot's history: rows of packed 64-cell chunks
   const int32_t* hdr;    // per reduced score t (score t << gsh): the row's header (sh_hdr)
   int base, nscores, nwf, gsh;
+  int64_t cap;           // cells in the slot's history (tested in WFA_DEVICE_CHECKS builds)
+  int32_t* chk;          // the launch's counters
+  __device__ __forceinline__ int cell(int i) const {
+    return cells[checked_index<int16_t>(i, cap, chk)];
+  }
   __device__ __forceinline__ bool m(int s, int k, int* out) const {
     if (s < 0 || s >= nscores || s >= nwf || (s & ((1 << gsh) - 1))) return false;
     const int i = sh_cell(hdr[s >> gsh], base, k);
     if (i < 0) return false;
-    const int v = cells[i];
+    const int v = cell(i);
     if (v < 0) return false;
     *out = v;
     return true;
@@ -121,8 +126,8 @@ static __device__ __forceinline__ void win_cells(const WinView& hv, const int32_
   if (s < 0 || s >= hv.nscores || s >= hv.nwf || (s & ((1 << hv.gsh) - 1))) return;
   const int32_t h = lhdr[s >> hv.gsh];
   const int i1 = sh_cell(h, hv.base, k1), i2 = sh_cell(h, hv.base, k2);
-  if (i1 >= 0) v1 = hv.cells[i1];
-  if (i2 >= 0) v2 = hv.cells[i2];
+  if (i1 >= 0) v1 = hv.cell(i1);
+  if (i2 >= 0) v2 = hv.cell(i2);
 }
 
 // Singletrack backtrace (src/backtrace.rs:751-858) of the short kernel's
@@ -315,6 +320,8 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
     return reinterpret_cast<int16_t*>(reinterpret_cast<char*>(rows) + byte_off) + 1 - base;
   };
   const int nwf = uniform(high_mem_nwf(p, plen, tlen));
+  // cells in the slot's history (index checks of WFA_DEVICE_CHECKS builds)
+  const int64_t hist_cells = CIG ? (int64_t)p.hdr_cap * SH_WIN : 0;
   int status = WFA_STATUS_OK, result = 0, t = 0;
   bool overflow = false;
   __syncthreads();
@@ -483,7 +490,7 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
       // reads M[t] in between), keep it, test the end cell
       // (termination_end2end, src/termination.rs:14-23) ----
       bool reached = false;
-      int16_t* hrow = CIG ? hcells + (int64_t)hnext * 64 : nullptr;
+      const int64_t hrow = (int64_t)hnext * 64;
 #ifdef WFA_PHASE_CLOCKS
       const uint64_t c0 = __builtin_readcyclecounter();
 #endif
@@ -493,7 +500,7 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
         const int off = sh_extend(P32, T32, mv[j], k, plen, tlen);
         const int16_t o16 = sh16(off);
         Wm[k] = o16;
-        if (CIG) hrow[64 * j + lane] = o16;
+        if (CIG) hcells[checked_index<int16_t>(hrow + 64 * j + lane, hist_cells, p.counters)] = o16;
         reached |= k == k_end && off >= tlen;
       }
 #ifdef WFA_PHASE_CLOCKS
@@ -545,7 +552,7 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
       __threadfence_block();
       uint8_t* out = p.cig + p.cig_off[pair];
       const int cap = plen + tlen;
-      const WinView hv{hcells, hhdr, base, result + 1, nwf, gsh};
+      const WinView hv{hcells, hhdr, base, result + 1, nwf, gsh, hist_cells, p.counters};
       // The ring rows are done with: the row headers (rows 0..t) and the ops
       // go there when they fit, and the ops from there to the pair's region
       // in order.
