@@ -130,7 +130,7 @@ def test_per_pair_path_equals_batch(pkg, N, engine, oracle):
 
 
 def test_wide_and_long_pairs_take_the_retry_path(pkg, N, engine, oracle):
-    """Pairs whose wavefronts outgrow the first launch's 512-diagonal window
+    """Pairs whose wavefronts outgrow the first launch's 448-diagonal window
     (long, divergent, unequal lengths) are re-run wider; results still match."""
     from wfa2lib_rs_amd.synth import Workload, generate
     w = Workload("wide", 40, 1, 600, 1200, 0.15, 0.25, 4.0)

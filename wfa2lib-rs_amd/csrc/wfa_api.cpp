@@ -138,7 +138,7 @@ int plan(wfa_engine_t* e, const KParams& p, int kind, int mode, int fmt, int max
   if (c->lds_bytes > e->lds_max) return WFA_ERR_UNSUPPORTED;
   // First pass of gap-affine end-to-end alignments without a heuristic,
   // score only or with high-memory CIGARs: the short-read kernel (a fixed
-  // 512-diagonal window per pair, 16-bit rows, a per-slot history of packed
+  // 448-diagonal window per pair, 16-bit rows, a per-slot history of packed
   // rows); pairs that leave its window come back for the general kernel.
   // Its rings hold a fixed number of rows (the ring class), enough for the
   // lookbacks of most penalty sets once scores step by their common power

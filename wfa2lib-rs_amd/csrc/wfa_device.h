@@ -179,7 +179,10 @@ size_t short_lds_row_bytes_host(int rc);
 // 2^gsh in *gsh): 2 for the built-in default penalties, else 0 or 1, or -1
 // when their reduced lookbacks exceed every class
 int short_ring_class(int x, int oe, int e, int* gsh);
-constexpr int kShortWindow = 512;
+// diagonals of the short-read kernel's window: 7 chunks of 64 (reads100_300
+// CIGAR launch 12.45 ms at 512, 11.86 ms at 448; at 384 one pair of 1M leaves
+// the window and its retry pass takes the step to 12.74 ms)
+constexpr int kShortWindow = 448;
 
 hipError_t launch_reset(int32_t* counters, hipStream_t stream);
 hipError_t launch_max_length(const int32_t* plen, const int32_t* tlen, int n, int32_t* out,

@@ -16,7 +16,7 @@
 //   penalties divided by g, and accounts for the real scores it passes over
 //   exactly: each counts towards the run of null steps (src/aligner.rs:
 //   2031-2043) and meets the per-score limits in order.
-// * Each pair gets a fixed window of 512 diagonals [base, base + 512),
+// * Each pair gets a fixed window of 448 diagonals [base, base + 448),
 //   centred on the diagonals its alignment must cross. Every ring row is an
 //   LDS row over the whole window plus pad cells on each side, all NULL when
 //   the pair starts. A step writes its three output rows over the
