@@ -7,7 +7,7 @@
 csrc/*.cpp; ``libwfa_synth.so`` with gcc. Outputs go to ``lib/``.
 ``--phase-clocks`` builds ``libwfa_mi355x_phases.so`` with WFA_PHASE_CLOCKS:
 the short-read kernel sums its waves' cycles for sequence staging, the
-extension and whole pairs (wfa_engine_last_phase_cycles).
+extension, the CIGAR backtrace and whole pairs (wfa_engine_last_phase_cycles).
 ``--checks`` builds ``libwfa_mi355x_checks.so`` with WFA_DEVICE_CHECKS: every
 ring and history index is tested on the device and a call that found one
 out of range fails with WFA_ERR_DEVICE; set WFA_MI355X_LIB to its file name
