@@ -130,6 +130,24 @@ static __device__ __forceinline__ void win_cells(const WinView& hv, const int32_
   if (i2 >= 0) v2 = hv.cell(i2);
 }
 
+// ops[pos], ops[pos - 1], ...: n1 copies of op1, then n2 of op2 (counts
+// below zero write none), in one masked store when they fit the wave;
+// pos moves past them
+static __device__ __forceinline__ void put_ops2(uint8_t* out, int& pos, uint8_t op1, int n1,
+                                                uint8_t op2, int n2) {
+  const int lane = threadIdx.x;
+  n1 = imax(n1, 0);
+  n2 = imax(n2, 0);
+  const int n = n1 + n2;
+  if (n <= 64) {
+    if (lane < n && pos - lane >= 0) out[pos - lane] = lane < n1 ? op1 : op2;
+    pos -= n;
+  } else {
+    write_ops(out, pos, op1, n1);
+    write_ops(out, pos, op2, n2);
+  }
+}
+
 // Singletrack backtrace (src/backtrace.rs:751-858) of the short kernel's
 // end-to-end gap-affine alignments: the steps of `singletrack` for one gap
 // piece, arranged for latency. At each M cell the wave loads the mismatch
@@ -147,8 +165,7 @@ static __device__ __forceinline__ int short_backtrace(const KParams& p, const Wi
   const int lane = threadIdx.x;
   const int x = p.x, o = p.o1, e = p.e1;
   int pos = cap - 1;
-  write_ops(ops, pos, 'D', plen - (offset - k));
-  write_ops(ops, pos, 'I', tlen - offset);
+  put_ops2(ops, pos, 'D', plen - (offset - k), 'I', tlen - offset);
   while (score != 0) {
     score = uniform(score);
     k = uniform(k);
@@ -162,8 +179,7 @@ static __device__ __forceinline__ int short_backtrace(const KParams& p, const Wi
     const int nm = backward_matches<2>(sP, sT, offset - k, offset);
     const int orig = offset - nm;
     if (mo >= 0 && mo == orig - 1) {
-      write_ops(ops, pos, 'M', nm);
-      write_ops(ops, pos, 'X', 1);
+      put_ops2(ops, pos, 'M', nm, 'X', 1);
       offset = orig - 1;
       score -= x;
       continue;
@@ -186,12 +202,10 @@ static __device__ __forceinline__ int short_backtrace(const KParams& p, const Wi
       const int fj = uniform(__shfl((ri ? 1 : 0) | (rd ? 2 : 0), j));
       const int m = uniform(__shfl(ri ? mi : md, j));
       if (fj & 1) {
-        write_ops(ops, pos, 'M', offset - (m + Lj));
-        write_ops(ops, pos, 'I', Lj);
+        put_ops2(ops, pos, 'M', offset - (m + Lj), 'I', Lj);
         k -= Lj;
       } else if (fj & 2) {
-        write_ops(ops, pos, 'M', offset - m);
-        write_ops(ops, pos, 'D', Lj);
+        put_ops2(ops, pos, 'M', offset - m, 'D', Lj);
         k += Lj;
       } else {
         break;  // no source
@@ -207,8 +221,7 @@ static __device__ __forceinline__ int short_backtrace(const KParams& p, const Wi
   const int nm = imin(v, h);
   v -= nm;
   h -= nm;
-  write_ops(ops, pos, 'M', nm);
-  write_ops(ops, pos, 'D', v);
+  put_ops2(ops, pos, 'M', nm, 'D', v);
   write_ops(ops, pos, 'I', h);
   return cap - 1 - pos;
 }
