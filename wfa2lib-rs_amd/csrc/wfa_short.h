@@ -243,17 +243,36 @@ static __device__ __forceinline__ uint32_t sh_w16(const uint32_t* s, uint32_t po
   return __builtin_amdgcn_alignbit(s[q + 1], s[q], (pos & 15u) * 2u);
 }
 
+// Bases [pos, pos + 32) of a 2-bit sequence staged as 32-bit words: two
+// funnel shifts over three consecutive words (the staged sequences carry
+// two zero words of padding past their end, so the third is in bounds).
+static __device__ __forceinline__ uint64_t sh_w32(const uint32_t* s, uint32_t pos) {
+  const uint32_t q = pos >> 4, sh = (pos & 15u) * 2u;
+  const uint32_t w0 = s[q], w1 = s[q + 1], w2 = s[q + 2];
+  return (uint64_t)__builtin_amdgcn_alignbit(w1, w0, sh) |
+         ((uint64_t)__builtin_amdgcn_alignbit(w2, w1, sh) << 32);
+}
+
 // Matching bases before the first difference of two 16-base words (16 when
 // they are equal).
 static __device__ __forceinline__ int sh_match16(uint32_t x) {
   return x ? (int)(__builtin_ctz(x) >> 1) : 16;
 }
 
+// Matching bases before the first difference of two 32-base words (32 when
+// they are equal).
+static __device__ __forceinline__ int sh_match32(uint64_t x) {
+  return x ? (int)(__builtin_ctzll(x) >> 1) : 32;
+}
+
 // Extend every lane's M cell `off` along its diagonal k (lanes with a NULL
 // cell keep it): the matching prefix of pattern[off - k..] and text[off..]
 // bounded by the sequence ends (extend_matches_packed_end2end's per-diagonal
 // LCP, src/extend.rs:15-38). The first 16 bases are compared for every lane
-// at once; only lanes still matching after them loop, 16 bases at a time.
+// at once; only lanes still matching after them loop, LOOP (16 or 32) bases
+// at a time. The score-only kernel loops 32 at a time (2% faster on 1M
+// 100-300 bp pairs); the CIGAR kernel, held to 80 VGPRs, was 3% slower so.
+template <int LOOP>
 static __device__ __forceinline__ int sh_extend(const uint32_t* P, const uint32_t* T, int off,
                                                 int k, int plen, int tlen) {
   const bool act = off >= 0;
@@ -265,9 +284,10 @@ static __device__ __forceinline__ int sh_extend(const uint32_t* P, const uint32_
   if (ballot(more)) {
     while (more) {
       const int left = rem - n;
-      const int d = sh_match16(sh_w16(P, v + n) ^ sh_w16(T, h + n));
+      const int d = LOOP == 32 ? sh_match32(sh_w32(P, v + n) ^ sh_w32(T, h + n))
+                               : sh_match16(sh_w16(P, v + n) ^ sh_w16(T, h + n));
       n += imin(d, left);
-      more = d == 16 && left > 16;
+      more = d == LOOP && left > LOOP;
     }
   }
   return off + n;
@@ -510,7 +530,7 @@ static __device__ void short_pair(const KParams& p, int pair, int plen, int tlen
 #pragma unroll
       for (int j = 0; j < NCH; ++j) {
         const int k = a + 64 * j + lane;
-        const int off = sh_extend(P32, T32, mv[j], k, plen, tlen);
+        const int off = sh_extend<CIG ? 16 : 32>(P32, T32, mv[j], k, plen, tlen);
         const int16_t o16 = sh16(off);
         Wm[k] = o16;
         if (CIG) hcells[checked_index<int16_t>(hrow + 64 * j + lane, hist_cells, p.counters)] = o16;
