@@ -105,6 +105,42 @@ def test_reads100_300_large_launch_match_oracle(pkg, N, engine, oracle):
         assert (st == N.STATUS_COMPLETED).all(), scope
 
 
+@pytest.mark.parametrize("fmt", ["ascii", "packed2bits"])
+@pytest.mark.parametrize("scope", ["score", "alignment"])
+def test_short_kernel_long_matches_at_sequence_ends(scope, fmt, pkg, N, engine, oracle):
+    """Matching runs that end at every distance from a 16- and 32-base
+    boundary and at the sequence ends: identical pairs of every length
+    1-300, single mismatches at and around multiples of 16 and at the last
+    bases, and one-base indels at the ends (the short kernel's extension
+    compares 16 bases first, then 16 or 32 at a time)."""
+    from wfa2lib_rs_amd.synth import from_ascii
+    rng = np.random.default_rng(16032)
+    flip = {"A": "C", "C": "G", "G": "T", "T": "A"}
+    P, T = [], []
+    for n in range(1, 301):
+        s = "".join(rng.choice(list("ACGT"), n))
+        P.append(s)
+        T.append(s)
+    for n in (33, 48, 64, 65, 96, 97, 128, 200, 299, 300):
+        s = "".join(rng.choice(list("ACGT"), n))
+        for m in sorted({15, 16, 17, 31, 32, 33, 47, 48, 49, 63, 64, 65, n - 2, n - 1}):
+            if 0 <= m < n:
+                P.append(s)
+                T.append(s[:m] + flip[s[m]] + s[m + 1:])
+        for tail in ("A", "CG", "TTT"):
+            P.append(s)
+            T.append(s + tail)
+            P.append(s + tail)
+            T.append(s)
+    pairs = from_ascii([x.encode() for x in P], [x.encode() for x in T],
+                       N.SEQ_ASCII if fmt == "ascii" else N.SEQ_PACKED2BITS)
+    attr = attr_for(N, "gap_affine", (0, 4, 6, 2, 0, 0), scope)
+    s, st, _ = _compare(engine, oracle, attr, pairs, scope == "alignment",
+                        label=f"{scope}/{fmt}")
+    assert (s[:300] == 0).all()
+    assert (st == N.STATUS_COMPLETED).all()
+
+
 @pytest.mark.parametrize("pen", [(0, 6, 4, 2, 0, 0), (0, 8, 12, 4, 0, 0), (0, 5, 3, 2, 0, 0),
                                  (0, 3, 1, 1, 0, 0), (0, 4, 20, 1, 0, 0)],
                          ids=["step2", "step4", "step1_wide_ring", "step1", "beyond_rings"])
