@@ -64,7 +64,7 @@ constexpr int SH_WIN = kShortWindow;     // diagonals of a pair's window
 constexpr int SH_CHUNKS = SH_WIN / 64;   // most 64-diagonal chunks a step computes
 constexpr int SH_ROW = SH_WIN + 8;       // LDS cells per row: pad, window, pad (16-byte rows)
 constexpr int SH_NULL = -16384;          // OFFSET_NULL as held in 16 bits
-constexpr int SH_GRAB = 4;               // score-only work items a wave takes at once
+constexpr int SH_GRAB = 4;               // score-only work items a wave takes at once (8: 1% slower)
 constexpr int SH_GRAB_TAIL = 2;          // ... until this many runs per workgroup are left
 constexpr int SH_HDR_EMPTY = 0;          // header of an absent M row: no chunks
 
